@@ -1,0 +1,262 @@
+/*
+ * uir.h -- C ABI of libuir, an MI355X-native (gfx950, HIP) implementation of
+ * ufo-ir's iterative parallel-beam reconstruction path.
+ *
+ * The boundary replaces ufo-ir's task plugins (one GObject per task, driven by
+ * ufo-core through UfoTaskIface) with plain C: opaque task handles created by
+ * per-plugin factories, string-keyed properties with the reference names,
+ * setup / get_requisition / process, and caller-owned device buffers.  No GLib,
+ * no torch types.  Every entry point cites the reference interface it replaces
+ * (paths relative to the ufo-ir source tree).
+ *
+ * Conventions
+ *   - Every function returns uir_status; on failure uir_last_error() holds a
+ *     thread-local message.
+ *   - Buffers are device pointers (hipMalloc'd or uir_device_malloc'd) with
+ *     UfoRequisition dims order: dims[0] = width (fastest), dims[1] = height.
+ *     Images are [y][x]; sinograms are [angle][detector].
+ *   - A task handle is thread-compatible, not thread-safe: one thread at a time
+ *     per task, as one reference task instance is not reentrant
+ *     (ufo-ir-parallel-projector-task.c:539-547 sets shared kernel args).
+ */
+#ifndef UIR_H
+#define UIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UIR_ABI_VERSION 1u
+
+typedef enum {
+    UIR_OK = 0,
+    UIR_ERR_INVALID_ARGUMENT = 1, /* g_error("Incorrect volume size.") etc. in the reference */
+    UIR_ERR_OUT_OF_MEMORY = 2,
+    UIR_ERR_DEVICE = 3,           /* a HIP runtime error */
+    UIR_ERR_UNSUPPORTED = 4,
+    UIR_ERR_SETUP = 5,            /* GError UFO_TASK_ERROR_SETUP in the reference */
+    UIR_ERR_NOT_FOUND = 6,        /* unknown plugin or property */
+    UIR_ERR_BUSY = 7,
+    UIR_ERR_CANCELLED = 8,
+    UIR_ERR_TIMEOUT = 9,
+    UIR_ERR_NO_DEVICE = 10
+} uir_status;
+
+typedef enum { UIR_DTYPE_F32 = 0, UIR_DTYPE_F16 = 1, UIR_DTYPE_BF16 = 2 } uir_dtype;
+
+uint32_t    uir_abi_version(void);
+const char *uir_last_error(void);
+const char *uir_status_string(uir_status s);
+
+/* ------------------------------------------------------------------------ */
+/* Device, memory, streams.  Replaces UfoResources / UfoGpuNode's command    */
+/* queue (ufo-ir-sirt-task.c:170-171) and UfoBuffer's device allocations.    */
+/* ------------------------------------------------------------------------ */
+typedef struct uir_resources uir_resources;
+
+uir_status uir_device_count(int *count);
+/* One resources object per device: owns an in-order stream (the reference's
+ * single in-order cl_command_queue per proc node) and device scratch. */
+uir_status uir_resources_new(int device, uir_resources **out);
+/* Drops the caller's reference; tasks set up on the resources keep their own,
+ * so the stream lives until the last such task is released. */
+void       uir_resources_free(uir_resources *res);
+void      *uir_resources_stream(uir_resources *res);   /* hipStream_t */
+int        uir_resources_device(const uir_resources *res);
+uir_status uir_resources_synchronize(uir_resources *res);
+
+uir_status uir_device_malloc(size_t bytes, void **ptr);
+uir_status uir_device_free(void *ptr);
+uir_status uir_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+uir_status uir_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+uir_status uir_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+uir_status uir_stream_synchronize(void *stream);
+uir_status uir_mem_get_info(size_t *free_bytes, size_t *total_bytes);
+
+/* Device event timing on a given stream (ms between two recorded points). */
+typedef struct uir_timer uir_timer;
+uir_status uir_timer_new(uir_timer **out);
+void       uir_timer_free(uir_timer *t);
+uir_status uir_timer_start(uir_timer *t, void *stream);
+uir_status uir_timer_stop(uir_timer *t, void *stream);
+uir_status uir_timer_elapsed_ms(uir_timer *t, float *ms);
+
+/* ------------------------------------------------------------------------ */
+/* Buffers and requisitions (UfoBuffer / UfoRequisition, 2-D float only).   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    void     *data;     /* device pointer, caller-owned */
+    uint32_t  dims[2];  /* dims[0] = width, dims[1] = height */
+    uir_dtype dtype;    /* tasks take UIR_DTYPE_F32 (the reference is FP32-only) */
+} uir_buffer;
+
+typedef struct {
+    uint32_t n_dims;
+    uint64_t dims[2];
+} uir_requisition;
+
+/* UfoIrProjectionsSubset, ufo-ir-parallel-projector-task.h:49-53 (12 bytes). */
+typedef struct {
+    uint32_t offset;
+    uint32_t n;
+    int32_t  direction; /* 1 = Vertical (|sin| <= |cos|), 0 = Horizontal */
+} uir_subset;
+
+/* UfoIrGeometryDims, ufo-ir-parallel-projector-task.h:55-61. */
+typedef struct {
+    uint64_t height;
+    uint64_t width;
+    uint64_t n_dets;
+    uint64_t n_angles;
+} uir_geometry_dims;
+
+/* ------------------------------------------------------------------------ */
+/* Tasks.  Factories replace the plugin constructors; properties keep the    */
+/* reference names ('-' and '_' are interchangeable, as GLib canonicalises). */
+/* ------------------------------------------------------------------------ */
+typedef struct uir_task uir_task;
+
+/* ufo_ir_parallel_projector_task_new, ufo-ir-parallel-projector-task.c:79-82.
+ * Properties: axis_position (float, -1 = n_dets/2), step (float rad),
+ * relaxation (float), correction_scale (float), is_forward (bool),
+ * model (string, "joseph"), angles_num (uint, 0 = from sinogram)
+ * (ufo-ir-projector-task.c:68-93, ufo-ir-state-dependent-task.c:75-80,
+ *  ufo-ir-parallel-projector-task.c:123-137).
+ * Extension property image_size (uint, 0 = n_dets): the N x N image edge. */
+uir_task *uir_ir_parallel_projector_task_new(void);
+/* ufo_ir_sirt_task_new, ufo-ir-sirt-task.c:142-146.  Properties: projector
+ * (object), num-iterations (uint >= 1, default 10, ufo-ir-method-task.c:74-85),
+ * relaxation_factor (float [0,1], default 0.25, ufo-ir-sirt-task.c:72-77).
+ * Extension: positive_constraint (bool, default false) clamps x >= 0 after
+ * each update. */
+uir_task *uir_ir_sirt_task_new(void);
+/* ufo_ir_sart_task_new, ufo-ir-sart-task.c:146-150 (relaxation_factor :76-81). */
+uir_task *uir_ir_sart_task_new(void);
+/* ufo_ir_asdpocs_task_new, ufo-ir-asdpocs-task.c:370-374.  Properties beta,
+ * beta_red, ng, alpha, alpha_red, r_max, positive_constraint, df_minimizer
+ * (ufo-ir-asdpocs-task.c:104-158). */
+uir_task *uir_ir_asdpocs_task_new(void);
+/* ufo_ir_sbtv_task_new, ufo-ir-sbtv-task.c:209-212.  Properties lambda, mu
+ * (ufo-ir-sbtv-task.c:90-101). */
+uir_task *uir_ir_sbtv_task_new(void);
+/* Factory by plugin name ("parallel-projector", "sirt", "sart", "asdpocs",
+ * "sbtv"), as ufo-core's plugin manager resolves package "ir" + name. */
+uir_task *uir_task_new(const char *plugin);
+
+uir_task   *uir_task_ref(uir_task *task);
+void        uir_task_unref(uir_task *task);
+const char *uir_task_plugin(const uir_task *task);
+
+/* g_object_set / g_object_get on the task's GParamSpecs; values outside a
+ * property's range are rejected with UIR_ERR_INVALID_ARGUMENT and leave the
+ * property unchanged (GLib's behaviour for out-of-range values). */
+uir_status uir_task_set_float(uir_task *task, const char *name, float value);
+uir_status uir_task_set_uint(uir_task *task, const char *name, uint32_t value);
+uir_status uir_task_set_bool(uir_task *task, const char *name, int value);
+uir_status uir_task_set_string(uir_task *task, const char *name, const char *value);
+uir_status uir_task_set_object(uir_task *task, const char *name, uir_task *value);
+uir_status uir_task_get_float(const uir_task *task, const char *name, float *value);
+uir_status uir_task_get_uint(const uir_task *task, const char *name, uint32_t *value);
+uir_status uir_task_get_bool(const uir_task *task, const char *name, int *value);
+uir_status uir_task_get_string(const uir_task *task, const char *name, char *buf, size_t buf_len);
+uir_status uir_task_get_object(const uir_task *task, const char *name, uir_task **value);
+
+/* UfoTaskIface.setup (e.g. ufo-ir-sirt-task.c:148-161). */
+uir_status uir_task_setup(uir_task *task, uir_resources *res);
+/* UfoTaskIface.get_requisition (ufo-ir-method-task.c:240-252 ->
+ * ufo-ir-parallel-projector-task.c:305-347): output (n_dets, n_angles) when the
+ * projector is forward, else the (N, N) image. */
+uir_status uir_task_get_requisition(uir_task *task, const uir_buffer *inputs, uir_requisition *out);
+/* UfoTaskIface.process (e.g. ufo-ir-sirt-task.c:163-222); output is written,
+ * inputs[0] is the sinogram (methods) or the projector's input.  The SBTV task
+ * min-max normalises inputs[0] in place, as the reference does
+ * (ufo-ir-sbtv-task.c:239). */
+uir_status uir_task_process(uir_task *task, const uir_buffer *inputs, uir_buffer *output);
+
+/* ufo_ir_state_dependent_task_forward / _backward,
+ * ufo-ir-state-dependent-task.c:99-137: accumulate into output
+ * (out = out + correction_scale * FP(in), out = out + relaxation * BP(in)). */
+uir_status uir_projector_forward(uir_task *projector, const uir_buffer *input, uir_buffer *output);
+uir_status uir_projector_backward(uir_task *projector, const uir_buffer *input, uir_buffer *output);
+/* ufo_ir_parallel_projector_subset_fp / _bp,
+ * ufo-ir-parallel-projector-task.h:93-94 (.c:167-195). */
+uir_status uir_parallel_projector_subset_fp(uir_task *projector, const uir_buffer *volume,
+                                            uir_buffer *sinogram, const uir_subset *subset);
+uir_status uir_parallel_projector_subset_bp(uir_task *projector, uir_buffer *volume,
+                                            const uir_buffer *sinogram, const uir_subset *subset);
+/* ufo_ir_parallel_projector_get_host_sin_vals / _cos_vals and
+ * generate_full_subsets_list, ufo-ir-parallel-projector-task.c:155-166,462-498;
+ * valid after get_requisition. */
+uir_status uir_parallel_projector_get_luts(uir_task *projector, const float **sin_vals,
+                                           const float **cos_vals, uint32_t *n_angles);
+uir_status uir_parallel_projector_get_subsets(uir_task *projector, const uir_subset **subsets,
+                                              uint32_t *n_subsets);
+
+/* ASD-POCS per-outer-iteration diagnostics (dd, dp, dg, dtgv) of the last
+ * process call, ufo-ir-asdpocs-task.c:478-503; n_rows = num-iterations. */
+uir_status uir_asdpocs_get_diagnostics(uir_task *task, const float **rows4, uint32_t *n_rows);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel-level ABI (the reference's inner boundary).  All pointers are      */
+/* device pointers; stream is a hipStream_t (NULL = default stream).         */
+/* ------------------------------------------------------------------------ */
+
+/* FP_hor / FP_vert, projector-parallel-joseph.cl:36-156 (args :37-45):
+ * w_sino = r_sino + correction_scale * ray_sum for the subset's angles.
+ * dims.width = dims.height = image edge, dims.n_dets = detectors.
+ * r_sino may equal w_sino (the reference binds the same image). */
+uir_status uir_fp_joseph(const float *volume, const float *r_sino, float *w_sino,
+                         const float *sin_lut, const float *cos_lut, uir_geometry_dims dims,
+                         float axis_pos, uir_subset part, float correction_scale, void *stream);
+/* BP, projector-parallel-joseph.cl:158-218 (args :159-167):
+ * w_vol = r_vol + relax * sum over the subset's angles. */
+uir_status uir_bp_joseph(const float *r_volume, float *w_volume, const float *sinogram,
+                         float relax, const float *sin_lut, const float *cos_lut,
+                         uir_geometry_dims dims, float axis_pos, uir_subset part, void *stream);
+
+/* Elementwise kernels, ufo-ir-basic-ops.cl:4-171,255-273 (wrappers
+ * ufo-ir-basic-ops.h:28-84).  n = element count. */
+uir_status uir_op_set(float *out, size_t n, float value, void *stream);                 /* operation_set */
+uir_status uir_op_inv(const float *in, float *out, size_t n, void *stream);             /* operation_inv */
+uir_status uir_op_mul(const float *a, const float *b, float *out, size_t n, void *stream); /* operation_mul */
+uir_status uir_op_add(const float *a, const float *b, float *out, size_t n, void *stream); /* operation_add */
+uir_status uir_op_deduction(const float *a, const float *b, float *out, size_t n, void *stream);
+uir_status uir_op_deduction2(const float *a, const float *b, float m, float *out, size_t n, void *stream);
+uir_status uir_op_add2(const float *a, const float *b, float m, float *out, size_t n, void *stream);
+uir_status uir_op_mul_rows(const float *a, const float *b, float *out, uint32_t width,
+                           uint32_t offset, uint32_t n_rows, void *stream);        /* op_mulRows */
+uir_status uir_op_positive_constraint(const float *in, float *out, size_t n, void *stream); /* POSC */
+
+/* Host-side ops of ufo-ir-basic-ops-processor.c moved to the device. */
+uir_status uir_op_mul_scalar(float *v, size_t n, float m, void *stream);          /* :324-340 */
+uir_status uir_op_sqrt(float *v, size_t n, void *stream);                         /* :415-430 */
+uir_status uir_op_max_ew(const float *a, const float *b, float *out, size_t n, void *stream); /* :590-596 */
+uir_status uir_op_div_ew(const float *a, const float *b, float *out, size_t n, void *stream); /* :598-604 */
+uir_status uir_op_normalization(float *v, size_t n, void *stream);                /* :342-367 */
+
+/* Reductions.  l1: ufo_ir_op_l1_norm (ufo-ir-basic-ops.c:259-277), summed in
+ * FP64 on the device and rounded to float.  dot_compat / l2_compat reproduce
+ * ufo_ir_basic_ops_processor_dot_product / _l2_norm bit for bit
+ * (ufo-ir-basic-ops-processor.c:148-196,239-247), including its index pattern.
+ * The result is written to host memory; the call synchronises the stream. */
+uir_status uir_op_l1_norm(const float *v, uint32_t width, uint32_t height, float *result, void *stream);
+uir_status uir_op_dot_compat(const float *a, const float *b, uint32_t width, uint32_t height,
+                             float *result, void *stream);
+uir_status uir_op_l2_compat(const float *a, uint32_t width, uint32_t height, float *result, void *stream);
+uir_status uir_op_dot_f64(const float *a, const float *b, size_t n, double *result, void *stream);
+
+/* Dx / Dxt / Dy / Dyt, ufo-ir-gradient-processor.cl:1-51 (periodic). */
+uir_status uir_op_dx(const float *in, float *out, uint32_t width, uint32_t height, void *stream);
+uir_status uir_op_dxt(const float *in, float *out, uint32_t width, uint32_t height, void *stream);
+uir_status uir_op_dy(const float *in, float *out, uint32_t width, uint32_t height, void *stream);
+uir_status uir_op_dyt(const float *in, float *out, uint32_t width, uint32_t height, void *stream);
+/* l1_grad, ufo-math-tvstd-method.cl:5-57. */
+uir_status uir_op_l1_grad(const float *in, float *out, uint32_t width, uint32_t height, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UIR_H */

@@ -1,0 +1,64 @@
+/*
+ * uir_internal.h -- shared declarations of libuir's C host layer.
+ */
+#ifndef UIR_INTERNAL_H
+#define UIR_INTERNAL_H
+
+#include <hip/hip_runtime_api.h>
+#include <pthread.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "uir.h"
+#include "uir_device.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* thread-local error message; returns s for tail calls */
+uir_status uir_set_error(uir_status s, const char *fmt, ...);
+uir_status uir_check_hip(int err, const char *what);
+
+/* Reduction workspace owned by a resources object (or the per-device default). */
+typedef struct {
+    int device;
+    void *d_ws;          /* device workspace */
+    size_t ws_bytes;
+    void *d_result;      /* 64 bytes device */
+    void *h_result;      /* 64 bytes pinned host */
+    pthread_mutex_t lock;
+} uir_workspace;
+
+struct uir_resources {
+    int refcount;
+    int device;
+    hipStream_t stream;
+    uir_workspace ws;
+};
+
+uir_status uir_workspace_reserve(uir_workspace *ws, size_t bytes);
+uir_resources *uir_resources_retain(uir_resources *res);
+void uir_workspace_release(uir_workspace *ws);
+
+/* Reductions through a workspace on a stream; results returned on the host
+ * (the stream is synchronised, as the reference's host-side reductions are). */
+uir_status uir_ws_l1(uir_workspace *ws, const float *v, size_t n, void *stream, double *out);
+uir_status uir_ws_dot_f64(uir_workspace *ws, const float *a, const float *b, size_t n, void *stream, double *out);
+uir_status uir_ws_dot_compat(uir_workspace *ws, const float *a, const float *b, uint32_t w, uint32_t h,
+                             void *stream, float *out);
+uir_status uir_ws_normalize(uir_workspace *ws, float *v, size_t n, void *stream);
+
+/* Device allocation with an optional failure-injection hook (tests). */
+uir_status uir_dmalloc(void **ptr, size_t bytes);
+void uir_dfree(void *ptr);
+
+/* geometry helpers (ufo-ir-parallel-projector-task.c:436-498) */
+void uir_geom_lut(uint32_t n_angles, float step, float *sin_lut, float *cos_lut);
+uint32_t uir_geom_runs(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uir_subset *out);
+float uir_geom_axis(float axis_property, uint32_t n_dets);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1123 @@
+/*
+ * uir_tasks.c -- the task objects of libuir: the parallel-beam projector and the
+ * SIRT, SART, ASD-POCS and split-Bregman TV methods, with the reference's
+ * property surface, requisition and process semantics (src/tasks/ufo-ir-*-task.c and
+ * src/core/ufo-ir-{state-dependent,projector,method}-task.c).
+ *
+ * Each process() enqueues on the task's resources stream; reductions the
+ * reference ran on the host (L1 norms, the CGS "dot") run on the device and
+ * synchronise the stream to hand their scalar to the host loop, as the
+ * reference's ufo_buffer_get_host_array did.
+ */
+#define _GNU_SOURCE
+#include <ctype.h>
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "uir_internal.h"
+
+typedef enum { K_PROJECTOR = 0, K_SIRT, K_SART, K_ASDPOCS, K_SBTV } task_kind;
+typedef enum { P_FLOAT, P_UINT, P_BOOL, P_STRING, P_OBJECT } prop_type;
+
+#define N_TMP 32
+
+struct uir_task {
+    int refcount;
+    task_kind kind;
+    uir_resources *res;
+
+    /* projector (ufo-ir-projector-task.c:68-110, state-dependent :75-80,
+     * parallel-projector :123-137) */
+    float axis_position, step, relaxation, correction_scale;
+    int is_forward;
+    char model[64];
+    uint32_t angles_num, image_size, detectors_num;
+    int first_run;
+    uint32_t n_dets;
+    float *h_sin, *h_cos, *d_sin, *d_cos;
+    uint32_t lut_n;
+    uir_subset *runs;
+    uint32_t n_runs;
+
+    /* methods (ufo-ir-method-task.c:74-85 and the method tasks) */
+    uir_task *projector;
+    uint32_t num_iterations;
+    float relaxation_factor;
+    int positive_constraint;
+    float beta, beta_red, alpha, alpha_red, r_max;
+    uint32_t ng;
+    uir_task *df_minimizer;
+    float *diag;
+    uint32_t diag_rows;
+    float lambda, mu;
+
+    /* cached device temporaries (ufo_buffer_dup in the reference) */
+    void *tmp[N_TMP];
+    size_t tmp_bytes[N_TMP];
+};
+
+typedef struct {
+    const char *name; /* canonical: '-' separated */
+    prop_type type;
+    size_t offset;
+    double min, max;
+} prop_def;
+
+#define OFF(f) offsetof(struct uir_task, f)
+
+static const prop_def g_projector_props[] = {
+    {"axis-position", P_FLOAT, OFF(axis_position), -1.0, FLT_MAX},   /* ufo-ir-projector-task.c:68-73 */
+    {"step", P_FLOAT, OFF(step), 0.0, FLT_MAX},                      /* :75-80 */
+    {"relaxation", P_FLOAT, OFF(relaxation), FLT_MIN, FLT_MAX},      /* :82-87 */
+    {"correction-scale", P_FLOAT, OFF(correction_scale), FLT_MIN, FLT_MAX}, /* :88-93 */
+    {"is-forward", P_BOOL, OFF(is_forward), 0, 1},                   /* ufo-ir-state-dependent-task.c:75-80 */
+    {"model", P_STRING, OFF(model), 0, 0},                           /* ufo-ir-parallel-projector-task.c:123-128 */
+    {"angles-num", P_UINT, OFF(angles_num), 0, 4294967295.0},        /* :132-137 */
+    {"image-size", P_UINT, OFF(image_size), 0, 1048576},             /* extension: N x N image edge (0 = n_dets) */
+    {"detectors-num", P_UINT, OFF(detectors_num), 0, 1048576},       /* extension: FP output width (0 = input width) */
+    {NULL, 0, 0, 0, 0}};
+
+#define METHOD_PROPS                                                               \
+    {"num-iterations", P_UINT, OFF(num_iterations), 1, 4294967295.0}, /* ufo-ir-method-task.c:74-79 */ \
+    {"projector", P_OBJECT, OFF(projector), 0, 0}                      /* :80-85 */
+
+static const prop_def g_sirt_props[] = {
+    METHOD_PROPS,
+    {"relaxation-factor", P_FLOAT, OFF(relaxation_factor), 0.0, 1.0}, /* ufo-ir-sirt-task.c:72-77 */
+    {"positive-constraint", P_BOOL, OFF(positive_constraint), 0, 1},   /* extension: clamp x >= 0 per update */
+    {NULL, 0, 0, 0, 0}};
+
+static const prop_def g_sart_props[] = {
+    METHOD_PROPS,
+    {"relaxation-factor", P_FLOAT, OFF(relaxation_factor), 0.0, 1.0}, /* ufo-ir-sart-task.c:76-81 */
+    {NULL, 0, 0, 0, 0}};
+
+static const prop_def g_asdpocs_props[] = {
+    METHOD_PROPS,
+    {"beta", P_FLOAT, OFF(beta), 0.0, FLT_MAX},                       /* ufo-ir-asdpocs-task.c:104-109 */
+    {"beta-red", P_FLOAT, OFF(beta_red), 0.0, FLT_MAX},               /* :111-116 */
+    {"ng", P_UINT, OFF(ng), 0, 1000},                                 /* :118-123 */
+    {"alpha", P_FLOAT, OFF(alpha), 0.0, FLT_MAX},                     /* :125-130 */
+    {"alpha-red", P_FLOAT, OFF(alpha_red), 0.0, FLT_MAX},             /* :132-137 */
+    {"r-max", P_FLOAT, OFF(r_max), 0.0, FLT_MAX},                     /* :139-144 */
+    {"positive-constraint", P_BOOL, OFF(positive_constraint), 0, 1},  /* :146-151 */
+    {"df-minimizer", P_OBJECT, OFF(df_minimizer), 0, 0},              /* :153-158 */
+    {NULL, 0, 0, 0, 0}};
+
+static const prop_def g_sbtv_props[] = {
+    METHOD_PROPS,
+    {"lambda", P_FLOAT, OFF(lambda), 0.0, FLT_MAX},                   /* ufo-ir-sbtv-task.c:90-95 */
+    {"mu", P_FLOAT, OFF(mu), 0.0, FLT_MAX},                           /* :96-101 */
+    {NULL, 0, 0, 0, 0}};
+
+static const char *g_plugin_names[] = {"parallel-projector", "sirt", "sart", "asdpocs", "sbtv"};
+
+static const prop_def *props_of(task_kind k)
+{
+    switch (k) {
+    case K_PROJECTOR: return g_projector_props;
+    case K_SIRT: return g_sirt_props;
+    case K_SART: return g_sart_props;
+    case K_ASDPOCS: return g_asdpocs_props;
+    default: return g_sbtv_props;
+    }
+}
+
+static int is_method(const uir_task *t) { return t && t->kind != K_PROJECTOR; }
+
+/* ------------------------------------------------------------------------ */
+/* construction                                                             */
+/* ------------------------------------------------------------------------ */
+
+static uir_task *task_alloc(task_kind k)
+{
+    uir_task *t = (uir_task *)calloc(1, sizeof *t);
+    if (!t) {
+        uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
+        return NULL;
+    }
+    t->refcount = 1;
+    t->kind = k;
+    /* projector defaults, ufo-ir-projector-task.c:101-110 and
+     * ufo-ir-parallel-projector-task.c:149-154 */
+    t->axis_position = -1.0f;
+    t->step = 0.0f;
+    t->relaxation = 1.0f;
+    t->correction_scale = 1.0f;
+    t->is_forward = 1;
+    strcpy(t->model, "joseph");
+    t->first_run = 1;
+    /* method defaults */
+    t->num_iterations = 10;          /* ufo-ir-method-task.c:109 */
+    t->relaxation_factor = 0.25f;    /* ufo-ir-sirt-task.c:89, ufo-ir-sart-task.c:93 */
+    t->positive_constraint = (k == K_ASDPOCS);
+    t->beta = 1.0f;                  /* ufo-ir-asdpocs-task.c:173-179 */
+    t->beta_red = 0.995f;
+    t->ng = 20;
+    t->alpha = 0.2f;
+    t->alpha_red = 0.95f;
+    t->r_max = 0.95f;
+    t->lambda = 0.1f;                /* ufo-ir-sbtv-task.c (class_init defaults) */
+    t->mu = 0.5f;
+    return t;
+}
+
+uir_task *uir_ir_parallel_projector_task_new(void) { return task_alloc(K_PROJECTOR); }
+uir_task *uir_ir_sirt_task_new(void) { return task_alloc(K_SIRT); }
+uir_task *uir_ir_sart_task_new(void) { return task_alloc(K_SART); }
+uir_task *uir_ir_asdpocs_task_new(void) { return task_alloc(K_ASDPOCS); }
+uir_task *uir_ir_sbtv_task_new(void) { return task_alloc(K_SBTV); }
+
+static void canon(const char *in, char *out, size_t n)
+{
+    size_t i = 0;
+    for (; in[i] && i + 1 < n; ++i)
+        out[i] = in[i] == '_' ? '-' : (char)tolower((unsigned char)in[i]);
+    out[i] = 0;
+}
+
+uir_task *uir_task_new(const char *plugin)
+{
+    if (!plugin) {
+        uir_set_error(UIR_ERR_INVALID_ARGUMENT, "plugin is NULL");
+        return NULL;
+    }
+    char c[64];
+    canon(plugin, c, sizeof c);
+    for (int k = 0; k < 5; ++k)
+        if (strcmp(c, g_plugin_names[k]) == 0)
+            return task_alloc((task_kind)k);
+    uir_set_error(UIR_ERR_NOT_FOUND, "no plugin '%s' in package ir", plugin);
+    return NULL;
+}
+
+uir_task *uir_task_ref(uir_task *t)
+{
+    if (t)
+        __atomic_add_fetch(&t->refcount, 1, __ATOMIC_SEQ_CST);
+    return t;
+}
+
+static void task_free_tmp(uir_task *t)
+{
+    for (int i = 0; i < N_TMP; ++i) {
+        uir_dfree(t->tmp[i]);
+        t->tmp[i] = NULL;
+        t->tmp_bytes[i] = 0;
+    }
+}
+
+void uir_task_unref(uir_task *t)
+{
+    if (!t)
+        return;
+    if (__atomic_sub_fetch(&t->refcount, 1, __ATOMIC_SEQ_CST) > 0)
+        return;
+    if (t->res)
+        (void)hipStreamSynchronize(t->res->stream);
+    task_free_tmp(t);
+    uir_resources_free(t->res);
+    free(t->h_sin);
+    free(t->h_cos);
+    uir_dfree(t->d_sin);
+    uir_dfree(t->d_cos);
+    free(t->runs);
+    free(t->diag);
+    uir_task_unref(t->projector);
+    uir_task_unref(t->df_minimizer);
+    free(t);
+}
+
+const char *uir_task_plugin(const uir_task *t) { return t ? g_plugin_names[t->kind] : NULL; }
+
+/* ------------------------------------------------------------------------ */
+/* properties                                                               */
+/* ------------------------------------------------------------------------ */
+
+static const prop_def *find_prop(const uir_task *t, const char *name, prop_type want)
+{
+    if (!t || !name) {
+        uir_set_error(UIR_ERR_INVALID_ARGUMENT, "task or property name is NULL");
+        return NULL;
+    }
+    char c[64];
+    canon(name, c, sizeof c);
+    for (const prop_def *p = props_of(t->kind); p->name; ++p) {
+        if (strcmp(p->name, c) == 0) {
+            if (p->type != want) {
+                uir_set_error(UIR_ERR_INVALID_ARGUMENT, "property '%s' of '%s' has another type", name,
+                              g_plugin_names[t->kind]);
+                return NULL;
+            }
+            return p;
+        }
+    }
+    uir_set_error(UIR_ERR_NOT_FOUND, "'%s' has no property '%s'", g_plugin_names[t->kind], name);
+    return NULL;
+}
+
+static uir_status prop_status(void)
+{
+    const char *e = uir_last_error();
+    return strstr(e, "has no property") ? UIR_ERR_NOT_FOUND : UIR_ERR_INVALID_ARGUMENT;
+}
+
+uir_status uir_task_set_float(uir_task *t, const char *name, float v)
+{
+    const prop_def *p = find_prop(t, name, P_FLOAT);
+    if (!p)
+        return prop_status();
+    if (!(v >= p->min && v <= p->max))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "value %g is out of range [%g, %g] for property '%s'",
+                             (double)v, p->min, p->max, name);
+    *(float *)((char *)t + p->offset) = v;
+    return UIR_OK;
+}
+
+uir_status uir_task_set_uint(uir_task *t, const char *name, uint32_t v)
+{
+    const prop_def *p = find_prop(t, name, P_UINT);
+    if (!p)
+        return prop_status();
+    if (!((double)v >= p->min && (double)v <= p->max))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "value %u is out of range [%g, %g] for property '%s'",
+                             v, p->min, p->max, name);
+    *(uint32_t *)((char *)t + p->offset) = v;
+    return UIR_OK;
+}
+
+uir_status uir_task_set_bool(uir_task *t, const char *name, int v)
+{
+    const prop_def *p = find_prop(t, name, P_BOOL);
+    if (!p)
+        return prop_status();
+    *(int *)((char *)t + p->offset) = v ? 1 : 0;
+    return UIR_OK;
+}
+
+uir_status uir_task_set_string(uir_task *t, const char *name, const char *v)
+{
+    const prop_def *p = find_prop(t, name, P_STRING);
+    if (!p)
+        return prop_status();
+    if (!v)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL string");
+    char *dst = (char *)t + p->offset;
+    size_t i = 0;
+    for (; v[i] && i + 1 < sizeof t->model; ++i) /* g_ascii_strdown, :283-287 */
+        dst[i] = (char)tolower((unsigned char)v[i]);
+    dst[i] = 0;
+    return UIR_OK;
+}
+
+uir_status uir_task_set_object(uir_task *t, const char *name, uir_task *v)
+{
+    const prop_def *p = find_prop(t, name, P_OBJECT);
+    if (!p)
+        return prop_status();
+    if (p->offset == OFF(projector) && v && v->kind != K_PROJECTOR)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "projector must be a projector task");
+    if (p->offset == OFF(df_minimizer) && v && !is_method(v))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "df_minimizer must be a method task");
+    uir_task **slot = (uir_task **)((char *)t + p->offset);
+    uir_task_ref(v);
+    uir_task_unref(*slot);
+    *slot = v;
+    return UIR_OK;
+}
+
+uir_status uir_task_get_float(const uir_task *t, const char *name, float *v)
+{
+    const prop_def *p = find_prop(t, name, P_FLOAT);
+    if (!p)
+        return prop_status();
+    *v = *(const float *)((const char *)t + p->offset);
+    return UIR_OK;
+}
+
+uir_status uir_task_get_uint(const uir_task *t, const char *name, uint32_t *v)
+{
+    const prop_def *p = find_prop(t, name, P_UINT);
+    if (!p)
+        return prop_status();
+    *v = *(const uint32_t *)((const char *)t + p->offset);
+    return UIR_OK;
+}
+
+uir_status uir_task_get_bool(const uir_task *t, const char *name, int *v)
+{
+    const prop_def *p = find_prop(t, name, P_BOOL);
+    if (!p)
+        return prop_status();
+    *v = *(const int *)((const char *)t + p->offset);
+    return UIR_OK;
+}
+
+uir_status uir_task_get_string(const uir_task *t, const char *name, char *buf, size_t len)
+{
+    const prop_def *p = find_prop(t, name, P_STRING);
+    if (!p)
+        return prop_status();
+    if (!buf || len == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "empty buffer");
+    snprintf(buf, len, "%s", (const char *)t + p->offset);
+    return UIR_OK;
+}
+
+uir_status uir_task_get_object(const uir_task *t, const char *name, uir_task **v)
+{
+    const prop_def *p = find_prop(t, name, P_OBJECT);
+    if (!p)
+        return prop_status();
+    *v = *(uir_task *const *)((const char *)t + p->offset);
+    return UIR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* temporaries                                                              */
+/* ------------------------------------------------------------------------ */
+
+static uir_status tmp_get(uir_task *t, int slot, size_t bytes, float **out)
+{
+    if (t->tmp_bytes[slot] < bytes) {
+        uir_dfree(t->tmp[slot]);
+        t->tmp[slot] = NULL;
+        t->tmp_bytes[slot] = 0;
+        uir_status s = uir_dmalloc(&t->tmp[slot], bytes);
+        if (s != UIR_OK)
+            return s;
+        t->tmp_bytes[slot] = bytes;
+    }
+    *out = (float *)t->tmp[slot];
+    return UIR_OK;
+}
+
+#define TRY(x) do { uir_status s_ = (x); if (s_ != UIR_OK) return s_; } while (0)
+#define HIPTRY(x, what) TRY(uir_check_hip((x), (what)))
+
+/* ------------------------------------------------------------------------ */
+/* setup                                                                    */
+/* ------------------------------------------------------------------------ */
+
+uir_status uir_task_setup(uir_task *t, uir_resources *res)
+{
+    if (!t || !res)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "task or resources is NULL");
+    TRY(uir_check_hip(hipSetDevice(res->device), "hipSetDevice"));
+    if (t->res != res) {
+        uir_resources_retain(res);
+        uir_resources_free(t->res);
+        t->res = res;
+    }
+    if (t->kind == K_PROJECTOR) {
+        /* ufo-ir-parallel-projector-task.c:360-388 loads projector-parallel-<model>.cl;
+         * the only model shipped is joseph. */
+        if (strcmp(t->model, "joseph") != 0)
+            return uir_set_error(UIR_ERR_SETUP, "no projection model '%s' (only 'joseph')", t->model);
+        return UIR_OK;
+    }
+    if (!t->projector) /* ufo-ir-method-task.c:212-215 */
+        return uir_set_error(UIR_ERR_SETUP, "No projector specified.");
+    TRY(uir_task_setup(t->projector, res));
+    if (t->kind == K_ASDPOCS) {
+        if (!t->df_minimizer) /* ufo-ir-asdpocs-task.c:389-392 */
+            return uir_set_error(UIR_ERR_SETUP, "df_minimizer is not defined");
+        if (t->df_minimizer->kind != K_SART && t->df_minimizer->kind != K_SIRT)
+            return uir_set_error(UIR_ERR_SETUP, "df_minimizer must be sart or sirt");
+        TRY(uir_task_set_object(t->df_minimizer, "projector", t->projector)); /* :395 */
+        TRY(uir_task_setup(t->df_minimizer, res));
+    }
+    return UIR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* projector geometry (get_requisition, ufo-ir-parallel-projector-task.c:305-347) */
+/* ------------------------------------------------------------------------ */
+
+static uir_status proj_build_luts(uir_task *p, uint32_t n_angles)
+{
+    free(p->h_sin);
+    free(p->h_cos);
+    free(p->runs);
+    p->h_sin = (float *)malloc(sizeof(float) * (n_angles ? n_angles : 1));
+    p->h_cos = (float *)malloc(sizeof(float) * (n_angles ? n_angles : 1));
+    p->runs = (uir_subset *)malloc(sizeof(uir_subset) * (n_angles ? n_angles : 1));
+    if (!p->h_sin || !p->h_cos || !p->runs)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+    uir_geom_lut(n_angles, p->step, p->h_sin, p->h_cos);
+    p->n_runs = uir_geom_runs(p->h_sin, p->h_cos, n_angles, p->runs);
+    uir_dfree(p->d_sin);
+    uir_dfree(p->d_cos);
+    p->d_sin = p->d_cos = NULL;
+    TRY(uir_dmalloc((void **)&p->d_sin, sizeof(float) * (n_angles ? n_angles : 1)));
+    TRY(uir_dmalloc((void **)&p->d_cos, sizeof(float) * (n_angles ? n_angles : 1)));
+    if (n_angles) {
+        hipStream_t st = p->res ? p->res->stream : 0;
+        HIPTRY(hipMemcpyAsync(p->d_sin, p->h_sin, sizeof(float) * n_angles, hipMemcpyHostToDevice, st), "LUT upload");
+        HIPTRY(hipMemcpyAsync(p->d_cos, p->h_cos, sizeof(float) * n_angles, hipMemcpyHostToDevice, st), "LUT upload");
+        HIPTRY(hipStreamSynchronize(st), "LUT upload sync");
+    }
+    p->lut_n = n_angles;
+    return UIR_OK;
+}
+
+static uir_status proj_requisition(uir_task *p, const uir_buffer *in, uir_requisition *out)
+{
+    if (!in || in->dims[0] == 0 || in->dims[1] == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "empty input buffer");
+    if (p->angles_num == 0)
+        p->angles_num = in->dims[1];
+    if (p->first_run || p->angles_num != in->dims[1] || p->lut_n != p->angles_num) {
+        p->first_run = 0;
+        TRY(proj_build_luts(p, p->angles_num));
+    }
+    if (p->is_forward) {
+        /* FP of an N x N image: n_dets from detectors-num (reference: input width). */
+        p->n_dets = p->detectors_num ? p->detectors_num : in->dims[0];
+        out->n_dims = 2;
+        out->dims[0] = p->n_dets;
+        out->dims[1] = p->angles_num;
+    } else {
+        p->n_dets = in->dims[0];
+        uint32_t n = p->image_size ? p->image_size : p->n_dets;
+        out->n_dims = 2;
+        out->dims[0] = n;
+        out->dims[1] = n;
+    }
+    return UIR_OK;
+}
+
+uir_status uir_task_get_requisition(uir_task *t, const uir_buffer *inputs, uir_requisition *out)
+{
+    if (!t || !inputs || !out)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (t->kind == K_PROJECTOR)
+        return proj_requisition(t, &inputs[0], out);
+    if (!t->projector) /* ufo-ir-method-task.c:247-251 */
+        return uir_set_error(UIR_ERR_SETUP, "Projector not specified");
+    return proj_requisition(t->projector, &inputs[0], out);
+}
+
+/* image edge of the projector's current geometry */
+static uint32_t proj_image_n(const uir_task *p, uint32_t fallback)
+{
+    return p->image_size ? p->image_size : fallback;
+}
+
+static float proj_axis(const uir_task *p) { return uir_geom_axis(p->axis_position, p->n_dets); }
+
+static void proj_runs_dev(const uir_task *p, const uir_subset *subs, uint32_t n, uirk_runs *r, uint32_t first)
+{
+    memset(r, 0, sizeof *r);
+    uint32_t k = 0;
+    for (uint32_t i = first; i < n && k < UIRK_MAX_RUNS; ++i, ++k) {
+        r->offset[k] = subs[i].offset;
+        r->n[k] = subs[i].n;
+    }
+    r->count = (int32_t)k;
+    (void)p;
+}
+
+/* forward over all runs (ufo-ir-parallel-projector-task.c:390-408): one launch
+ * with each angle's own direction; ray_w != NULL fuses the SIRT ray weighting. */
+static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float *r_sino, float *w_sino,
+                          const float *ray_w)
+{
+    return uir_check_hip(uirk_fp_ref(vol, r_sino, w_sino, ray_w, p->d_sin, p->d_cos, (int)n, (int)p->n_dets,
+                                     proj_axis(p), p->correction_scale, 0, (int)p->lut_n, -1, p->res->stream),
+                         "FP launch");
+}
+
+/* backward over all runs (:410-429), accumulate form: vol += relax * BP per run. */
+static uir_status proj_bp(uir_task *p, uint32_t n, const float *sino, float *vol)
+{
+    for (uint32_t first = 0; first < p->n_runs; first += UIRK_MAX_RUNS) {
+        uirk_runs r;
+        proj_runs_dev(p, p->runs, p->n_runs, &r, first);
+        TRY(uir_check_hip(uirk_bp_ref(vol, vol, sino, (int)p->lut_n, p->d_sin, p->d_cos, (int)n, (int)p->n_dets,
+                                      proj_axis(p), p->relaxation, &r, NULL, 0, p->res->stream), "BP launch"));
+    }
+    return UIR_OK;
+}
+
+static uir_status check_buf(const uir_buffer *b, uint64_t w, uint64_t h, const char *what)
+{
+    if (!b || !b->data)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "%s: NULL buffer", what);
+    if (b->dtype != UIR_DTYPE_F32)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "%s: tasks take FP32 buffers", what);
+    if (b->dims[0] != w || b->dims[1] != h)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "%s: dims %ux%u, expected %llux%llu", what, b->dims[0],
+                             b->dims[1], (unsigned long long)w, (unsigned long long)h);
+    return UIR_OK;
+}
+
+static uir_status proj_ready(uir_task *p)
+{
+    if (!p || p->kind != K_PROJECTOR)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "not a projector task");
+    if (!p->res)
+        return uir_set_error(UIR_ERR_SETUP, "projector is not set up");
+    if (!p->h_sin)
+        return uir_set_error(UIR_ERR_SETUP, "projector geometry unknown: call get_requisition first");
+    return UIR_OK;
+}
+
+uir_status uir_projector_forward(uir_task *p, const uir_buffer *in, uir_buffer *out)
+{
+    TRY(proj_ready(p));
+    uint32_t n = in ? in->dims[0] : 0;
+    TRY(check_buf(in, n, n, "volume"));
+    TRY(check_buf(out, p->n_dets, p->lut_n, "sinogram"));
+    return proj_fp(p, n, (const float *)in->data, (const float *)out->data, (float *)out->data, NULL);
+}
+
+uir_status uir_projector_backward(uir_task *p, const uir_buffer *in, uir_buffer *out)
+{
+    TRY(proj_ready(p));
+    uint32_t n = out ? out->dims[0] : 0;
+    TRY(check_buf(in, p->n_dets, p->lut_n, "sinogram"));
+    TRY(check_buf(out, n, n, "volume"));
+    return proj_bp(p, n, (const float *)in->data, (float *)out->data);
+}
+
+uir_status uir_parallel_projector_subset_fp(uir_task *p, const uir_buffer *volume, uir_buffer *sinogram,
+                                            const uir_subset *sub)
+{
+    TRY(proj_ready(p));
+    if (!sub || (uint64_t)sub->offset + sub->n > p->lut_n)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "subset outside the angle range");
+    uint32_t n = volume ? volume->dims[0] : 0;
+    TRY(check_buf(volume, n, n, "volume"));
+    TRY(check_buf(sinogram, p->n_dets, p->lut_n, "sinogram"));
+    return uir_check_hip(uirk_fp_ref((const float *)volume->data, (const float *)sinogram->data,
+                                     (float *)sinogram->data, NULL, p->d_sin, p->d_cos, (int)n, (int)p->n_dets,
+                                     proj_axis(p), p->correction_scale, (int)sub->offset, (int)sub->n,
+                                     sub->direction, p->res->stream), "FP launch");
+}
+
+uir_status uir_parallel_projector_subset_bp(uir_task *p, uir_buffer *volume, const uir_buffer *sinogram,
+                                            const uir_subset *sub)
+{
+    TRY(proj_ready(p));
+    if (!sub || (uint64_t)sub->offset + sub->n > p->lut_n)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "subset outside the angle range");
+    uint32_t n = volume ? volume->dims[0] : 0;
+    TRY(check_buf(volume, n, n, "volume"));
+    TRY(check_buf(sinogram, p->n_dets, p->lut_n, "sinogram"));
+    uirk_runs r;
+    proj_runs_dev(p, sub, 1, &r, 0);
+    return uir_check_hip(uirk_bp_ref((const float *)volume->data, (float *)volume->data,
+                                     (const float *)sinogram->data, (int)p->lut_n, p->d_sin, p->d_cos, (int)n,
+                                     (int)p->n_dets, proj_axis(p), p->relaxation, &r, NULL, 0, p->res->stream),
+                         "BP launch");
+}
+
+uir_status uir_parallel_projector_get_luts(uir_task *p, const float **s, const float **c, uint32_t *n)
+{
+    if (!p || p->kind != K_PROJECTOR || !p->h_sin)
+        return uir_set_error(UIR_ERR_SETUP, "projector geometry unknown");
+    if (s) *s = p->h_sin;
+    if (c) *c = p->h_cos;
+    if (n) *n = p->lut_n;
+    return UIR_OK;
+}
+
+uir_status uir_parallel_projector_get_subsets(uir_task *p, const uir_subset **subs, uint32_t *n)
+{
+    if (!p || p->kind != K_PROJECTOR || !p->runs)
+        return uir_set_error(UIR_ERR_SETUP, "projector geometry unknown");
+    if (subs) *subs = p->runs;
+    if (n) *n = p->n_runs;
+    return UIR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* methods                                                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uir_task *p;       /* projector */
+    uint32_t n, D, A;  /* image edge, detectors, angles */
+    size_t nv, ns;
+    hipStream_t st;
+} mctx;
+
+static uir_status method_begin(uir_task *t, const uir_buffer *inputs, uir_buffer *out, mctx *c)
+{
+    if (!t->res)
+        return uir_set_error(UIR_ERR_SETUP, "task is not set up");
+    if (!t->projector)
+        return uir_set_error(UIR_ERR_SETUP, "No projector specified.");
+    if (!inputs)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "no input");
+    uir_requisition req;
+    TRY(uir_task_get_requisition(t, inputs, &req));
+    c->p = t->projector;
+    if (!c->p->res)
+        c->p->res = uir_resources_retain(t->res);
+    c->D = c->p->n_dets;
+    c->A = c->p->lut_n;
+    c->n = proj_image_n(c->p, c->D);
+    c->nv = (size_t)c->n * c->n;
+    c->ns = (size_t)c->A * c->D;
+    c->st = t->res->stream;
+    TRY(check_buf(&inputs[0], c->D, c->A, "sinogram"));
+    TRY(check_buf(out, c->n, c->n, "output"));
+    if (c->p->n_runs > UIRK_MAX_RUNS)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "more than %d direction runs", UIRK_MAX_RUNS);
+    return UIR_OK;
+}
+
+/* SIRT ray and pixel weights, ufo-ir-sirt-task.c:176-193. */
+static uir_status sirt_weights(uir_task *t, mctx *c, float **ray_w, float **pix_w)
+{
+    float *vol_tmp, *sino_tmp;
+    TRY(tmp_get(t, 0, c->nv * 4, &vol_tmp));
+    TRY(tmp_get(t, 1, c->ns * 4, ray_w));
+    TRY(tmp_get(t, 2, c->ns * 4, &sino_tmp));
+    TRY(tmp_get(t, 3, c->nv * 4, pix_w));
+    c->p->relaxation = 1.0f;
+    c->p->correction_scale = 1.0f;
+    HIPTRY(uirk_set(vol_tmp, c->nv, 1.0f, c->st), "set");
+    HIPTRY(uirk_set(*ray_w, c->ns, 0.0f, c->st), "set");
+    TRY(proj_fp(c->p, c->n, vol_tmp, *ray_w, *ray_w, NULL));
+    HIPTRY(uirk_inv(*ray_w, *ray_w, c->ns, c->st), "inv");
+    HIPTRY(uirk_set(sino_tmp, c->ns, 1.0f, c->st), "set");
+    HIPTRY(uirk_set(*pix_w, c->nv, 0.0f, c->st), "set");
+    TRY(proj_bp(c->p, c->n, sino_tmp, *pix_w));
+    HIPTRY(uirk_inv(*pix_w, *pix_w, c->nv, c->st), "inv");
+    return UIR_OK;
+}
+
+/* SIRT process, ufo-ir-sirt-task.c:163-222.  One iteration is two launches:
+ * FP with the residual and ray weighting fused (b - A x) * R, and BP with the
+ * pixel weighting and update fused x + C * relax * A^T r (+ POSC if enabled). */
+static uir_status sirt_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, int posc,
+                           mctx *c)
+{
+    float *ray_w, *pix_w, *sino_tmp;
+    TRY(sirt_weights(t, c, &ray_w, &pix_w));
+    TRY(tmp_get(t, 2, c->ns * 4, &sino_tmp));
+    c->p->relaxation = relax;            /* :195-197 */
+    c->p->correction_scale = -1.0f;
+    HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
+    uirk_runs r;
+    proj_runs_dev(c->p, c->p->runs, c->p->n_runs, &r, 0);
+    for (uint32_t it = 0; it < iters; ++it) { /* :202-215 */
+        TRY(proj_fp(c->p, c->n, x, b, sino_tmp, ray_w));
+        HIPTRY(uirk_bp_ref(x, x, sino_tmp, (int)c->A, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D,
+                           proj_axis(c->p), c->p->relaxation, &r, pix_w, posc, c->st), "BP launch");
+    }
+    return UIR_OK;
+}
+
+/* SART process, ufo-ir-sart-task.c:173-228: per angle, FP of the row with the
+ * ray weighting fused (subset_fp + op_mulRows), then BP of that row into x. */
+static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, mctx *c)
+{
+    float *sino_tmp, *vol_tmp, *ray_w;
+    TRY(tmp_get(t, 4, c->ns * 4, &sino_tmp));
+    TRY(tmp_get(t, 5, c->nv * 4, &vol_tmp));
+    TRY(tmp_get(t, 6, c->ns * 4, &ray_w));
+    HIPTRY(uirk_set(vol_tmp, c->nv, 1.0f, c->st), "set");          /* :193-200 */
+    HIPTRY(uirk_set(ray_w, c->ns, 0.0f, c->st), "set");
+    c->p->correction_scale = 1.0f;
+    TRY(proj_fp(c->p, c->n, vol_tmp, ray_w, ray_w, NULL));
+    HIPTRY(uirk_inv(ray_w, ray_w, c->ns, c->st), "inv");
+    c->p->correction_scale = -1.0f;                                 /* :205-207 */
+    c->p->relaxation = relax;
+    HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
+    const float axis = proj_axis(c->p);
+    for (uint32_t it = 0; it < iters; ++it) {                       /* :208-220 */
+        HIPTRY(hipMemcpyAsync(sino_tmp, b, c->ns * 4, hipMemcpyDeviceToDevice, c->st), "copy");
+        for (uint32_t a = 0; a < c->A; ++a) {
+            HIPTRY(uirk_fp_ref(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
+                               c->p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
+            uirk_runs r;
+            memset(&r, 0, sizeof r);
+            r.count = 1;
+            r.offset[0] = a;
+            r.n[0] = 1;
+            HIPTRY(uirk_bp_ref(x, x, sino_tmp, (int)c->A, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
+                               c->p->relaxation, &r, NULL, 0, c->st), "BP launch");
+        }
+    }
+    return UIR_OK;
+}
+
+static uir_status method_process_df(uir_task *t, const float *b, float *x, mctx *c, float relax)
+{
+    if (t->kind == K_SIRT)
+        return sirt_run(t, b, x, relax, t->num_iterations, t->positive_constraint, c);
+    return sart_run(t, b, x, relax, t->num_iterations, c);
+}
+
+/* ASD-POCS process, ufo-ir-asdpocs-task.c:411-509 (tvstd :550-580). */
+static uir_status asdpocs_run(uir_task *t, const float *b, float *out, mctx *c)
+{
+    uir_task *df = t->df_minimizer;
+    if (!df)
+        return uir_set_error(UIR_ERR_SETUP, "df_minimizer is not defined");
+    if (df->projector != t->projector)
+        TRY(uir_task_set_object(df, "projector", t->projector)); /* :434 */
+    if (!df->res)
+        df->res = uir_resources_retain(t->res);
+    float *x, *x_prev, *x_res, *b_res, *grad;
+    TRY(tmp_get(t, 7, c->nv * 4, &x));
+    TRY(tmp_get(t, 8, c->nv * 4, &x_prev));
+    TRY(tmp_get(t, 9, c->nv * 4, &x_res));
+    TRY(tmp_get(t, 10, c->ns * 4, &b_res));
+    TRY(tmp_get(t, 11, c->nv * 4, &grad));
+    HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(x_prev, c->nv, 0.0f, c->st), "set");
+    free(t->diag);
+    t->diag = (float *)calloc(4 * (size_t)(t->num_iterations ? t->num_iterations : 1), sizeof(float));
+    t->diag_rows = 0;
+    float dp = 1.0f, dd = 1.0f, dg = 1.0f, dtgv = 1.0f;
+    float beta = t->beta;
+    uir_workspace *ws = &t->res->ws;
+    for (uint32_t it = 0; it < t->num_iterations; ++it) {
+        /* g_object_set(df, "relaxation_factor", beta): GLib rejects values outside [0,1]. */
+        if (beta >= 0.0f && beta <= 1.0f)
+            df->relaxation_factor = beta;
+        TRY(method_process_df(df, b, x, c, df->relaxation_factor));        /* :460 */
+        if (t->positive_constraint)                                         /* :463-465 */
+            HIPTRY(uirk_unary(UIRK_POSC, x, x, c->nv, 0, c->st), "POSC");
+        HIPTRY(hipMemcpyAsync(out, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy"); /* :468 */
+        c->p->correction_scale = -1.0f;                                     /* :471-475 */
+        TRY(proj_fp(c->p, c->n, x, b, b_res, NULL));
+        double d;
+        TRY(uir_ws_l1(ws, b_res, c->ns, c->st, &d));                       /* :478 */
+        dd = (float)d;
+        HIPTRY(uirk_binary(UIRK_SUB, x, x_prev, 0, x_res, c->nv, c->st), "deduction"); /* :481-482 */
+        TRY(uir_ws_l1(ws, x_res, c->nv, c->st, &d));
+        dp = (float)d;
+        if (it == 0)                                                        /* :485-487 */
+            dtgv = t->alpha * dp;
+        HIPTRY(hipMemcpyAsync(x_prev, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy"); /* :490 */
+        HIPTRY(uirk_set(grad, c->nv, 0.0f, c->st), "set");                 /* tvstd :559 */
+        for (int k = 0; k < 20; ++k) {                                      /* :573-579 */
+            HIPTRY(uirk_l1_grad(x, grad, c->n, c->n, c->st), "l1_grad");
+            TRY(uir_ws_l1(ws, grad, c->nv, c->st, &d));
+            const float factor = dtgv / (float)d;
+            HIPTRY(uirk_binary(UIRK_SUB_SCALED, x, grad, factor, x, c->nv, c->st), "deduction2");
+        }
+        HIPTRY(uirk_binary(UIRK_SUB, x, x_prev, 0, x_res, c->nv, c->st), "deduction"); /* :496-497 */
+        TRY(uir_ws_l1(ws, x_res, c->nv, c->st, &d));
+        dg = (float)d;
+        beta *= t->beta_red;                                                /* :498 */
+        t->diag[4 * it + 0] = dd;
+        t->diag[4 * it + 1] = dp;
+        t->diag[4 * it + 2] = dg;
+        t->diag[4 * it + 3] = dtgv;
+        t->diag_rows = it + 1;
+        const float epsilon = 0.001f;                                       /* :495,501-503 */
+        if (dg > t->r_max * dp && dd > epsilon)
+            dtgv *= t->alpha_red;
+    }
+    return UIR_OK;
+}
+
+uir_status uir_asdpocs_get_diagnostics(uir_task *t, const float **rows4, uint32_t *n_rows)
+{
+    if (!t || t->kind != K_ASDPOCS)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "not an asdpocs task");
+    if (rows4) *rows4 = t->diag;
+    if (n_rows) *n_rows = t->diag_rows;
+    return UIR_OK;
+}
+
+/* ---- SBTV (ufo-ir-sbtv-task.c) ---- */
+
+typedef struct {
+    uir_task *t;
+    mctx *c;
+    float *tempA, *tempAt, *tempD, *tempDxt;
+} sbctx;
+
+static uir_status sb_l2(sbctx *s, const float *v, float *out)
+{
+    float d;
+    TRY(uir_ws_dot_compat(&s->t->res->ws, v, v, s->c->n, s->c->n, s->c->st, &d));
+    *out = (float)sqrt(d);
+    return UIR_OK;
+}
+
+static uir_status sb_dot(sbctx *s, const float *a, const float *b, float *out)
+{
+    return uir_ws_dot_compat(&s->t->res->ws, a, b, s->c->n, s->c->n, s->c->st, out);
+}
+
+/* processA, :587-635 */
+static uir_status sb_process_a(sbctx *s, const float *in, float *out)
+{
+    mctx *c = s->c;
+    const uint32_t n = c->n;
+    HIPTRY(uirk_set(out, c->nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(s->tempA, c->ns, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(s->tempAt, c->nv, 0.0f, c->st), "set");
+    TRY(proj_fp(c->p, n, in, s->tempA, s->tempA, NULL));
+    TRY(proj_bp(c->p, n, s->tempA, s->tempAt));
+    HIPTRY(uirk_unary(UIRK_SCALE, s->tempAt, s->tempAt, c->nv, s->t->mu, c->st), "mul_scalar");
+    HIPTRY(uirk_diff(UIRK_DY, in, s->tempD, n, n, c->st), "Dy");
+    HIPTRY(uirk_diff(UIRK_DYT, s->tempD, out, n, n, c->st), "Dyt");
+    HIPTRY(uirk_diff(UIRK_DX, in, s->tempD, n, n, c->st), "Dx");
+    HIPTRY(uirk_diff(UIRK_DXT, s->tempD, s->tempDxt, n, n, c->st), "Dxt");
+    HIPTRY(uirk_binary(UIRK_ADD, out, s->tempDxt, 0, out, c->nv, c->st), "add");
+    HIPTRY(uirk_unary(UIRK_SCALE, out, out, c->nv, s->t->lambda, c->st), "mul_scalar");
+    HIPTRY(uirk_binary(UIRK_ADD, out, s->tempAt, 0, out, c->nv, c->st), "add");
+    return UIR_OK;
+}
+
+/* cgs, :404-585 */
+static uir_status sb_cgs(sbctx *s, const float *b, float *x, const float *x0, uint32_t max_iter)
+{
+    mctx *c = s->c;
+    uir_task *t = s->t;
+    const size_t nv = c->nv, nb = nv * 4;
+    const float tol = 1E-06f;
+    const double eps = 2.2204E-16;
+    float n2b;
+    TRY(sb_l2(s, b, &n2b));
+    HIPTRY(hipMemcpyAsync(x, x0, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+    int flag = 1;
+    float *xmin, *r, *rt, *u, *tmpa, *pp, *ph, *q, *vh, *temp_sum, *uh, *qh;
+    TRY(tmp_get(t, 12, nb, &xmin));
+    TRY(tmp_get(t, 13, nb, &r));
+    TRY(tmp_get(t, 14, nb, &rt));
+    TRY(tmp_get(t, 15, nb, &u));
+    TRY(tmp_get(t, 16, nb, &tmpa));
+    TRY(tmp_get(t, 17, nb, &pp));
+    TRY(tmp_get(t, 18, nb, &ph));
+    TRY(tmp_get(t, 19, nb, &q));
+    TRY(tmp_get(t, 20, nb, &vh));
+    TRY(tmp_get(t, 21, nb, &temp_sum));
+    TRY(tmp_get(t, 22, nb, &uh));
+    TRY(tmp_get(t, 23, nb, &qh));
+    HIPTRY(hipMemcpyAsync(xmin, x, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+    const float tolb = tol * n2b;
+    TRY(sb_process_a(s, x, r));
+    HIPTRY(uirk_binary(UIRK_SUB, b, r, 0, r, nv, c->st), "deduction");
+    float normr;
+    TRY(sb_l2(s, r, &normr));
+    float normr_act = normr;
+    if (normr <= tolb)
+        return UIR_OK; /* :435-440 */
+    HIPTRY(hipMemcpyAsync(rt, r, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+    float normmin = normr, rho = 1.0f;
+    unsigned stag = 0, moresteps = 0;
+    const unsigned maxmsteps = 5, maxstagsteps = 3;
+    HIPTRY(uirk_set(u, nv, 0, c->st), "set");
+    HIPTRY(uirk_set(tmpa, nv, 0, c->st), "set");
+    HIPTRY(uirk_set(pp, nv, 0, c->st), "set");
+    HIPTRY(uirk_set(q, nv, 0, c->st), "set");
+    HIPTRY(uirk_set(temp_sum, nv, 0, c->st), "set");
+    for (uint32_t it = 0; it < max_iter; ++it) {
+        const float rho1 = rho;
+        TRY(sb_dot(s, rt, r, &rho));
+        if (rho == 0 || isinf(rho) || isnan(rho)) {
+            flag = 4;
+            break;
+        }
+        if (it == 0) {
+            HIPTRY(hipMemcpyAsync(u, r, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+            HIPTRY(hipMemcpyAsync(pp, u, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+        } else {
+            const float beta = rho / rho1;
+            if (beta == 0 || isinf(beta) || isnan(beta)) {
+                flag = 4;
+                break;
+            }
+            HIPTRY(uirk_binary(UIRK_ADD_SCALED, r, q, beta, u, nv, c->st), "add2");
+            HIPTRY(uirk_binary(UIRK_ADD_SCALED, q, pp, beta, temp_sum, nv, c->st), "add2");
+            HIPTRY(uirk_binary(UIRK_ADD_SCALED, u, temp_sum, beta, pp, nv, c->st), "add2");
+        }
+        HIPTRY(hipMemcpyAsync(ph, pp, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+        TRY(sb_process_a(s, ph, vh));
+        float rtvh;
+        TRY(sb_dot(s, rt, vh, &rtvh));
+        float alpha = 0;
+        if (rtvh == 0) {
+            flag = 4;
+            break;
+        }
+        alpha = rho / rtvh;
+        if (isinf(alpha) || isnan(alpha)) {
+            flag = 4;
+            break;
+        }
+        HIPTRY(uirk_binary(UIRK_ADD_SCALED, u, vh, -alpha, q, nv, c->st), "add2");
+        HIPTRY(uirk_binary(UIRK_ADD, u, q, 0, uh, nv, c->st), "add");
+        float l2uh, l2x;
+        TRY(sb_l2(s, uh, &l2uh));
+        TRY(sb_l2(s, x, &l2x));
+        if (fabs(alpha) * l2uh < eps * l2x) /* :521 */
+            stag += 1;
+        else
+            stag = 0;
+        HIPTRY(uirk_binary(UIRK_ADD_SCALED, x, uh, alpha, x, nv, c->st), "add2");
+        TRY(sb_process_a(s, uh, qh));
+        HIPTRY(uirk_binary(UIRK_ADD_SCALED, r, qh, -alpha, r, nv, c->st), "add2");
+        TRY(sb_l2(s, r, &normr));
+        normr_act = normr;
+        if (normr <= tolb || stag >= maxstagsteps || moresteps) {
+            TRY(sb_process_a(s, r, tmpa)); /* :535 applies A to r */
+            HIPTRY(uirk_binary(UIRK_SUB, b, tmpa, 0, r, nv, c->st), "deduction");
+            TRY(sb_l2(s, r, &normr_act));
+            if (normr_act <= tolb) {
+                flag = 0;
+                break;
+            }
+            if (stag >= maxstagsteps && moresteps == 0)
+                stag = 0;
+            moresteps += 1;
+            if (moresteps >= maxmsteps) {
+                flag = 3;
+                break;
+            }
+        }
+        if (normr_act < normmin) {
+            normmin = normr_act;
+            HIPTRY(hipMemcpyAsync(xmin, x, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+        }
+        if (stag >= maxstagsteps) {
+            flag = 3;
+            break;
+        }
+    }
+    if (flag != 0) { /* :566-572 */
+        TRY(sb_process_a(s, xmin, tmpa));
+        HIPTRY(uirk_binary(UIRK_SUB, b, tmpa, 0, r, nv, c->st), "deduction");
+        float l2r;
+        TRY(sb_l2(s, r, &l2r));
+        if (l2r <= normr_act)
+            HIPTRY(hipMemcpyAsync(x, xmin, nb, hipMemcpyDeviceToDevice, c->st), "copy");
+    }
+    return UIR_OK;
+}
+
+/* SBTV process, :230-302 with calculate_b :304-334 and update_db :336-402. */
+static uir_status sbtv_run(uir_task *t, float *input, float *u, mctx *c)
+{
+    const size_t nv = c->nv, ns = c->ns;
+    const uint32_t n = c->n;
+    sbctx s;
+    s.t = t;
+    s.c = c;
+    float *f, *fbp, *up, *bb, *bx, *by, *dx, *dy, *tmpx, *tmpy, *tmp;
+    TRY(tmp_get(t, 0, ns * 4, &s.tempA));
+    TRY(tmp_get(t, 1, nv * 4, &s.tempAt));
+    TRY(tmp_get(t, 2, nv * 4, &s.tempD));
+    TRY(tmp_get(t, 3, nv * 4, &s.tempDxt));
+    TRY(tmp_get(t, 4, ns * 4, &f));
+    TRY(tmp_get(t, 5, nv * 4, &fbp));
+    TRY(tmp_get(t, 6, nv * 4, &up));
+    TRY(tmp_get(t, 7, nv * 4, &bb));
+    TRY(tmp_get(t, 8, nv * 4, &bx));
+    TRY(tmp_get(t, 9, nv * 4, &by));
+    TRY(tmp_get(t, 10, nv * 4, &dx));
+    TRY(tmp_get(t, 11, nv * 4, &dy));
+    /* calculate_b / update_db temporaries (slot 24; CGS owns slots 12-23) */
+    float *sbuf;
+    TRY(tmp_get(t, 24, nv * 4 * 3, &sbuf));
+    tmpx = sbuf;
+    tmpy = sbuf + nv;
+    tmp = sbuf + 2 * nv;
+    uir_workspace *ws = &t->res->ws;
+    TRY(uir_ws_normalize(ws, input, ns, c->st));                                  /* :239 */
+    HIPTRY(hipMemcpyAsync(f, input, ns * 4, hipMemcpyDeviceToDevice, c->st), "copy"); /* :246-248 */
+    TRY(uir_ws_normalize(ws, f, ns, c->st));
+    HIPTRY(uirk_set(fbp, nv, 0.0f, c->st), "set");                                /* :251-253 */
+    TRY(proj_bp(c->p, n, f, fbp));
+    HIPTRY(uirk_set(u, nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(bx, nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(by, nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(dx, nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_set(dy, nv, 0.0f, c->st), "set");
+    HIPTRY(uirk_unary(UIRK_SCALE, fbp, fbp, nv, t->mu, c->st), "mul_scalar");     /* :279 */
+    const float d_lambda = -1 / t->lambda;                                         /* :357 */
+    float *ub;
+    for (uint32_t it = 0; it < t->num_iterations; ++it) {
+        HIPTRY(hipMemcpyAsync(up, u, nv * 4, hipMemcpyDeviceToDevice, c->st), "copy");
+        /* calculate_b: tmpx = Dxt(dx - bx), tmpy = Dyt(dy - by), b = fbp + lambda (tmpx + tmpy) */
+        HIPTRY(uirk_binary(UIRK_SUB, dx, bx, 0, tmp, nv, c->st), "deduction");
+        HIPTRY(uirk_diff(UIRK_DXT, tmp, tmpx, n, n, c->st), "Dxt");
+        HIPTRY(uirk_binary(UIRK_SUB, dy, by, 0, tmp, nv, c->st), "deduction");
+        HIPTRY(uirk_diff(UIRK_DYT, tmp, tmpy, n, n, c->st), "Dyt");
+        HIPTRY(uirk_binary(UIRK_ADD, tmpx, tmpy, 0, bb, nv, c->st), "add");
+        HIPTRY(uirk_unary(UIRK_SCALE, bb, bb, nv, t->lambda, c->st), "mul_scalar");
+        HIPTRY(uirk_binary(UIRK_ADD, fbp, bb, 0, bb, nv, c->st), "add");
+        TRY(sb_cgs(&s, bb, u, up, 30));                                            /* :287 */
+        /* update_db */
+        ub = s.tempDxt; /* s (the magnitude) */
+        HIPTRY(uirk_diff(UIRK_DX, u, tmpx, n, n, c->st), "Dx");
+        HIPTRY(uirk_binary(UIRK_ADD, tmpx, bx, 0, tmpx, nv, c->st), "add");
+        HIPTRY(uirk_diff(UIRK_DY, u, tmpy, n, n, c->st), "Dy");
+        HIPTRY(uirk_binary(UIRK_ADD, tmpy, by, 0, tmpy, nv, c->st), "add");
+        HIPTRY(uirk_binary(UIRK_MUL, tmpx, tmpx, 0, tmp, nv, c->st), "mul");
+        HIPTRY(hipMemcpyAsync(ub, tmp, nv * 4, hipMemcpyDeviceToDevice, c->st), "copy");
+        HIPTRY(uirk_binary(UIRK_MUL, tmpy, tmpy, 0, tmp, nv, c->st), "mul");
+        HIPTRY(uirk_binary(UIRK_ADD, ub, tmp, 0, ub, nv, c->st), "add");
+        HIPTRY(uirk_unary(UIRK_SQRT, ub, ub, nv, 0, c->st), "sqrt");
+        HIPTRY(uirk_set(tmp, nv, d_lambda, c->st), "set");                        /* temp_s_top */
+        HIPTRY(uirk_binary(UIRK_ADD, tmp, ub, 0, tmp, nv, c->st), "add");
+        HIPTRY(uirk_set(s.tempD, nv, 0.0f, c->st), "set");                        /* Z */
+        HIPTRY(uirk_binary(UIRK_MAX, tmp, s.tempD, 0, tmp, nv, c->st), "max");
+        HIPTRY(uirk_set(s.tempAt, nv, 1E-12f, c->st), "set");                     /* e12 */
+        HIPTRY(uirk_binary(UIRK_MAX, ub, s.tempAt, 0, s.tempAt, nv, c->st), "max");
+        HIPTRY(uirk_binary(UIRK_DIV, tmp, s.tempAt, 0, ub, nv, c->st), "div");    /* tresh */
+        HIPTRY(uirk_binary(UIRK_MUL, ub, tmpx, 0, dx, nv, c->st), "mul");
+        HIPTRY(uirk_binary(UIRK_MUL, ub, tmpy, 0, dy, nv, c->st), "mul");
+        HIPTRY(uirk_binary(UIRK_SUB, tmpx, dx, 0, bx, nv, c->st), "deduction");
+        HIPTRY(uirk_binary(UIRK_SUB, tmpy, dy, 0, by, nv, c->st), "deduction");
+    }
+    return UIR_OK;
+}
+
+/* ---- process ---- */
+
+/* ufo_ir_state_dependent_task_process, ufo-ir-state-dependent-task.c:194-212:
+ * zero the output, then FP or BP over all runs. */
+static uir_status projector_process(uir_task *p, const uir_buffer *inputs, uir_buffer *out)
+{
+    if (!p->res)
+        return uir_set_error(UIR_ERR_SETUP, "task is not set up");
+    uir_requisition req;
+    TRY(proj_requisition(p, &inputs[0], &req));
+    TRY(check_buf(out, req.dims[0], req.dims[1], "output"));
+    const size_t n_out = (size_t)req.dims[0] * req.dims[1];
+    HIPTRY(uirk_set((float *)out->data, n_out, 0.0f, p->res->stream), "set");
+    if (p->is_forward)
+        return uir_projector_forward(p, &inputs[0], out);
+    return uir_projector_backward(p, &inputs[0], out);
+}
+
+uir_status uir_task_process(uir_task *t, const uir_buffer *inputs, uir_buffer *output)
+{
+    if (!t || !inputs || !output)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (t->res)
+        TRY(uir_check_hip(hipSetDevice(t->res->device), "hipSetDevice"));
+    if (t->kind == K_PROJECTOR)
+        return projector_process(t, inputs, output);
+    mctx c;
+    TRY(method_begin(t, inputs, output, &c));
+    const float *b = (const float *)inputs[0].data;
+    float *x = (float *)output->data;
+    switch (t->kind) {
+    case K_SIRT:
+        return sirt_run(t, b, x, t->relaxation_factor, t->num_iterations, t->positive_constraint, &c);
+    case K_SART:
+        return sart_run(t, b, x, t->relaxation_factor, t->num_iterations, &c);
+    case K_ASDPOCS:
+        return asdpocs_run(t, b, x, &c);
+    case K_SBTV:
+        return sbtv_run(t, (float *)inputs[0].data, x, &c);
+    default:
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "unknown task kind");
+    }
+}

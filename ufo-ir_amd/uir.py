@@ -1,0 +1,404 @@
+"""Python mirror of libuir's C ABI (include/uir.h), over ctypes.
+
+It exposes the reference's plugin surface: tasks created by plugin name
+("parallel-projector", "sirt", "sart", "asdpocs", "sbtv") with the reference's
+property names, set up on a device, asked for their requisition and run with
+process(), plus the kernel-level entry points.  Device memory is owned by
+DeviceArray (uir_device_malloc); numpy is used on the host side only.
+
+The library must be built (make -C ufo-ir_amd, or __graft_entry__.build());
+load() raises if libuir.so is missing, and nothing here falls back to CPU code.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libuir.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "uir.h")
+
+UIR_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE", 4: "UNSUPPORTED",
+                5: "SETUP", 6: "NOT_FOUND", 7: "BUSY", 8: "CANCELLED", 9: "TIMEOUT", 10: "NO_DEVICE"}
+DTYPE_F32, DTYPE_F16, DTYPE_BF16 = 0, 1, 2
+
+
+class UirError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+class Buffer(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("dims", C.c_uint32 * 2), ("dtype", C.c_int)]
+
+
+class Requisition(C.Structure):
+    _fields_ = [("n_dims", C.c_uint32), ("dims", C.c_uint64 * 2)]
+
+
+class Subset(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("n", C.c_uint32), ("direction", C.c_int32)]
+
+
+class GeometryDims(C.Structure):
+    _fields_ = [("height", C.c_uint64), ("width", C.c_uint64), ("n_dets", C.c_uint64), ("n_angles", C.c_uint64)]
+
+
+_lib = None
+
+
+def load():
+    """Load libuir.so (built in-tree); raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libuir.so is not built (%s); run make -C ufo-ir_amd" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, st, sz, u32, f32 = C.c_void_p, C.c_int, C.c_size_t, C.c_uint32, C.c_float
+    L.uir_last_error.restype = C.c_char_p
+    L.uir_status_string.restype = C.c_char_p
+    L.uir_abi_version.restype = u32
+    L.uir_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.uir_resources_new.argtypes = [C.c_int, C.POINTER(vp)]
+    L.uir_resources_free.argtypes = [vp]
+    L.uir_resources_free.restype = None
+    L.uir_resources_stream.argtypes = [vp]
+    L.uir_resources_stream.restype = vp
+    L.uir_resources_synchronize.argtypes = [vp]
+    L.uir_device_malloc.argtypes = [sz, C.POINTER(vp)]
+    L.uir_device_free.argtypes = [vp]
+    for n in ("uir_memcpy_htod", "uir_memcpy_dtoh", "uir_memcpy_dtod"):
+        getattr(L, n).argtypes = [vp, vp, sz, vp]
+    L.uir_stream_synchronize.argtypes = [vp]
+    L.uir_mem_get_info.argtypes = [C.POINTER(sz), C.POINTER(sz)]
+    L.uir_timer_new.argtypes = [C.POINTER(vp)]
+    L.uir_timer_free.argtypes = [vp]
+    L.uir_timer_free.restype = None
+    L.uir_timer_start.argtypes = [vp, vp]
+    L.uir_timer_stop.argtypes = [vp, vp]
+    L.uir_timer_elapsed_ms.argtypes = [vp, C.POINTER(f32)]
+    for n in ("uir_ir_parallel_projector_task_new", "uir_ir_sirt_task_new", "uir_ir_sart_task_new",
+              "uir_ir_asdpocs_task_new", "uir_ir_sbtv_task_new"):
+        getattr(L, n).restype = vp
+        getattr(L, n).argtypes = []
+    L.uir_task_new.argtypes = [C.c_char_p]
+    L.uir_task_new.restype = vp
+    L.uir_task_ref.argtypes = [vp]
+    L.uir_task_ref.restype = vp
+    L.uir_task_unref.argtypes = [vp]
+    L.uir_task_unref.restype = None
+    L.uir_task_plugin.argtypes = [vp]
+    L.uir_task_plugin.restype = C.c_char_p
+    L.uir_task_set_float.argtypes = [vp, C.c_char_p, f32]
+    L.uir_task_set_uint.argtypes = [vp, C.c_char_p, u32]
+    L.uir_task_set_bool.argtypes = [vp, C.c_char_p, C.c_int]
+    L.uir_task_set_string.argtypes = [vp, C.c_char_p, C.c_char_p]
+    L.uir_task_set_object.argtypes = [vp, C.c_char_p, vp]
+    L.uir_task_get_float.argtypes = [vp, C.c_char_p, C.POINTER(f32)]
+    L.uir_task_get_uint.argtypes = [vp, C.c_char_p, C.POINTER(u32)]
+    L.uir_task_get_bool.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+    L.uir_task_get_string.argtypes = [vp, C.c_char_p, C.c_char_p, sz]
+    L.uir_task_get_object.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
+    L.uir_task_setup.argtypes = [vp, vp]
+    L.uir_task_get_requisition.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Requisition)]
+    L.uir_task_process.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer)]
+    L.uir_projector_forward.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer)]
+    L.uir_projector_backward.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer)]
+    L.uir_parallel_projector_subset_fp.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer), C.POINTER(Subset)]
+    L.uir_parallel_projector_subset_bp.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer), C.POINTER(Subset)]
+    L.uir_parallel_projector_get_luts.argtypes = [vp, C.POINTER(C.POINTER(f32)), C.POINTER(C.POINTER(f32)),
+                                                  C.POINTER(u32)]
+    L.uir_parallel_projector_get_subsets.argtypes = [vp, C.POINTER(C.POINTER(Subset)), C.POINTER(u32)]
+    L.uir_asdpocs_get_diagnostics.argtypes = [vp, C.POINTER(C.POINTER(f32)), C.POINTER(u32)]
+    L.uir_fp_joseph.argtypes = [vp, vp, vp, vp, vp, GeometryDims, f32, Subset, f32, vp]
+    L.uir_bp_joseph.argtypes = [vp, vp, vp, f32, vp, vp, GeometryDims, f32, Subset, vp]
+    L.uir_op_set.argtypes = [vp, sz, f32, vp]
+    L.uir_op_inv.argtypes = [vp, vp, sz, vp]
+    for n in ("uir_op_mul", "uir_op_add", "uir_op_deduction", "uir_op_max_ew", "uir_op_div_ew"):
+        getattr(L, n).argtypes = [vp, vp, vp, sz, vp]
+    for n in ("uir_op_deduction2", "uir_op_add2"):
+        getattr(L, n).argtypes = [vp, vp, f32, vp, sz, vp]
+    L.uir_op_mul_rows.argtypes = [vp, vp, vp, u32, u32, u32, vp]
+    L.uir_op_positive_constraint.argtypes = [vp, vp, sz, vp]
+    L.uir_op_mul_scalar.argtypes = [vp, sz, f32, vp]
+    L.uir_op_sqrt.argtypes = [vp, sz, vp]
+    L.uir_op_normalization.argtypes = [vp, sz, vp]
+    L.uir_op_l1_norm.argtypes = [vp, u32, u32, C.POINTER(f32), vp]
+    L.uir_op_dot_compat.argtypes = [vp, vp, u32, u32, C.POINTER(f32), vp]
+    L.uir_op_l2_compat.argtypes = [vp, u32, u32, C.POINTER(f32), vp]
+    L.uir_op_dot_f64.argtypes = [vp, vp, sz, C.POINTER(C.c_double), vp]
+    for n in ("uir_op_dx", "uir_op_dxt", "uir_op_dy", "uir_op_dyt", "uir_op_l1_grad"):
+        getattr(L, n).argtypes = [vp, vp, u32, u32, vp]
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != UIR_OK:
+        raise UirError(status, load().uir_last_error().decode(errors="replace"))
+
+
+def device_count():
+    n = C.c_int(0)
+    load().uir_device_count(C.byref(n))
+    return n.value
+
+
+def abi_version():
+    return load().uir_abi_version()
+
+
+def mem_get_info():
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    check(load().uir_mem_get_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+class Resources:
+    """A device plus its in-order stream (UfoResources + UfoGpuNode queue)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(load().uir_resources_new(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @property
+    def stream(self):
+        return load().uir_resources_stream(self.handle)
+
+    def synchronize(self):
+        check(load().uir_resources_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            load().uir_resources_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_NP_OF = {DTYPE_F32: np.float32, DTYPE_F16: np.float16, DTYPE_BF16: np.uint16}
+
+
+class DeviceArray:
+    """A 2-D device buffer ([height][width]) with UfoRequisition dims order."""
+
+    def __init__(self, shape, dtype=DTYPE_F32):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = dtype
+        self.itemsize = 2 if dtype in (DTYPE_F16, DTYPE_BF16) else 4
+        self.nbytes = int(np.prod(self.shape)) * self.itemsize
+        p = C.c_void_p()
+        check(load().uir_device_malloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=np.float32)
+        d = cls(arr.shape)
+        check(load().uir_memcpy_htod(d.ptr, arr.ctypes.data, d.nbytes, stream))
+        return d
+
+    @classmethod
+    def zeros(cls, shape, stream=None):
+        d = cls(shape)
+        check(load().uir_op_set(d.ptr, int(np.prod(shape)), 0.0, stream))
+        check(load().uir_stream_synchronize(stream))
+        return d
+
+    def numpy(self, stream=None):
+        out = np.empty(self.shape, dtype=_NP_OF[self.dtype])
+        check(load().uir_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes, stream))
+        return out
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=_NP_OF[self.dtype])
+        assert arr.nbytes == self.nbytes
+        check(load().uir_memcpy_htod(self.ptr, arr.ctypes.data, self.nbytes, stream))
+
+    def buffer(self):
+        b = Buffer()
+        b.data = self.ptr
+        b.dims[0] = self.shape[1] if len(self.shape) > 1 else self.shape[0]
+        b.dims[1] = self.shape[0] if len(self.shape) > 1 else 1
+        b.dtype = self.dtype
+        return b
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def free(self):
+        if self.ptr:
+            load().uir_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _c(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+class Task:
+    """A reference task plugin: Task('sirt', num_iterations=50, relaxation_factor=1.0)."""
+
+    def __init__(self, plugin, **props):
+        h = load().uir_task_new(_c(plugin))
+        if not h:
+            raise UirError(6, load().uir_last_error().decode())
+        self.handle = h
+        self._children = []
+        self.set(**props)
+
+    def set(self, **props):
+        for k, v in props.items():
+            self[k] = v
+        return self
+
+    def __setitem__(self, name, v):
+        L = load()
+        n = _c(name)
+        if isinstance(v, Task):
+            check(L.uir_task_set_object(self.handle, n, v.handle))
+            self._children.append(v)
+        elif isinstance(v, bool):
+            check(L.uir_task_set_bool(self.handle, n, int(v)))
+        elif isinstance(v, str):
+            check(L.uir_task_set_string(self.handle, n, _c(v)))
+        elif isinstance(v, int) and not isinstance(v, bool):
+            st = L.uir_task_set_uint(self.handle, n, v) if v >= 0 else 1
+            if st != UIR_OK:
+                check(L.uir_task_set_float(self.handle, n, float(v)))
+        else:
+            check(L.uir_task_set_float(self.handle, n, float(v)))
+
+    def get_float(self, name):
+        v = C.c_float()
+        check(load().uir_task_get_float(self.handle, _c(name), C.byref(v)))
+        return v.value
+
+    def get_uint(self, name):
+        v = C.c_uint32()
+        check(load().uir_task_get_uint(self.handle, _c(name), C.byref(v)))
+        return v.value
+
+    def get_bool(self, name):
+        v = C.c_int()
+        check(load().uir_task_get_bool(self.handle, _c(name), C.byref(v)))
+        return bool(v.value)
+
+    def get_string(self, name):
+        buf = C.create_string_buffer(256)
+        check(load().uir_task_get_string(self.handle, _c(name), buf, 256))
+        return buf.value.decode()
+
+    @property
+    def plugin(self):
+        return load().uir_task_plugin(self.handle).decode()
+
+    def setup(self, res):
+        check(load().uir_task_setup(self.handle, res.handle))
+        self._res = res
+        return self
+
+    def requisition(self, inp):
+        r = Requisition()
+        b = inp.buffer()
+        check(load().uir_task_get_requisition(self.handle, C.byref(b), C.byref(r)))
+        return (int(r.dims[1]), int(r.dims[0]))  # (height, width)
+
+    def process(self, inp, out):
+        bi, bo = inp.buffer(), out.buffer()
+        check(load().uir_task_process(self.handle, C.byref(bi), C.byref(bo)))
+
+    def luts(self):
+        s, c, n = C.POINTER(C.c_float)(), C.POINTER(C.c_float)(), C.c_uint32()
+        check(load().uir_parallel_projector_get_luts(self.handle, C.byref(s), C.byref(c), C.byref(n)))
+        return (np.ctypeslib.as_array(s, (n.value,)).copy(), np.ctypeslib.as_array(c, (n.value,)).copy())
+
+    def subsets(self):
+        p, n = C.POINTER(Subset)(), C.c_uint32()
+        check(load().uir_parallel_projector_get_subsets(self.handle, C.byref(p), C.byref(n)))
+        return [(p[i].offset, p[i].n, p[i].direction) for i in range(n.value)]
+
+    def forward(self, vol, sino):
+        bi, bo = vol.buffer(), sino.buffer()
+        check(load().uir_projector_forward(self.handle, C.byref(bi), C.byref(bo)))
+
+    def backward(self, sino, vol):
+        bi, bo = sino.buffer(), vol.buffer()
+        check(load().uir_projector_backward(self.handle, C.byref(bi), C.byref(bo)))
+
+    def subset_fp(self, vol, sino, subset):
+        bv, bs = vol.buffer(), sino.buffer()
+        s = Subset(*subset)
+        check(load().uir_parallel_projector_subset_fp(self.handle, C.byref(bv), C.byref(bs), C.byref(s)))
+
+    def subset_bp(self, vol, sino, subset):
+        bv, bs = vol.buffer(), sino.buffer()
+        s = Subset(*subset)
+        check(load().uir_parallel_projector_subset_bp(self.handle, C.byref(bv), C.byref(bs), C.byref(s)))
+
+    def asdpocs_diagnostics(self):
+        p, n = C.POINTER(C.c_float)(), C.c_uint32()
+        check(load().uir_asdpocs_get_diagnostics(self.handle, C.byref(p), C.byref(n)))
+        if n.value == 0:
+            return np.zeros((0, 4), np.float32)
+        return np.ctypeslib.as_array(p, (n.value * 4,)).reshape(-1, 4).copy()
+
+    def close(self):
+        if self.handle:
+            load().uir_task_unref(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Timer:
+    def __init__(self):
+        h = C.c_void_p()
+        check(load().uir_timer_new(C.byref(h)))
+        self.handle = h
+
+    def start(self, stream):
+        check(load().uir_timer_start(self.handle, stream))
+
+    def stop(self, stream):
+        check(load().uir_timer_stop(self.handle, stream))
+
+    def elapsed_ms(self):
+        v = C.c_float()
+        check(load().uir_timer_elapsed_ms(self.handle, C.byref(v)))
+        return v.value
+
+    def __del__(self):
+        try:
+            load().uir_timer_free(self.handle)
+        except Exception:
+            pass
+
+
+def header_functions(path=HEADER_PATH):
+    """Names of the functions include/uir.h declares."""
+    import re
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(uir_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
