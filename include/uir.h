@@ -200,6 +200,53 @@ uir_status uir_parallel_projector_get_subsets(uir_task *projector, const uir_sub
 uir_status uir_asdpocs_get_diagnostics(uir_task *task, const float **rows4, uint32_t *n_rows);
 
 /* ------------------------------------------------------------------------ */
+/* Batched SIRT.  No counterpart in the reference, which reconstructs one    */
+/* slice per process() call (ufo-ir-method-task.c:226-232); this runs the    */
+/* same SIRT update (ufo-ir-sirt-task.c:202-215) on many slices of uneven    */
+/* geometry at once: images and sinograms stay resident in FP32, FP16 or     */
+/* BF16 storage, interpolation and accumulation are FP32, one FP and one BP  */
+/* launch per iteration cover every slice through a work table, and the      */
+/* iterations are replayed from a captured hipGraph.                         */
+/* ------------------------------------------------------------------------ */
+typedef struct uir_batch uir_batch;
+
+typedef struct {
+    uint32_t n;              /* image edge (N x N) */
+    uint32_t n_dets;         /* detector bins */
+    uint32_t n_angles;       /* angles (sinogram rows) */
+    float    step;           /* angular step, rad (projector "step") */
+    float    axis_position;  /* projector "axis_position" (-1 = n_dets / 2) */
+    float    relaxation;     /* SIRT "relaxation_factor" */
+    int32_t  positive_constraint;
+} uir_slice_spec;
+
+typedef struct {
+    uint64_t device_bytes;       /* resident device memory of the batch */
+    uint32_t n_slices, n_geometries;
+    uint32_t fp_items, bp_items;
+    uint32_t fp_grid, bp_grid;
+    double   fp_samples, bp_samples; /* interpolation samples per iteration (in-image rays) */
+} uir_batch_info;
+
+uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_spec *specs, uint32_t n_slices,
+                         uir_batch **out);
+void       uir_batch_free(uir_batch *batch);
+/* Load slice `slice`'s sinogram ([n_angles][n_dets], FP32, host or device
+ * memory) into the batch's storage dtype. */
+uir_status uir_batch_set_sinogram(uir_batch *batch, uint32_t slice, const float *sino, int on_device);
+/* x = 0 for every slice (ufo-ir-sirt-task.c:197). */
+uir_status uir_batch_reset(uir_batch *batch);
+/* `iterations` SIRT iterations on every slice; use_graph replays a captured
+ * hipGraph, else launches directly.  Asynchronous on the batch stream. */
+uir_status uir_batch_iterate(uir_batch *batch, uint32_t iterations, int use_graph);
+/* Slice `slice`'s image as FP32 into host or device memory (synchronises). */
+uir_status uir_batch_get_image(uir_batch *batch, uint32_t slice, float *dst, int on_device);
+uir_status uir_batch_get_info(const uir_batch *batch, uir_batch_info *info);
+/* Average device duration (ms) of one FP and one BP launch over `reps`
+ * launches each, timed with events on the batch stream. */
+uir_status uir_batch_time_kernels(uir_batch *batch, uint32_t reps, float *fp_ms, float *bp_ms);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel-level ABI (the reference's inner boundary).  All pointers are      */
 /* device pointers; stream is a hipStream_t (NULL = default stream).         */
 /* ------------------------------------------------------------------------ */

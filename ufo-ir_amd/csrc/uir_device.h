@@ -66,6 +66,72 @@ int uirk_dot_compat(const float *a, const float *b, uint32_t w, uint32_t h, floa
 int uirk_minmax(const float *v, size_t n, float *ws, float *minmax_dev, void *stream);
 int uirk_normalize_apply(float *v, size_t n, const float *minmax_dev, void *stream);
 
+/* ---- Table-driven LDS-staged projectors (uir_fast.hip) ---- */
+
+/* Per-slice descriptor.  Images are n x n, sinograms n_angles x n_dets, all in
+ * the storage dtype of the launch except ray_w / pix_w (FP32). */
+typedef struct {
+    const void *x;       /* FP input image; BP reads the previous image here */
+    void *x_out;         /* BP output image (may equal x) */
+    const void *b;       /* FP input sinogram (r_sinogram / measured data) */
+    void *r;             /* FP output sinogram (may equal b) */
+    const float *ray_w;  /* FP: multiply the result by ray_w (NULL: none) */
+    const float *pix_w;  /* ref BP: SIRT pixel weights (NULL: accumulate form) */
+    const float *trig;   /* per angle (sin, cos) interleaved, FP32 */
+    int32_t n, n_dets, n_angles, posc;
+    float axis, corr, relax, pix_scale;
+    int32_t n_runs;
+    uint32_t run_off[UIRK_MAX_RUNS], run_n[UIRK_MAX_RUNS];
+} uirk_slice;
+
+/* FP work item: angles [a0, a0 + na) of one direction family (0 = horizontal,
+ * 1 = vertical) and detectors [d0, d0 + 64) of one slice. */
+typedef struct {
+    int32_t slice, dir, a0, na, d0, pad0, pad1, pad2;
+} uirk_fp_item;
+
+/* BP work item: the 64 x 64 pixel tile at (x0, y0) of one slice. */
+typedef struct {
+    int32_t slice, x0, y0, pad;
+} uirk_bp_item;
+
+#define UIRK_FP_DETS 64     /* detectors per FP work item */
+#define UIRK_FP_ANGLES 16   /* max angles per FP work item */
+#define UIRK_FP_LINES 32    /* lines per staged chunk (divides the 64-sample block) */
+#define UIRK_FP_WMAX 256    /* max staged footprint width */
+#define UIRK_BP_TILE 64     /* BP tile edge */
+
+/* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
+ * FP32 / FP16 / BF16 storage).  dtype: uir_dtype of the image / sinogram
+ * buffers (the reference-compatible mode takes FP32 only).  d_slices: device
+ * descriptor array indexed by item.slice, or NULL to use *one (host memory,
+ * passed by value as a kernel argument; item.slice must then be 0).
+ * err: device int, set non-zero if a footprint exceeded the staging window. */
+int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
+                  int mode, int dtype, int grid, int *d_err, void *stream);
+int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
+                  int mode, int dtype, int grid, int *d_err, void *stream);
+
+/* Work-table construction (host/uir_plan.c). */
+typedef struct {
+    uirk_fp_item *v;
+    size_t n, cap;
+} uirk_fp_vec;
+typedef struct {
+    uirk_bp_item *v;
+    size_t n, cap;
+} uirk_bp_vec;
+/* Appends the FP items of one slice for the given subsets (runs or single
+ * angles); angle blocks are sized so that each footprint fits the staging
+ * window.  Returns 0 on success, -1 on allocation failure. */
+int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs);
+int uirk_plan_bp(uirk_bp_vec *out, int slice, int n);
+void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig);
+int uirk_plan_grid(int n_items, int per_cu);
+/* FP32 <-> storage conversions */
+int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream);
+
 /* Finite differences (ufo-ir-gradient-processor.cl) and l1_grad. */
 int uirk_diff(int kind, const float *in, float *out, uint32_t w, uint32_t h, void *stream);
 enum { UIRK_DX = 0, UIRK_DXT = 1, UIRK_DY = 2, UIRK_DYT = 3 };

@@ -1,0 +1,612 @@
+// uir_fast.hip -- table-driven, LDS-staged Joseph projector pair for gfx950.
+//
+// One kernel family serves two paths:
+//   REF  = true : the reference's FP32 arithmetic operation for operation
+//                 (exact incremental float ray stepping from the double start
+//                 coordinate, 64-sample blocked sums, per-run BP sums, clamped
+//                 detector coordinate), bit-identical to the one-thread-per-ray
+//                 restatement of projector-parallel-joseph.cl.
+//   REF  = false: the batched fast path.  FP32 interpolation and accumulation
+//                 on FP32 / FP16 / BF16 storage, exact (double-anchored) ray
+//                 coordinates, fract / floor-convert sampling, fused SIRT
+//                 epilogues.
+// Work is described by tables (uirk_fp_item / uirk_bp_item) over any number
+// of slices of any geometry; the kernels are persistent (grid-stride over the
+// table) and every item writes disjoint outputs, so results do not depend on
+// the grid or on which other slices share the launch.
+//
+// FP: a workgroup owns 64 detectors x up to 16 angles of one direction family
+//     (one detector and four angles per thread).  It walks the image in
+//     chunks of 32 lines (rows for the vertical family, columns for the
+//     horizontal one); each chunk's footprint is staged into LDS line-major
+//     (transposed for the horizontal family), so both families read their two
+//     taps as adjacent LDS words.
+// BP: a workgroup owns a 64 x 64 pixel tile (one column and 16 rows per
+//     thread); sinogram rows and their sin/cos are staged 32 angles at a time
+//     over the tile's detector footprint.
+// Inner loops issue the LDS reads of all independent rays / rows of a step
+// before consuming any, so LDS latency overlaps within the wave.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "uir_device.h"
+
+namespace {
+
+constexpr int kFpAngles = UIRK_FP_ANGLES;
+constexpr int kFpApt = kFpAngles / 4;      // angles per thread (4 waves)
+constexpr int kFpLines = UIRK_FP_LINES;
+constexpr int kFpW = UIRK_FP_WMAX;
+constexpr int kFpLs = kFpW + 1;            // LDS line stride (odd: conflict-free transposed stores)
+constexpr int kBpT = UIRK_BP_TILE;
+constexpr int kBpRows = kBpT / 4;          // rows per thread
+constexpr int kBpNa = 32;                  // angles per staged batch
+constexpr int kBpWb = 100;                 // staged bins per angle (>= 64 * sqrt(2) + margin)
+
+template <typename T> __device__ __forceinline__ float to_f(T v);
+template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ float to_f<__half>(__half v) { return __half2float(v); }
+template <> __device__ __forceinline__ float to_f<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+template <typename T> __device__ __forceinline__ T from_f(float v);
+template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half from_f<__half>(float v) { return __float2half(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 from_f<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
+// floor(x) converted to int in one instruction (no memory access).
+__device__ __forceinline__ int cvt_flr(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// x - floor(x); exact for x >= 0 (the callers guarantee it).
+__device__ __forceinline__ float fract_pos(float x) { return __builtin_amdgcn_fractf(x); }
+
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ------------------------------------------------------------------------
+// Forward projection
+// ------------------------------------------------------------------------
+template <typename T, bool REF>
+__global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
+                                                const uirk_fp_item *__restrict__ items, int n_items,
+                                                int *__restrict__ err)
+{
+    __shared__ float lds[kFpLines * kFpLs];
+    __shared__ float red[2][4];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const uirk_fp_item item = items[it];
+        const uirk_slice *S = slices ? &slices[item.slice] : &one;
+        const int N = S->n, D = S->n_dets;
+        const T *img = (const T *)S->x;
+        const float *trig = S->trig;
+        const int s = item.d0 + lane;
+        const bool hor = item.dir == 0;
+
+        // projector-parallel-joseph.cl:51-66 (detector quantities use n_dets)
+        const float required_width = S->axis * 2;
+        const float diff = (float)D - required_width;
+        const float rot = diff / 2.0f;
+        const float origin_shift = (float)N / 2.0f;
+        const bool in_range = s < D;
+        const bool det_on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
+        const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
+
+        float pos[kFpApt], stp[kFpApt], acc[kFpApt], inner[kFpApt];
+        double p0d[kFpApt];
+        bool on[kFpApt];
+#pragma unroll
+        for (int j = 0; j < kFpApt; ++j) {
+            const int k = wave * kFpApt + j;
+            on[j] = det_on && k < item.na;
+            const int ang = item.a0 + (k < item.na ? k : 0);
+            const float st = trig[2 * ang], ct = trig[2 * ang + 1];
+            float det_step, step;
+            if (hor) {
+                det_step = -1.0f / st;
+                step = ct / st;
+            } else {
+                det_step = 1.0f / ct;
+                step = st / ct;
+            }
+            // start coordinate: double expression rounded to float (:73, :134)
+            const double p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
+            p0d[j] = p0;
+            pos[j] = (float)p0;
+            stp[j] = step;
+            acc[j] = 0.0f;
+            inner[j] = 0.0f;
+        }
+
+        for (int l0 = 0; l0 < N; l0 += kFpLines) {
+            const int nl = min(kFpLines, N - l0);
+            // Footprint of this block's rays over the chunk.
+            float lo = 3.0e38f, hi = -3.0e38f;
+#pragma unroll
+            for (int j = 0; j < kFpApt; ++j) {
+                if (!REF)
+                    pos[j] = (float)(p0d[j] + (double)l0 * (double)stp[j]);
+                if (on[j]) {
+                    const float a = pos[j], b = pos[j] + (float)(nl - 1) * stp[j];
+                    lo = fminf(lo, fminf(a, b));
+                    hi = fmaxf(hi, fmaxf(a, b));
+                }
+            }
+            lo = wave_min(lo);
+            hi = wave_max(hi);
+            __syncthreads(); // previous chunk's LDS reads are done
+            if (lane == 0) {
+                red[0][wave] = lo;
+                red[1][wave] = hi;
+            }
+            __syncthreads();
+            lo = fminf(fminf(red[0][0], red[0][1]), fminf(red[0][2], red[0][3]));
+            hi = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
+            if (!(lo <= hi))
+                break; // no active ray in this workgroup
+            const int pmin = (int)floorf(lo - 0.5f) - 1;
+            const int W = (int)floorf(hi - 0.5f) + 3 - pmin;
+            // A footprint wider than the staging window is reported and the
+            // chunk's samples are skipped (the host sizes angle blocks so that
+            // this does not happen); no sample ever reads outside the window.
+            const bool overflow = W > kFpW;
+            if (overflow && threadIdx.x == 0)
+                atomicOr(err, 1);
+            const bool outside = overflow || pmin + W <= 0 || pmin >= N;
+            if (!outside) {
+                if (hor) {
+                    // line = column l0 + l, position = row pmin + p; coalesced along columns
+                    const int l = threadIdx.x & 31;
+                    const int col = l0 + l;
+                    for (int p = threadIdx.x >> 5; p < W; p += 8) {
+                        const int row = pmin + p;
+                        float v = 0.0f;
+                        if (row >= 0 && row < N && l < nl)
+                            v = to_f<T>(img[(size_t)row * N + col]);
+                        lds[l * kFpLs + p] = v;
+                    }
+                } else {
+                    // line = row l0 + l, position = column pmin + p
+                    for (int l = 0; l < nl; ++l) {
+                        const T *rowp = img + (size_t)(l0 + l) * N;
+                        for (int p = threadIdx.x; p < W; p += 256) {
+                            const int col = pmin + p;
+                            lds[l * kFpLs + p] = (col >= 0 && col < N) ? to_f<T>(rowp[col]) : 0.0f;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+
+            // Inactive rays sample a fixed in-window position with zero step;
+            // their sums are never written.
+            const float park = lo + 1.0f;
+            if (REF) {
+                if (!outside) {
+                    float pp[kFpApt], se[kFpApt];
+#pragma unroll
+                    for (int j = 0; j < kFpApt; ++j) {
+                        pp[j] = on[j] ? pos[j] : park;
+                        se[j] = on[j] ? stp[j] : 0.0f;
+                    }
+                    for (int l = 0; l < nl; ++l) {
+                        const float *L = lds + l * kFpLs - pmin;
+                        float t0[kFpApt], t1[kFpApt], b[kFpApt];
+#pragma unroll
+                        for (int j = 0; j < kFpApt; ++j) {
+                            const float v = pp[j] - 0.5f;
+                            const float fv = floorf(v);
+                            const int i0 = (int)fv;
+                            b[j] = v - fv;
+                            t0[j] = L[i0];
+                            t1[j] = L[i0 + 1];
+                        }
+#pragma unroll
+                        for (int j = 0; j < kFpApt; ++j) {
+                            const float omb = 1.0f - b[j];
+                            inner[j] += omb * t0[j] + b[j] * t1[j];
+                            pp[j] += se[j];
+                        }
+                        if (((l0 + l + 1) & 63) == 0) { // 64-sample blocks (:82-90)
+#pragma unroll
+                            for (int j = 0; j < kFpApt; ++j) {
+                                acc[j] += inner[j];
+                                inner[j] = 0.0f;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < kFpApt; ++j)
+                        if (on[j])
+                            pos[j] = pp[j];
+                } else {
+                    // outside the image: the samples are zero, the exact
+                    // float stepping of the coordinates continues
+                    for (int l = 0; l < nl; ++l) {
+#pragma unroll
+                        for (int j = 0; j < kFpApt; ++j)
+                            pos[j] += stp[j];
+                        if (((l0 + l + 1) & 63) == 0) {
+#pragma unroll
+                            for (int j = 0; j < kFpApt; ++j) {
+                                acc[j] += inner[j];
+                                inner[j] = 0.0f;
+                            }
+                        }
+                    }
+                }
+            } else if (!outside) {
+                const float off = 0.5f + (float)pmin;
+                float q[kFpApt], st[kFpApt];
+#pragma unroll
+                for (int j = 0; j < kFpApt; ++j) {
+                    q[j] = on[j] ? pos[j] - off : park - off;
+                    st[j] = on[j] ? stp[j] : 0.0f;
+                }
+                for (int l = 0; l < nl; ++l) {
+                    const float *L = lds + l * kFpLs;
+                    float t0[kFpApt], t1[kFpApt], a[kFpApt];
+#pragma unroll
+                    for (int j = 0; j < kFpApt; ++j) {
+                        const int i0 = cvt_flr(q[j]);
+                        a[j] = fract_pos(q[j]);
+                        t0[j] = L[i0];
+                        t1[j] = L[i0 + 1];
+                    }
+#pragma unroll
+                    for (int j = 0; j < kFpApt; ++j) {
+                        acc[j] = __builtin_fmaf(a[j], t1[j] - t0[j], acc[j] + t0[j]);
+                        q[j] += st[j];
+                    }
+                }
+            }
+        }
+
+        const T *bsrc = (const T *)S->b;
+        T *rdst = (T *)S->r;
+        const float *ray_w = S->ray_w;
+        const float corr = S->corr;
+#pragma unroll
+        for (int j = 0; j < kFpApt; ++j) {
+            const int k = wave * kFpApt + j;
+            if (!in_range || k >= item.na)
+                continue;
+            const int ang = item.a0 + k;
+            const size_t o = (size_t)ang * D + s;
+            const float bval = to_f<T>(bsrc[o]);
+            float w;
+            if (on[j]) {
+                const float sum = REF ? acc[j] + inner[j] : acc[j];
+                w = bval + sum * corr; // :92-94
+            } else {
+                w = bval; // inactive detector: the reference leaves the value as read
+            }
+            if (ray_w)
+                w = w * ray_w[o];
+            rdst[o] = from_f<T>(w);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Backprojection
+// ------------------------------------------------------------------------
+template <typename T, bool REF>
+__global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
+                                                const uirk_bp_item *__restrict__ items, int n_items,
+                                                int *__restrict__ err)
+{
+    __shared__ float lds[kBpNa * kBpWb];
+    __shared__ int s_umin[kBpNa];
+    __shared__ float s_trig[2 * kBpNa];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const uirk_bp_item item = items[it];
+        const uirk_slice *S = slices ? &slices[item.slice] : &one;
+        const int N = S->n, D = S->n_dets;
+        const T *sino = (const T *)S->b;
+        const float *trig = S->trig;
+        const float relax = S->relax;
+        const int n_runs = S->n_runs;
+        const int px = item.x0 + lane;
+
+        // projector-parallel-joseph.cl:174-190 (detector quantities use n_dets)
+        const float axis = S->axis;
+        const float required_width = axis * 2;
+        const float diff = (float)D - required_width;
+        const float half_active = diff < 0 ? (float)D - axis : axis;
+        const float edge0 = axis - half_active + 0.5f;
+        const float edge1 = axis + half_active - 0.5f;
+        const float rot = diff / 2.0f;
+        const float det_shift = (float)D / 2.0f;
+        const float origin_shift = (float)N / 2.0f;
+        const float shift = det_shift - rot;
+        const float fX = (float)px + 0.5f - origin_shift;
+        // active bins for the fast path's clamp-to-edge staging
+        const int bin_lo = (int)floorf(edge0 - 0.5f);
+        const int bin_hi = (int)floorf(edge1 - 0.5f);
+        // tile corner coordinates (pixel centres)
+        const float cx0 = (float)item.x0 + 0.5f - origin_shift;
+        const float cx1 = cx0 + (float)(kBpT - 1);
+        const float cy0 = (float)item.y0 + 0.5f - origin_shift;
+        const float cy1 = cy0 + (float)(kBpT - 1);
+        const float fY0 = (float)(item.y0 + wave) + 0.5f - origin_shift;
+
+        float acc[kBpRows], val[kBpRows];
+#pragma unroll
+        for (int k = 0; k < kBpRows; ++k) {
+            acc[k] = 0.0f;
+            val[k] = 0.0f;
+        }
+        if (REF && !S->pix_w) {
+            const T *xin = (const T *)S->x;
+#pragma unroll
+            for (int k = 0; k < kBpRows; ++k) {
+                const int py = item.y0 + wave + 4 * k;
+                acc[k] = (px < N && py < N) ? to_f<T>(xin[(size_t)py * N + px]) : 0.0f;
+            }
+        }
+
+        for (int r = 0; r < n_runs; ++r) {
+            const int roff = (int)S->run_off[r], rn = (int)S->run_n[r];
+            for (int b0 = 0; b0 < rn; b0 += kBpNa) {
+                const int na = min(kBpNa, rn - b0);
+                const int abase = roff + b0;
+                __syncthreads(); // previous batch's reads are done
+                if (threadIdx.x < na) {
+                    const int ang = abase + threadIdx.x;
+                    const float st = trig[2 * ang], ct = trig[2 * ang + 1];
+                    const float u00 = cx0 * ct - cy0 * st, u10 = cx1 * ct - cy0 * st;
+                    const float u01 = cx0 * ct - cy1 * st, u11 = cx1 * ct - cy1 * st;
+                    float umn = fminf(fminf(u00, u10), fminf(u01, u11)) + shift;
+                    float umx = fmaxf(fmaxf(u00, u10), fmaxf(u01, u11)) + shift;
+                    if (REF) {
+                        umn = fminf(fmaxf(umn, edge0), edge1);
+                        umx = fminf(fmaxf(umx, edge0), edge1);
+                    }
+                    const int umin = (int)floorf(umn - 0.5f) - 1;
+                    if ((int)floorf(umx - 0.5f) + 2 - umin >= kBpWb)
+                        atomicOr(err, 2);
+                    s_umin[threadIdx.x] = umin;
+                    s_trig[2 * threadIdx.x] = st;
+                    s_trig[2 * threadIdx.x + 1] = ct;
+                }
+                __syncthreads();
+                for (int k = wave; k < na; k += 4) {
+                    const int ang = abase + k;
+                    const int umin = s_umin[k];
+                    const T *row = sino + (size_t)ang * D;
+                    for (int i = lane; i < kBpWb; i += 64) {
+                        int bin = umin + i;
+                        float v;
+                        if (REF) {
+                            v = (bin >= 0 && bin < D) ? to_f<T>(row[bin]) : 0.0f;
+                        } else {
+                            bin = bin < bin_lo ? bin_lo : (bin > bin_hi ? bin_hi : bin);
+                            v = to_f<T>(row[bin]);
+                        }
+                        lds[k * kBpWb + i] = v;
+                    }
+                }
+                __syncthreads();
+                for (int k = 0; k < na; ++k) {
+                    const float st = s_trig[2 * k], ct = s_trig[2 * k + 1];
+                    const float *L = lds + k * kBpWb - s_umin[k];
+                    if (REF) {
+                        const float fXc = fX * ct;
+#pragma unroll
+                        for (int g = 0; g < kBpRows; g += 4) {
+                            float t0[4], t1[4], a[4];
+#pragma unroll
+                            for (int h = 0; h < 4; ++h) {
+                                const float fY = fY0 + (float)(4 * (g + h));
+                                float u = fXc - fY * st + shift;
+                                u = __builtin_amdgcn_fmed3f(u, edge0, edge1); // the clamp of :206-210
+                                const float w = u - 0.5f;                     // >= 0: edge0 >= 0.5
+                                const int i0 = cvt_flr(w);
+                                a[h] = fract_pos(w);
+                                t0[h] = L[i0];
+                                t1[h] = L[i0 + 1];
+                            }
+#pragma unroll
+                            for (int h = 0; h < 4; ++h) {
+                                const float oma = 1.0f - a[h];
+                                val[g + h] += oma * t0[h] + a[h] * t1[h];
+                            }
+                        }
+                    } else {
+                        float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f));
+                        const float du = -4.0f * st;
+#pragma unroll
+                        for (int g = 0; g < kBpRows; g += 8) {
+                            float t0[8], t1[8], a[8];
+#pragma unroll
+                            for (int h = 0; h < 8; ++h) {
+                                const float uu = __builtin_fmaf((float)h, du, u);
+                                const int i0 = cvt_flr(uu);
+                                a[h] = __builtin_amdgcn_fractf(uu);
+                                t0[h] = L[i0];
+                                t1[h] = L[i0 + 1];
+                            }
+#pragma unroll
+                            for (int h = 0; h < 8; ++h)
+                                acc[g + h] = __builtin_fmaf(a[h], t1[h] - t0[h], acc[g + h] + t0[h]);
+                            u = __builtin_fmaf(8.0f, du, u);
+                        }
+                    }
+                }
+            }
+            if (REF) { // per-run accumulation (:216)
+#pragma unroll
+                for (int q = 0; q < kBpRows; ++q) {
+                    acc[q] = acc[q] + relax * val[q];
+                    val[q] = 0.0f;
+                }
+            }
+        }
+
+        if (px < N) {
+            const T *xin = (const T *)S->x;
+            T *xout = (T *)S->x_out;
+            const float *pix_w = S->pix_w;
+            const int posc = S->posc;
+            const float gain = S->relax * S->pix_scale;
+#pragma unroll
+            for (int q = 0; q < kBpRows; ++q) {
+                const int py = item.y0 + wave + 4 * q;
+                if (py >= N)
+                    continue;
+                const size_t o = (size_t)py * N + px;
+                float out;
+                if (REF) {
+                    out = acc[q];
+                    if (pix_w) { // SIRT form (ufo-ir-sirt-task.c:208-212)
+                        out = out * pix_w[o];
+                        out = out + to_f<T>(xin[o]);
+                    }
+                } else {
+                    out = __builtin_fmaf(acc[q], gain, to_f<T>(xin[o]));
+                }
+                if (posc)
+                    out = out > 0 ? out : 0.0f;
+                xout[o] = from_f<T>(out);
+            }
+        }
+    }
+}
+
+__global__ void k_convert_f32_to(const float *src, void *dst, int dtype, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (dtype == UIR_DTYPE_F16)
+            ((__half *)dst)[i] = __float2half(src[i]);
+        else if (dtype == UIR_DTYPE_BF16)
+            ((__hip_bfloat16 *)dst)[i] = __float2bfloat16(src[i]);
+        else
+            ((float *)dst)[i] = src[i];
+    }
+}
+
+__global__ void k_convert_to_f32(const void *src, int dtype, float *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (dtype == UIR_DTYPE_F16)
+            dst[i] = __half2float(((const __half *)src)[i]);
+        else if (dtype == UIR_DTYPE_BF16)
+            dst[i] = __bfloat162float(((const __hip_bfloat16 *)src)[i]);
+        else
+            dst[i] = ((const float *)src)[i];
+    }
+}
+
+template <typename T, bool REF>
+int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n, int grid, int *err,
+              hipStream_t st)
+{
+    hipLaunchKernelGGL((k_fp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, err);
+    return (int)hipGetLastError();
+}
+
+template <typename T, bool REF>
+int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n, int grid, int *err,
+              hipStream_t st)
+{
+    hipLaunchKernelGGL((k_bp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, err);
+    return (int)hipGetLastError();
+}
+
+} // namespace
+
+extern "C" {
+
+int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
+                  int mode, int dtype, int grid, int *d_err, void *stream)
+{
+    uirk_slice single{};
+    if (!d_slices) {
+        if (!one)
+            return (int)hipErrorInvalidValue;
+        single = *one;
+    }
+    if (n_items <= 0)
+        return 0;
+    if (grid <= 0 || grid > n_items)
+        grid = n_items;
+    hipStream_t st = (hipStream_t)stream;
+    if (mode == 0) {
+        if (dtype != UIR_DTYPE_F32)
+            return (int)hipErrorInvalidValue;
+        return launch_fp<float, true>(d_slices, single, d_items, n_items, grid, d_err, st);
+    }
+    switch (dtype) {
+    case UIR_DTYPE_F16: return launch_fp<__half, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    case UIR_DTYPE_BF16: return launch_fp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    default: return launch_fp<float, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    }
+}
+
+int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
+                  int mode, int dtype, int grid, int *d_err, void *stream)
+{
+    uirk_slice single{};
+    if (!d_slices) {
+        if (!one)
+            return (int)hipErrorInvalidValue;
+        single = *one;
+    }
+    if (n_items <= 0)
+        return 0;
+    if (grid <= 0 || grid > n_items)
+        grid = n_items;
+    hipStream_t st = (hipStream_t)stream;
+    if (mode == 0) {
+        if (dtype != UIR_DTYPE_F32)
+            return (int)hipErrorInvalidValue;
+        return launch_bp<float, true>(d_slices, single, d_items, n_items, grid, d_err, st);
+    }
+    switch (dtype) {
+    case UIR_DTYPE_F16: return launch_bp<__half, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    case UIR_DTYPE_BF16: return launch_bp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    default: return launch_bp<float, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    }
+}
+
+int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream)
+{
+    if (!n)
+        return 0;
+    size_t g = (n + 255) / 256;
+    if (g > 65536)
+        g = 65536;
+    if (src_dtype == UIR_DTYPE_F32)
+        hipLaunchKernelGGL(k_convert_f32_to, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream,
+                           (const float *)src, dst, dst_dtype, n);
+    else if (dst_dtype == UIR_DTYPE_F32)
+        hipLaunchKernelGGL(k_convert_to_f32, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, src, src_dtype,
+                           (float *)dst, n);
+    else
+        return (int)hipErrorInvalidValue;
+    return (int)hipGetLastError();
+}
+
+} // extern "C"

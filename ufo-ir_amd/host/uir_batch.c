@@ -1,0 +1,510 @@
+/*
+ * uir_batch.c -- batched SIRT over many slices of uneven geometry.
+ *
+ * Layout in HBM: one allocation holds, per slice, the image x (N*N), the
+ * measured sinogram b (A*D) and the weighted residual r (A*D) in the storage
+ * dtype, and, per distinct geometry, the FP32 ray weights R = 1/FP(1) (A*D)
+ * and the (sin, cos) table.  The pixel weight of the fast projector pair is
+ * the scalar 1/A (its backprojection of ones is exactly A).
+ *
+ * One iteration = one FP launch over every slice's FP work items
+ * (r = (b - A x) * R) and one BP launch over every slice's tiles
+ * (x += relaxation / A * A^T r, optionally clamped at 0), i.e. the SIRT update
+ * of ufo-ir-sirt-task.c:202-215.  Iterations are captured once into a hipGraph
+ * and replayed.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "uir_internal.h"
+
+typedef struct {
+    uint32_t n, D, A;
+    float step, axis;        /* resolved axis */
+    float *h_sin, *h_cos;
+    float *d_trig, *d_ray_w; /* carved from the batch allocation */
+    uir_subset *runs;
+    uint32_t n_runs;
+} geom_t;
+
+struct uir_batch {
+    uir_resources *res;
+    uir_dtype dtype;
+    size_t es; /* storage element size */
+    uint32_t n_slices, n_geoms;
+    uir_slice_spec *specs;
+    uint32_t *geom_of;
+    geom_t *geoms;
+    void *mem;
+    size_t mem_bytes;
+    void **x, **b, **r;
+    uirk_slice *d_fp_sl, *d_bp_sl;
+    uirk_fp_item *d_fp_items;
+    uirk_bp_item *d_bp_items;
+    int n_fp_items, n_bp_items, fp_grid, bp_grid;
+    int *d_err;
+    double fp_samples, bp_samples;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    uint32_t graph_iters;
+};
+
+#define TRY(x) do { uir_status s_ = (x); if (s_ != UIR_OK) return s_; } while (0)
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int same_geom(const uir_slice_spec *s, const geom_t *g)
+{
+    return s->n == g->n && s->n_dets == g->D && s->n_angles == g->A && s->step == g->step &&
+           uir_geom_axis(s->axis_position, s->n_dets) == g->axis;
+}
+
+void uir_batch_free(uir_batch *b)
+{
+    if (!b)
+        return;
+    if (b->res)
+        (void)hipStreamSynchronize(b->res->stream);
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    for (uint32_t i = 0; i < b->n_geoms; ++i) {
+        free(b->geoms[i].h_sin);
+        free(b->geoms[i].h_cos);
+        free(b->geoms[i].runs);
+    }
+    free(b->geoms);
+    free(b->geom_of);
+    free(b->specs);
+    free(b->x);
+    free(b->b);
+    free(b->r);
+    uir_dfree(b->mem);
+    uir_dfree(b->d_fp_sl);
+    uir_dfree(b->d_bp_sl);
+    uir_dfree(b->d_fp_items);
+    uir_dfree(b->d_bp_items);
+    uir_dfree(b->d_err);
+    uir_resources_free(b->res);
+    free(b);
+}
+
+static int cmp_fp_cost_desc(const void *pa, const void *pb, void *ctx)
+{
+    const uir_batch *b = (const uir_batch *)ctx;
+    const uirk_fp_item *x = (const uirk_fp_item *)pa, *y = (const uirk_fp_item *)pb;
+    const double cx = (double)b->specs[x->slice].n * x->na, cy = (double)b->specs[y->slice].n * y->na;
+    if (cx != cy)
+        return cx > cy ? -1 : 1;
+    return (x->slice != y->slice) ? x->slice - y->slice : (x->a0 != y->a0 ? x->a0 - y->a0 : x->d0 - y->d0);
+}
+
+static int cmp_bp_cost_desc(const void *pa, const void *pb, void *ctx)
+{
+    const uir_batch *b = (const uir_batch *)ctx;
+    const uirk_bp_item *x = (const uirk_bp_item *)pa, *y = (const uirk_bp_item *)pb;
+    const uint32_t cx = b->specs[x->slice].n_angles, cy = b->specs[y->slice].n_angles;
+    if (cx != cy)
+        return cx > cy ? -1 : 1;
+    return (x->slice != y->slice) ? x->slice - y->slice : (x->y0 != y->y0 ? x->y0 - y->y0 : x->x0 - y->x0);
+}
+
+static uir_status upload(void **dst, const void *src, size_t bytes)
+{
+    TRY(uir_dmalloc(dst, bytes));
+    return uir_check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+/* R = 1 / FP(1) for one geometry with the fast FP32 projector (inv(0) = 0). */
+static uir_status ray_weights(uir_batch *bt, geom_t *g)
+{
+    const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->A * g->D;
+    float *ones = NULL, *zeros = NULL;
+    uirk_fp_item *d_items = NULL;
+    uirk_fp_vec fv = {0};
+    uir_status s = uir_dmalloc((void **)&ones, nv * 4);
+    if (s == UIR_OK)
+        s = uir_dmalloc((void **)&zeros, ns * 4);
+    if (s == UIR_OK && uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs))
+        s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+    if (s == UIR_OK)
+        s = upload((void **)&d_items, fv.v, fv.n * sizeof(uirk_fp_item));
+    hipStream_t st = bt->res->stream;
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_set(ones, nv, 1.0f, st), "set");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_set(zeros, ns, 0.0f, st), "set");
+    if (s == UIR_OK) {
+        uirk_slice sl;
+        memset(&sl, 0, sizeof sl);
+        sl.x = ones;
+        sl.b = zeros;
+        sl.r = g->d_ray_w;
+        sl.trig = g->d_trig;
+        sl.n = (int32_t)g->n;
+        sl.n_dets = (int32_t)g->D;
+        sl.n_angles = (int32_t)g->A;
+        sl.axis = g->axis;
+        sl.corr = 1.0f;
+        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, 1, UIR_DTYPE_F32,
+                                        uirk_plan_grid((int)fv.n, 4), bt->d_err, st), "FP launch");
+    }
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_inv(g->d_ray_w, g->d_ray_w, ns, st), "inv");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uir_dfree(ones);
+    uir_dfree(zeros);
+    uir_dfree(d_items);
+    free(fv.v);
+    return s;
+}
+
+uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_spec *specs, uint32_t n_slices,
+                         uir_batch **out)
+{
+    if (!res || !specs || !out || n_slices == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument or empty batch");
+    if (storage != UIR_DTYPE_F32 && storage != UIR_DTYPE_F16 && storage != UIR_DTYPE_BF16)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
+    *out = NULL;
+    for (uint32_t i = 0; i < n_slices; ++i) {
+        const uir_slice_spec *s = &specs[i];
+        if (s->n == 0 || s->n_dets == 0 || s->n_angles == 0 || s->n > 16384 || s->n_dets > 32768 ||
+            s->n_angles > 65536)
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad geometry %ux%u, %u angles", i, s->n,
+                                 s->n_dets, s->n_angles);
+        if (!(s->relaxation >= 0.0f && s->relaxation <= 1.0f))
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: relaxation outside [0, 1]", i);
+        if (!(s->step >= 0.0f) || !(s->axis_position >= -1.0f))
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad step or axis", i);
+    }
+    TRY(uir_check_hip(hipSetDevice(res->device), "hipSetDevice"));
+    uir_batch *b = (uir_batch *)calloc(1, sizeof *b);
+    if (!b)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
+    b->res = uir_resources_retain(res);
+    b->dtype = storage;
+    b->es = storage == UIR_DTYPE_F32 ? 4 : 2;
+    b->n_slices = n_slices;
+    b->specs = (uir_slice_spec *)malloc(sizeof *specs * n_slices);
+    b->geom_of = (uint32_t *)calloc(n_slices, sizeof(uint32_t));
+    b->geoms = (geom_t *)calloc(n_slices, sizeof(geom_t));
+    b->x = (void **)calloc(n_slices, sizeof(void *));
+    b->b = (void **)calloc(n_slices, sizeof(void *));
+    b->r = (void **)calloc(n_slices, sizeof(void *));
+    uir_status s = UIR_OK;
+    uirk_slice *fp_sl = NULL, *bp_sl = NULL;
+    uirk_fp_vec fv = {0};
+    uirk_bp_vec bv = {0};
+    if (!b->specs || !b->geom_of || !b->geoms || !b->x || !b->b || !b->r) {
+        s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+        goto fail;
+    }
+    memcpy(b->specs, specs, sizeof *specs * n_slices);
+
+    /* distinct geometries: LUTs and direction runs as the projector builds them */
+    for (uint32_t i = 0; i < n_slices; ++i) {
+        uint32_t g = 0;
+        while (g < b->n_geoms && !same_geom(&specs[i], &b->geoms[g]))
+            ++g;
+        if (g == b->n_geoms) {
+            geom_t *G = &b->geoms[b->n_geoms++];
+            G->n = specs[i].n;
+            G->D = specs[i].n_dets;
+            G->A = specs[i].n_angles;
+            G->step = specs[i].step;
+            G->axis = uir_geom_axis(specs[i].axis_position, specs[i].n_dets);
+            G->h_sin = (float *)malloc(sizeof(float) * G->A);
+            G->h_cos = (float *)malloc(sizeof(float) * G->A);
+            G->runs = (uir_subset *)malloc(sizeof(uir_subset) * G->A);
+            if (!G->h_sin || !G->h_cos || !G->runs) {
+                s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+                goto fail;
+            }
+            uir_geom_lut(G->A, G->step, G->h_sin, G->h_cos);
+            G->n_runs = uir_geom_runs(G->h_sin, G->h_cos, G->A, G->runs);
+        }
+        b->geom_of[i] = g;
+    }
+
+    /* one allocation for everything resident */
+    size_t total = 0;
+    for (uint32_t i = 0; i < n_slices; ++i) {
+        const size_t nv = (size_t)specs[i].n * specs[i].n, ns = (size_t)specs[i].n_angles * specs[i].n_dets;
+        total += align_up(nv * b->es) + 2 * align_up(ns * b->es);
+    }
+    for (uint32_t g = 0; g < b->n_geoms; ++g)
+        total += align_up((size_t)b->geoms[g].A * b->geoms[g].D * 4) + align_up((size_t)b->geoms[g].A * 8);
+    if ((s = uir_dmalloc(&b->mem, total)) != UIR_OK)
+        goto fail;
+    b->mem_bytes = total;
+    {
+        char *p = (char *)b->mem;
+        for (uint32_t i = 0; i < n_slices; ++i) {
+            const size_t nv = (size_t)specs[i].n * specs[i].n, ns = (size_t)specs[i].n_angles * specs[i].n_dets;
+            b->x[i] = p;
+            p += align_up(nv * b->es);
+            b->b[i] = p;
+            p += align_up(ns * b->es);
+            b->r[i] = p;
+            p += align_up(ns * b->es);
+        }
+        for (uint32_t g = 0; g < b->n_geoms; ++g) {
+            geom_t *G = &b->geoms[g];
+            G->d_ray_w = (float *)p;
+            p += align_up((size_t)G->A * G->D * 4);
+            G->d_trig = (float *)p;
+            p += align_up((size_t)G->A * 8);
+        }
+    }
+    if ((s = uir_check_hip(hipMemset(b->mem, 0, total), "hipMemset")) != UIR_OK)
+        goto fail;
+    if ((s = uir_dmalloc((void **)&b->d_err, sizeof(int))) != UIR_OK ||
+        (s = uir_check_hip(hipMemset(b->d_err, 0, sizeof(int)), "hipMemset")) != UIR_OK)
+        goto fail;
+    for (uint32_t g = 0; g < b->n_geoms; ++g) {
+        geom_t *G = &b->geoms[g];
+        float *trig = (float *)malloc(sizeof(float) * 2 * G->A);
+        if (!trig) {
+            s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+            goto fail;
+        }
+        uirk_plan_trig(G->h_sin, G->h_cos, G->A, trig);
+        s = uir_check_hip(hipMemcpy(G->d_trig, trig, sizeof(float) * 2 * G->A, hipMemcpyHostToDevice), "hipMemcpy");
+        free(trig);
+        if (s != UIR_OK || (s = ray_weights(b, G)) != UIR_OK)
+            goto fail;
+    }
+
+    /* descriptors and work tables */
+    fp_sl = (uirk_slice *)calloc(n_slices, sizeof *fp_sl);
+    bp_sl = (uirk_slice *)calloc(n_slices, sizeof *bp_sl);
+    if (!fp_sl || !bp_sl) {
+        s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
+        goto fail;
+    }
+    for (uint32_t i = 0; i < n_slices; ++i) {
+        const geom_t *G = &b->geoms[b->geom_of[i]];
+        uirk_slice *f = &fp_sl[i], *k = &bp_sl[i];
+        f->x = b->x[i];
+        f->b = b->b[i];
+        f->r = b->r[i];
+        f->ray_w = G->d_ray_w;
+        f->trig = G->d_trig;
+        f->n = (int32_t)G->n;
+        f->n_dets = (int32_t)G->D;
+        f->n_angles = (int32_t)G->A;
+        f->axis = G->axis;
+        f->corr = -1.0f;
+        *k = *f;
+        k->x = b->x[i];
+        k->x_out = b->x[i];
+        k->b = b->r[i];
+        k->r = NULL;
+        k->ray_w = NULL;
+        k->relax = specs[i].relaxation;
+        k->pix_scale = 1.0f / (float)G->A;
+        k->posc = specs[i].positive_constraint ? 1 : 0;
+        k->n_runs = 1;
+        k->run_off[0] = 0;
+        k->run_n[0] = G->A;
+        if (uirk_plan_fp(&fv, (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos, G->runs, G->n_runs) ||
+            uirk_plan_bp(&bv, (int)i, (int)G->n)) {
+            s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+            goto fail;
+        }
+        double m = 0;
+        for (uint32_t a = 0; a < G->A; ++a)
+            m += fabs(G->h_sin[a]) > fabs(G->h_cos[a]) ? fabs(G->h_sin[a]) : fabs(G->h_cos[a]);
+        b->fp_samples += m * (double)G->n * G->n;
+        b->bp_samples += (double)G->A * G->n * G->n;
+    }
+    qsort_r(fv.v, fv.n, sizeof(uirk_fp_item), cmp_fp_cost_desc, b);
+    qsort_r(bv.v, bv.n, sizeof(uirk_bp_item), cmp_bp_cost_desc, b);
+    if ((s = upload((void **)&b->d_fp_sl, fp_sl, n_slices * sizeof *fp_sl)) != UIR_OK ||
+        (s = upload((void **)&b->d_bp_sl, bp_sl, n_slices * sizeof *bp_sl)) != UIR_OK ||
+        (s = upload((void **)&b->d_fp_items, fv.v, fv.n * sizeof(uirk_fp_item))) != UIR_OK ||
+        (s = upload((void **)&b->d_bp_items, bv.v, bv.n * sizeof(uirk_bp_item))) != UIR_OK)
+        goto fail;
+    b->n_fp_items = (int)fv.n;
+    b->n_bp_items = (int)bv.n;
+    b->fp_grid = uirk_plan_grid(b->n_fp_items, 4);
+    b->bp_grid = uirk_plan_grid(b->n_bp_items, 8);
+    free(fp_sl);
+    free(bp_sl);
+    free(fv.v);
+    free(bv.v);
+    *out = b;
+    return UIR_OK;
+fail:
+    free(fp_sl);
+    free(bp_sl);
+    free(fv.v);
+    free(bv.v);
+    uir_batch_free(b);
+    return s;
+}
+
+uir_status uir_batch_set_sinogram(uir_batch *b, uint32_t slice, const float *sino, int on_device)
+{
+    if (!b || !sino || slice >= b->n_slices)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad slice or NULL sinogram");
+    const size_t ns = (size_t)b->specs[slice].n_angles * b->specs[slice].n_dets;
+    hipStream_t st = b->res->stream;
+    if (on_device)
+        return uir_check_hip(uirk_convert(sino, UIR_DTYPE_F32, b->b[slice], b->dtype, ns, st), "convert");
+    float *tmp = NULL;
+    TRY(uir_dmalloc((void **)&tmp, ns * 4));
+    uir_status s = uir_check_hip(hipMemcpyAsync(tmp, sino, ns * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_convert(tmp, UIR_DTYPE_F32, b->b[slice], b->dtype, ns, st), "convert");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uir_dfree(tmp);
+    return s;
+}
+
+uir_status uir_batch_reset(uir_batch *b)
+{
+    if (!b)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    for (uint32_t i = 0; i < b->n_slices; ++i) {
+        const size_t nv = (size_t)b->specs[i].n * b->specs[i].n;
+        TRY(uir_check_hip(hipMemsetAsync(b->x[i], 0, nv * b->es, b->res->stream), "hipMemsetAsync"));
+    }
+    return UIR_OK;
+}
+
+static uir_status launch_iteration(uir_batch *b)
+{
+    hipStream_t st = b->res->stream;
+    TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, 1, b->dtype, b->fp_grid,
+                                    b->d_err, st), "FP launch"));
+    return uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, 1, b->dtype, b->bp_grid,
+                                       b->d_err, st), "BP launch");
+}
+
+uir_status uir_batch_iterate(uir_batch *b, uint32_t iterations, int use_graph)
+{
+    if (!b)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (iterations == 0)
+        return UIR_OK;
+    TRY(uir_check_hip(hipSetDevice(b->res->device), "hipSetDevice"));
+    if (!use_graph) {
+        for (uint32_t i = 0; i < iterations; ++i)
+            TRY(launch_iteration(b));
+        return UIR_OK;
+    }
+    if (!b->exec || b->graph_iters != iterations) {
+        if (b->exec) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = NULL;
+        }
+        if (b->graph) {
+            (void)hipGraphDestroy(b->graph);
+            b->graph = NULL;
+        }
+        hipStream_t st = b->res->stream;
+        TRY(uir_check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"));
+        uir_status s = UIR_OK;
+        for (uint32_t i = 0; i < iterations && s == UIR_OK; ++i)
+            s = launch_iteration(b);
+        hipGraph_t g = NULL;
+        uir_status s2 = uir_check_hip(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+        if (s != UIR_OK || s2 != UIR_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return s != UIR_OK ? s : s2;
+        }
+        b->graph = g;
+        TRY(uir_check_hip(hipGraphInstantiate(&b->exec, g, NULL, NULL, 0), "hipGraphInstantiate"));
+        b->graph_iters = iterations;
+    }
+    return uir_check_hip(hipGraphLaunch(b->exec, b->res->stream), "hipGraphLaunch");
+}
+
+uir_status uir_batch_get_image(uir_batch *b, uint32_t slice, float *dst, int on_device)
+{
+    if (!b || !dst || slice >= b->n_slices)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad slice or NULL destination");
+    const size_t nv = (size_t)b->specs[slice].n * b->specs[slice].n;
+    hipStream_t st = b->res->stream;
+    int err = 0;
+    TRY(uir_check_hip(hipMemcpyAsync(&err, b->d_err, sizeof err, hipMemcpyDeviceToHost, st), "flag read"));
+    TRY(uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize"));
+    if (err)
+        return uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window (flag %d)", err);
+    if (on_device) {
+        TRY(uir_check_hip(uirk_convert(b->x[slice], b->dtype, dst, UIR_DTYPE_F32, nv, st), "convert"));
+        return uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+    float *tmp = NULL;
+    TRY(uir_dmalloc((void **)&tmp, nv * 4));
+    uir_status s = uir_check_hip(uirk_convert(b->x[slice], b->dtype, tmp, UIR_DTYPE_F32, nv, st), "convert");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipMemcpyAsync(dst, tmp, nv * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uir_dfree(tmp);
+    return s;
+}
+
+uir_status uir_batch_get_info(const uir_batch *b, uir_batch_info *info)
+{
+    if (!b || !info)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    memset(info, 0, sizeof *info);
+    info->device_bytes = b->mem_bytes + (uint64_t)b->n_fp_items * sizeof(uirk_fp_item) +
+                         (uint64_t)b->n_bp_items * sizeof(uirk_bp_item) + 2ull * b->n_slices * sizeof(uirk_slice);
+    info->n_slices = b->n_slices;
+    info->n_geometries = b->n_geoms;
+    info->fp_items = (uint32_t)b->n_fp_items;
+    info->bp_items = (uint32_t)b->n_bp_items;
+    info->fp_grid = (uint32_t)b->fp_grid;
+    info->bp_grid = (uint32_t)b->bp_grid;
+    info->fp_samples = b->fp_samples;
+    info->bp_samples = b->bp_samples;
+    return UIR_OK;
+}
+
+uir_status uir_batch_time_kernels(uir_batch *b, uint32_t reps, float *fp_ms, float *bp_ms)
+{
+    if (!b || reps == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad arguments");
+    hipStream_t st = b->res->stream;
+    hipEvent_t e0, e1, e2;
+    TRY(uir_check_hip(hipEventCreate(&e0), "hipEventCreate"));
+    TRY(uir_check_hip(hipEventCreate(&e1), "hipEventCreate"));
+    TRY(uir_check_hip(hipEventCreate(&e2), "hipEventCreate"));
+    uir_status s = uir_check_hip(hipEventRecord(e0, st), "hipEventRecord");
+    for (uint32_t i = 0; i < reps && s == UIR_OK; ++i)
+        s = uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, 1, b->dtype, b->fp_grid,
+                                        b->d_err, st), "FP launch");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipEventRecord(e1, st), "hipEventRecord");
+    for (uint32_t i = 0; i < reps && s == UIR_OK; ++i)
+        s = uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, 1, b->dtype, b->bp_grid,
+                                        b->d_err, st), "BP launch");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipEventRecord(e2, st), "hipEventRecord");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipEventSynchronize(e2), "hipEventSynchronize");
+    float a = 0, c = 0;
+    if (s == UIR_OK)
+        s = uir_check_hip(hipEventElapsedTime(&a, e0, e1), "hipEventElapsedTime");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipEventElapsedTime(&c, e1, e2), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (s == UIR_OK) {
+        if (fp_ms) *fp_ms = a / reps;
+        if (bp_ms) *bp_ms = c / reps;
+    }
+    return s;
+}

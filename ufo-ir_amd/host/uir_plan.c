@@ -1,0 +1,157 @@
+/*
+ * uir_plan.c -- work-table construction for the table-driven projectors.
+ *
+ * FP items: for every subset (a direction run, or one angle), angle blocks of
+ * up to UIRK_FP_ANGLES angles crossed with detector blocks of UIRK_FP_DETS.
+ * An angle block is accepted only if the footprint of its rays over any chunk
+ * of UIRK_FP_LINES lines fits the LDS staging window.  Ray positions are
+ * affine in the detector index and the line index, so the footprint width of
+ * a (detector block, chunk) window is a convex function of the window's
+ * position and attains its maximum at one of the four extreme windows; those
+ * four are evaluated exactly here (with a margin for float stepping).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "uir_internal.h"
+
+static int fp_push(uirk_fp_vec *v, uirk_fp_item it)
+{
+    if (v->n == v->cap) {
+        size_t cap = v->cap ? 2 * v->cap : 256;
+        uirk_fp_item *p = (uirk_fp_item *)realloc(v->v, cap * sizeof *p);
+        if (!p)
+            return -1;
+        v->v = p;
+        v->cap = cap;
+    }
+    v->v[v->n++] = it;
+    return 0;
+}
+
+static int bp_push(uirk_bp_vec *v, uirk_bp_item it)
+{
+    if (v->n == v->cap) {
+        size_t cap = v->cap ? 2 * v->cap : 256;
+        uirk_bp_item *p = (uirk_bp_item *)realloc(v->v, cap * sizeof *p);
+        if (!p)
+            return -1;
+        v->v = p;
+        v->cap = cap;
+    }
+    v->v[v->n++] = it;
+    return 0;
+}
+
+/* Footprint width (in staged positions, as the kernel computes it) of the rays
+ * of angles [a0, a0 + k) and detectors [d0, d0 + 64) over lines [l0, l0 + C). */
+static int window_width(int n, int n_dets, double axis, const float *s, const float *c, int dir, int a0, int k,
+                        int d0, int l0)
+{
+    const int C = UIRK_FP_LINES;
+    const int d1 = d0 + UIRK_FP_DETS - 1 < n_dets - 1 ? d0 + UIRK_FP_DETS - 1 : n_dets - 1;
+    const int l1 = l0 + C - 1;
+    double lo = 1e300, hi = -1e300;
+    for (int a = a0; a < a0 + k; ++a) {
+        double det_step, step;
+        if (dir == 0) {
+            det_step = -1.0 / s[a];
+            step = (double)c[a] / (double)s[a];
+        } else {
+            det_step = 1.0 / c[a];
+            step = (double)s[a] / (double)c[a];
+        }
+        const int ds[2] = {d0, d1};
+        const int ls[2] = {l0, l1};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                const double t = (double)ds[i] + 0.5 - axis;
+                const double p = t * det_step - 0.5 * n * step + 0.5 * n + (double)ls[j] * step;
+                if (p < lo) lo = p;
+                if (p > hi) hi = p;
+            }
+    }
+    /* kernel: pmin = floor(lo - 0.5) - 1, W = floor(hi - 0.5) + 3 - pmin;
+     * +2 margin for the float rounding of stepped coordinates */
+    return (int)(floor(hi - 0.5) - floor(lo - 0.5)) + 4 + 2;
+}
+
+static int block_fits(int n, int n_dets, float axis, const float *s, const float *c, int dir, int a0, int k)
+{
+    const int dlast = ((n_dets - 1) / UIRK_FP_DETS) * UIRK_FP_DETS;
+    const int llast = ((n - 1) / UIRK_FP_LINES) * UIRK_FP_LINES;
+    const int dd[2] = {0, dlast};
+    const int ll[2] = {0, llast};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            if (window_width(n, n_dets, axis, s, c, dir, a0, k, dd[i], ll[j]) > UIRK_FP_WMAX)
+                return 0;
+    return 1;
+}
+
+int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs)
+{
+    for (uint32_t si = 0; si < n_subs; ++si) {
+        const int off = (int)subs[si].offset, cnt = (int)subs[si].n, dir = subs[si].direction;
+        int a = off;
+        while (a < off + cnt) {
+            int k = off + cnt - a < UIRK_FP_ANGLES ? off + cnt - a : UIRK_FP_ANGLES;
+            while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k))
+                k = k > 2 ? k - (k + 3) / 4 : 1;
+            for (int d0 = 0; d0 < n_dets; d0 += UIRK_FP_DETS) {
+                uirk_fp_item it;
+                memset(&it, 0, sizeof it);
+                it.slice = slice;
+                it.dir = dir;
+                it.a0 = a;
+                it.na = k;
+                it.d0 = d0;
+                if (fp_push(out, it))
+                    return -1;
+            }
+            a += k;
+        }
+    }
+    return 0;
+}
+
+int uirk_plan_bp(uirk_bp_vec *out, int slice, int n)
+{
+    for (int y0 = 0; y0 < n; y0 += UIRK_BP_TILE)
+        for (int x0 = 0; x0 < n; x0 += UIRK_BP_TILE) {
+            uirk_bp_item it;
+            memset(&it, 0, sizeof it);
+            it.slice = slice;
+            it.x0 = x0;
+            it.y0 = y0;
+            if (bp_push(out, it))
+                return -1;
+        }
+    return 0;
+}
+
+void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        trig[2 * i] = sin_lut[i];
+        trig[2 * i + 1] = cos_lut[i];
+    }
+}
+
+/* Persistent-grid size: enough workgroups to fill every CU per_cu times. */
+int uirk_plan_grid(int n_items, int per_cu)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0)
+            cus = 256;
+    }
+    long g = (long)cus * per_cu;
+    return (int)(g < n_items ? g : n_items);
+}

@@ -41,6 +41,14 @@ struct uir_task {
     uint32_t lut_n;
     uir_subset *runs;
     uint32_t n_runs;
+    /* work tables of the table-driven kernels for the current geometry */
+    uint32_t plan_n, plan_dets, plan_angles;
+    float plan_axis;
+    uirk_fp_item *d_fp_items;
+    uirk_bp_item *d_bp_items;
+    int n_fp_items, n_bp_items;
+    float *d_trig;
+    int *d_err;
 
     /* methods (ufo-ir-method-task.c:74-85 and the method tasks) */
     uir_task *projector;
@@ -225,6 +233,10 @@ void uir_task_unref(uir_task *t)
     uir_dfree(t->d_sin);
     uir_dfree(t->d_cos);
     free(t->runs);
+    uir_dfree(t->d_fp_items);
+    uir_dfree(t->d_bp_items);
+    uir_dfree(t->d_trig);
+    uir_dfree(t->d_err);
     free(t->diag);
     uir_task_unref(t->projector);
     uir_task_unref(t->df_minimizer);
@@ -521,25 +533,131 @@ static void proj_runs_dev(const uir_task *p, const uir_subset *subs, uint32_t n,
     (void)p;
 }
 
-/* forward over all runs (ufo-ir-parallel-projector-task.c:390-408): one launch
- * with each angle's own direction; ray_w != NULL fuses the SIRT ray weighting. */
+/* Work tables of the projector's current geometry for an n x n image. */
+static uir_status proj_plan(uir_task *p, uint32_t n)
+{
+    const float axis = proj_axis(p);
+    if (p->d_fp_items && p->plan_n == n && p->plan_dets == p->n_dets && p->plan_angles == p->lut_n &&
+        p->plan_axis == axis)
+        return UIR_OK;
+    uir_dfree(p->d_fp_items);
+    uir_dfree(p->d_bp_items);
+    uir_dfree(p->d_trig);
+    p->d_fp_items = NULL;
+    p->d_bp_items = NULL;
+    p->d_trig = NULL;
+    uirk_fp_vec fv = {0};
+    uirk_bp_vec bv = {0};
+    float *trig = (float *)malloc(sizeof(float) * 2 * (p->lut_n ? p->lut_n : 1));
+    uir_status s = UIR_OK;
+    if (!trig || uirk_plan_fp(&fv, 0, (int)n, (int)p->n_dets, axis, p->h_sin, p->h_cos, p->runs, p->n_runs) ||
+        uirk_plan_bp(&bv, 0, (int)n)) {
+        s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+        goto done;
+    }
+    uirk_plan_trig(p->h_sin, p->h_cos, p->lut_n, trig);
+    if ((s = uir_dmalloc((void **)&p->d_fp_items, fv.n * sizeof(uirk_fp_item))) != UIR_OK ||
+        (s = uir_dmalloc((void **)&p->d_bp_items, bv.n * sizeof(uirk_bp_item))) != UIR_OK ||
+        (s = uir_dmalloc((void **)&p->d_trig, 2 * sizeof(float) * (p->lut_n ? p->lut_n : 1))) != UIR_OK)
+        goto done;
+    if (!p->d_err && (s = uir_dmalloc((void **)&p->d_err, sizeof(int))) != UIR_OK)
+        goto done;
+    if ((s = uir_check_hip(hipMemcpy(p->d_fp_items, fv.v, fv.n * sizeof(uirk_fp_item), hipMemcpyHostToDevice),
+                           "table upload")) != UIR_OK ||
+        (s = uir_check_hip(hipMemcpy(p->d_bp_items, bv.v, bv.n * sizeof(uirk_bp_item), hipMemcpyHostToDevice),
+                           "table upload")) != UIR_OK ||
+        (s = uir_check_hip(hipMemcpy(p->d_trig, trig, 2 * sizeof(float) * p->lut_n, hipMemcpyHostToDevice),
+                           "table upload")) != UIR_OK ||
+        (s = uir_check_hip(hipMemset(p->d_err, 0, sizeof(int)), "memset")) != UIR_OK)
+        goto done;
+    p->n_fp_items = (int)fv.n;
+    p->n_bp_items = (int)bv.n;
+    p->plan_n = n;
+    p->plan_dets = p->n_dets;
+    p->plan_angles = p->lut_n;
+    p->plan_axis = axis;
+done:
+    free(fv.v);
+    free(bv.v);
+    free(trig);
+    if (s != UIR_OK) {
+        uir_dfree(p->d_fp_items);
+        p->d_fp_items = NULL;
+    }
+    return s;
+}
+
+static void proj_slice(const uir_task *p, uint32_t n, uirk_slice *sl)
+{
+    memset(sl, 0, sizeof *sl);
+    sl->trig = p->d_trig;
+    sl->n = (int32_t)n;
+    sl->n_dets = (int32_t)p->n_dets;
+    sl->n_angles = (int32_t)p->lut_n;
+    sl->axis = proj_axis(p);
+    sl->corr = p->correction_scale;
+    sl->relax = p->relaxation;
+}
+
+/* forward over all runs (ufo-ir-parallel-projector-task.c:390-408);
+ * ray_w != NULL fuses the SIRT ray weighting. */
 static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float *r_sino, float *w_sino,
                           const float *ray_w)
 {
-    return uir_check_hip(uirk_fp_ref(vol, r_sino, w_sino, ray_w, p->d_sin, p->d_cos, (int)n, (int)p->n_dets,
-                                     proj_axis(p), p->correction_scale, 0, (int)p->lut_n, -1, p->res->stream),
-                         "FP launch");
+    TRY(proj_plan(p, n));
+    uirk_slice sl;
+    proj_slice(p, n, &sl);
+    sl.x = vol;
+    sl.b = r_sino;
+    sl.r = w_sino;
+    sl.ray_w = ray_w;
+    return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_fp_items, p->n_fp_items, 0, UIR_DTYPE_F32,
+                                       uirk_plan_grid(p->n_fp_items, 8), p->d_err, p->res->stream), "FP launch");
 }
 
-/* backward over all runs (:410-429), accumulate form: vol += relax * BP per run. */
+/* backward over all runs (:410-429).  pix_w == NULL: accumulate form,
+ * vol += relax * BP per run; else the fused SIRT update with POSC option. */
+static uir_status proj_bp_ex(uir_task *p, uint32_t n, const float *sino, float *vol, const float *pix_w, int posc)
+{
+    TRY(proj_plan(p, n));
+    if (pix_w && p->n_runs > UIRK_MAX_RUNS)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "more than %d direction runs", UIRK_MAX_RUNS);
+    for (uint32_t first = 0; first < p->n_runs; first += UIRK_MAX_RUNS) {
+        uirk_slice sl;
+        proj_slice(p, n, &sl);
+        sl.b = sino;
+        sl.x = vol;
+        sl.x_out = vol;
+        sl.pix_w = pix_w;
+        sl.posc = posc;
+        uint32_t k = 0;
+        for (uint32_t i = first; i < p->n_runs && k < UIRK_MAX_RUNS; ++i, ++k) {
+            sl.run_off[k] = p->runs[i].offset;
+            sl.run_n[k] = p->runs[i].n;
+        }
+        sl.n_runs = (int32_t)k;
+        TRY(uir_check_hip(uirk_bp_table(NULL, &sl, p->d_bp_items, p->n_bp_items, 0, UIR_DTYPE_F32,
+                                        uirk_plan_grid(p->n_bp_items, 8), p->d_err, p->res->stream), "BP launch"));
+    }
+    return UIR_OK;
+}
+
 static uir_status proj_bp(uir_task *p, uint32_t n, const float *sino, float *vol)
 {
-    for (uint32_t first = 0; first < p->n_runs; first += UIRK_MAX_RUNS) {
-        uirk_runs r;
-        proj_runs_dev(p, p->runs, p->n_runs, &r, first);
-        TRY(uir_check_hip(uirk_bp_ref(vol, vol, sino, (int)p->lut_n, p->d_sin, p->d_cos, (int)n, (int)p->n_dets,
-                                      proj_axis(p), p->relaxation, &r, NULL, 0, p->res->stream), "BP launch"));
-    }
+    return proj_bp_ex(p, n, sino, vol, NULL, 0);
+}
+
+/* The staging-window overflow flag of the table kernels (never set when the
+ * work tables come from uirk_plan_fp); synchronises the stream. */
+static uir_status uir_check_err_flag(uir_task *p)
+{
+    if (!p->d_err)
+        return UIR_OK;
+    int e = 0;
+    TRY(uir_check_hip(hipMemcpyAsync(&e, p->d_err, sizeof e, hipMemcpyDeviceToHost, p->res->stream), "flag read"));
+    TRY(uir_check_hip(hipStreamSynchronize(p->res->stream), "hipStreamSynchronize"));
+    if (e)
+        return uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window (flag %d)", e);
     return UIR_OK;
 }
 
@@ -705,14 +823,11 @@ static uir_status sirt_run(uir_task *t, const float *b, float *x, float relax, u
     c->p->relaxation = relax;            /* :195-197 */
     c->p->correction_scale = -1.0f;
     HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
-    uirk_runs r;
-    proj_runs_dev(c->p, c->p->runs, c->p->n_runs, &r, 0);
     for (uint32_t it = 0; it < iters; ++it) { /* :202-215 */
         TRY(proj_fp(c->p, c->n, x, b, sino_tmp, ray_w));
-        HIPTRY(uirk_bp_ref(x, x, sino_tmp, (int)c->A, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D,
-                           proj_axis(c->p), c->p->relaxation, &r, pix_w, posc, c->st), "BP launch");
+        TRY(proj_bp_ex(c->p, c->n, sino_tmp, x, pix_w, posc));
     }
-    return UIR_OK;
+    return uir_check_err_flag(c->p);
 }
 
 /* SART process, ufo-ir-sart-task.c:173-228: per angle, FP of the row with the

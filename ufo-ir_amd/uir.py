@@ -46,6 +46,17 @@ class GeometryDims(C.Structure):
     _fields_ = [("height", C.c_uint64), ("width", C.c_uint64), ("n_dets", C.c_uint64), ("n_angles", C.c_uint64)]
 
 
+class SliceSpec(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("n_dets", C.c_uint32), ("n_angles", C.c_uint32), ("step", C.c_float),
+                ("axis_position", C.c_float), ("relaxation", C.c_float), ("positive_constraint", C.c_int32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("device_bytes", C.c_uint64), ("n_slices", C.c_uint32), ("n_geometries", C.c_uint32),
+                ("fp_items", C.c_uint32), ("bp_items", C.c_uint32), ("fp_grid", C.c_uint32),
+                ("bp_grid", C.c_uint32), ("fp_samples", C.c_double), ("bp_samples", C.c_double)]
+
+
 _lib = None
 
 
@@ -115,6 +126,15 @@ def load():
     L.uir_asdpocs_get_diagnostics.argtypes = [vp, C.POINTER(C.POINTER(f32)), C.POINTER(u32)]
     L.uir_fp_joseph.argtypes = [vp, vp, vp, vp, vp, GeometryDims, f32, Subset, f32, vp]
     L.uir_bp_joseph.argtypes = [vp, vp, vp, f32, vp, vp, GeometryDims, f32, Subset, vp]
+    L.uir_batch_new.argtypes = [vp, C.c_int, C.POINTER(SliceSpec), u32, C.POINTER(vp)]
+    L.uir_batch_free.argtypes = [vp]
+    L.uir_batch_free.restype = None
+    L.uir_batch_set_sinogram.argtypes = [vp, u32, vp, C.c_int]
+    L.uir_batch_reset.argtypes = [vp]
+    L.uir_batch_iterate.argtypes = [vp, u32, C.c_int]
+    L.uir_batch_get_image.argtypes = [vp, u32, vp, C.c_int]
+    L.uir_batch_get_info.argtypes = [vp, C.POINTER(BatchInfo)]
+    L.uir_batch_time_kernels.argtypes = [vp, u32, C.POINTER(f32), C.POINTER(f32)]
     L.uir_op_set.argtypes = [vp, sz, f32, vp]
     L.uir_op_inv.argtypes = [vp, vp, sz, vp]
     for n in ("uir_op_mul", "uir_op_add", "uir_op_deduction", "uir_op_max_ew", "uir_op_div_ew"):
@@ -402,3 +422,67 @@ def header_functions(path=HEADER_PATH):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b(uir_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
+
+
+DTYPES = {"f32": DTYPE_F32, "f16": DTYPE_F16, "bf16": DTYPE_BF16}
+
+
+class Batch:
+    """Batched SIRT over slices of uneven geometry (uir_batch_*).
+
+    specs: list of dicts with n, n_dets, n_angles, step, axis_position (-1),
+    relaxation, positive_constraint.  storage: "f32", "f16" or "bf16"."""
+
+    def __init__(self, res, specs, storage="f32"):
+        arr = (SliceSpec * len(specs))()
+        for i, s in enumerate(specs):
+            arr[i] = SliceSpec(int(s["n"]), int(s["n_dets"]), int(s["n_angles"]), float(s["step"]),
+                               float(s.get("axis_position", -1.0)), float(s.get("relaxation", 0.25)),
+                               int(bool(s.get("positive_constraint", False))))
+        h = C.c_void_p()
+        check(load().uir_batch_new(res.handle, DTYPES[storage], arr, len(specs), C.byref(h)))
+        self.handle = h
+        self.specs = [dict(s) for s in specs]
+        self.res = res
+
+    def set_sinogram(self, i, sino):
+        if isinstance(sino, DeviceArray):
+            check(load().uir_batch_set_sinogram(self.handle, i, sino.ptr, 1))
+        else:
+            a = np.ascontiguousarray(sino, np.float32)
+            check(load().uir_batch_set_sinogram(self.handle, i, a.ctypes.data, 0))
+
+    def reset(self):
+        check(load().uir_batch_reset(self.handle))
+
+    def iterate(self, iterations, use_graph=True):
+        check(load().uir_batch_iterate(self.handle, int(iterations), int(bool(use_graph))))
+
+    def image(self, i):
+        n = int(self.specs[i]["n"])
+        out = np.empty((n, n), np.float32)
+        check(load().uir_batch_get_image(self.handle, i, out.ctypes.data, 0))
+        return out
+
+    def info(self):
+        inf = BatchInfo()
+        check(load().uir_batch_get_info(self.handle, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in BatchInfo._fields_}
+
+    def time_kernels(self, reps=10):
+        """Average FP and BP launch durations (ms).  Runs `reps` FP and BP
+        launches, so it changes the images: reset() afterwards."""
+        a, b = C.c_float(), C.c_float()
+        check(load().uir_batch_time_kernels(self.handle, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.handle:
+            load().uir_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
