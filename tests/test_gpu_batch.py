@@ -1,0 +1,103 @@
+"""Batched SIRT engine (uir_batch_*): accuracy against the FP64 evaluation of
+the same operators, independence of each slice's result from the batch mix,
+and graph replay equal to direct launches."""
+import numpy as np
+import pytest
+
+import oracle as O
+import uir
+from phantoms import ellipse_phantom
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [  # (n, n_dets, n_angles)
+    (128, 182, 90),
+    (96, 136, 120),
+    (160, 227, 75),
+]
+
+
+def _spec(n, d, a, relax=0.5, posc=False):
+    return dict(n=n, n_dets=d, n_angles=a, step=float(np.float32(np.pi / a)), axis_position=-1.0,
+                relaxation=relax, positive_constraint=posc)
+
+
+def _data(n, d, a, seed):
+    g = O.Geometry(n, d, a, np.float32(np.pi / a))
+    x = ellipse_phantom(n, seed=seed).astype(np.float64)
+    b = O.fp64(g, x)
+    return g, b
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16", 1e-3)])
+def test_batch_matches_fp64(res, storage, tol):
+    iters = 20
+    specs = [_spec(*gm) for gm in GEOMS]
+    bt = uir.Batch(res, specs, storage)
+    refs = []
+    for i, (n, d, a) in enumerate(GEOMS):
+        g, b = _data(n, d, a, 100 + i)
+        bt.set_sinogram(i, b.astype(np.float32))
+        refs.append(O.sirt64(g, b, iters, 0.5))
+    bt.reset()
+    bt.iterate(iters)
+    for i in range(len(GEOMS)):
+        err = O.rel_l2(bt.image(i), refs[i])
+        print(storage, GEOMS[i], "rel-L2 vs FP64", err)
+        assert err <= tol
+
+
+def test_batch_result_independent_of_mix(res):
+    iters = 5
+    (n, d, a) = GEOMS[0]
+    g, b = _data(n, d, a, 7)
+    alone = uir.Batch(res, [_spec(n, d, a)], "f16")
+    alone.set_sinogram(0, b.astype(np.float32))
+    alone.reset()
+    alone.iterate(iters)
+    mix_specs = [_spec(*GEOMS[1]), _spec(n, d, a), _spec(*GEOMS[2]), _spec(n, d, a, relax=0.25)]
+    mix = uir.Batch(res, mix_specs, "f16")
+    for i, s in enumerate(mix_specs):
+        _, bb = _data(s["n"], s["n_dets"], s["n_angles"], 50 + i)
+        mix.set_sinogram(i, bb.astype(np.float32))
+    mix.set_sinogram(1, b.astype(np.float32))
+    mix.reset()
+    mix.iterate(iters)
+    assert np.array_equal(alone.image(0), mix.image(1))
+
+
+def test_graph_replay_equals_direct_launches(res):
+    specs = [_spec(*gm) for gm in GEOMS]
+    out = []
+    for use_graph in (False, True):
+        bt = uir.Batch(res, specs, "f32")
+        for i, (n, d, a) in enumerate(GEOMS):
+            _, b = _data(n, d, a, 200 + i)
+            bt.set_sinogram(i, b.astype(np.float32))
+        bt.reset()
+        for _ in range(3):          # several replays of the same graph
+            bt.iterate(4, use_graph=use_graph)
+        out.append([bt.image(i) for i in range(len(GEOMS))])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_positive_constraint_and_info(res):
+    n, d, a = GEOMS[0]
+    bt = uir.Batch(res, [_spec(n, d, a, relax=1.0, posc=True)], "f32")
+    g, b = _data(n, d, a, 3)
+    rng = np.random.default_rng(0)
+    bt.set_sinogram(0, (b + rng.normal(0, 0.05 * b.max(), b.shape)).astype(np.float32))
+    bt.reset()
+    bt.iterate(10)
+    assert bt.image(0).min() >= 0.0
+    info = bt.info()
+    assert info["n_slices"] == 1 and info["fp_items"] > 0 and info["bp_items"] == 4
+    assert info["bp_samples"] == a * n * n
+
+
+def test_batch_rejects_bad_specs(res):
+    with pytest.raises(uir.UirError):
+        uir.Batch(res, [_spec(0, 10, 10)], "f32")
+    with pytest.raises(uir.UirError):
+        uir.Batch(res, [_spec(64, 91, 30, relax=1.5)], "f32")
