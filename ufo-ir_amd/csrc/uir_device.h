@@ -85,9 +85,11 @@ typedef struct {
 } uirk_slice;
 
 /* FP work item: angles [a0, a0 + na) of one direction family (0 = horizontal,
- * 1 = vertical) and detectors [d0, d0 + 64) of one slice. */
+ * 1 = vertical) and detectors [d0, d0 + 64) of one slice; lines [l_lo, l_hi)
+ * hold every sample of these rays that can touch the image (the other
+ * samples read the zero border). */
 typedef struct {
-    int32_t slice, dir, a0, na, d0, pad0, pad1, pad2;
+    int32_t slice, dir, a0, na, d0, l_lo, l_hi, pad;
 } uirk_fp_item;
 
 /* BP work item: the 64 x 64 pixel tile at (x0, y0) of one slice. */

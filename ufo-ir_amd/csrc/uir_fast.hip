@@ -44,15 +44,54 @@ constexpr int kBpRows = kBpT / 4;          // rows per thread
 constexpr int kBpNa = 32;                  // angles per staged batch
 constexpr int kBpWb = 100;                 // staged bins per angle (>= 64 * sqrt(2) + margin)
 
-template <typename T> __device__ __forceinline__ float to_f(T v);
-template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
-template <> __device__ __forceinline__ float to_f<__half>(__half v) { return __half2float(v); }
-template <> __device__ __forceinline__ float to_f<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+// Loads and stores through global-address-space pointers: the buffers come
+// from descriptors in memory, so without the cast the compiler can only emit
+// flat accesses (which also hold the LDS counter).
+#define UIRK_GLOBAL __attribute__((address_space(1)))
 
-template <typename T> __device__ __forceinline__ T from_f(float v);
-template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
-template <> __device__ __forceinline__ __half from_f<__half>(float v) { return __float2half(v); }
-template <> __device__ __forceinline__ __hip_bfloat16 from_f<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+// Offsets are 32-bit (every buffer holds < 2^32 elements) so that, with the
+// base pointer uniform (see uniform()), loads use a scalar base plus a 32-bit
+// vector offset.
+// Every buffer is < 4 GiB (checked on the host), so byte offsets are 32-bit.
+template <typename E> __device__ __forceinline__ const UIRK_GLOBAL E *gat(const void *p, uint32_t i)
+{
+    return (const UIRK_GLOBAL E *)((const UIRK_GLOBAL char *)p + i * (uint32_t)sizeof(E));
+}
+template <typename E> __device__ __forceinline__ UIRK_GLOBAL E *gat_w(void *p, uint32_t i)
+{
+    return (UIRK_GLOBAL E *)((UIRK_GLOBAL char *)p + i * (uint32_t)sizeof(E));
+}
+__device__ __forceinline__ float gload(const float *p, uint32_t i) { return *gat<float>(p, i); }
+__device__ __forceinline__ float gload(const __half *p, uint32_t i)
+{
+    return __half2float(__ushort_as_half(*gat<unsigned short>(p, i)));
+}
+__device__ __forceinline__ float gload(const __hip_bfloat16 *p, uint32_t i)
+{
+    return __uint_as_float((unsigned)*gat<unsigned short>(p, i) << 16);
+}
+__device__ __forceinline__ void gstore(float *p, uint32_t i, float v) { *gat_w<float>(p, i) = v; }
+__device__ __forceinline__ void gstore(__half *p, uint32_t i, float v)
+{
+    *gat_w<unsigned short>(p, i) = __half_as_ushort(__float2half(v));
+}
+__device__ __forceinline__ void gstore(__hip_bfloat16 *p, uint32_t i, float v)
+{
+    const __hip_bfloat16 h = __float2bfloat16(v);
+    unsigned short bits;
+    __builtin_memcpy(&bits, &h, sizeof bits);
+    *gat_w<unsigned short>(p, i) = bits;
+}
+
+// A pointer that is the same for every lane (read from a descriptor), moved
+// to scalar registers.
+template <typename P> __device__ __forceinline__ P *uniform(P *p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (P *)(((unsigned long long)hi << 32) | lo);
+}
 
 // floor(x) converted to int in one instruction (no memory access).
 __device__ __forceinline__ int cvt_flr(float x)
@@ -96,8 +135,8 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         const uirk_fp_item item = items[it];
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
-        const T *img = (const T *)S->x;
-        const float *trig = S->trig;
+        const T *img = uniform((const T *)S->x);
+        const float *trig = uniform(S->trig);
         const int s = item.d0 + lane;
         const bool hor = item.dir == 0;
 
@@ -118,7 +157,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             const int k = wave * kFpApt + j;
             on[j] = det_on && k < item.na;
             const int ang = item.a0 + (k < item.na ? k : 0);
-            const float st = trig[2 * ang], ct = trig[2 * ang + 1];
+            const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
             float det_step, step;
             if (hor) {
                 det_step = -1.0f / st;
@@ -136,8 +175,17 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             inner[j] = 0.0f;
         }
 
-        for (int l0 = 0; l0 < N; l0 += kFpLines) {
-            const int nl = min(kFpLines, N - l0);
+        // Lines before l_lo read only the zero border: the reference-compatible
+        // path replays their float steps exactly, the fast path re-anchors.
+        if (REF) {
+            for (int l = 0; l < item.l_lo; ++l) {
+#pragma unroll
+                for (int j = 0; j < kFpApt; ++j)
+                    pos[j] += stp[j];
+            }
+        }
+        for (int l0 = item.l_lo; l0 < item.l_hi; l0 += kFpLines) {
+            const int nl = min(kFpLines, item.l_hi - l0);
             // Footprint of this block's rays over the chunk.
             float lo = 3.0e38f, hi = -3.0e38f;
 #pragma unroll
@@ -172,24 +220,58 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                 atomicOr(err, 1);
             const bool outside = overflow || pmin + W <= 0 || pmin >= N;
             if (!outside) {
+                // Every thread issues all of its loads (from clamped, always
+                // valid indices, no branches) before writing LDS; texels
+                // outside the image are then replaced by the zero border.
+                // Two halves of 16 batched loads per thread keep the register
+                // footprint within four waves per SIMD.
+                constexpr int kPer = kFpLines * kFpW / 256;
+                constexpr int kHalf = kPer / 2;
                 if (hor) {
                     // line = column l0 + l, position = row pmin + p; coalesced along columns
                     const int l = threadIdx.x & 31;
-                    const int col = l0 + l;
-                    for (int p = threadIdx.x >> 5; p < W; p += 8) {
-                        const int row = pmin + p;
-                        float v = 0.0f;
-                        if (row >= 0 && row < N && l < nl)
-                            v = to_f<T>(img[(size_t)row * N + col]);
-                        lds[l * kFpLs + p] = v;
+                    const int col = min(l0 + l, N - 1);
+                    for (int h0 = 0; h0 < kPer; h0 += kHalf) {
+                        if (8 * h0 >= W)
+                            break;
+                        float v[kHalf];
+#pragma unroll
+                        for (int e = 0; e < kHalf; ++e) {
+                            const int p = (threadIdx.x >> 5) + 8 * (h0 + e);
+                            const int row = pmin + p;
+                            const int rc = min(max(row, 0), N - 1);
+                            const float t = gload(img, (uint32_t)rc * (uint32_t)N + (uint32_t)col);
+                            v[e] = (row == rc && l < nl) ? t : 0.0f;
+                        }
+#pragma unroll
+                        for (int e = 0; e < kHalf; ++e) {
+                            const int p = (threadIdx.x >> 5) + 8 * (h0 + e);
+                            if (p < W)
+                                lds[l * kFpLs + p] = v[e];
+                        }
                     }
                 } else {
                     // line = row l0 + l, position = column pmin + p
-                    for (int l = 0; l < nl; ++l) {
-                        const T *rowp = img + (size_t)(l0 + l) * N;
-                        for (int p = threadIdx.x; p < W; p += 256) {
+                    for (int h0 = 0; h0 < kPer; h0 += kHalf) {
+                        if (4 * (h0 >> 2) >= nl)
+                            break;
+                        float v[kHalf];
+#pragma unroll
+                        for (int e = 0; e < kHalf; ++e) {
+                            const int l = (threadIdx.x >> 6) + 4 * ((h0 + e) >> 2);
+                            const int p = (threadIdx.x & 63) + 64 * ((h0 + e) & 3);
                             const int col = pmin + p;
-                            lds[l * kFpLs + p] = (col >= 0 && col < N) ? to_f<T>(rowp[col]) : 0.0f;
+                            const int cc = min(max(col, 0), N - 1);
+                            const int row = min(l0 + l, N - 1);
+                            const float t = gload(img, (uint32_t)row * (uint32_t)N + (uint32_t)cc);
+                            v[e] = (col == cc && l < nl) ? t : 0.0f;
+                        }
+#pragma unroll
+                        for (int e = 0; e < kHalf; ++e) {
+                            const int l = (threadIdx.x >> 6) + 4 * ((h0 + e) >> 2);
+                            const int p = (threadIdx.x & 63) + 64 * ((h0 + e) & 3);
+                            if (p < W)
+                                lds[l * kFpLs + p] = v[e];
                         }
                     }
                 }
@@ -280,9 +362,9 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             }
         }
 
-        const T *bsrc = (const T *)S->b;
-        T *rdst = (T *)S->r;
-        const float *ray_w = S->ray_w;
+        const T *bsrc = uniform((const T *)S->b);
+        T *rdst = uniform((T *)S->r);
+        const float *ray_w = uniform(S->ray_w);
         const float corr = S->corr;
 #pragma unroll
         for (int j = 0; j < kFpApt; ++j) {
@@ -290,8 +372,8 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             if (!in_range || k >= item.na)
                 continue;
             const int ang = item.a0 + k;
-            const size_t o = (size_t)ang * D + s;
-            const float bval = to_f<T>(bsrc[o]);
+            const uint32_t o = (uint32_t)ang * (uint32_t)D + (uint32_t)s;
+            const float bval = gload(bsrc, o);
             float w;
             if (on[j]) {
                 const float sum = REF ? acc[j] + inner[j] : acc[j];
@@ -300,8 +382,8 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                 w = bval; // inactive detector: the reference leaves the value as read
             }
             if (ray_w)
-                w = w * ray_w[o];
-            rdst[o] = from_f<T>(w);
+                w = w * gload(ray_w, o);
+            gstore(rdst, o, w);
         }
     }
 }
@@ -314,7 +396,9 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                                                 const uirk_bp_item *__restrict__ items, int n_items,
                                                 int *__restrict__ err)
 {
-    __shared__ float lds[kBpNa * kBpWb];
+    // reference-compatible path: plain staged bins; fast path: (t, dt) pairs
+    __shared__ float lds[REF ? kBpNa * kBpWb : 1];
+    __shared__ float2 ldp[REF ? 1 : kBpNa * kBpWb];
     __shared__ int s_umin[kBpNa];
     __shared__ float s_trig[2 * kBpNa];
     const int lane = threadIdx.x & 63;
@@ -324,8 +408,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         const uirk_bp_item item = items[it];
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
-        const T *sino = (const T *)S->b;
-        const float *trig = S->trig;
+        const T *sino = uniform((const T *)S->b);
+        const float *trig = uniform(S->trig);
         const float relax = S->relax;
         const int n_runs = S->n_runs;
         const int px = item.x0 + lane;
@@ -359,11 +443,11 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
             val[k] = 0.0f;
         }
         if (REF && !S->pix_w) {
-            const T *xin = (const T *)S->x;
+            const T *xin = uniform((const T *)S->x);
 #pragma unroll
             for (int k = 0; k < kBpRows; ++k) {
                 const int py = item.y0 + wave + 4 * k;
-                acc[k] = (px < N && py < N) ? to_f<T>(xin[(size_t)py * N + px]) : 0.0f;
+                acc[k] = (px < N && py < N) ? gload(xin, (uint32_t)py * (uint32_t)N + (uint32_t)px) : 0.0f;
             }
         }
 
@@ -375,7 +459,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                 __syncthreads(); // previous batch's reads are done
                 if (threadIdx.x < na) {
                     const int ang = abase + threadIdx.x;
-                    const float st = trig[2 * ang], ct = trig[2 * ang + 1];
+                    const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
                     const float u00 = cx0 * ct - cy0 * st, u10 = cx1 * ct - cy0 * st;
                     const float u01 = cx0 * ct - cy1 * st, u11 = cx1 * ct - cy1 * st;
                     float umn = fminf(fminf(u00, u10), fminf(u01, u11)) + shift;
@@ -392,27 +476,48 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                     s_trig[2 * threadIdx.x + 1] = ct;
                 }
                 __syncthreads();
-                for (int k = wave; k < na; k += 4) {
-                    const int ang = abase + k;
-                    const int umin = s_umin[k];
-                    const T *row = sino + (size_t)ang * D;
-                    for (int i = lane; i < kBpWb; i += 64) {
-                        int bin = umin + i;
-                        float v;
-                        if (REF) {
-                            v = (bin >= 0 && bin < D) ? to_f<T>(row[bin]) : 0.0f;
+                {
+                    // angle k = wave + 4 * (e / 2), bin i = lane + 64 * (e % 2);
+                    // all loads first, then the LDS writes
+                    constexpr int kPer = (kBpNa / 4) * 2;
+                    static_assert(kBpWb <= 128, "two bins per lane");
+                    float v0[kPer], v1[kPer];
+#pragma unroll
+                    for (int e = 0; e < kPer; ++e) {
+                        // clamped, always valid indices: no branches around the loads
+                        const int k = min(wave + 4 * (e >> 1), na - 1);
+                        const int i = lane + 64 * (e & 1);
+                        const uint32_t rowo = (uint32_t)(abase + k) * (uint32_t)D;
+                        const int bin = s_umin[k] + i;
+                        if constexpr (REF) {
+                            const int bc = min(max(bin, 0), D - 1);
+                            const float t = gload(sino, rowo + (uint32_t)bc);
+                            v0[e] = bin == bc ? t : 0.0f;
                         } else {
-                            bin = bin < bin_lo ? bin_lo : (bin > bin_hi ? bin_hi : bin);
-                            v = to_f<T>(row[bin]);
+                            // (t[i], t[i+1]) with clamp-to-edge bins
+                            const int b0 = bin < bin_lo ? bin_lo : (bin > bin_hi ? bin_hi : bin);
+                            const int b1 = bin + 1 < bin_lo ? bin_lo : (bin + 1 > bin_hi ? bin_hi : bin + 1);
+                            v0[e] = gload(sino, rowo + (uint32_t)b0);
+                            v1[e] = gload(sino, rowo + (uint32_t)b1);
                         }
-                        lds[k * kBpWb + i] = v;
+                    }
+#pragma unroll
+                    for (int e = 0; e < kPer; ++e) {
+                        const int k = wave + 4 * (e >> 1);
+                        const int i = lane + 64 * (e & 1);
+                        if (k < na && i < kBpWb) {
+                            if constexpr (REF)
+                                lds[k * kBpWb + i] = v0[e];
+                            else
+                                ldp[k * kBpWb + i] = make_float2(v0[e], v1[e] - v0[e]);
+                        }
                     }
                 }
                 __syncthreads();
                 for (int k = 0; k < na; ++k) {
                     const float st = s_trig[2 * k], ct = s_trig[2 * k + 1];
-                    const float *L = lds + k * kBpWb - s_umin[k];
-                    if (REF) {
+                    if constexpr (REF) {
+                        const float *L = lds + k * kBpWb - s_umin[k];
                         const float fXc = fX * ct;
 #pragma unroll
                         for (int g = 0; g < kBpRows; g += 4) {
@@ -435,22 +540,22 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                             }
                         }
                     } else {
+                        const float2 *P = ldp + k * kBpWb - s_umin[k];
                         float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f));
                         const float du = -4.0f * st;
 #pragma unroll
                         for (int g = 0; g < kBpRows; g += 8) {
-                            float t0[8], t1[8], a[8];
+                            float2 p[8];
+                            float a[8];
 #pragma unroll
                             for (int h = 0; h < 8; ++h) {
                                 const float uu = __builtin_fmaf((float)h, du, u);
-                                const int i0 = cvt_flr(uu);
                                 a[h] = __builtin_amdgcn_fractf(uu);
-                                t0[h] = L[i0];
-                                t1[h] = L[i0 + 1];
+                                p[h] = P[cvt_flr(uu)];
                             }
 #pragma unroll
                             for (int h = 0; h < 8; ++h)
-                                acc[g + h] = __builtin_fmaf(a[h], t1[h] - t0[h], acc[g + h] + t0[h]);
+                                acc[g + h] = __builtin_fmaf(a[h], p[h].y, acc[g + h] + p[h].x);
                             u = __builtin_fmaf(8.0f, du, u);
                         }
                     }
@@ -466,9 +571,9 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         }
 
         if (px < N) {
-            const T *xin = (const T *)S->x;
-            T *xout = (T *)S->x_out;
-            const float *pix_w = S->pix_w;
+            const T *xin = uniform((const T *)S->x);
+            T *xout = uniform((T *)S->x_out);
+            const float *pix_w = uniform(S->pix_w);
             const int posc = S->posc;
             const float gain = S->relax * S->pix_scale;
 #pragma unroll
@@ -476,20 +581,20 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                 const int py = item.y0 + wave + 4 * q;
                 if (py >= N)
                     continue;
-                const size_t o = (size_t)py * N + px;
+                const uint32_t o = (uint32_t)py * (uint32_t)N + (uint32_t)px;
                 float out;
                 if (REF) {
                     out = acc[q];
                     if (pix_w) { // SIRT form (ufo-ir-sirt-task.c:208-212)
-                        out = out * pix_w[o];
-                        out = out + to_f<T>(xin[o]);
+                        out = out * gload(pix_w, o);
+                        out = out + gload(xin, o);
                     }
                 } else {
-                    out = __builtin_fmaf(acc[q], gain, to_f<T>(xin[o]));
+                    out = __builtin_fmaf(acc[q], gain, gload(xin, o));
                 }
                 if (posc)
                     out = out > 0 ? out : 0.0f;
-                xout[o] = from_f<T>(out);
+                gstore(xout, o, out);
             }
         }
     }

@@ -177,6 +177,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             s->n_angles > 65536)
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad geometry %ux%u, %u angles", i, s->n,
                                  s->n_dets, s->n_angles);
+        if ((uint64_t)s->n * s->n * 4 >= (1ull << 32) || (uint64_t)s->n_angles * s->n_dets * 4 >= (1ull << 32))
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: image or sinogram of 4 GiB or more", i);
         if (!(s->relaxation >= 0.0f && s->relaxation <= 1.0f))
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: relaxation outside [0, 1]", i);
         if (!(s->step >= 0.0f) || !(s->axis_position >= -1.0f))

@@ -90,6 +90,67 @@ static int block_fits(int n, int n_dets, float axis, const float *s, const float
     return 1;
 }
 
+/* Lines [*lo, *hi) outside which no ray of the block (angles [a0, a0 + k),
+ * detectors [d0, d0 + 64)) samples inside the image.  A sample reads a
+ * non-border texel iff its position lies in [-0.5, n + 0.5); positions are
+ * affine in the detector and line indices, so each angle's set of lines over
+ * the detector block is bounded by its two extreme rays.  A margin of 2
+ * positions covers the float rounding of stepped coordinates. */
+static void line_range(int n, int n_dets, double axis, const float *s, const float *c, int dir, int a0, int k,
+                       int d0, int *lo, int *hi)
+{
+    const int d1 = d0 + UIRK_FP_DETS - 1 < n_dets - 1 ? d0 + UIRK_FP_DETS - 1 : n_dets - 1;
+    const double plo = -0.5 - 2.0, phi = n + 0.5 + 2.0;
+    double lmin = 1e300, lmax = -1e300;
+    for (int a = a0; a < a0 + k; ++a) {
+        double det_step, step;
+        if (dir == 0) {
+            det_step = -1.0 / s[a];
+            step = (double)c[a] / (double)s[a];
+        } else {
+            det_step = 1.0 / c[a];
+            step = (double)s[a] / (double)c[a];
+        }
+        const int ds[2] = {d0, d1};
+        for (int i = 0; i < 2; ++i) {
+            const double p0 = ((double)ds[i] + 0.5 - axis) * det_step - 0.5 * n * step + 0.5 * n;
+            double l_a, l_b;
+            if (fabs(step) < 1e-12) {
+                if (p0 < plo || p0 > phi)
+                    continue;
+                l_a = 0;
+                l_b = n;
+            } else {
+                l_a = (plo - p0) / step;
+                l_b = (phi - p0) / step;
+                if (l_a > l_b) {
+                    const double t = l_a;
+                    l_a = l_b;
+                    l_b = t;
+                }
+            }
+            if (l_a < lmin) lmin = l_a;
+            if (l_b > lmax) lmax = l_b;
+        }
+    }
+    /* Rays between the two extremes are covered: for one angle the entry and
+     * exit lines are affine in the detector index, so the union over the
+     * block lies within [min, max] of the extreme rays. */
+    if (!(lmin <= lmax)) {
+        *lo = *hi = 0;
+        return;
+    }
+    double a = floor(lmin) - 1, b = ceil(lmax) + 2;
+    if (a < 0) a = 0;
+    if (b > n) b = n;
+    if (a >= b) {
+        *lo = *hi = 0;
+        return;
+    }
+    *lo = (int)a;
+    *hi = (int)b;
+}
+
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
                  const float *cos_lut, const uir_subset *subs, uint32_t n_subs)
 {
@@ -108,6 +169,7 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
                 it.a0 = a;
                 it.na = k;
                 it.d0 = d0;
+                line_range(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, &it.l_lo, &it.l_hi);
                 if (fp_push(out, it))
                     return -1;
             }

@@ -537,6 +537,8 @@ static void proj_runs_dev(const uir_task *p, const uir_subset *subs, uint32_t n,
 static uir_status proj_plan(uir_task *p, uint32_t n)
 {
     const float axis = proj_axis(p);
+    if ((uint64_t)n * n * 4 >= (1ull << 32) || (uint64_t)p->lut_n * p->n_dets * 4 >= (1ull << 32))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "image or sinogram of 4 GiB or more");
     if (p->d_fp_items && p->plan_n == n && p->plan_dets == p->n_dets && p->plan_angles == p->lut_n &&
         p->plan_axis == axis)
         return UIR_OK;
