@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--dets", type=int, default=1448)
     ap.add_argument("--accuracy-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 accuracy check")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="for counter collection: build the batch, run 3 iterations with direct launches and one "
+                         "512 MiB device-to-device copy (the calibration of the byte counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (slices shard across GPUs with no "
@@ -104,6 +107,23 @@ def main():
         img.free()
         s.free()
     t_data = time.time() - t0
+
+    if args.profile_pass:
+        specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax)] * args.batch
+        bt = uir.Batch(res, specs, args.storage)
+        for i in range(args.batch):
+            bt.set_sinogram(i, sinos[i % n_distinct])
+        bt.reset()
+        bt.iterate(3, use_graph=False)
+        res.synchronize()
+        # calibration kernel: in-place scale of 512 MiB (reads and writes 512 MiB)
+        nbytes = 512 << 20
+        buf = uir.DeviceArray((nbytes // 4,))
+        uir.check(uir.load().uir_op_mul_scalar(buf.ptr, nbytes // 4, 1.0, None))
+        res.synchronize()
+        print("profile pass: batch %d x %dx%d, %d angles, %d bins, %s; 3 iterations; calibration %d bytes"
+              % (args.batch, n, n, a, d, args.storage, nbytes), flush=True)
+        return
 
     # ---- plain FP32 path: the SIRT task, one slice per process()
     ds = uir.DeviceArray.from_numpy(sinos[0])

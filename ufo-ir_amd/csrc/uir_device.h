@@ -97,6 +97,17 @@ typedef struct {
     int32_t slice, x0, y0, pad;
 } uirk_bp_item;
 
+/* XCD-aware partition of a work table: workgroup g runs on XCD g % n_xcd
+ * (round-robin dispatch) and processes only items [start[x], start[x + 1])
+ * of its XCD x, so each XCD's L2 holds the data of a contiguous,
+ * cost-balanced share of the table (a few slices, not all of them).  n_xcd <= 1
+ * or a grid that is not a multiple of n_xcd falls back to plain striding. */
+#define UIRK_MAX_XCD 16
+typedef struct {
+    int32_t n_xcd;
+    int32_t start[UIRK_MAX_XCD + 1];
+} uirk_xcd_part;
+
 #define UIRK_FP_DETS 64     /* detectors per FP work item */
 #define UIRK_FP_ANGLES 16   /* max angles per FP work item */
 #define UIRK_FP_LINES 32    /* lines per staged chunk (divides the 64-sample block) */
@@ -110,9 +121,9 @@ typedef struct {
  * passed by value as a kernel argument; item.slice must then be 0).
  * err: device int, set non-zero if a footprint exceeded the staging window. */
 int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
-                  int mode, int dtype, int grid, int *d_err, void *stream);
+                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream);
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                  int mode, int dtype, int grid, int *d_err, void *stream);
+                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream);
 
 /* Work-table construction (host/uir_plan.c). */
 typedef struct {
@@ -131,6 +142,11 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
 int uirk_plan_bp(uirk_bp_vec *out, int slice, int n);
 void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig);
 int uirk_plan_grid(int n_items, int per_cu);
+int uirk_plan_num_xcd(void);
+/* Splits items [0, n) into n_xcd contiguous ranges of about equal total cost
+ * (cost[i] >= 0; NULL = unit cost). */
+void uirk_plan_xcd(const double *cost, int n, int n_xcd, uirk_xcd_part *out);
+double uirk_fp_item_cost(const uirk_fp_item *it);
 /* FP32 <-> storage conversions */
 int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream);
 

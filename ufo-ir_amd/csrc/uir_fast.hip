@@ -104,6 +104,21 @@ __device__ __forceinline__ int cvt_flr(float x)
 // x - floor(x); exact for x >= 0 (the callers guarantee it).
 __device__ __forceinline__ float fract_pos(float x) { return __builtin_amdgcn_fractf(x); }
 
+// Item range of this workgroup under an XCD-aware partition (see uirk_xcd_part).
+struct ItemRange {
+    int first, end, stride;
+};
+
+__device__ __forceinline__ ItemRange item_range(const uirk_xcd_part &part, int n_items)
+{
+    const int nx = part.n_xcd;
+    if (nx > 1 && (gridDim.x % nx) == 0) {
+        const int x = blockIdx.x % nx;
+        return ItemRange{part.start[x] + (int)(blockIdx.x / nx), part.start[x + 1], (int)(gridDim.x / nx)};
+    }
+    return ItemRange{(int)blockIdx.x, n_items, (int)gridDim.x};
+}
+
 __device__ __forceinline__ float wave_min(float v)
 {
     for (int o = 32; o > 0; o >>= 1)
@@ -124,14 +139,15 @@ __device__ __forceinline__ float wave_max(float v)
 template <typename T, bool REF>
 __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_fp_item *__restrict__ items, int n_items,
-                                                int *__restrict__ err)
+                                                uirk_xcd_part part, int *__restrict__ err)
 {
     __shared__ float lds[kFpLines * kFpLs];
     __shared__ float red[2][4];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const ItemRange range = item_range(part, n_items);
 
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    for (int it = range.first; it < range.end; it += range.stride) {
         const uirk_fp_item item = items[it];
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
@@ -394,7 +410,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 template <typename T, bool REF>
 __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
-                                                int *__restrict__ err)
+                                                uirk_xcd_part part, int *__restrict__ err)
 {
     // reference-compatible path: plain staged bins; fast path: (t, dt) pairs
     __shared__ float lds[REF ? kBpNa * kBpWb : 1];
@@ -403,8 +419,9 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
     __shared__ float s_trig[2 * kBpNa];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const ItemRange range = item_range(part, n_items);
 
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    for (int it = range.first; it < range.end; it += range.stride) {
         const uirk_bp_item item = items[it];
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
@@ -625,18 +642,18 @@ __global__ void k_convert_to_f32(const void *src, int dtype, float *dst, size_t 
 }
 
 template <typename T, bool REF>
-int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n, int grid, int *err,
-              hipStream_t st)
+int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n,
+              const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, err);
+    hipLaunchKernelGGL((k_fp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
 template <typename T, bool REF>
-int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n, int grid, int *err,
-              hipStream_t st)
+int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
+              const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, err);
+    hipLaunchKernelGGL((k_bp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
@@ -645,7 +662,7 @@ int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *i
 extern "C" {
 
 int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
-                  int mode, int dtype, int grid, int *d_err, void *stream)
+                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -657,21 +674,25 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         return 0;
     if (grid <= 0 || grid > n_items)
         grid = n_items;
+    uirk_xcd_part pt{};
+    if (part)
+        pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
         if (dtype != UIR_DTYPE_F32)
             return (int)hipErrorInvalidValue;
-        return launch_fp<float, true>(d_slices, single, d_items, n_items, grid, d_err, st);
+        return launch_fp<float, true>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
     switch (dtype) {
-    case UIR_DTYPE_F16: return launch_fp<__half, false>(d_slices, single, d_items, n_items, grid, d_err, st);
-    case UIR_DTYPE_BF16: return launch_fp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, grid, d_err, st);
-    default: return launch_fp<float, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    case UIR_DTYPE_F16: return launch_fp<__half, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    case UIR_DTYPE_BF16:
+        return launch_fp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    default: return launch_fp<float, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
 }
 
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                  int mode, int dtype, int grid, int *d_err, void *stream)
+                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -683,16 +704,20 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         return 0;
     if (grid <= 0 || grid > n_items)
         grid = n_items;
+    uirk_xcd_part pt{};
+    if (part)
+        pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
         if (dtype != UIR_DTYPE_F32)
             return (int)hipErrorInvalidValue;
-        return launch_bp<float, true>(d_slices, single, d_items, n_items, grid, d_err, st);
+        return launch_bp<float, true>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
     switch (dtype) {
-    case UIR_DTYPE_F16: return launch_bp<__half, false>(d_slices, single, d_items, n_items, grid, d_err, st);
-    case UIR_DTYPE_BF16: return launch_bp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, grid, d_err, st);
-    default: return launch_bp<float, false>(d_slices, single, d_items, n_items, grid, d_err, st);
+    case UIR_DTYPE_F16: return launch_bp<__half, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    case UIR_DTYPE_BF16:
+        return launch_bp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    default: return launch_bp<float, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
 }
 

@@ -44,6 +44,7 @@ struct uir_batch {
     uirk_fp_item *d_fp_items;
     uirk_bp_item *d_bp_items;
     int n_fp_items, n_bp_items, fp_grid, bp_grid;
+    uirk_xcd_part fp_part, bp_part;
     int *d_err;
     double fp_samples, bp_samples;
     hipGraph_t graph;
@@ -92,26 +93,6 @@ void uir_batch_free(uir_batch *b)
     free(b);
 }
 
-static int cmp_fp_cost_desc(const void *pa, const void *pb, void *ctx)
-{
-    const uir_batch *b = (const uir_batch *)ctx;
-    const uirk_fp_item *x = (const uirk_fp_item *)pa, *y = (const uirk_fp_item *)pb;
-    const double cx = (double)b->specs[x->slice].n * x->na, cy = (double)b->specs[y->slice].n * y->na;
-    if (cx != cy)
-        return cx > cy ? -1 : 1;
-    return (x->slice != y->slice) ? x->slice - y->slice : (x->a0 != y->a0 ? x->a0 - y->a0 : x->d0 - y->d0);
-}
-
-static int cmp_bp_cost_desc(const void *pa, const void *pb, void *ctx)
-{
-    const uir_batch *b = (const uir_batch *)ctx;
-    const uirk_bp_item *x = (const uirk_bp_item *)pa, *y = (const uirk_bp_item *)pb;
-    const uint32_t cx = b->specs[x->slice].n_angles, cy = b->specs[y->slice].n_angles;
-    if (cx != cy)
-        return cx > cy ? -1 : 1;
-    return (x->slice != y->slice) ? x->slice - y->slice : (x->y0 != y->y0 ? x->y0 - y->y0 : x->x0 - y->x0);
-}
-
 static uir_status upload(void **dst, const void *src, size_t bytes)
 {
     TRY(uir_dmalloc(dst, bytes));
@@ -149,7 +130,7 @@ static uir_status ray_weights(uir_batch *bt, geom_t *g)
         sl.n_angles = (int32_t)g->A;
         sl.axis = g->axis;
         sl.corr = 1.0f;
-        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, 1, UIR_DTYPE_F32,
+        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32,
                                         uirk_plan_grid((int)fv.n, 4), bt->d_err, st), "FP launch");
     }
     if (s == UIR_OK)
@@ -325,8 +306,23 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         b->fp_samples += m * (double)G->n * G->n;
         b->bp_samples += (double)G->A * G->n * G->n;
     }
-    qsort_r(fv.v, fv.n, sizeof(uirk_fp_item), cmp_fp_cost_desc, b);
-    qsort_r(bv.v, bv.n, sizeof(uirk_bp_item), cmp_bp_cost_desc, b);
+    /* Tables stay slice-major; each XCD takes a contiguous, cost-balanced range,
+     * so its L2 holds the images and sinograms of a few slices only. */
+    {
+        const size_t nmax = fv.n > bv.n ? fv.n : bv.n;
+        double *cost = (double *)malloc(sizeof(double) * (nmax ? nmax : 1));
+        if (!cost) {
+            s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+            goto fail;
+        }
+        for (size_t i = 0; i < fv.n; ++i)
+            cost[i] = uirk_fp_item_cost(&fv.v[i]);
+        uirk_plan_xcd(cost, (int)fv.n, uirk_plan_num_xcd(), &b->fp_part);
+        for (size_t i = 0; i < bv.n; ++i)
+            cost[i] = (double)specs[bv.v[i].slice].n_angles;
+        uirk_plan_xcd(cost, (int)bv.n, uirk_plan_num_xcd(), &b->bp_part);
+        free(cost);
+    }
     if ((s = upload((void **)&b->d_fp_sl, fp_sl, n_slices * sizeof *fp_sl)) != UIR_OK ||
         (s = upload((void **)&b->d_bp_sl, bp_sl, n_slices * sizeof *bp_sl)) != UIR_OK ||
         (s = upload((void **)&b->d_fp_items, fv.v, fv.n * sizeof(uirk_fp_item))) != UIR_OK ||
@@ -384,9 +380,9 @@ uir_status uir_batch_reset(uir_batch *b)
 static uir_status launch_iteration(uir_batch *b)
 {
     hipStream_t st = b->res->stream;
-    TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, 1, b->dtype, b->fp_grid,
+    TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, &b->fp_part, 1, b->dtype, b->fp_grid,
                                     b->d_err, st), "FP launch"));
-    return uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, 1, b->dtype, b->bp_grid,
+    return uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, &b->bp_part, 1, b->dtype, b->bp_grid,
                                        b->d_err, st), "BP launch");
 }
 
@@ -485,12 +481,12 @@ uir_status uir_batch_time_kernels(uir_batch *b, uint32_t reps, float *fp_ms, flo
     TRY(uir_check_hip(hipEventCreate(&e2), "hipEventCreate"));
     uir_status s = uir_check_hip(hipEventRecord(e0, st), "hipEventRecord");
     for (uint32_t i = 0; i < reps && s == UIR_OK; ++i)
-        s = uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, 1, b->dtype, b->fp_grid,
+        s = uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, &b->fp_part, 1, b->dtype, b->fp_grid,
                                         b->d_err, st), "FP launch");
     if (s == UIR_OK)
         s = uir_check_hip(hipEventRecord(e1, st), "hipEventRecord");
     for (uint32_t i = 0; i < reps && s == UIR_OK; ++i)
-        s = uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, 1, b->dtype, b->bp_grid,
+        s = uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, &b->bp_part, 1, b->dtype, b->bp_grid,
                                         b->d_err, st), "BP launch");
     if (s == UIR_OK)
         s = uir_check_hip(hipEventRecord(e2, st), "hipEventRecord");

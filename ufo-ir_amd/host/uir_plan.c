@@ -202,6 +202,48 @@ void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, floa
     }
 }
 
+int uirk_plan_num_xcd(void)
+{
+    static int nx = 0;
+    if (nx == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeNumberOfXccs, dev) == hipSuccess && v > 0)
+            nx = v > UIRK_MAX_XCD ? UIRK_MAX_XCD : v;
+        else
+            nx = 1;
+    }
+    return nx;
+}
+
+double uirk_fp_item_cost(const uirk_fp_item *it)
+{
+    return (double)it->na * (double)(it->l_hi - it->l_lo) + 16.0; /* + epilogue */
+}
+
+void uirk_plan_xcd(const double *cost, int n, int n_xcd, uirk_xcd_part *out)
+{
+    memset(out, 0, sizeof *out);
+    if (n_xcd < 1)
+        n_xcd = 1;
+    if (n_xcd > UIRK_MAX_XCD)
+        n_xcd = UIRK_MAX_XCD;
+    out->n_xcd = n_xcd;
+    double total = 0;
+    for (int i = 0; i < n; ++i)
+        total += cost ? cost[i] : 1.0;
+    double acc = 0;
+    int x = 1;
+    out->start[0] = 0;
+    for (int i = 0; i < n && x < n_xcd; ++i) {
+        acc += cost ? cost[i] : 1.0;
+        while (x < n_xcd && acc >= total * x / n_xcd)
+            out->start[x++] = i + 1;
+    }
+    while (x <= n_xcd)
+        out->start[x++] = n;
+}
+
 /* Persistent-grid size: enough workgroups to fill every CU per_cu times. */
 int uirk_plan_grid(int n_items, int per_cu)
 {

@@ -47,6 +47,7 @@ struct uir_task {
     uirk_fp_item *d_fp_items;
     uirk_bp_item *d_bp_items;
     int n_fp_items, n_bp_items;
+    uirk_xcd_part fp_part, bp_part;
     float *d_trig;
     int *d_err;
 
@@ -574,6 +575,18 @@ static uir_status proj_plan(uir_task *p, uint32_t n)
         goto done;
     p->n_fp_items = (int)fv.n;
     p->n_bp_items = (int)bv.n;
+    {
+        double *cost = (double *)malloc(sizeof(double) * (fv.n ? fv.n : 1));
+        if (!cost) {
+            s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+            goto done;
+        }
+        for (size_t i = 0; i < fv.n; ++i)
+            cost[i] = uirk_fp_item_cost(&fv.v[i]);
+        uirk_plan_xcd(cost, (int)fv.n, uirk_plan_num_xcd(), &p->fp_part);
+        uirk_plan_xcd(NULL, (int)bv.n, uirk_plan_num_xcd(), &p->bp_part);
+        free(cost);
+    }
     p->plan_n = n;
     p->plan_dets = p->n_dets;
     p->plan_angles = p->lut_n;
@@ -613,8 +626,8 @@ static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float
     sl.b = r_sino;
     sl.r = w_sino;
     sl.ray_w = ray_w;
-    return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_fp_items, p->n_fp_items, 0, UIR_DTYPE_F32,
-                                       uirk_plan_grid(p->n_fp_items, 8), p->d_err, p->res->stream), "FP launch");
+    return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_fp_items, p->n_fp_items, &p->fp_part, 0, UIR_DTYPE_F32,
+                                       uirk_plan_grid(p->n_fp_items, 4), p->d_err, p->res->stream), "FP launch");
 }
 
 /* backward over all runs (:410-429).  pix_w == NULL: accumulate form,
@@ -638,7 +651,7 @@ static uir_status proj_bp_ex(uir_task *p, uint32_t n, const float *sino, float *
             sl.run_n[k] = p->runs[i].n;
         }
         sl.n_runs = (int32_t)k;
-        TRY(uir_check_hip(uirk_bp_table(NULL, &sl, p->d_bp_items, p->n_bp_items, 0, UIR_DTYPE_F32,
+        TRY(uir_check_hip(uirk_bp_table(NULL, &sl, p->d_bp_items, p->n_bp_items, &p->bp_part, 0, UIR_DTYPE_F32,
                                         uirk_plan_grid(p->n_bp_items, 8), p->d_err, p->res->stream), "BP launch"));
     }
     return UIR_OK;
