@@ -86,6 +86,7 @@ def lib():
         L.orc_sbtv_cgs.argtypes = [G, f32p, f32p, f32p, C.c_uint32, C.c_float, C.c_float, C.c_int]
         L.orc_sbtv_cgs.restype = C.c_int
         L.orc_sbtv_apply.argtypes = [G, f32p, f32p, C.c_float, C.c_float]
+        L.orc_sbtv_shrink.argtypes = [f32p, f32p, C.c_size_t, C.c_float, f32p, f32p]
         L.orc64_fp.argtypes = [G, f64p, f64p]
         L.orc64_bp.argtypes = [G, f64p, f64p]
         L.orc64_sirt.argtypes = [G, f64p, f64p, C.c_uint32, C.c_double, C.c_int]
@@ -232,6 +233,13 @@ def sbtv_cgs(geom, rhs, x0, max_iter, lam, mu, dot_mode=0):
     x = np.zeros(geom.vol_shape(), np.float32)
     flag = lib().orc_sbtv_cgs(geom.ptr, _f32(rhs), x, _f32(x0), int(max_iter), float(lam), float(mu), int(dot_mode))
     return x, flag
+
+
+def sbtv_shrink(tx, ty, lam):
+    tx, ty = _f32(tx), _f32(ty)
+    dx, dy = np.zeros_like(tx), np.zeros_like(ty)
+    lib().orc_sbtv_shrink(tx, ty, tx.size, float(lam), dx, dy)
+    return dx, dy
 
 
 def l1_norm(v):

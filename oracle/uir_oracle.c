@@ -787,6 +787,25 @@ int orc_sbtv_cgs(const orc_geom *g, const float *rhs, float *x, const float *x0,
     return flag;
 }
 
+/* The shrinkage of update_db, ufo-ir-sbtv-task.c:367-391, on t = (tx, ty):
+ * s = sqrt(tx^2 + ty^2), k = max(s - 1/lambda, 0) / max(s, 1e-12),
+ * d = k t, returned in (dx, dy). */
+void orc_sbtv_shrink(const float *tx, const float *ty, size_t n, float lambda, float *dx, float *dy)
+{
+    const float d_lambda = -1 / lambda;
+    for (size_t i = 0; i < n; ++i) {
+        float s = tx[i] * tx[i];
+        s = s + ty[i] * ty[i];
+        s = (float)sqrt(s);
+        float top = d_lambda + s;
+        top = (float)fmax(top, 0.0f);
+        const float den = (float)fmax(s, 1E-12f);
+        const float k = top / den;
+        dx[i] = k * tx[i];
+        dy[i] = k * ty[i];
+    }
+}
+
 /* SBTV process, ufo-ir-sbtv-task.c:230-302 with calculate_b :304-334 and
  * update_db :336-402. */
 void orc_sbtv(const orc_geom *g, float *input, float *u, uint32_t iters, float lambda, float mu, int dot_mode)
@@ -810,11 +829,7 @@ void orc_sbtv(const orc_geom *g, float *input, float *u, uint32_t iters, float l
     float *tmpx = (float *)malloc(nv * sizeof(float));
     float *tmpy = (float *)malloc(nv * sizeof(float));
     float *tmp = (float *)malloc(nv * sizeof(float));
-    float *s = (float *)malloc(nv * sizeof(float));
-    float *e12 = (float *)malloc(nv * sizeof(float));
-    float *zero = (float *)calloc(nv, sizeof(float));
     orc_mul_scalar(fbp, nv, mu);                            /* :279 */
-    const float d_lambda = -1 / lambda;                     /* :357 */
 
     for (uint32_t it = 0; it < iters; ++it) {               /* :282-289 */
         memcpy(up, u, nv * sizeof(float));
@@ -833,24 +848,12 @@ void orc_sbtv(const orc_geom *g, float *input, float *u, uint32_t iters, float l
         orc_op_add(tmpx, bx, tmpx, nv);
         orc_dy(u, tmpy, N, N);
         orc_op_add(tmpy, by, tmpy, nv);
-        orc_mul_ew(tmpx, tmpx, tmp, nv);
-        memcpy(s, tmp, nv * sizeof(float));
-        orc_mul_ew(tmpy, tmpy, tmp, nv);
-        orc_op_add(s, tmp, s, nv);
-        orc_sqrt(s, nv);
-        orc_op_set(tmp, nv, d_lambda);                       /* temp_s_top */
-        orc_op_add(tmp, s, tmp, nv);
-        orc_max_ew(tmp, zero, tmp, nv);
-        orc_op_set(e12, nv, 1E-12f);
-        orc_max_ew(s, e12, e12, nv);
-        orc_div_ew(tmp, e12, s, nv);                         /* tresh (reuses s) */
-        orc_mul_ew(s, tmpx, dx, nv);
-        orc_mul_ew(s, tmpy, dy, nv);
+        orc_sbtv_shrink(tmpx, tmpy, nv, lambda, dx, dy);     /* :367-385 */
         orc_op_sub(tmpx, dx, bx, nv);
         orc_op_sub(tmpy, dy, by, nv);
     }
     free(f); free(fbp); free(up); free(bb); free(bx); free(by); free(dx); free(dy);
-    free(tmpx); free(tmpy); free(tmp); free(s); free(e12); free(zero);
+    free(tmpx); free(tmpy); free(tmp);
     sb_free(&c);
 }
 

@@ -118,6 +118,8 @@ void orc_sbtv(const orc_geom *g, float *b, float *u, uint32_t iters, float lambd
 int orc_sbtv_cgs(const orc_geom *g, const float *rhs, float *x, const float *x0, uint32_t max_iter,
                  float lambda, float mu, int dot_mode);
 void orc_sbtv_apply(const orc_geom *g, const float *in, float *out, float lambda, float mu);
+/* the shrinkage step of update_db on (tx, ty) -> (dx, dy) */
+void orc_sbtv_shrink(const float *tx, const float *ty, size_t n, float lambda, float *dx, float *dy);
 
 /* ---- FP64 evaluation of the same operators (exact coordinates) ---- */
 void orc64_fp(const orc_geom *g, const double *vol, double *sino);
