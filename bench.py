@@ -15,12 +15,28 @@ slice per process() call, reference-compatible arithmetic) is measured beside
 it, and accuracy is reported against the FP64 evaluation of the same
 operators.
 
+Serving leg (default run): the reconstruction server (continuous batching)
+takes SIRT requests of the same shape (20 iterations each) arriving as a
+Poisson process at --serve-qps; p50 / p99 request latency (submission to
+result on the host) and the served slice-iterations/s are reported beside the
+batched throughput.  --serve-c5 runs the split-Bregman TV serving workload
+instead (image edge N in {512, 1024, 2048}, angles U{90..900}, bins
+ceil(N sqrt 2), lambda ~ logU(1e-3, 1e-1), mu = 1, 30 outer x 5 inner sweeps,
+FP16 storage) and prints its own JSON line.
+
+Every batched or served request uses ray_weight_min = 0.1: rays that cross
+less than a tenth of a pixel (image-corner clips) get weight 0; the FP64
+reference uses the same floor.
+
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 import argparse
 import json
+import math
 import os
+import queue
 import sys
+import threading
 import time
 
 import numpy as np
@@ -60,6 +76,82 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v), q))
 
 
+RAY_WEIGHT_MIN = 0.1
+
+
+def run_serving(uir, storage, make, qps, n_requests, seed):
+    """Poisson arrivals at `qps` of requests make(k) -> (request, sinogram);
+    returns latency percentiles and served slice-iterations/s."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / qps, n_requests)
+    work = [make(k) for k in range(n_requests)]
+    tickets = queue.Queue()
+    with uir.Server(0, storage, hbm_reserve=4 << 30) as srv:
+        def producer():
+            t_next = time.perf_counter()
+            for k in range(n_requests):
+                t_next += gaps[k]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets.put(srv.submit(*work[k]))
+            tickets.put(None)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        stats = []
+        while True:
+            t = tickets.get()
+            if t is None:
+                break
+            stats.append(srv.wait(t, timeout=600)[1])
+        th.join()
+        info = srv.info()
+    lat = [(s["t_done"] - s["t_submit"]) * 1e3 for s in stats]
+    t0 = min(s["t_submit"] for s in stats)
+    t1 = max(s["t_done"] for s in stats)
+    return {"qps": qps, "requests": n_requests, "duration_s": round(t1 - t0, 2),
+            "latency_ms": {"p50": round(percentile(lat, 50), 2), "p99": round(percentile(lat, 99), 2),
+                           "mean": round(float(np.mean(lat)), 2), "max": round(max(lat), 2)},
+            "slice_iters_per_s": round(info["slice_iterations"] / (t1 - t0), 1),
+            "mean_batch": round(info["slice_iterations"] / max(info["steps"], 1), 2),
+            "max_batch": max(s["max_batch"] for s in stats),
+            "steps": info["steps"]}
+
+
+def serve_c5(args, uir):
+    """Split-Bregman TV serving at a fixed Poisson rate (one JSON line)."""
+    rng = np.random.default_rng(20261205)
+    sizes = [512, 1024, 2048]
+    pool = {}
+    for n in sizes:
+        d = math.ceil(n * math.sqrt(2))
+        # content does not change the work: smooth synthetic sinograms, one per size
+        t = np.linspace(-1, 1, d, dtype=np.float32)
+        th = np.linspace(0, np.pi, 900, dtype=np.float32)[:, None]
+        pool[n] = (np.maximum(0.0, 1 - (t[None, :] - 0.3 * np.cos(th)) ** 2) * n / 4).astype(np.float32)
+
+    def make(k):
+        n = sizes[rng.integers(0, 3)]
+        a = int(rng.integers(90, 901))
+        d = math.ceil(n * math.sqrt(2))
+        lam = float(10 ** rng.uniform(-3, -1))
+        req = uir.make_request(n, d, a, float(np.float32(np.pi / a)), "sbtv-sirt", iterations=30,
+                               inner_iterations=5, relaxation=1.0, lam=lam, mu=1.0,
+                               ray_weight_min=RAY_WEIGHT_MIN)
+        return req, np.ascontiguousarray(pool[n][:a])
+
+    out = run_serving(uir, "f16", make, args.serve_qps_c5, args.serve_requests_c5, 20261206)
+    line = {"metric": "split-Bregman TV serving: p50/p99 latency and slice-iters/s", "value": out["slice_iters_per_s"],
+            "unit": "slice-iters/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f16-storage/f32-math",
+            "data": "synthetic smooth sinograms (content does not change the work)",
+            "config": {"workload": "SBTV_SIRT requests, N in {512, 1024, 2048}, angles U{90..900}, bins ceil(N sqrt 2), "
+                                   "lambda logU(1e-3, 1e-1), mu 1, 30 outer x 5 inner sweeps, Poisson arrivals",
+                       "qps": args.serve_qps_c5, "requests": args.serve_requests_c5, "storage": "f16"},
+            "serving": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +164,13 @@ def main():
     ap.add_argument("--dets", type=int, default=1448)
     ap.add_argument("--accuracy-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 accuracy check")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-iters", type=int, default=20)
+    ap.add_argument("--serve-c5", action="store_true", help="run the split-Bregman TV serving workload only")
+    ap.add_argument("--serve-qps-c5", type=float, default=10.0)
+    ap.add_argument("--serve-requests-c5", type=int, default=200)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for counter collection: build the batch, run 3 iterations with direct launches and one "
                          "512 MiB device-to-device copy (the calibration of the byte counters), then exit")
@@ -85,6 +184,9 @@ def main():
     if uir.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.serve_c5:
+        serve_c5(args, uir)
+        return
     res = uir.Resources(0)
     n, a, d = args.n, args.angles, args.dets
     step = float(np.float32(np.pi / a))
@@ -109,7 +211,7 @@ def main():
     t_data = time.time() - t0
 
     if args.profile_pass:
-        specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax)] * args.batch
+        specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
         bt = uir.Batch(res, specs, args.storage)
         for i in range(args.batch):
             bt.set_sinogram(i, sinos[i % n_distinct])
@@ -147,7 +249,7 @@ def main():
     plain_img = out.numpy()
 
     # ---- batched path
-    specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax)] * args.batch
+    specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
     bt = uir.Batch(res, specs, args.storage)
     for i in range(args.batch):
         bt.set_sinogram(i, sinos[i % n_distinct])
@@ -218,14 +320,28 @@ def main():
         cpu = {"value": round(cpu_rate, 4), "unit": "slice-iters/s", "cores": O.num_threads(), "kind": "port",
                "sample": "1 slice at 1024x1024x720x1448, ref-compat FP32 SIRT, (t(3 it) - t(1 it)) / 2"}
         t1 = time.time()
-        ref64 = O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax)
+        ref64 = O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax, ray_weight_min=RAY_WEIGHT_MIN)
         t_ref = time.time() - t1
+        ref64_plain = O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax)
         acc = {"iterations": args.accuracy_iters,
                "batched_%s_rel_l2_vs_fp64" % args.storage: O.rel_l2(batched_img, ref64),
                "batched_%s_psnr_db" % args.storage: round(O.psnr(batched_img, ref64), 2),
-               "plain_fp32_rel_l2_vs_fp64": O.rel_l2(plain_img, ref64),
-               "plain_fp32_psnr_db": round(O.psnr(plain_img, ref64), 2),
+               "batched_ray_weight_min": RAY_WEIGHT_MIN,
+               "plain_fp32_rel_l2_vs_fp64": O.rel_l2(plain_img, ref64_plain),
+               "plain_fp32_psnr_db": round(O.psnr(plain_img, ref64_plain), 2),
+               "plain_reference": "FP64 SIRT without a ray weight floor (the reference's inv)",
                "fp64_oracle_s": round(t_ref, 1)}
+
+    # ---- serving: Poisson arrivals of SIRT requests of the same shape
+    serving = None
+    if not args.no_serve:
+        del bt
+        def make(k):
+            return (uir.make_request(n, d, a, step, "sirt", iterations=args.serve_iters, relaxation=relax,
+                                     ray_weight_min=RAY_WEIGHT_MIN), sinos[k % n_distinct])
+        serving = run_serving(uir, args.storage, make, args.serve_qps, args.serve_requests, 20261204)
+        serving["workload"] = "SIRT requests of %dx%d, %d angles, %d bins, %d iterations each, Poisson arrivals" % (
+            n, n, a, d, args.serve_iters)
 
     line = {
         "metric": METRIC,
@@ -248,6 +364,7 @@ def main():
         "speedup_vs_plain": round(value / plain_rate, 3),
         "latency_ms": {"p50": round(percentile(step_ms, 50), 4), "p99": round(percentile(step_ms, 99), 4),
                        "what": "one batched SIRT iteration (all slices)"},
+        "serving": serving,
         "kernels_ms": {"fp": round(fp_ms, 4), "bp": round(bp_ms, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,

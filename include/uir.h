@@ -218,6 +218,12 @@ typedef struct {
     float    axis_position;  /* projector "axis_position" (-1 = n_dets / 2) */
     float    relaxation;     /* SIRT "relaxation_factor" */
     int32_t  positive_constraint;
+    /* Ray weights R = 1 / FP(1) are set to 0 for rays with FP(1) < ray_weight_min,
+     * as for rays that miss the image.  0 = the reference's inv (only FP(1) = 0
+     * gives 0, ufo-ir-basic-ops.cl:34).  Rays that clip an image corner by a tiny
+     * fraction of a pixel otherwise get weights up to 1/FP(1) ~ 1e4-1e5 that
+     * amplify noise into the corner pixels. */
+    float    ray_weight_min;
 } uir_slice_spec;
 
 typedef struct {
@@ -245,6 +251,74 @@ uir_status uir_batch_get_info(const uir_batch *batch, uir_batch_info *info);
 /* Average device duration (ms) of one FP and one BP launch over `reps`
  * launches each, timed with events on the batch stream. */
 uir_status uir_batch_time_kernels(uir_batch *batch, uint32_t reps, float *fp_ms, float *bp_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Reconstruction server: continuous batching.  No counterpart in the        */
+/* reference (one slice per process() call in a ufo-core graph).  Requests   */
+/* of any geometry arrive from any thread; a worker thread owning one device */
+/* admits them in arrival order while the device budget and the slot limit   */
+/* allow, and advances every admitted request by one iteration per step:     */
+/* one FP and one BP launch over the work tables of all admitted requests.   */
+/* A request leaves as soon as its iterations are done.  Each request's      */
+/* arithmetic is independent of the others in the step, so its result is     */
+/* bit-identical alone and in any mix.                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct uir_server uir_server;
+
+typedef enum {
+    UIR_METHOD_SIRT = 0,      /* the SIRT update of ufo-ir-sirt-task.c:202-215 */
+    UIR_METHOD_SBTV_SIRT = 1  /* split-Bregman TV (ufo-ir-sbtv-task.c's outer loop) with SIRT-preconditioned
+                                 gradient sweeps for the u-subproblem in place of CGS */
+} uir_method;
+
+typedef struct {
+    uir_slice_spec slice;      /* geometry, relaxation (SBTV_SIRT: in (0, 1]), positive_constraint */
+    uint32_t method;           /* uir_method */
+    uint32_t iterations;       /* SIRT: iterations; SBTV_SIRT: outer (Bregman) iterations */
+    uint32_t inner_iterations; /* SBTV_SIRT: sweeps per outer iteration (>= 1) */
+    float    lambda, mu;       /* SBTV_SIRT: ufo-ir-sbtv-task.c "lambda" and "mu" (> 0) */
+} uir_request;
+
+typedef struct {
+    uir_dtype storage;         /* image / sinogram storage of every request */
+    uint32_t  max_active;      /* most requests in one step (0 = no limit) */
+    uint64_t  hbm_budget;      /* device bytes the server may hold (0 = free HBM at creation - hbm_reserve) */
+    uint64_t  hbm_reserve;     /* bytes left free when hbm_budget = 0 */
+} uir_server_config;
+
+typedef struct {
+    double   t_submit, t_admit, t_done; /* seconds on uir_clock() */
+    uint64_t device_bytes;              /* what the request held while admitted */
+    uint32_t steps;                     /* steps it took part in (= its iterations) */
+    uint32_t max_batch;                 /* most requests that shared one of its steps */
+} uir_request_stats;
+
+typedef struct {
+    uint64_t budget, used;              /* device bytes */
+    uint32_t active, pending;
+    uint64_t steps, completed;
+    double   busy_s;                    /* worker time with at least one request admitted */
+    double   slice_iterations;          /* sum over steps of requests in the step */
+} uir_server_info;
+
+double     uir_clock(void);  /* monotonic seconds */
+/* Starts the worker thread on `device`. */
+uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server **out);
+/* Copies the request and its FP32 sinogram ([n_angles][n_dets], host memory)
+ * and queues it.  A request whose footprint exceeds the whole budget is
+ * refused with UIR_ERR_OUT_OF_MEMORY; bad parameters with
+ * UIR_ERR_INVALID_ARGUMENT. */
+uir_status uir_server_submit(uir_server *s, const uir_request *req, const float *sinogram, uint64_t *ticket);
+/* Waits up to timeout_s (< 0: forever) for the request, copies its N x N FP32
+ * image to host memory `image` (NULL: discard) and releases the ticket.
+ * UIR_ERR_TIMEOUT leaves the ticket valid; a failed request returns its error. */
+uir_status uir_server_wait(uir_server *s, uint64_t ticket, float *image, double timeout_s, uir_request_stats *stats);
+uir_status uir_server_get_info(uir_server *s, uir_server_info *info);
+/* Device bytes a request holds while admitted (its footprint for admission). */
+uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint64_t *bytes);
+/* Cancels queued requests (their waits return UIR_ERR_CANCELLED), finishes
+ * the admitted ones, joins the worker and frees everything. */
+void       uir_server_free(uir_server *s);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel-level ABI (the reference's inner boundary).  All pointers are      */

@@ -89,8 +89,12 @@ def lib():
         L.orc_sbtv_shrink.argtypes = [f32p, f32p, C.c_size_t, C.c_float, f32p, f32p]
         L.orc64_fp.argtypes = [G, f64p, f64p]
         L.orc64_bp.argtypes = [G, f64p, f64p]
-        L.orc64_sirt.argtypes = [G, f64p, f64p, C.c_uint32, C.c_double, C.c_int]
+        L.orc64_sirt.argtypes = [G, f64p, f64p, C.c_uint32, C.c_double, C.c_int, C.c_double]
         L.orc64_sart.argtypes = [G, f64p, f64p, C.c_uint32, C.c_double]
+        L.orc64_sb_tv.argtypes = [f64p, f64p, f64p, f64p, C.c_int]
+        L.orc64_sb_shrink.argtypes = [f64p, f64p, f64p, f64p, f64p, C.c_int, C.c_double]
+        L.orc64_sbsirt.argtypes = [G, f64p, f64p, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_int,
+                                   C.c_double]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -193,9 +197,11 @@ def sart(geom, b, iters, relax=0.25):
     return x
 
 
-def sirt64(geom, b, iters, relax=0.25, posc=False):
+def sirt64(geom, b, iters, relax=0.25, posc=False, ray_weight_min=0.0):
+    """SIRT in FP64; rays with FP(1) < ray_weight_min get weight 0."""
     x = np.zeros(geom.vol_shape(), np.float64)
-    lib().orc64_sirt(geom.ptr, np.ascontiguousarray(b, np.float64), x, int(iters), float(relax), int(bool(posc)))
+    lib().orc64_sirt(geom.ptr, np.ascontiguousarray(b, np.float64), x, int(iters), float(relax), int(bool(posc)),
+                     float(ray_weight_min))
     return x
 
 
@@ -203,6 +209,34 @@ def sart64(geom, b, iters, relax=0.25):
     x = np.zeros(geom.vol_shape(), np.float64)
     lib().orc64_sart(geom.ptr, np.ascontiguousarray(b, np.float64), x, int(iters), float(relax))
     return x
+
+
+def sbsirt64(geom, f, outer, inner, relax=1.0, lam=0.1, mu=1.0, posc=False, ray_weight_min=0.0):
+    """Split-Bregman TV with SIRT-preconditioned inner sweeps, FP64."""
+    u = np.zeros(geom.vol_shape(), np.float64)
+    lib().orc64_sbsirt(geom.ptr, np.ascontiguousarray(f, np.float64), u, int(outer), int(inner), float(relax),
+                       float(lam), float(mu), int(bool(posc)), float(ray_weight_min))
+    return u
+
+
+def sb_tv64(u, ex, ey):
+    """tv = Dx^T (ex - Dx u) + Dy^T (ey - Dy u) (FP64, periodic differences)."""
+    n = u.shape[0]
+    tv = np.zeros((n, n), np.float64)
+    c = lambda a: np.ascontiguousarray(a, np.float64)
+    lib().orc64_sb_tv(c(u), c(ex), c(ey), tv, n)
+    return tv
+
+
+def sb_shrink64(u, bx, by, lam):
+    """One shrink + Bregman update from (u, bx, by): returns (bx, by, ex, ey)."""
+    n = u.shape[0]
+    bx = np.array(bx, np.float64)
+    by = np.array(by, np.float64)
+    ex = np.zeros((n, n), np.float64)
+    ey = np.zeros((n, n), np.float64)
+    lib().orc64_sb_shrink(np.ascontiguousarray(u, np.float64), bx, by, ex, ey, n, float(lam))
+    return bx, by, ex, ey
 
 
 def asdpocs(geom, b, iters, beta=1.0, beta_red=0.995, alpha=0.2, alpha_red=0.95, r_max=0.95,

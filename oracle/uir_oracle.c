@@ -944,7 +944,8 @@ void orc64_bp(const orc_geom *g, double *vol, const double *sino)
     }
 }
 
-void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc)
+void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc,
+                double r_min)
 {
     const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
     double *R = (double *)malloc(ns * sizeof(double));
@@ -956,7 +957,7 @@ void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, d
     for (size_t i = 0; i < nv; ++i) ones_v[i] = 1.0;
     for (size_t i = 0; i < ns; ++i) ones_s[i] = 1.0;
     orc64_fp(g, ones_v, R);
-    for (size_t i = 0; i < ns; ++i) R[i] = R[i] != 0 ? 1.0 / R[i] : 0.0;
+    for (size_t i = 0; i < ns; ++i) R[i] = (R[i] != 0 && R[i] >= r_min) ? 1.0 / R[i] : 0.0;
     orc64_bp(g, C, ones_s);
     for (size_t i = 0; i < nv; ++i) C[i] = C[i] != 0 ? 1.0 / C[i] : 0.0;
     for (size_t i = 0; i < nv; ++i) x[i] = 0.0;
@@ -1014,4 +1015,82 @@ void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, d
         }
     }
     free(R); free(ones_v); free(row); free(tmp);
+}
+
+/* ---- split-Bregman TV with SIRT-preconditioned sweeps (FP64) ----
+ * The outer loop of ufo-ir-sbtv-task.c:230-300 (u-subproblem, shrink,
+ * Bregman update, update_db :360-402) with the u-subproblem
+ *   min_u mu/2 |A u - f|^2_R + lambda/2 |e - D u|^2,  e = d - b,
+ * approximated by `inner` sweeps u <- u + w P (mu A^T R (f - A u) + lambda D^T (e - D u)),
+ * P = 1 / (mu A_n + 8 lambda), A_n = n_angles; differences as orc_dx .. orc_dyt. */
+void orc64_sb_tv(const double *u, const double *ex, const double *ey, double *tv, int n)
+{
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            const int xl = x == 0 ? n - 1 : x - 1, xr = x == n - 1 ? 0 : x + 1;
+            const int yu = y == 0 ? n - 1 : y - 1, yd = y == n - 1 ? 0 : y + 1;
+            const size_t o = (size_t)y * n + x;
+            /* (Dx^T h)[x] = h[x] - h[x+1], h = e - Dx u, (Dx u)[x] = u[x] - u[x-1] */
+            const double hx0 = ex[o] - (u[o] - u[(size_t)y * n + xl]);
+            const double hx1 = ex[(size_t)y * n + xr] - (u[(size_t)y * n + xr] - u[o]);
+            const double hy0 = ey[o] - (u[o] - u[(size_t)yu * n + x]);
+            const double hy1 = ey[(size_t)yd * n + x] - (u[(size_t)yd * n + x] - u[o]);
+            tv[o] = (hx0 - hx1) + (hy0 - hy1);
+        }
+}
+
+void orc64_sb_shrink(const double *u, double *bx, double *by, double *ex, double *ey, int n, double lambda)
+{
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            const int xl = x == 0 ? n - 1 : x - 1, yu = y == 0 ? n - 1 : y - 1;
+            const size_t o = (size_t)y * n + x;
+            const double gx = (u[o] - u[(size_t)y * n + xl]) + bx[o];
+            const double gy = (u[o] - u[(size_t)yu * n + x]) + by[o];
+            const double s = sqrt(gx * gx + gy * gy);
+            const double t = fmax(s - 1.0 / lambda, 0.0) / fmax(s, 1e-12);
+            const double dx = t * gx, dy = t * gy;
+            bx[o] = gx - dx;
+            by[o] = gy - dy;
+            ex[o] = dx - bx[o];
+            ey[o] = dy - by[o];
+        }
+}
+
+void orc64_sbsirt(const orc_geom *g, const double *f, double *u, uint32_t outer, uint32_t inner, double relax,
+                  double lambda, double mu, int posc, double r_min)
+{
+    const int n = g->n;
+    const size_t nv = (size_t)n * n, ns = (size_t)g->n_angles * g->n_dets;
+    double *R = (double *)calloc(ns, sizeof(double));
+    double *ones_v = (double *)malloc(nv * sizeof(double));
+    double *r = (double *)calloc(ns, sizeof(double));
+    double *bp = (double *)calloc(nv, sizeof(double));
+    double *tv = (double *)calloc(nv, sizeof(double));
+    double *st = (double *)calloc(4 * nv, sizeof(double)); /* bx, by, ex, ey */
+    if (!R || !ones_v || !r || !bp || !tv || !st) {
+        free(R); free(ones_v); free(r); free(bp); free(tv); free(st);
+        return;
+    }
+    double *bx = st, *by = st + nv, *ex = st + 2 * nv, *ey = st + 3 * nv;
+    for (size_t i = 0; i < nv; ++i) ones_v[i] = 1.0;
+    orc64_fp(g, ones_v, R);
+    for (size_t i = 0; i < ns; ++i) R[i] = (R[i] != 0 && R[i] >= r_min) ? 1.0 / R[i] : 0.0;
+    const double P = 1.0 / (mu * (double)g->n_angles + 8.0 * lambda);
+    for (size_t i = 0; i < nv; ++i) u[i] = 0.0;
+    for (uint32_t k = 0; k < outer; ++k) {
+        for (uint32_t j = 0; j < inner; ++j) {
+            orc64_fp(g, u, r);
+            for (size_t i = 0; i < ns; ++i) r[i] = (f[i] - r[i]) * R[i];
+            orc64_bp(g, bp, r);
+            orc64_sb_tv(u, ex, ey, tv, n);
+            for (size_t i = 0; i < nv; ++i) {
+                u[i] += relax * P * (mu * bp[i] + lambda * tv[i]);
+                if (posc && u[i] < 0) u[i] = 0;
+            }
+        }
+        if (k + 1 < outer)
+            orc64_sb_shrink(u, bx, by, ex, ey, n, lambda);
+    }
+    free(R); free(ones_v); free(r); free(bp); free(tv); free(st);
 }

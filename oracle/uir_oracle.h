@@ -124,8 +124,18 @@ void orc_sbtv_shrink(const float *tx, const float *ty, size_t n, float lambda, f
 /* ---- FP64 evaluation of the same operators (exact coordinates) ---- */
 void orc64_fp(const orc_geom *g, const double *vol, double *sino);
 void orc64_bp(const orc_geom *g, double *vol, const double *sino);
-void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc);
+/* r_min: rays with FP(1) < r_min get weight 0 (0 = only FP(1) = 0 does) */
+void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc,
+                double r_min);
 void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax);
+
+/* split-Bregman TV with SIRT-preconditioned sweeps for the u-subproblem (FP64):
+ * outer Bregman iterations x inner sweeps, relaxation w, TV weight lambda,
+ * data weight mu; orc64_sb_tv / orc64_sb_shrink are its two TV steps */
+void orc64_sb_tv(const double *u, const double *ex, const double *ey, double *tv, int n);
+void orc64_sb_shrink(const double *u, double *bx, double *by, double *ex, double *ey, int n, double lambda);
+void orc64_sbsirt(const orc_geom *g, const double *f, double *u, uint32_t outer, uint32_t inner, double relax,
+                  double lambda, double mu, int posc, double r_min);
 
 int orc_num_threads(void);
 

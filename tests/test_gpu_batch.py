@@ -47,6 +47,28 @@ def test_batch_matches_fp64(res, storage, tol):
         assert err <= tol
 
 
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16", 1e-3)])
+def test_batch_noisy_with_ray_weight_floor_matches_fp64(res, storage, tol):
+    """1% noise: with ray_weight_min = 0.1 in both the batch and the oracle the
+    corner-clipping rays drop out and the FP32 / FP16 results stay at the
+    clean-data tolerance."""
+    iters = 20
+    specs = [dict(_spec(*gm), ray_weight_min=0.1) for gm in GEOMS]
+    bt = uir.Batch(res, specs, storage)
+    refs = []
+    for i, (n, d, a) in enumerate(GEOMS):
+        g, b = _data(n, d, a, 300 + i)
+        b = (b + np.random.default_rng(i).normal(0, 0.01 * b.max(), b.shape)).astype(np.float32)
+        bt.set_sinogram(i, b)
+        refs.append(O.sirt64(g, b, iters, 0.5, ray_weight_min=0.1))
+    bt.reset()
+    bt.iterate(iters)
+    for i in range(len(GEOMS)):
+        err = O.rel_l2(bt.image(i), refs[i])
+        print(storage, GEOMS[i], "noisy rel-L2 vs FP64", err)
+        assert err <= tol
+
+
 def test_batch_result_independent_of_mix(res):
     iters = 5
     (n, d, a) = GEOMS[0]

@@ -401,3 +401,113 @@ def test_cgs_matches_scipy_with_a_true_dot():
         assert np.linalg.norm(rhs - M @ got.ravel()) <= 2 * min(resid[:it + 1])
         if it == 3:
             assert int(np.argmin(errs)) == 3
+
+
+# ------------------- split-Bregman TV with SIRT sweeps (this build's method) --
+
+def test_sbsirt_without_tv_is_sirt():
+    """lambda = 0 removes the TV term: the sweeps are SIRT with relaxation w and
+    pixel weight 1/A (catches a wrong preconditioner or a missing R)."""
+    g = geom(24, 34, 30)
+    f = O.fp64(g, gaussian_blobs(24, [(-3.0, 2.0, 4.0, 1.0), (5.0, -4.0, 2.5, 0.6)]))
+    got = O.sbsirt64(g, f, 4, 3, relax=0.8, lam=0.0, mu=2.0)
+    want = O.sirt64(g, f, 12, relax=0.8)
+    assert O.rel_l2(got, want) < 1e-12
+
+
+def test_sb_tv_is_negative_gradient_of_coupling_term():
+    """tv = D^T (e - D u) is minus the gradient of 1/2 |e - D u|^2 with the
+    periodic backward differences of ufo-ir-gradient-processor.cl (catches a
+    transposed or sign-flipped difference)."""
+    rng = np.random.default_rng(11)
+    n = 6
+    u, ex, ey = (rng.standard_normal((n, n)) for _ in range(3))
+
+    def J(v):
+        dx = v - np.roll(v, 1, axis=1)
+        dy = v - np.roll(v, 1, axis=0)
+        return 0.5 * (np.sum((ex - dx) ** 2) + np.sum((ey - dy) ** 2))
+
+    num = np.zeros((n, n))
+    h = 1e-6
+    for k in range(n * n):
+        e = np.zeros(n * n)
+        e[k] = h
+        num.ravel()[k] = (J(u + e.reshape(n, n)) - J(u - e.reshape(n, n))) / (2 * h)
+    assert np.allclose(O.sb_tv64(u, ex, ey), -num, atol=1e-6)
+
+
+def test_sb_shrink_matches_reference_update_db():
+    """The shrink + Bregman update equals update_db's shrinkage
+    (ufo-ir-sbtv-task.c:360-402, the FP32 restatement pinned above) applied to
+    g = D u + b, with b <- g - d and e = d - b."""
+    rng = np.random.default_rng(12)
+    n = 16
+    u = rng.standard_normal((n, n))
+    bx, by = rng.standard_normal((n, n)) * 0.3, rng.standard_normal((n, n)) * 0.3
+    lam = 1.7
+    nbx, nby, ex, ey = O.sb_shrink64(u, bx, by, lam)
+    gx = u - np.roll(u, 1, axis=1) + bx
+    gy = u - np.roll(u, 1, axis=0) + by
+    dx, dy = O.sbtv_shrink(gx.astype(np.float32).ravel(), gy.astype(np.float32).ravel(), lam)
+    assert np.allclose(gx - nbx, dx.reshape(n, n), atol=1e-5)
+    assert np.allclose(gy - nby, dy.reshape(n, n), atol=1e-5)
+    assert np.allclose(ex, (gx - nbx) - nbx, atol=1e-12)
+
+
+def test_sbsirt_as_matrix_iteration():
+    """The whole loop from dense FP64 system matrices and periodic difference
+    matrices: inner sweeps u += w P (mu B R (f - A u) + lambda D^T (e - D u)),
+    then the shrink (catches a shrink after the last outer iteration, a
+    swapped mu / lambda, or a sweep count off by one)."""
+    n, d, a = 8, 12, 10
+    g = geom(n, d, a)
+    A = np.stack([O.fp64(g, np.eye(n * n)[k].reshape(n, n)).ravel() for k in range(n * n)], axis=1)
+    B = np.stack([O.bp64(g, np.eye(a * d)[k].reshape(a, d)).ravel() for k in range(a * d)], axis=1)
+    I = np.eye(n)
+    d1 = I - np.roll(I, 1, axis=0)   # (d1 v)[x] = v[x] - v[x-1], periodic
+    Dx = np.kron(I, d1)              # along x (fastest index)
+    Dy = np.kron(d1, I)              # along y
+    x_true = np.random.default_rng(13).random(n * n)
+    f = A @ x_true
+    R = 1 / np.where(A.sum(1) != 0, A.sum(1), np.inf)
+    w, lam, mu, outer, inner = 0.9, 0.3, 1.5, 3, 2
+    P = 1 / (mu * a + 8 * lam)
+    u = np.zeros(n * n)
+    bx = np.zeros(n * n)
+    by = np.zeros(n * n)
+    ex = np.zeros(n * n)
+    ey = np.zeros(n * n)
+    for k in range(outer):
+        for _ in range(inner):
+            tv = Dx.T @ (ex - Dx @ u) + Dy.T @ (ey - Dy @ u)
+            u = u + w * P * (mu * (B @ (R * (f - A @ u))) + lam * tv)
+        if k + 1 < outer:
+            gx, gy = Dx @ u + bx, Dy @ u + by
+            s = np.hypot(gx, gy)
+            t = np.maximum(s - 1 / lam, 0) / np.maximum(s, 1e-12)
+            bx, by = gx - t * gx, gy - t * gy
+            ex, ey = t * gx - bx, t * gy - by
+    got = O.sbsirt64(g, f.reshape(a, d), outer, inner, relax=w, lam=lam, mu=mu)
+    assert O.rel_l2(got.ravel(), u) < 1e-12
+
+
+def test_sirt64_ray_weight_floor_as_matrix_iteration():
+    """ray_weight_min drops exactly the rays whose FP(1) is below it (R = 0, as
+    for rays that miss the image); catches a floor applied to the pixel
+    weights or compared against R instead of FP(1)."""
+    n, d, a = 8, 13, 10
+    g = geom(n, d, a)
+    A = np.stack([O.fp64(g, np.eye(n * n)[k].reshape(n, n)).ravel() for k in range(n * n)], axis=1)
+    B = np.stack([O.bp64(g, np.eye(a * d)[k].reshape(a, d)).ravel() for k in range(a * d)], axis=1)
+    rs = A.sum(1)
+    tau = float(np.sort(rs[rs > 0])[5]) + 1e-9      # drops the six weakest rays
+    R = np.where((rs != 0) & (rs >= tau), 1 / np.where(rs != 0, rs, 1), 0)
+    C = 1 / B.sum(1)
+    f = np.random.default_rng(14).random(a * d)
+    x = np.zeros(n * n)
+    for _ in range(5):
+        x = x + C * (0.8 * (B @ (R * (f - A @ x))))
+    got = O.sirt64(g, f.reshape(a, d), 5, 0.8, ray_weight_min=tau)
+    assert O.rel_l2(got.ravel(), x) < 1e-12
+    assert O.rel_l2(got, O.sirt64(g, f.reshape(a, d), 5, 0.8)) > 1e-3

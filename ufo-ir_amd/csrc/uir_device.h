@@ -46,6 +46,8 @@ int uirk_bp_ref(const float *r_vol, float *w_vol, const float *sino, int n_angle
 /* Elementwise (ufo-ir-basic-ops.cl). */
 int uirk_set(float *out, size_t n, float v, void *stream);
 int uirk_inv(const float *in, float *out, size_t n, void *stream);
+/* out = (in != 0 && in >= vmin) ? 1 / in : 0 */
+int uirk_inv_min(const float *in, float *out, size_t n, float vmin, void *stream);
 int uirk_binary(int op, const float *a, const float *b, float m, float *out, size_t n, void *stream);
 int uirk_mul_rows(const float *a, const float *b, float *out, uint32_t width, uint32_t offset,
                   uint32_t rows, void *stream);
@@ -82,6 +84,11 @@ typedef struct {
     float axis, corr, relax, pix_scale;
     int32_t n_runs;
     uint32_t run_off[UIRK_MAX_RUNS], run_n[UIRK_MAX_RUNS];
+    /* fast BP only: x_out = x + relax * pix_scale * sum + tv_scale * tv
+     * (tv: FP32 N x N, NULL = no term) */
+    const float *tv;
+    float tv_scale;
+    int32_t pad_;
 } uirk_slice;
 
 /* FP work item: angles [a0, a0 + na) of one direction family (0 = horizontal,
@@ -149,6 +156,17 @@ void uirk_plan_xcd(const double *cost, int n, int n_xcd, uirk_xcd_part *out);
 double uirk_fp_item_cost(const uirk_fp_item *it);
 /* FP32 <-> storage conversions */
 int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream);
+
+/* Split-Bregman TV with SIRT inner sweeps, per N x N slice.  u is in the
+ * storage dtype; ex = dx - bx, ey = dy - by, bx, by and tv are FP32.
+ * sb_tv:     tv = Dx^T (ex - Dx u) + Dy^T (ey - Dy u)
+ * sb_shrink: g = D u + b, t = max(|g| - 1/lambda, 0) / max(|g|, 1e-12),
+ *            d = t g, b = g - d, e = d - b
+ * (the periodic differences of ufo-ir-gradient-processor.cl:1-51:
+ * (Dx u)[y][x] = u[y][x] - u[y][x-1], (Dx^T v)[y][x] = v[y][x] - v[y][x+1]). */
+int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream);
+int uirk_sb_shrink(const void *u, int dtype, float *bx, float *by, float *ex, float *ey, uint32_t n, float lambda,
+                   void *stream);
 
 /* Finite differences (ufo-ir-gradient-processor.cl) and l1_grad. */
 int uirk_diff(int kind, const float *in, float *out, uint32_t w, uint32_t h, void *stream);

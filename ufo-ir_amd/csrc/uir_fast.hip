@@ -593,6 +593,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
             const float *pix_w = uniform(S->pix_w);
             const int posc = S->posc;
             const float gain = S->relax * S->pix_scale;
+            const float *tv = REF ? nullptr : uniform(S->tv);
+            const float tv_scale = S->tv_scale;
 #pragma unroll
             for (int q = 0; q < kBpRows; ++q) {
                 const int py = item.y0 + wave + 4 * q;
@@ -608,6 +610,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                     }
                 } else {
                     out = __builtin_fmaf(acc[q], gain, gload(xin, o));
+                    if (tv) // split-Bregman TV term, evaluated at the same x
+                        out = __builtin_fmaf(tv_scale, gload(tv, o), out);
                 }
                 if (posc)
                     out = out > 0 ? out : 0.0f;
@@ -639,6 +643,54 @@ __global__ void k_convert_to_f32(const void *src, int dtype, float *dst, size_t 
         else
             dst[i] = ((const float *)src)[i];
     }
+}
+
+// Split-Bregman TV term of the u-subproblem, tv = Dx^T (ex - Dx u) + Dy^T (ey - Dy u),
+// with the periodic differences of ufo-ir-gradient-processor.cl:1-51
+// ((Dx u)[x] = u[x] - u[x-1], (Dx^T v)[x] = v[x] - v[x+1]).  One thread per
+// pixel, rows of 256 pixels per block; neighbours come through L2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sb_tv(const T *__restrict__ u, const float *__restrict__ ex,
+                                               const float *__restrict__ ey, float *__restrict__ tv, uint32_t n)
+{
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x, y = blockIdx.y;
+    if (x >= n)
+        return;
+    const uint32_t xl = x == 0 ? n - 1 : x - 1, xr = x + 1 == n ? 0 : x + 1;
+    const uint32_t yu = y == 0 ? n - 1 : y - 1, yd = y + 1 == n ? 0 : y + 1;
+    const uint32_t o = y * n + x;
+    const float c = gload(u, o);
+    // h(p) = e(p) - (u(p) - u(p - 1)) along each axis; tv = h(p) - h(p + 1)
+    const float hx0 = ex[o] - (c - gload(u, y * n + xl));
+    const float hx1 = ex[y * n + xr] - (gload(u, y * n + xr) - c);
+    const float hy0 = ey[o] - (c - gload(u, yu * n + x));
+    const float hy1 = ey[yd * n + x] - (gload(u, yd * n + x) - c);
+    tv[o] = (hx0 - hx1) + (hy0 - hy1);
+}
+
+// Split-Bregman shrinkage and Bregman update (ufo-ir-sbtv-task.c:360-402 form):
+// g = D u + b, t = max(|g| - 1/lambda, 0) / max(|g|, 1e-12), d = t g,
+// b <- g - d; stores b and e = d - b.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sb_shrink(const T *__restrict__ u, float *__restrict__ bx,
+                                                   float *__restrict__ by, float *__restrict__ ex,
+                                                   float *__restrict__ ey, uint32_t n, float inv_lambda)
+{
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x, y = blockIdx.y;
+    if (x >= n)
+        return;
+    const uint32_t xl = x == 0 ? n - 1 : x - 1, yu = y == 0 ? n - 1 : y - 1;
+    const uint32_t o = y * n + x;
+    const float c = gload(u, o);
+    const float gx = (c - gload(u, y * n + xl)) + bx[o];
+    const float gy = (c - gload(u, yu * n + x)) + by[o];
+    const float s = sqrtf(gx * gx + gy * gy);
+    const float t = fmaxf(s - inv_lambda, 0.0f) / fmaxf(s, 1e-12f);
+    const float dx = t * gx, dy = t * gy;
+    bx[o] = gx - dx;
+    by[o] = gy - dy;
+    ex[o] = dx - (gx - dx);
+    ey[o] = dy - (gy - dy);
 }
 
 template <typename T, bool REF>
@@ -719,6 +771,45 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         return launch_bp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     default: return launch_bp<float, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
+}
+
+int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)
+{
+    if (!n)
+        return 0;
+    const dim3 grid((n + 255) / 256, n);
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case UIR_DTYPE_F16:
+        hipLaunchKernelGGL(k_sb_tv<__half>, grid, dim3(256), 0, st, (const __half *)u, ex, ey, tv, n);
+        break;
+    case UIR_DTYPE_BF16:
+        hipLaunchKernelGGL(k_sb_tv<__hip_bfloat16>, grid, dim3(256), 0, st, (const __hip_bfloat16 *)u, ex, ey, tv, n);
+        break;
+    default: hipLaunchKernelGGL(k_sb_tv<float>, grid, dim3(256), 0, st, (const float *)u, ex, ey, tv, n);
+    }
+    return (int)hipGetLastError();
+}
+
+int uirk_sb_shrink(const void *u, int dtype, float *bx, float *by, float *ex, float *ey, uint32_t n, float lambda,
+                   void *stream)
+{
+    if (!n)
+        return 0;
+    const dim3 grid((n + 255) / 256, n);
+    hipStream_t st = (hipStream_t)stream;
+    const float il = 1.0f / lambda;
+    switch (dtype) {
+    case UIR_DTYPE_F16:
+        hipLaunchKernelGGL(k_sb_shrink<__half>, grid, dim3(256), 0, st, (const __half *)u, bx, by, ex, ey, n, il);
+        break;
+    case UIR_DTYPE_BF16:
+        hipLaunchKernelGGL(k_sb_shrink<__hip_bfloat16>, grid, dim3(256), 0, st, (const __hip_bfloat16 *)u, bx, by, ex,
+                           ey, n, il);
+        break;
+    default: hipLaunchKernelGGL(k_sb_shrink<float>, grid, dim3(256), 0, st, (const float *)u, bx, by, ex, ey, n, il);
+    }
+    return (int)hipGetLastError();
 }
 
 int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream)

@@ -215,6 +215,16 @@ __global__ void k_inv(const float *in, float *out, size_t n)
     }
 }
 
+// Ray weights with a floor: rays whose FP(1) is below vmin are treated as
+// missing the image (weight 0).
+__global__ void k_inv_min(const float *in, float *out, size_t n, float vmin)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = in[i];
+        out[i] = (v != 0 && v >= vmin) ? 1.0f / v : 0.0f;
+    }
+}
+
 __global__ void k_binary(int op, const float *a, const float *b, float m, float *out, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -487,6 +497,14 @@ int uirk_copy(float *dst, const float *src, size_t n, void *stream)
     if (!n)
         return 0;
     hipLaunchKernelGGL(k_copy, dim3(grid1d(n)), dim3(256), 0, (hipStream_t)stream, dst, src, n);
+    return (int)hipGetLastError();
+}
+
+int uirk_inv_min(const float *in, float *out, size_t n, float vmin, void *stream)
+{
+    if (!n)
+        return 0;
+    hipLaunchKernelGGL(k_inv_min, dim3(grid1d(n)), dim3(256), 0, (hipStream_t)stream, in, out, n, vmin);
     return (int)hipGetLastError();
 }
 

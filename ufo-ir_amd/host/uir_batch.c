@@ -23,6 +23,7 @@
 typedef struct {
     uint32_t n, D, A;
     float step, axis;        /* resolved axis */
+    float r_min;             /* ray weight floor */
     float *h_sin, *h_cos;
     float *d_trig, *d_ray_w; /* carved from the batch allocation */
     uir_subset *runs;
@@ -59,7 +60,7 @@ static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 static int same_geom(const uir_slice_spec *s, const geom_t *g)
 {
     return s->n == g->n && s->n_dets == g->D && s->n_angles == g->A && s->step == g->step &&
-           uir_geom_axis(s->axis_position, s->n_dets) == g->axis;
+           uir_geom_axis(s->axis_position, s->n_dets) == g->axis && s->ray_weight_min == g->r_min;
 }
 
 void uir_batch_free(uir_batch *b)
@@ -99,21 +100,25 @@ static uir_status upload(void **dst, const void *src, size_t bytes)
     return uir_check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
 }
 
-/* R = 1 / FP(1) for one geometry with the fast FP32 projector (inv(0) = 0). */
-static uir_status ray_weights(uir_batch *bt, geom_t *g)
+/* R = 1 / FP(1) for one geometry with the fast FP32 projector (inv(0) = 0),
+ * written to d_ray_w (A x D FP32); d_trig holds the geometry's (sin, cos)
+ * table.  Synchronises the stream. */
+uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uint32_t A, float axis,
+                           const float *h_sin, const float *h_cos, const uir_subset *runs, uint32_t n_runs,
+                           const float *d_trig, float r_min, float *d_ray_w)
 {
-    const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->A * g->D;
+    const size_t nv = (size_t)n * n, ns = (size_t)A * D;
     float *ones = NULL, *zeros = NULL;
     uirk_fp_item *d_items = NULL;
     uirk_fp_vec fv = {0};
     uir_status s = uir_dmalloc((void **)&ones, nv * 4);
     if (s == UIR_OK)
         s = uir_dmalloc((void **)&zeros, ns * 4);
-    if (s == UIR_OK && uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs))
+    if (s == UIR_OK && uirk_plan_fp(&fv, 0, (int)n, (int)D, axis, h_sin, h_cos, runs, n_runs))
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
     if (s == UIR_OK)
         s = upload((void **)&d_items, fv.v, fv.n * sizeof(uirk_fp_item));
-    hipStream_t st = bt->res->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (s == UIR_OK)
         s = uir_check_hip(uirk_set(ones, nv, 1.0f, st), "set");
     if (s == UIR_OK)
@@ -123,18 +128,18 @@ static uir_status ray_weights(uir_batch *bt, geom_t *g)
         memset(&sl, 0, sizeof sl);
         sl.x = ones;
         sl.b = zeros;
-        sl.r = g->d_ray_w;
-        sl.trig = g->d_trig;
-        sl.n = (int32_t)g->n;
-        sl.n_dets = (int32_t)g->D;
-        sl.n_angles = (int32_t)g->A;
-        sl.axis = g->axis;
+        sl.r = d_ray_w;
+        sl.trig = d_trig;
+        sl.n = (int32_t)n;
+        sl.n_dets = (int32_t)D;
+        sl.n_angles = (int32_t)A;
+        sl.axis = axis;
         sl.corr = 1.0f;
         s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32,
-                                        uirk_plan_grid((int)fv.n, 4), bt->d_err, st), "FP launch");
+                                        uirk_plan_grid((int)fv.n, 4), d_err, st), "FP launch");
     }
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_inv(g->d_ray_w, g->d_ray_w, ns, st), "inv");
+        s = uir_check_hip(uirk_inv_min(d_ray_w, d_ray_w, ns, r_min, st), "inv");
     if (s == UIR_OK)
         s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     uir_dfree(ones);
@@ -164,6 +169,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: relaxation outside [0, 1]", i);
         if (!(s->step >= 0.0f) || !(s->axis_position >= -1.0f))
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad step or axis", i);
+        if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad ray_weight_min", i);
     }
     TRY(uir_check_hip(hipSetDevice(res->device), "hipSetDevice"));
     uir_batch *b = (uir_batch *)calloc(1, sizeof *b);
@@ -201,6 +208,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             G->A = specs[i].n_angles;
             G->step = specs[i].step;
             G->axis = uir_geom_axis(specs[i].axis_position, specs[i].n_dets);
+            G->r_min = specs[i].ray_weight_min;
             G->h_sin = (float *)malloc(sizeof(float) * G->A);
             G->h_cos = (float *)malloc(sizeof(float) * G->A);
             G->runs = (uir_subset *)malloc(sizeof(uir_subset) * G->A);
@@ -222,6 +230,15 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     }
     for (uint32_t g = 0; g < b->n_geoms; ++g)
         total += align_up((size_t)b->geoms[g].A * b->geoms[g].D * 4) + align_up((size_t)b->geoms[g].A * 8);
+    {
+        size_t free_b = 0;
+        if ((s = uir_mem_get_info(&free_b, NULL)) != UIR_OK)
+            goto fail;
+        if (total > free_b) {
+            s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "batch needs %zu device bytes, %zu free", total, free_b);
+            goto fail;
+        }
+    }
     if ((s = uir_dmalloc(&b->mem, total)) != UIR_OK)
         goto fail;
     b->mem_bytes = total;
@@ -259,7 +276,9 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         uirk_plan_trig(G->h_sin, G->h_cos, G->A, trig);
         s = uir_check_hip(hipMemcpy(G->d_trig, trig, sizeof(float) * 2 * G->A, hipMemcpyHostToDevice), "hipMemcpy");
         free(trig);
-        if (s != UIR_OK || (s = ray_weights(b, G)) != UIR_OK)
+        if (s != UIR_OK ||
+            (s = uir_ray_weights(b->res->stream, b->d_err, G->n, G->D, G->A, G->axis, G->h_sin, G->h_cos, G->runs,
+                                 G->n_runs, G->d_trig, G->r_min, G->d_ray_w)) != UIR_OK)
             goto fail;
     }
 

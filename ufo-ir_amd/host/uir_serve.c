@@ -1,0 +1,1024 @@
+/*
+ * uir_serve.c -- reconstruction server with continuous batching.
+ *
+ * Clients submit requests (geometry, method, iteration counts, parameters and
+ * an FP32 sinogram) from any thread.  One worker thread owns the device and a
+ * blocking stream.  Its loop:
+ *   1. admit queued requests in arrival order while the device budget and the
+ *      slot limit allow (a request that does not fit waits at the head of the
+ *      queue, so large requests are not starved by small ones);
+ *   2. for each admitted request allocate its buffers (one allocation), upload
+ *      its sinogram into the storage dtype, and attach its geometry (LUTs,
+ *      (sin, cos) table, ray weights R = 1/FP(1) and work-table template),
+ *      shared with every other request of the same geometry;
+ *   3. when the set of admitted requests changed, rebuild the slice
+ *      descriptors and work tables (the geometry templates re-indexed by slot)
+ *      and their XCD partitions;
+ *   4. run one step: one FP launch and one BP launch over all admitted
+ *      requests (plus the split-Bregman TV kernels of SBTV_SIRT requests);
+ *   5. retire requests whose iterations are done: copy the image to the
+ *      host, release its memory, wake its waiter.
+ * At most two steps are in flight, so arrivals are seen within about two step
+ * times.  Every FP and BP work item writes only its own slice, so a request's
+ * result does not depend on what shares its steps.
+ *
+ * SBTV_SIRT is split-Bregman TV (the outer loop of ufo-ir-sbtv-task.c:
+ * b = mu A^T f + lambda D^T (d - b), u-subproblem, shrink, Bregman update)
+ * whose u-subproblem  min_u mu/2 |A u - f|^2_R + lambda/2 |d - b - D u|^2  is
+ * approximated by `inner_iterations` preconditioned gradient sweeps
+ *   u <- u + w P (mu A^T R (f - A u) + lambda D^T (d - b - D u)),
+ *   P = 1 / (mu A_n + 8 lambda),
+ * A_n = the angle count (= A^T R A 1 for the fast projector pair) and 8 the
+ * row-sum bound of D^T D, so P^-1 majorises the subproblem's Hessian and any
+ * relaxation w in (0, 1] converges.  u, d and b start at 0.
+ */
+#define _GNU_SOURCE
+#include <errno.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "uir_internal.h"
+
+enum { RQ_PENDING = 0, RQ_ACTIVE = 1, RQ_DONE = 2, RQ_FAILED = 3, RQ_CANCELLED = 4 };
+
+#define GEOM_CACHE_IDLE 16   /* unreferenced geometries kept for reuse */
+#define REQ_SLACK (1u << 20) /* per-request allowance for its work-table entries */
+
+typedef struct sgeom {
+    uint32_t n, D, A;
+    float step, axis, r_min;
+    float *h_sin, *h_cos;
+    uir_subset *runs;
+    uint32_t n_runs;
+    void *mem;          /* ray weights + trig */
+    float *d_ray_w, *d_trig;
+    uirk_fp_item *fp;   /* template items (slice 0) */
+    double *fp_cost;
+    size_t n_fp;
+    uirk_bp_item *bp;
+    size_t n_bp;
+    uint64_t bytes;
+    int refs;
+    double last_used;
+    struct sgeom *next;
+} sgeom;
+
+typedef struct sreq {
+    uint64_t ticket;
+    uir_request rq;
+    float *h_sino;
+    float *h_image;
+    int state;
+    uir_status status;
+    char msg[256];
+    uir_request_stats stats;
+    uint64_t bytes;
+    sgeom *g;
+    void *mem;
+    void *x, *b, *r;
+    float *bx, *by, *ex, *ey, *tv;
+    uint32_t done_steps, total_steps, inner;
+    struct sreq *next_pending;
+    struct sreq *next_all;
+} sreq;
+
+struct uir_server {
+    int device;
+    uir_dtype dtype;
+    size_t es;
+    uint32_t max_active;
+    uint64_t budget, used;
+    hipStream_t stream;
+    hipEvent_t ev[2];
+    int *d_err;
+    float *d_tmp;
+    size_t tmp_cap;
+
+    pthread_t thread;
+    int thread_started;
+    pthread_mutex_t mu;
+    pthread_cond_t cv_work, cv_done;
+    int stop;
+    uir_status fatal;
+    char fatal_msg[256];
+
+    sreq *pend_head, *pend_tail;
+    uint32_t n_pending;
+    sreq *all;
+    uint64_t next_ticket;
+    sreq **active;
+    uint32_t n_active, cap_active;
+    sgeom *geoms;
+
+    /* tables (worker thread only) */
+    uirk_slice *h_fp_sl, *h_bp_sl;
+    uirk_fp_item *h_fp;
+    uirk_bp_item *h_bp;
+    double *h_cost;
+    size_t hcap_fsl, hcap_bsl, hcap_fp, hcap_bp, hcap_cost;
+    uirk_slice *d_fp_sl, *d_bp_sl;
+    uirk_fp_item *d_fp;
+    uirk_bp_item *d_bp;
+    size_t dcap_fsl, dcap_bsl, dcap_fp, dcap_bp;
+    int n_fp, n_bp, fp_grid, bp_grid;
+    uirk_xcd_part fp_part, bp_part;
+
+    uint64_t steps, completed;
+    double busy_s, slice_its;
+};
+
+#define TRY(x) do { uir_status s_ = (x); if (s_ != UIR_OK) return s_; } while (0)
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+double uir_clock(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ---- requests ---- */
+
+static int is_sb(const uir_request *r) { return r->method == UIR_METHOD_SBTV_SIRT; }
+
+static uint64_t req_bytes(const uir_request *r, size_t es)
+{
+    const uint64_t nv = (uint64_t)r->slice.n * r->slice.n, ns = (uint64_t)r->slice.n_angles * r->slice.n_dets;
+    uint64_t b = align_up(nv * es) + 2 * align_up(ns * es);
+    if (is_sb(r))
+        b += 5 * align_up(nv * 4);
+    return b + REQ_SLACK;
+}
+
+static uint64_t geom_bytes(uint32_t D, uint32_t A) { return align_up((size_t)A * D * 4) + align_up((size_t)A * 8); }
+
+uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint64_t *bytes)
+{
+    if (!req || !bytes)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t es = storage == UIR_DTYPE_F32 ? 4 : 2;
+    *bytes = req_bytes(req, es) + geom_bytes(req->slice.n_dets, req->slice.n_angles);
+    return UIR_OK;
+}
+
+static uir_status check_request(const uir_request *r)
+{
+    const uir_slice_spec *s = &r->slice;
+    if (s->n == 0 || s->n_dets == 0 || s->n_angles == 0 || s->n > 16384 || s->n_dets > 32768 || s->n_angles > 65536)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad geometry %ux%u, %u angles", s->n, s->n_dets, s->n_angles);
+    if ((uint64_t)s->n * s->n * 4 >= (1ull << 32) || (uint64_t)s->n_angles * s->n_dets * 4 >= (1ull << 32))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "image or sinogram of 4 GiB or more");
+    if (!(s->step >= 0.0f) || !(s->axis_position >= -1.0f))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad step or axis");
+    if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad ray_weight_min");
+    if (r->iterations == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
+    if (r->method == UIR_METHOD_SIRT) {
+        if (!(s->relaxation >= 0.0f && s->relaxation <= 1.0f))
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "relaxation outside [0, 1]");
+        return UIR_OK;
+    }
+    if (r->method != UIR_METHOD_SBTV_SIRT)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown method %u", r->method);
+    if (!(s->relaxation > 0.0f && s->relaxation <= 1.0f))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "relaxation outside (0, 1]");
+    if (r->inner_iterations == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "inner_iterations must be >= 1");
+    if ((uint64_t)r->iterations * r->inner_iterations > 0xffffffffull)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "too many iterations");
+    if (!(r->lambda > 0.0f && isfinite(r->lambda)) || !(r->mu > 0.0f && isfinite(r->mu)))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "lambda and mu must be > 0");
+    return UIR_OK;
+}
+
+static void req_fail(sreq *q, uir_status st, const char *msg)
+{
+    q->state = st == UIR_ERR_CANCELLED ? RQ_CANCELLED : RQ_FAILED;
+    q->status = st;
+    snprintf(q->msg, sizeof q->msg, "%s", msg ? msg : "");
+}
+
+/* ---- geometries ---- */
+
+static void geom_free(sgeom *g)
+{
+    if (!g)
+        return;
+    uir_dfree(g->mem);
+    free(g->h_sin);
+    free(g->h_cos);
+    free(g->runs);
+    free(g->fp);
+    free(g->fp_cost);
+    free(g->bp);
+    free(g);
+}
+
+static sgeom *geom_find(uir_server *s, const uir_slice_spec *sp)
+{
+    const float axis = uir_geom_axis(sp->axis_position, sp->n_dets);
+    for (sgeom *g = s->geoms; g; g = g->next)
+        if (g->n == sp->n && g->D == sp->n_dets && g->A == sp->n_angles && g->step == sp->step && g->axis == axis &&
+            g->r_min == sp->ray_weight_min)
+            return g;
+    return NULL;
+}
+
+/* Creates the geometry of spec (worker thread; the caller has reserved its bytes). */
+static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **out)
+{
+    sgeom *g = (sgeom *)calloc(1, sizeof *g);
+    if (!g)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
+    g->n = sp->n;
+    g->D = sp->n_dets;
+    g->A = sp->n_angles;
+    g->step = sp->step;
+    g->axis = uir_geom_axis(sp->axis_position, sp->n_dets);
+    g->r_min = sp->ray_weight_min;
+    g->bytes = geom_bytes(g->D, g->A);
+    g->h_sin = (float *)malloc(sizeof(float) * g->A);
+    g->h_cos = (float *)malloc(sizeof(float) * g->A);
+    g->runs = (uir_subset *)malloc(sizeof(uir_subset) * g->A);
+    float *trig = (float *)malloc(sizeof(float) * 2 * g->A);
+    uir_status st = UIR_OK;
+    if (!g->h_sin || !g->h_cos || !g->runs || !trig) {
+        st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+        goto fail;
+    }
+    uir_geom_lut(g->A, g->step, g->h_sin, g->h_cos);
+    g->n_runs = uir_geom_runs(g->h_sin, g->h_cos, g->A, g->runs);
+    uirk_plan_trig(g->h_sin, g->h_cos, g->A, trig);
+    {
+        uirk_fp_vec fv = {0};
+        uirk_bp_vec bv = {0};
+        if (uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs) ||
+            uirk_plan_bp(&bv, 0, (int)g->n)) {
+            free(fv.v);
+            free(bv.v);
+            st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+            goto fail;
+        }
+        g->fp = fv.v;
+        g->n_fp = fv.n;
+        g->bp = bv.v;
+        g->n_bp = bv.n;
+        g->fp_cost = (double *)malloc(sizeof(double) * (g->n_fp ? g->n_fp : 1));
+        if (!g->fp_cost) {
+            st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+            goto fail;
+        }
+        for (size_t i = 0; i < g->n_fp; ++i)
+            g->fp_cost[i] = uirk_fp_item_cost(&g->fp[i]);
+    }
+    if ((st = uir_dmalloc(&g->mem, g->bytes)) != UIR_OK)
+        goto fail;
+    g->d_ray_w = (float *)g->mem;
+    g->d_trig = (float *)((char *)g->mem + align_up((size_t)g->A * g->D * 4));
+    if ((st = uir_check_hip(hipMemcpyAsync(g->d_trig, trig, sizeof(float) * 2 * g->A, hipMemcpyHostToDevice,
+                                           s->stream), "hipMemcpyAsync")) != UIR_OK)
+        goto fail;
+    if ((st = uir_ray_weights(s->stream, s->d_err, g->n, g->D, g->A, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs,
+                              g->d_trig, g->r_min, g->d_ray_w)) != UIR_OK)
+        goto fail;
+    free(trig);
+    *out = g;
+    return UIR_OK;
+fail:
+    free(trig);
+    geom_free(g);
+    return st;
+}
+
+/* Frees the least recently used idle geometry; returns its bytes (0: none). */
+static uint64_t geom_evict_one(uir_server *s)
+{
+    sgeom **best = NULL;
+    for (sgeom **pp = &s->geoms; *pp; pp = &(*pp)->next)
+        if ((*pp)->refs == 0 && (!best || (*pp)->last_used < (*best)->last_used))
+            best = pp;
+    if (!best)
+        return 0;
+    sgeom *g = *best;
+    *best = g->next;
+    const uint64_t b = g->bytes;
+    geom_free(g);
+    return b;
+}
+
+static void geom_trim_idle(uir_server *s)
+{
+    for (;;) {
+        int idle = 0;
+        for (sgeom *g = s->geoms; g; g = g->next)
+            idle += g->refs == 0;
+        if (idle <= GEOM_CACHE_IDLE)
+            return;
+        pthread_mutex_lock(&s->mu);
+        s->used -= geom_evict_one(s);
+        pthread_mutex_unlock(&s->mu);
+    }
+}
+
+/* ---- worker: admission ---- */
+
+/* Bytes admitting q needs now (its buffers, plus its geometry unless resident). */
+static uint64_t admit_bytes(uir_server *s, const sreq *q)
+{
+    return q->bytes + (geom_find(s, &q->rq.slice) ? 0 : geom_bytes(q->rq.slice.n_dets, q->rq.slice.n_angles));
+}
+
+/* Under the lock: pops the requests that fit, in arrival order. */
+static uint32_t admit_locked(uir_server *s, sreq **out, uint32_t max_out)
+{
+    uint32_t k = 0;
+    while (s->pend_head && k < max_out) {
+        if (s->max_active && s->n_active + k >= s->max_active)
+            break;
+        sreq *q = s->pend_head;
+        uint64_t need = admit_bytes(s, q);
+        while (s->used + need > s->budget) {
+            const uint64_t freed = geom_evict_one(s);
+            if (!freed)
+                break;
+            s->used -= freed;
+            need = admit_bytes(s, q);
+        }
+        if (s->used + need > s->budget)
+            break;
+        s->used += need;
+        /* pin a resident geometry now, so no later eviction in this pass frees it */
+        sgeom *g = geom_find(s, &q->rq.slice);
+        if (g) {
+            g->refs++;
+            q->g = g;
+        }
+        s->pend_head = q->next_pending;
+        if (!s->pend_head)
+            s->pend_tail = NULL;
+        --s->n_pending;
+        q->next_pending = NULL;
+        q->state = RQ_ACTIVE;
+        q->stats.t_admit = uir_clock();
+        out[k++] = q;
+    }
+    return k;
+}
+
+static uir_status tmp_reserve(uir_server *s, size_t elems)
+{
+    if (elems <= s->tmp_cap)
+        return UIR_OK;
+    uir_dfree(s->d_tmp);
+    s->d_tmp = NULL;
+    s->tmp_cap = 0;
+    TRY(uir_dmalloc((void **)&s->d_tmp, elems * 4));
+    s->tmp_cap = elems;
+    return UIR_OK;
+}
+
+/* Worker thread: allocate, upload and attach the geometry of an admitted request. */
+static uir_status req_setup(uir_server *s, sreq *q)
+{
+    const uir_slice_spec *sp = &q->rq.slice;
+    const size_t nv = (size_t)sp->n * sp->n, ns = (size_t)sp->n_angles * sp->n_dets;
+    if (!q->g) {
+        sgeom *g = geom_find(s, sp); /* created by an earlier request of this admission pass */
+        if (!g) {
+            TRY(geom_create(s, sp, &g));
+            g->next = s->geoms;
+            s->geoms = g;
+        }
+        g->refs++;
+        q->g = g;
+    }
+    q->g->last_used = uir_clock();
+    size_t total = align_up(nv * s->es) + 2 * align_up(ns * s->es);
+    if (is_sb(&q->rq))
+        total += 5 * align_up(nv * 4);
+    TRY(uir_dmalloc(&q->mem, total));
+    char *p = (char *)q->mem;
+    q->x = p;
+    p += align_up(nv * s->es);
+    q->b = p;
+    p += align_up(ns * s->es);
+    q->r = p;
+    p += align_up(ns * s->es);
+    if (is_sb(&q->rq)) {
+        float **arr[5] = {&q->bx, &q->by, &q->ex, &q->ey, &q->tv};
+        for (int i = 0; i < 5; ++i) {
+            *arr[i] = (float *)p;
+            p += align_up(nv * 4);
+        }
+    }
+    /* x = 0 and the SB state = 0 (a fill kernel: no memset nodes on this stream) */
+    TRY(uir_check_hip(uirk_set((float *)q->mem, total / 4, 0.0f, s->stream), "set"));
+    TRY(tmp_reserve(s, ns > nv ? ns : nv));
+    TRY(uir_check_hip(hipMemcpyAsync(s->d_tmp, q->h_sino, ns * 4, hipMemcpyHostToDevice, s->stream), "upload"));
+    TRY(uir_check_hip(uirk_convert(s->d_tmp, UIR_DTYPE_F32, q->b, s->dtype, ns, s->stream), "convert"));
+    TRY(uir_check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize"));
+    free(q->h_sino);
+    q->h_sino = NULL;
+    q->total_steps = is_sb(&q->rq) ? q->rq.iterations * q->rq.inner_iterations : q->rq.iterations;
+    q->done_steps = 0;
+    q->inner = 0;
+    return UIR_OK;
+}
+
+/* Releases an active request's device memory and geometry reference. */
+static void req_release(uir_server *s, sreq *q)
+{
+    uir_dfree(q->mem);
+    q->mem = NULL;
+    if (q->g) {
+        q->g->refs--;
+        q->g->last_used = uir_clock();
+        q->g = NULL;
+    }
+    pthread_mutex_lock(&s->mu);
+    s->used -= q->bytes;
+    pthread_mutex_unlock(&s->mu);
+}
+
+/* ---- worker: tables ---- */
+
+static int grow(void **p, size_t *cap, size_t need, size_t es)
+{
+    if (need <= *cap)
+        return 0;
+    size_t c = *cap ? *cap : 64;
+    while (c < need)
+        c *= 2;
+    void *np = realloc(*p, c * es);
+    if (!np)
+        return -1;
+    *p = np;
+    *cap = c;
+    return 0;
+}
+
+static uir_status dgrow(void **p, size_t *cap, size_t need, size_t es)
+{
+    if (need <= *cap)
+        return UIR_OK;
+    size_t c = *cap ? *cap : 64;
+    while (c < need)
+        c *= 2;
+    uir_dfree(*p);
+    *p = NULL;
+    *cap = 0;
+    TRY(uir_dmalloc(p, c * es));
+    *cap = c;
+    return UIR_OK;
+}
+
+/* Rebuilds descriptors and work tables of the admitted set.  The stream is
+ * idle (the caller synchronised it), so the device tables can be rewritten. */
+static uir_status tables_build(uir_server *s)
+{
+    const uint32_t na = s->n_active;
+    size_t nfp = 0, nbp = 0;
+    for (uint32_t i = 0; i < na; ++i) {
+        nfp += s->active[i]->g->n_fp;
+        nbp += s->active[i]->g->n_bp;
+    }
+    const size_t ncost = nfp > nbp ? nfp : nbp;
+    if (grow((void **)&s->h_fp_sl, &s->hcap_fsl, na, sizeof(uirk_slice)) ||
+        grow((void **)&s->h_bp_sl, &s->hcap_bsl, na, sizeof(uirk_slice)) ||
+        grow((void **)&s->h_fp, &s->hcap_fp, nfp, sizeof(uirk_fp_item)) ||
+        grow((void **)&s->h_bp, &s->hcap_bp, nbp, sizeof(uirk_bp_item)) ||
+        grow((void **)&s->h_cost, &s->hcap_cost, ncost, sizeof(double)))
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+    nfp = nbp = 0;
+    for (uint32_t i = 0; i < na; ++i) {
+        const sreq *q = s->active[i];
+        const sgeom *g = q->g;
+        uirk_slice *f = &s->h_fp_sl[i], *k = &s->h_bp_sl[i];
+        memset(f, 0, sizeof *f);
+        f->x = q->x;
+        f->b = q->b;
+        f->r = q->r;
+        f->ray_w = g->d_ray_w;
+        f->trig = g->d_trig;
+        f->n = (int32_t)g->n;
+        f->n_dets = (int32_t)g->D;
+        f->n_angles = (int32_t)g->A;
+        f->axis = g->axis;
+        f->corr = -1.0f;
+        f->n_runs = 1;
+        f->run_off[0] = 0;
+        f->run_n[0] = g->A;
+        *k = *f;
+        k->x_out = q->x;
+        k->b = q->r;
+        k->r = NULL;
+        k->ray_w = NULL;
+        k->posc = q->rq.slice.positive_constraint ? 1 : 0;
+        k->relax = q->rq.slice.relaxation;
+        if (is_sb(&q->rq)) {
+            const float P = 1.0f / (q->rq.mu * (float)g->A + 8.0f * q->rq.lambda);
+            k->pix_scale = q->rq.mu * P;
+            k->tv = q->tv;
+            k->tv_scale = q->rq.slice.relaxation * q->rq.lambda * P;
+        } else {
+            k->pix_scale = 1.0f / (float)g->A;
+        }
+        for (size_t j = 0; j < g->n_fp; ++j) {
+            s->h_fp[nfp] = g->fp[j];
+            s->h_fp[nfp].slice = (int32_t)i;
+            ++nfp;
+        }
+        for (size_t j = 0; j < g->n_bp; ++j) {
+            s->h_bp[nbp] = g->bp[j];
+            s->h_bp[nbp].slice = (int32_t)i;
+            ++nbp;
+        }
+    }
+    {
+        size_t o = 0;
+        for (uint32_t i = 0; i < na; ++i)
+            for (size_t j = 0; j < s->active[i]->g->n_fp; ++j)
+                s->h_cost[o++] = s->active[i]->g->fp_cost[j];
+        uirk_plan_xcd(s->h_cost, (int)nfp, uirk_plan_num_xcd(), &s->fp_part);
+        o = 0;
+        for (uint32_t i = 0; i < na; ++i)
+            for (size_t j = 0; j < s->active[i]->g->n_bp; ++j)
+                s->h_cost[o++] = (double)s->active[i]->g->A;
+        uirk_plan_xcd(s->h_cost, (int)nbp, uirk_plan_num_xcd(), &s->bp_part);
+    }
+    TRY(dgrow((void **)&s->d_fp_sl, &s->dcap_fsl, na, sizeof(uirk_slice)));
+    TRY(dgrow((void **)&s->d_bp_sl, &s->dcap_bsl, na, sizeof(uirk_slice)));
+    TRY(dgrow((void **)&s->d_fp, &s->dcap_fp, nfp, sizeof(uirk_fp_item)));
+    TRY(dgrow((void **)&s->d_bp, &s->dcap_bp, nbp, sizeof(uirk_bp_item)));
+    TRY(uir_check_hip(hipMemcpy(s->d_fp_sl, s->h_fp_sl, na * sizeof(uirk_slice), hipMemcpyHostToDevice), "tables"));
+    TRY(uir_check_hip(hipMemcpy(s->d_bp_sl, s->h_bp_sl, na * sizeof(uirk_slice), hipMemcpyHostToDevice), "tables"));
+    TRY(uir_check_hip(hipMemcpy(s->d_fp, s->h_fp, nfp * sizeof(uirk_fp_item), hipMemcpyHostToDevice), "tables"));
+    TRY(uir_check_hip(hipMemcpy(s->d_bp, s->h_bp, nbp * sizeof(uirk_bp_item), hipMemcpyHostToDevice), "tables"));
+    s->n_fp = (int)nfp;
+    s->n_bp = (int)nbp;
+    s->fp_grid = uirk_plan_grid(s->n_fp, 4);
+    s->bp_grid = uirk_plan_grid(s->n_bp, 8);
+    return UIR_OK;
+}
+
+/* ---- worker: steps ---- */
+
+static uir_status step_launch(uir_server *s)
+{
+    for (uint32_t i = 0; i < s->n_active; ++i) {
+        sreq *q = s->active[i];
+        if (is_sb(&q->rq))
+            TRY(uir_check_hip(uirk_sb_tv(q->x, s->dtype, q->ex, q->ey, q->tv, q->rq.slice.n, s->stream), "sb_tv"));
+    }
+    TRY(uir_check_hip(uirk_fp_table(s->d_fp_sl, NULL, s->d_fp, s->n_fp, &s->fp_part, 1, s->dtype, s->fp_grid, s->d_err,
+                                    s->stream), "FP launch"));
+    TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, s->d_bp, s->n_bp, &s->bp_part, 1, s->dtype, s->bp_grid, s->d_err,
+                                    s->stream), "BP launch"));
+    for (uint32_t i = 0; i < s->n_active; ++i) {
+        sreq *q = s->active[i];
+        q->done_steps++;
+        q->stats.steps++;
+        if (s->n_active > q->stats.max_batch)
+            q->stats.max_batch = s->n_active;
+        if (is_sb(&q->rq) && ++q->inner == q->rq.inner_iterations) {
+            q->inner = 0;
+            /* the last outer iteration's Bregman update does not change u */
+            if (q->done_steps < q->total_steps)
+                TRY(uir_check_hip(uirk_sb_shrink(q->x, s->dtype, q->bx, q->by, q->ex, q->ey, q->rq.slice.n,
+                                                 q->rq.lambda, s->stream), "sb_shrink"));
+        }
+    }
+    return UIR_OK;
+}
+
+/* Copies a finished request's image to the host (synchronises the stream). */
+static uir_status req_finish(uir_server *s, sreq *q)
+{
+    const size_t nv = (size_t)q->rq.slice.n * q->rq.slice.n;
+    int err = 0;
+    TRY(uir_check_hip(hipMemcpyAsync(&err, s->d_err, sizeof err, hipMemcpyDeviceToHost, s->stream), "flag read"));
+    TRY(tmp_reserve(s, nv));
+    TRY(uir_check_hip(uirk_convert(q->x, s->dtype, s->d_tmp, UIR_DTYPE_F32, nv, s->stream), "convert"));
+    q->h_image = (float *)malloc(nv * 4);
+    if (!q->h_image)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+    TRY(uir_check_hip(hipMemcpyAsync(q->h_image, s->d_tmp, nv * 4, hipMemcpyDeviceToHost, s->stream), "download"));
+    TRY(uir_check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize"));
+    if (err) {
+        (void)hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream);
+        return uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window (flag %d)", err);
+    }
+    return UIR_OK;
+}
+
+static void fail_all_locked(uir_server *s, uir_status st, const char *msg)
+{
+    s->fatal = st;
+    snprintf(s->fatal_msg, sizeof s->fatal_msg, "%s", msg);
+    for (uint32_t i = 0; i < s->n_active; ++i)
+        req_fail(s->active[i], st, msg);
+    for (sreq *q = s->pend_head; q; q = q->next_pending)
+        req_fail(q, st, msg);
+    s->pend_head = s->pend_tail = NULL;
+    s->n_pending = 0;
+    pthread_cond_broadcast(&s->cv_done);
+}
+
+static void *worker(void *arg)
+{
+    uir_server *s = (uir_server *)arg;
+    (void)hipSetDevice(s->device);
+    sreq **adm = NULL;
+    uint32_t adm_cap = 0;
+    int in_flight = 0;   /* steps enqueued since the last synchronisation (<= 2) */
+    uint64_t ev_step = 0;
+    double t_busy0 = 0;
+
+    pthread_mutex_lock(&s->mu);
+    for (;;) {
+        while (!s->stop && !s->pend_head && s->n_active == 0)
+            pthread_cond_wait(&s->cv_work, &s->mu);
+        if (s->stop && s->n_active == 0 && !s->pend_head)
+            break;
+        if (s->fatal != UIR_OK) {
+            fail_all_locked(s, s->fatal, s->fatal_msg);
+            if (s->stop)
+                break;
+            pthread_cond_wait(&s->cv_work, &s->mu);
+            continue;
+        }
+        /* admission */
+        if (adm_cap < s->n_pending + 1) {
+            sreq **na = (sreq **)realloc(adm, sizeof(sreq *) * (s->n_pending + 16));
+            if (na) {
+                adm = na;
+                adm_cap = s->n_pending + 16;
+            }
+        }
+        uint32_t n_adm = adm ? admit_locked(s, adm, adm_cap) : 0;
+        if (n_adm && s->n_active + n_adm > s->cap_active) {
+            uint32_t c = s->cap_active ? s->cap_active : 16;
+            while (c < s->n_active + n_adm)
+                c *= 2;
+            sreq **na = (sreq **)realloc(s->active, sizeof(sreq *) * c);
+            if (!na) {
+                fail_all_locked(s, UIR_ERR_OUT_OF_MEMORY, "active list");
+                continue;
+            }
+            s->active = na;
+            s->cap_active = c;
+        }
+        pthread_mutex_unlock(&s->mu);
+
+        uir_status st = UIR_OK;
+        int changed = 0;
+        if (n_adm) {
+            if (in_flight) {
+                st = uir_check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+                in_flight = 0;
+            }
+            for (uint32_t i = 0; i < n_adm && st == UIR_OK; ++i) {
+                sreq *q = adm[i];
+                uir_status rs = req_setup(s, q);
+                if (rs != UIR_OK) {
+                    char msg[256];
+                    snprintf(msg, sizeof msg, "%s", uir_last_error());
+                    req_release(s, q);
+                    pthread_mutex_lock(&s->mu);
+                    /* its geometry bytes were reserved too if the geometry was missing */
+                    req_fail(q, rs, msg);
+                    pthread_cond_broadcast(&s->cv_done);
+                    pthread_mutex_unlock(&s->mu);
+                    if (rs == UIR_ERR_DEVICE)
+                        st = rs;
+                    continue;
+                }
+                if (s->n_active == 0)
+                    t_busy0 = uir_clock();
+                s->active[s->n_active++] = q;
+                changed = 1;
+            }
+            /* account geometry bytes exactly: used = requests + resident geometries + ... */
+            pthread_mutex_lock(&s->mu);
+            {
+                uint64_t u = 0;
+                for (uint32_t i = 0; i < s->n_active; ++i)
+                    u += s->active[i]->bytes;
+                for (sgeom *g = s->geoms; g; g = g->next)
+                    u += g->bytes;
+                s->used = u;
+            }
+            pthread_mutex_unlock(&s->mu);
+        }
+        if (st == UIR_OK && changed && s->n_active)
+            st = tables_build(s);
+
+        /* one step */
+        if (st == UIR_OK && s->n_active) {
+            if (in_flight == 2) {
+                st = uir_check_hip(hipEventSynchronize(s->ev[ev_step & 1]), "hipEventSynchronize");
+                in_flight = 1;
+            }
+            if (st == UIR_OK)
+                st = step_launch(s);
+            if (st == UIR_OK)
+                st = uir_check_hip(hipEventRecord(s->ev[ev_step & 1], s->stream), "hipEventRecord");
+            ++ev_step;
+            ++in_flight;
+            s->steps++;
+            s->slice_its += s->n_active;
+        }
+        /* retire */
+        if (st == UIR_OK) {
+            uint32_t keep = 0;
+            int retired = 0;
+            for (uint32_t i = 0; i < s->n_active; ++i) {
+                sreq *q = s->active[i];
+                if (q->done_steps < q->total_steps) {
+                    s->active[keep++] = q;
+                    continue;
+                }
+                uir_status rs = req_finish(s, q);
+                in_flight = 0;
+                char msg[256];
+                snprintf(msg, sizeof msg, "%s", rs == UIR_OK ? "" : uir_last_error());
+                req_release(s, q);
+                pthread_mutex_lock(&s->mu);
+                if (rs == UIR_OK) {
+                    q->state = RQ_DONE;
+                    q->status = UIR_OK;
+                    q->stats.t_done = uir_clock();
+                    s->completed++;
+                } else {
+                    req_fail(q, rs, msg);
+                }
+                pthread_cond_broadcast(&s->cv_done);
+                pthread_mutex_unlock(&s->mu);
+                retired = 1;
+                if (rs == UIR_ERR_DEVICE)
+                    st = rs;
+            }
+            if (retired) {
+                s->n_active = keep;
+                if (keep)
+                    st = st == UIR_OK ? tables_build(s) : st;
+                else
+                    s->busy_s += uir_clock() - t_busy0;
+                geom_trim_idle(s);
+            }
+        }
+        if (st != UIR_OK) {
+            /* release device memory first: a woken waiter frees its request */
+            char msg[256];
+            snprintf(msg, sizeof msg, "%s", uir_last_error());
+            (void)hipStreamSynchronize(s->stream);
+            in_flight = 0;
+            for (uint32_t i = 0; i < s->n_active; ++i)
+                req_release(s, s->active[i]);
+            pthread_mutex_lock(&s->mu);
+            fail_all_locked(s, st, msg);
+            s->n_active = 0;
+        } else {
+            pthread_mutex_lock(&s->mu);
+        }
+    }
+    pthread_mutex_unlock(&s->mu);
+    free(adm);
+    return NULL;
+}
+
+/* ---- public API ---- */
+
+uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server **out)
+{
+    if (!cfg || !out)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = NULL;
+    if (cfg->storage != UIR_DTYPE_F32 && cfg->storage != UIR_DTYPE_F16 && cfg->storage != UIR_DTYPE_BF16)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
+    int nd = 0;
+    TRY(uir_device_count(&nd));
+    if (device < 0 || device >= nd)
+        return uir_set_error(UIR_ERR_NO_DEVICE, "device %d of %d", device, nd);
+    TRY(uir_check_hip(hipSetDevice(device), "hipSetDevice"));
+    uint64_t budget = cfg->hbm_budget;
+    if (!budget) {
+        size_t f = 0;
+        TRY(uir_mem_get_info(&f, NULL));
+        if (f <= cfg->hbm_reserve)
+            return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "%zu bytes free, %llu reserved", f,
+                                 (unsigned long long)cfg->hbm_reserve);
+        budget = f - cfg->hbm_reserve;
+    }
+    uir_server *s = (uir_server *)calloc(1, sizeof *s);
+    if (!s)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
+    s->device = device;
+    s->dtype = cfg->storage;
+    s->es = cfg->storage == UIR_DTYPE_F32 ? 4 : 2;
+    s->max_active = cfg->max_active;
+    s->budget = budget;
+    s->next_ticket = 1;
+    s->fatal = UIR_OK;
+    pthread_mutex_init(&s->mu, NULL);
+    pthread_cond_init(&s->cv_work, NULL);
+    pthread_cond_init(&s->cv_done, NULL);
+    uir_status st = uir_check_hip(hipStreamCreate(&s->stream), "hipStreamCreate");
+    if (st == UIR_OK)
+        st = uir_check_hip(hipEventCreateWithFlags(&s->ev[0], hipEventDisableTiming), "hipEventCreate");
+    if (st == UIR_OK)
+        st = uir_check_hip(hipEventCreateWithFlags(&s->ev[1], hipEventDisableTiming), "hipEventCreate");
+    if (st == UIR_OK)
+        st = uir_dmalloc((void **)&s->d_err, sizeof(int));
+    if (st == UIR_OK)
+        st = uir_check_hip(hipMemset(s->d_err, 0, sizeof(int)), "hipMemset");
+    if (st == UIR_OK) {
+        if (pthread_create(&s->thread, NULL, worker, s) != 0)
+            st = uir_set_error(UIR_ERR_DEVICE, "pthread_create failed");
+        else
+            s->thread_started = 1;
+    }
+    if (st != UIR_OK) {
+        uir_server_free(s);
+        return st;
+    }
+    *out = s;
+    return UIR_OK;
+}
+
+uir_status uir_server_submit(uir_server *s, const uir_request *req, const float *sinogram, uint64_t *ticket)
+{
+    if (!s || !req || !sinogram || !ticket)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    TRY(check_request(req));
+    const uint64_t need = req_bytes(req, s->es) + geom_bytes(req->slice.n_dets, req->slice.n_angles);
+    if (need > s->budget)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "request needs %llu device bytes, the budget is %llu",
+                             (unsigned long long)need, (unsigned long long)s->budget);
+    const size_t ns = (size_t)req->slice.n_angles * req->slice.n_dets;
+    sreq *q = (sreq *)calloc(1, sizeof *q);
+    if (!q)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
+    q->h_sino = (float *)malloc(ns * 4);
+    if (!q->h_sino) {
+        free(q);
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+    }
+    memcpy(q->h_sino, sinogram, ns * 4);
+    q->rq = *req;
+    q->bytes = req_bytes(req, s->es);
+    q->stats.device_bytes = q->bytes;
+    q->stats.t_submit = uir_clock();
+    q->state = RQ_PENDING;
+    pthread_mutex_lock(&s->mu);
+    if (s->stop || s->fatal != UIR_OK) {
+        const uir_status fs = s->fatal != UIR_OK ? s->fatal : UIR_ERR_CANCELLED;
+        char msg[256];
+        snprintf(msg, sizeof msg, "%s", s->fatal != UIR_OK ? s->fatal_msg : "server is stopping");
+        pthread_mutex_unlock(&s->mu);
+        free(q->h_sino);
+        free(q);
+        return uir_set_error(fs, "%s", msg);
+    }
+    q->ticket = s->next_ticket++;
+    q->next_all = s->all;
+    s->all = q;
+    if (s->pend_tail)
+        s->pend_tail->next_pending = q;
+    else
+        s->pend_head = q;
+    s->pend_tail = q;
+    s->n_pending++;
+    *ticket = q->ticket;
+    pthread_cond_signal(&s->cv_work);
+    pthread_mutex_unlock(&s->mu);
+    return UIR_OK;
+}
+
+uir_status uir_server_wait(uir_server *s, uint64_t ticket, float *image, double timeout_s, uir_request_stats *stats)
+{
+    if (!s)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL server");
+    struct timespec dl;
+    if (timeout_s >= 0) {
+        clock_gettime(CLOCK_REALTIME, &dl);
+        double t = (double)dl.tv_sec + 1e-9 * dl.tv_nsec + timeout_s;
+        dl.tv_sec = (time_t)t;
+        dl.tv_nsec = (long)((t - (double)dl.tv_sec) * 1e9);
+    }
+    pthread_mutex_lock(&s->mu);
+    sreq **pp = &s->all;
+    while (*pp && (*pp)->ticket != ticket)
+        pp = &(*pp)->next_all;
+    if (!*pp) {
+        pthread_mutex_unlock(&s->mu);
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+    }
+    sreq *q = *pp;
+    while (q->state == RQ_PENDING || q->state == RQ_ACTIVE) {
+        int rc = timeout_s >= 0 ? pthread_cond_timedwait(&s->cv_done, &s->mu, &dl)
+                                : pthread_cond_wait(&s->cv_done, &s->mu);
+        if (rc == ETIMEDOUT && (q->state == RQ_PENDING || q->state == RQ_ACTIVE)) {
+            pthread_mutex_unlock(&s->mu);
+            return uir_set_error(UIR_ERR_TIMEOUT, "request %llu not done", (unsigned long long)ticket);
+        }
+    }
+    /* unlink: the ticket is released */
+    pp = &s->all;
+    while (*pp != q)
+        pp = &(*pp)->next_all;
+    *pp = q->next_all;
+    pthread_mutex_unlock(&s->mu);
+    uir_status st = q->status;
+    if (st == UIR_OK) {
+        if (image)
+            memcpy(image, q->h_image, (size_t)q->rq.slice.n * q->rq.slice.n * 4);
+    } else {
+        uir_set_error(st, "%s", q->msg);
+    }
+    if (stats)
+        *stats = q->stats;
+    free(q->h_image);
+    free(q->h_sino);
+    free(q);
+    return st;
+}
+
+uir_status uir_server_get_info(uir_server *s, uir_server_info *info)
+{
+    if (!s || !info)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    pthread_mutex_lock(&s->mu);
+    info->budget = s->budget;
+    info->used = s->used;
+    info->active = s->n_active;
+    info->pending = s->n_pending;
+    info->steps = s->steps;
+    info->completed = s->completed;
+    info->busy_s = s->busy_s;
+    info->slice_iterations = s->slice_its;
+    pthread_mutex_unlock(&s->mu);
+    return UIR_OK;
+}
+
+void uir_server_free(uir_server *s)
+{
+    if (!s)
+        return;
+    if (s->thread_started) {
+        pthread_mutex_lock(&s->mu);
+        for (sreq *q = s->pend_head; q; q = q->next_pending)
+            req_fail(q, UIR_ERR_CANCELLED, "server freed");
+        s->pend_head = s->pend_tail = NULL;
+        s->n_pending = 0;
+        s->stop = 1;
+        pthread_cond_broadcast(&s->cv_work);
+        pthread_cond_broadcast(&s->cv_done);
+        pthread_mutex_unlock(&s->mu);
+        pthread_join(s->thread, NULL);
+    }
+    if (s->stream)
+        (void)hipStreamSynchronize(s->stream);
+    for (uint32_t i = 0; i < s->n_active; ++i)
+        uir_dfree(s->active[i]->mem);
+    while (s->all) {
+        sreq *q = s->all;
+        s->all = q->next_all;
+        free(q->h_image);
+        free(q->h_sino);
+        free(q);
+    }
+    while (s->geoms) {
+        sgeom *g = s->geoms;
+        s->geoms = g->next;
+        geom_free(g);
+    }
+    free(s->active);
+    free(s->h_fp_sl);
+    free(s->h_bp_sl);
+    free(s->h_fp);
+    free(s->h_bp);
+    free(s->h_cost);
+    uir_dfree(s->d_fp_sl);
+    uir_dfree(s->d_bp_sl);
+    uir_dfree(s->d_fp);
+    uir_dfree(s->d_bp);
+    uir_dfree(s->d_tmp);
+    uir_dfree(s->d_err);
+    if (s->ev[0])
+        (void)hipEventDestroy(s->ev[0]);
+    if (s->ev[1])
+        (void)hipEventDestroy(s->ev[1]);
+    if (s->stream)
+        (void)hipStreamDestroy(s->stream);
+    pthread_mutex_destroy(&s->mu);
+    pthread_cond_destroy(&s->cv_work);
+    pthread_cond_destroy(&s->cv_done);
+    free(s);
+}

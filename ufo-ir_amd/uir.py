@@ -48,7 +48,29 @@ class GeometryDims(C.Structure):
 
 class SliceSpec(C.Structure):
     _fields_ = [("n", C.c_uint32), ("n_dets", C.c_uint32), ("n_angles", C.c_uint32), ("step", C.c_float),
-                ("axis_position", C.c_float), ("relaxation", C.c_float), ("positive_constraint", C.c_int32)]
+                ("axis_position", C.c_float), ("relaxation", C.c_float), ("positive_constraint", C.c_int32),
+                ("ray_weight_min", C.c_float)]
+
+
+class Request(C.Structure):
+    _fields_ = [("slice", SliceSpec), ("method", C.c_uint32), ("iterations", C.c_uint32),
+                ("inner_iterations", C.c_uint32), ("lambda_", C.c_float), ("mu", C.c_float)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("storage", C.c_int), ("max_active", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("hbm_reserve", C.c_uint64)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("device_bytes", C.c_uint64), ("steps", C.c_uint32), ("max_batch", C.c_uint32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("budget", C.c_uint64), ("used", C.c_uint64), ("active", C.c_uint32), ("pending", C.c_uint32),
+                ("steps", C.c_uint64), ("completed", C.c_uint64), ("busy_s", C.c_double),
+                ("slice_iterations", C.c_double)]
 
 
 class BatchInfo(C.Structure):
@@ -135,6 +157,15 @@ def load():
     L.uir_batch_get_image.argtypes = [vp, u32, vp, C.c_int]
     L.uir_batch_get_info.argtypes = [vp, C.POINTER(BatchInfo)]
     L.uir_batch_time_kernels.argtypes = [vp, u32, C.POINTER(f32), C.POINTER(f32)]
+    L.uir_clock.restype = C.c_double
+    L.uir_clock.argtypes = []
+    L.uir_server_new.argtypes = [C.c_int, C.POINTER(ServerConfig), C.POINTER(vp)]
+    L.uir_server_submit.argtypes = [vp, C.POINTER(Request), vp, C.POINTER(C.c_uint64)]
+    L.uir_server_wait.argtypes = [vp, C.c_uint64, vp, C.c_double, C.POINTER(RequestStats)]
+    L.uir_server_get_info.argtypes = [vp, C.POINTER(ServerInfo)]
+    L.uir_request_footprint.argtypes = [C.POINTER(Request), C.c_int, C.POINTER(C.c_uint64)]
+    L.uir_server_free.argtypes = [vp]
+    L.uir_server_free.restype = None
     L.uir_op_set.argtypes = [vp, sz, f32, vp]
     L.uir_op_inv.argtypes = [vp, vp, sz, vp]
     for n in ("uir_op_mul", "uir_op_add", "uir_op_deduction", "uir_op_max_ew", "uir_op_div_ew"):
@@ -438,7 +469,7 @@ class Batch:
         for i, s in enumerate(specs):
             arr[i] = SliceSpec(int(s["n"]), int(s["n_dets"]), int(s["n_angles"]), float(s["step"]),
                                float(s.get("axis_position", -1.0)), float(s.get("relaxation", 0.25)),
-                               int(bool(s.get("positive_constraint", False))))
+                               int(bool(s.get("positive_constraint", False))), float(s.get("ray_weight_min", 0.0)))
         h = C.c_void_p()
         check(load().uir_batch_new(res.handle, DTYPES[storage], arr, len(specs), C.byref(h)))
         self.handle = h
@@ -480,6 +511,87 @@ class Batch:
         if self.handle:
             load().uir_batch_free(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+METHODS = {"sirt": 0, "sbtv-sirt": 1}
+STATUS_TIMEOUT = 9
+
+
+def make_request(n, n_dets, n_angles, step, method="sirt", iterations=10, inner_iterations=1, relaxation=0.25,
+                 lam=0.1, mu=1.0, axis_position=-1.0, positive_constraint=False, ray_weight_min=0.0):
+    """A uir_request (method "sirt" or "sbtv-sirt")."""
+    return Request(SliceSpec(int(n), int(n_dets), int(n_angles), float(step), float(axis_position),
+                             float(relaxation), int(bool(positive_constraint)), float(ray_weight_min)),
+                   METHODS[method], int(iterations), int(inner_iterations), float(lam), float(mu))
+
+
+def request_footprint(req, storage="f16"):
+    v = C.c_uint64()
+    check(load().uir_request_footprint(C.byref(req), DTYPES[storage], C.byref(v)))
+    return v.value
+
+
+def clock():
+    return load().uir_clock()
+
+
+class Server:
+    """Continuous-batching reconstruction server (uir_server_*).
+
+    submit() copies the request and its FP32 sinogram and returns a ticket;
+    wait(ticket) returns (image, stats).  ctypes releases the GIL inside the
+    calls, so submitters and waiters may run in different Python threads."""
+
+    def __init__(self, device=0, storage="f16", max_active=0, hbm_budget=0, hbm_reserve=0):
+        cfg = ServerConfig(DTYPES[storage], int(max_active), int(hbm_budget), int(hbm_reserve))
+        h = C.c_void_p()
+        check(load().uir_server_new(int(device), C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self.sizes = {}
+
+    def submit(self, req, sinogram):
+        a = np.ascontiguousarray(sinogram, np.float32)
+        if a.shape != (req.slice.n_angles, req.slice.n_dets):
+            raise ValueError("sinogram shape %s, request wants %s" % (a.shape, (req.slice.n_angles, req.slice.n_dets)))
+        t = C.c_uint64()
+        check(load().uir_server_submit(self.handle, C.byref(req), a.ctypes.data, C.byref(t)))
+        self.sizes[t.value] = int(req.slice.n)
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """(image, stats) of a submitted request; raises UirError with the
+        request's status (TIMEOUT leaves the ticket valid)."""
+        n = self.sizes.get(ticket)
+        img = np.empty((n, n), np.float32) if n else None
+        st = RequestStats()
+        rc = load().uir_server_wait(self.handle, C.c_uint64(ticket), img.ctypes.data if n else None,
+                                    float(timeout), C.byref(st))
+        if rc != STATUS_TIMEOUT:
+            self.sizes.pop(ticket, None)
+        check(rc)
+        return img, {k: getattr(st, k) for k, _ in RequestStats._fields_}
+
+    def info(self):
+        inf = ServerInfo()
+        check(load().uir_server_get_info(self.handle, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in ServerInfo._fields_}
+
+    def close(self):
+        if self.handle:
+            load().uir_server_free(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:
