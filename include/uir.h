@@ -42,7 +42,8 @@ typedef enum {
     UIR_ERR_BUSY = 7,
     UIR_ERR_CANCELLED = 8,
     UIR_ERR_TIMEOUT = 9,
-    UIR_ERR_NO_DEVICE = 10
+    UIR_ERR_NO_DEVICE = 10,
+    UIR_ERR_IO = 11               /* file could not be written */
 } uir_status;
 
 typedef enum { UIR_DTYPE_F32 = 0, UIR_DTYPE_F16 = 1, UIR_DTYPE_BF16 = 2 } uir_dtype;
@@ -158,6 +159,11 @@ uir_status uir_task_set_uint(uir_task *task, const char *name, uint32_t value);
 uir_status uir_task_set_bool(uir_task *task, const char *name, int value);
 uir_status uir_task_set_string(uir_task *task, const char *name, const char *value);
 uir_status uir_task_set_object(uir_task *task, const char *name, uir_task *value);
+/* g_object_class_find_property: the value type of a property (the JSON
+ * loader uses it to convert numbers, as GValue transforms do). */
+typedef enum { UIR_PROP_FLOAT = 0, UIR_PROP_UINT = 1, UIR_PROP_BOOL = 2, UIR_PROP_STRING = 3,
+               UIR_PROP_OBJECT = 4 } uir_prop_type;
+uir_status uir_task_property_type(const uir_task *task, const char *name, uir_prop_type *type);
 uir_status uir_task_get_float(const uir_task *task, const char *name, float *value);
 uir_status uir_task_get_uint(const uir_task *task, const char *name, uint32_t *value);
 uir_status uir_task_get_bool(const uir_task *task, const char *name, int *value);
@@ -317,8 +323,41 @@ uir_status uir_server_get_info(uir_server *s, uir_server_info *info);
 /* Device bytes a request holds while admitted (its footprint for admission). */
 uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint64_t *bytes);
 /* Cancels queued requests (their waits return UIR_ERR_CANCELLED), finishes
- * the admitted ones, joins the worker and frees everything. */
+ * the admitted ones, joins the worker and frees everything.  No other thread
+ * may be inside uir_server_submit / uir_server_wait on this server meanwhile. */
 void       uir_server_free(uir_server *s);
+
+/* ------------------------------------------------------------------------ */
+/* Frame I/O and graph runner.  What ufo-core's "read" / "write" plugins and */
+/* its JSON graph loader do for an ufo-ir graph (json-sample.json:1-56): one  */
+/* 2-D FP32 frame per TIFF page, and a chain read -> ir tasks -> write.      */
+/* ------------------------------------------------------------------------ */
+
+/* Pages and the first page's size of a TIFF file. */
+uir_status uir_tiff_info(const char *path, uint32_t *n_pages, uint32_t *width, uint32_t *height);
+/* Page `page` as FP32 into host memory dst (capacity in floats; NULL dst:
+ * size only).  Uncompressed 8/16/32-bit integer or 32/64-bit float samples. */
+uir_status uir_tiff_read(const char *path, uint32_t page, float *dst, size_t capacity, uint32_t *width,
+                         uint32_t *height);
+/* Writes an FP32 page (append != 0: as a further page of an existing file). */
+uir_status uir_tiff_write(const char *path, const float *data, uint32_t width, uint32_t height, int append);
+
+typedef struct {
+    uint32_t frames;        /* frames read, processed and written */
+    uint32_t tasks;         /* ir tasks in the chain (nested ones not counted) */
+    double   seconds;       /* wall time of the frame loop */
+} uir_json_report;
+
+/* Runs a ufo-core JSON graph whose edges form one chain: a "read" node
+ * ("path": file, glob pattern or directory of TIFFs; optional "start",
+ * "number"), ufo-ir task nodes (package "ir"; object-valued properties with a
+ * "plugin" key become nested tasks, e.g. "projector", "df_minimizer"), and a
+ * "write" node ("filename" with one printf integer conversion for one file
+ * per frame, e.g. "out/out%03i.tif", or without one for a multi-page file).
+ * Relative paths are taken from base_dir (NULL: the working directory).
+ * Unknown properties are reported on stderr and skipped, as GLib does;
+ * out-of-range values fail the run. */
+uir_status uir_json_run(const char *json_text, const char *base_dir, int device, uir_json_report *report);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel-level ABI (the reference's inner boundary).  All pointers are      */

@@ -40,6 +40,7 @@ const char *uir_status_string(uir_status s)
     case UIR_ERR_CANCELLED: return "cancelled";
     case UIR_ERR_TIMEOUT: return "timeout";
     case UIR_ERR_NO_DEVICE: return "no device";
+    case UIR_ERR_IO: return "i/o error";
     }
     return "unknown status";
 }

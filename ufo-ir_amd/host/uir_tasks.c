@@ -20,7 +20,8 @@
 #include "uir_internal.h"
 
 typedef enum { K_PROJECTOR = 0, K_SIRT, K_SART, K_ASDPOCS, K_SBTV } task_kind;
-typedef enum { P_FLOAT, P_UINT, P_BOOL, P_STRING, P_OBJECT } prop_type;
+typedef enum { P_FLOAT = UIR_PROP_FLOAT, P_UINT = UIR_PROP_UINT, P_BOOL = UIR_PROP_BOOL, P_STRING = UIR_PROP_STRING,
+               P_OBJECT = UIR_PROP_OBJECT } prop_type;
 
 #define N_TMP 32
 
@@ -340,6 +341,20 @@ uir_status uir_task_set_object(uir_task *t, const char *name, uir_task *v)
     uir_task_unref(*slot);
     *slot = v;
     return UIR_OK;
+}
+
+uir_status uir_task_property_type(const uir_task *t, const char *name, uir_prop_type *type)
+{
+    if (!t || !name || !type)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    char c[64];
+    canon(name, c, sizeof c);
+    for (const prop_def *p = props_of(t->kind); p->name; ++p)
+        if (strcmp(p->name, c) == 0) {
+            *type = (uir_prop_type)p->type;
+            return UIR_OK;
+        }
+    return uir_set_error(UIR_ERR_NOT_FOUND, "'%s' has no property '%s'", g_plugin_names[t->kind], name);
 }
 
 uir_status uir_task_get_float(const uir_task *t, const char *name, float *v)

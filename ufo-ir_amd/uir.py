@@ -20,7 +20,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "uir.h")
 
 UIR_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE", 4: "UNSUPPORTED",
-                5: "SETUP", 6: "NOT_FOUND", 7: "BUSY", 8: "CANCELLED", 9: "TIMEOUT", 10: "NO_DEVICE"}
+                5: "SETUP", 6: "NOT_FOUND", 7: "BUSY", 8: "CANCELLED", 9: "TIMEOUT", 10: "NO_DEVICE", 11: "IO"}
 DTYPE_F32, DTYPE_F16, DTYPE_BF16 = 0, 1, 2
 
 
@@ -71,6 +71,10 @@ class ServerInfo(C.Structure):
     _fields_ = [("budget", C.c_uint64), ("used", C.c_uint64), ("active", C.c_uint32), ("pending", C.c_uint32),
                 ("steps", C.c_uint64), ("completed", C.c_uint64), ("busy_s", C.c_double),
                 ("slice_iterations", C.c_double)]
+
+
+class JsonReport(C.Structure):
+    _fields_ = [("frames", C.c_uint32), ("tasks", C.c_uint32), ("seconds", C.c_double)]
 
 
 class BatchInfo(C.Structure):
@@ -166,6 +170,11 @@ def load():
     L.uir_request_footprint.argtypes = [C.POINTER(Request), C.c_int, C.POINTER(C.c_uint64)]
     L.uir_server_free.argtypes = [vp]
     L.uir_server_free.restype = None
+    L.uir_task_property_type.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+    L.uir_tiff_info.argtypes = [C.c_char_p, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    L.uir_tiff_read.argtypes = [C.c_char_p, u32, vp, sz, C.POINTER(u32), C.POINTER(u32)]
+    L.uir_tiff_write.argtypes = [C.c_char_p, vp, u32, u32, C.c_int]
+    L.uir_json_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(JsonReport)]
     L.uir_op_set.argtypes = [vp, sz, f32, vp]
     L.uir_op_inv.argtypes = [vp, vp, sz, vp]
     for n in ("uir_op_mul", "uir_op_add", "uir_op_deduction", "uir_op_max_ew", "uir_op_div_ew"):
@@ -598,3 +607,41 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- frame I/O and the graph runner ----
+
+PROP_TYPES = {0: "float", 1: "uint", 2: "bool", 3: "string", 4: "object"}
+
+
+def tiff_info(path):
+    """(pages, width, height) of a TIFF file."""
+    n, w, h = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    check(load().uir_tiff_info(os.fsencode(path), C.byref(n), C.byref(w), C.byref(h)))
+    return n.value, w.value, h.value
+
+
+def tiff_read(path, page=0):
+    """Page `page` of a TIFF file as a float32 array [height][width]."""
+    w, h = C.c_uint32(), C.c_uint32()
+    check(load().uir_tiff_read(os.fsencode(path), int(page), None, 0, C.byref(w), C.byref(h)))
+    out = np.empty((h.value, w.value), np.float32)
+    check(load().uir_tiff_read(os.fsencode(path), int(page), out.ctypes.data, out.size, C.byref(w), C.byref(h)))
+    return out
+
+
+def tiff_write(path, image, append=False):
+    a = np.ascontiguousarray(image, np.float32)
+    if a.ndim != 2:
+        raise ValueError("a 2-D image is expected")
+    check(load().uir_tiff_write(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0], int(bool(append))))
+
+
+def json_run(graph, base_dir=None, device=0):
+    """Runs a ufo-core JSON graph (text or dict); returns the run report."""
+    import json as _json
+    text = graph if isinstance(graph, str) else _json.dumps(graph)
+    rep = JsonReport()
+    check(load().uir_json_run(text.encode(), os.fsencode(base_dir) if base_dir else None, int(device),
+                              C.byref(rep)))
+    return {k: getattr(rep, k) for k, _ in JsonReport._fields_}
