@@ -38,6 +38,31 @@ def test_fp_bp_joseph_per_subset(res, n, a):
     assert O.rel_l2(vol.numpy(), O.bp(g, ref, vol=x, relax=0.5)) <= 1e-5
 
 
+@pytest.mark.parametrize("n,d,a,axis", [(256, 256, 60, -1.0), (200, 283, 45, -1.0), (128, 128, 30, 70.0),
+                                        (1100, 1556, 6, -1.0)])
+def test_fp_joseph_single_angle_split_is_bit_identical(res, n, d, a, axis):
+    """Subsets of <= 4 angles take the split kernel (blocks of a ray over 16
+    threads, replayed float stepping); it must equal the one-thread-per-ray
+    kernel bit for bit, inactive detectors (shifted axis) included."""
+    step = np.float32(np.pi / a)
+    g = O.Geometry(n, d, a, step, axis)
+    x = ellipse_phantom(n, seed=4)
+    L = uir.load()
+    dx, dsin, dcos = _dev(x), _dev(g.sin[None, :]), _dev(g.cos[None, :])
+    r0 = np.random.default_rng(5).random((a, d)).astype(np.float32)
+    per_angle, per_run = _dev(r0), _dev(r0)
+    dims = uir.GeometryDims(n, n, d, a)
+    for (o, k, dr) in g.runs:
+        for ang in range(o, o + k):
+            uir.check(L.uir_fp_joseph(dx.ptr, per_angle.ptr, per_angle.ptr, dsin.ptr, dcos.ptr, dims, g.axis,
+                                      uir.Subset(ang, 1, dr), -1.0, None))
+        for o2 in range(o, o + k, 5):       # blocks of 5 angles take the one-thread-per-ray kernel
+            uir.check(L.uir_fp_joseph(dx.ptr, per_run.ptr, per_run.ptr, dsin.ptr, dcos.ptr, dims, g.axis,
+                                      uir.Subset(o2, min(5, o + k - o2), dr), -1.0, None))
+    assert np.array_equal(per_angle.numpy(), per_run.numpy())
+    assert O.rel_l2(per_angle.numpy(), O.fp(g, x, sino=r0, corr=-1.0)) <= 1e-5
+
+
 def test_elementwise_ops(res):
     rng = np.random.default_rng(0)
     a = rng.standard_normal((37, 53)).astype(np.float32)

@@ -34,6 +34,12 @@ int uirk_fp_ref(const float *vol, const float *r_sino, float *w_sino, const floa
                 const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
                 float corr, int ang0, int n_ang, int dir, void *stream);
 
+/* The same FP, bit-identical, with each ray's 64-sample blocks spread over 16
+ * threads (for launches over few rays, e.g. one angle); n <= 16384. */
+int uirk_fp_ref_split(const float *vol, const float *r_sino, float *w_sino, const float *ray_w,
+                      const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
+                      float corr, int ang0, int n_ang, int dir, void *stream);
+
 /* Reference-compatible BP (projector-parallel-joseph.cl:158-218) over a list of
  * subsets (runs), one per-run accumulation each: w = r + relax * sum_run.
  * pix_w == NULL: accumulate mode (r_vol may equal w_vol).

@@ -29,36 +29,22 @@ struct FpRefArgs {
     float axis, corr;
 };
 
-__global__ __launch_bounds__(256) void k_fp_ref(FpRefArgs p)
+// Per-ray set-up of projector-parallel-joseph.cl:51-75 / :112-136: returns
+// false for an inactive detector; else the start coordinate along the ray's
+// interpolation axis (a double expression rounded to float, :73 / :134), the
+// per-line step and the family (0 = FP_hor, 1 = FP_vert).
+__device__ __forceinline__ bool fp_ray_setup(int s, int ang, int N, int D, float axis, const float *sin_lut,
+                                             const float *cos_lut, int dir_in, float &pos, float &step, int &dir)
 {
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    const int k = blockIdx.y * 4 + threadIdx.y;
-    if (s >= p.n_dets || k >= p.n_ang)
-        return;
-    const int ang = p.ang0 + k;
-    const int N = p.n, D = p.n_dets;
-    const size_t o = (size_t)ang * D + s;
-
-    // projector-parallel-joseph.cl:51-66 (the reference's width here is n_dets)
-    const float required_width = p.axis * 2;
+    const float required_width = axis * 2;
     const float diff = (float)D - required_width;
-    if ((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width)) {
-        // Inactive detector: the reference kernel returns and the (aliased)
-        // sinogram keeps r; the SIRT form then multiplies by the ray weight.
-        if (p.ray_w)
-            p.w_sino[o] = p.r_sino[o] * p.ray_w[o];
-        else if (p.w_sino != p.r_sino)
-            p.w_sino[o] = p.r_sino[o];
-        return;
-    }
+    if ((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width))
+        return false;
     const float rot = diff / 2.0f;
     const float origin_shift = (float)N / 2.0f;
-    const float st = p.sin_lut[ang], ct = p.cos_lut[ang];
-    const int dir = p.dir >= 0 ? p.dir : (fabsf(st) <= fabsf(ct));
-
-    // Start coordinate: a double expression rounded to float (:73, :134).
+    const float st = sin_lut[ang], ct = cos_lut[ang];
+    dir = dir_in >= 0 ? dir_in : (fabsf(st) <= fabsf(ct));
     const double t = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
-    float pos, step;
     if (dir == 0) {
         const float det_step = -1.0f / st;
         step = ct / st;
@@ -68,50 +54,127 @@ __global__ __launch_bounds__(256) void k_fp_ref(FpRefArgs p)
         step = st / ct;
         pos = (float)(t * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N));
     }
+    return true;
+}
 
-    // Blocked sums of 64 samples (:81-90).  Samples past the image edge read
-    // the zero border, so the loop stops at N without changing the sum.
-    const float *img = p.vol;
-    float acc = 0.0f;
-    for (int b0 = 0; b0 < N; b0 += kBlockSamples) {
-        const int bend = min(N, b0 + kBlockSamples);
-        float inner = 0.0f;
-        if (dir == 0) {
-            // FP_hor: column i, linear in y (x = i + 0.5 gives zero x weight).
-            for (int i = b0; i < bend; ++i) {
-                const float v = pos - 0.5f;
-                const float fv = floorf(v);
-                const int r0 = (int)fv;
-                const float b = v - fv;
-                const float omb = 1.0f - b;
-                const float t0 = (r0 >= 0 && r0 < N) ? img[(size_t)r0 * N + i] : 0.0f;
-                const float t1 = (r0 + 1 >= 0 && r0 + 1 < N) ? img[(size_t)(r0 + 1) * N + i] : 0.0f;
-                inner += omb * t0 + b * t1;
-                pos += step;
-            }
-        } else {
-            // FP_vert: row i, linear in x.
-            const float *row_base = img;
-            for (int i = b0; i < bend; ++i) {
-                const float u = pos - 0.5f;
-                const float fu = floorf(u);
-                const int c0 = (int)fu;
-                const float a = u - fu;
-                const float oma = 1.0f - a;
-                const float *row = row_base + (size_t)i * N;
-                const float t0 = (c0 >= 0 && c0 < N) ? row[c0] : 0.0f;
-                const float t1 = (c0 + 1 >= 0 && c0 + 1 < N) ? row[c0 + 1] : 0.0f;
-                inner += oma * t0 + a * t1;
-                pos += step;
-            }
+// One block sum of :82-90: samples i in [b0, bend) of the ray, pos advanced
+// by step per sample (the incremental float stepping of the reference).
+// FP_hor walks column i linear in y; FP_vert walks row i linear in x.  Samples
+// past the image edge read the zero border and do not change the sum.
+__device__ __forceinline__ float fp_block_sum(const float *img, int N, int dir, int b0, int bend, float &pos, float step)
+{
+    float inner = 0.0f;
+    if (dir == 0) {
+        for (int i = b0; i < bend; ++i) {
+            const float v = pos - 0.5f;
+            const float fv = floorf(v);
+            const int r0 = (int)fv;
+            const float b = v - fv;
+            const float omb = 1.0f - b;
+            const float t0 = (r0 >= 0 && r0 < N) ? img[(size_t)r0 * N + i] : 0.0f;
+            const float t1 = (r0 + 1 >= 0 && r0 + 1 < N) ? img[(size_t)(r0 + 1) * N + i] : 0.0f;
+            inner += omb * t0 + b * t1;
+            pos += step;
         }
-        acc += inner;
+    } else {
+        for (int i = b0; i < bend; ++i) {
+            const float u = pos - 0.5f;
+            const float fu = floorf(u);
+            const int c0 = (int)fu;
+            const float a = u - fu;
+            const float oma = 1.0f - a;
+            const float *row = img + (size_t)i * N;
+            const float t0 = (c0 >= 0 && c0 < N) ? row[c0] : 0.0f;
+            const float t1 = (c0 + 1 >= 0 && c0 + 1 < N) ? row[c0 + 1] : 0.0f;
+            inner += oma * t0 + a * t1;
+            pos += step;
+        }
     }
-    const float r = p.r_sino[o];
-    float w = r + acc * p.corr; // :92-94
+    return inner;
+}
+
+// Inactive detector (:63-66): the reference kernel returns and the (aliased)
+// sinogram keeps r; the SIRT / SART form then multiplies by the ray weight.
+__device__ __forceinline__ void fp_inactive(const FpRefArgs &p, size_t o)
+{
+    if (p.ray_w)
+        p.w_sino[o] = p.r_sino[o] * p.ray_w[o];
+    else if (p.w_sino != p.r_sino)
+        p.w_sino[o] = p.r_sino[o];
+}
+
+__device__ __forceinline__ void fp_epilogue(const FpRefArgs &p, size_t o, float acc)
+{
+    float w = p.r_sino[o] + acc * p.corr; // :92-94
     if (p.ray_w)
         w = w * p.ray_w[o];
     p.w_sino[o] = w;
+}
+
+__global__ __launch_bounds__(256) void k_fp_ref(FpRefArgs p)
+{
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    const int k = blockIdx.y * 4 + threadIdx.y;
+    if (s >= p.n_dets || k >= p.n_ang)
+        return;
+    const int ang = p.ang0 + k;
+    const int N = p.n;
+    const size_t o = (size_t)ang * p.n_dets + s;
+    float pos, step;
+    int dir;
+    if (!fp_ray_setup(s, ang, N, p.n_dets, p.axis, p.sin_lut, p.cos_lut, p.dir, pos, step, dir)) {
+        fp_inactive(p, o);
+        return;
+    }
+    float acc = 0.0f;
+    for (int b0 = 0; b0 < N; b0 += kBlockSamples)
+        acc += fp_block_sum(p.vol, N, dir, b0, min(N, b0 + kBlockSamples), pos, step);
+    fp_epilogue(p, o, acc);
+}
+
+// The same projection with each ray split over 16 threads: thread slot q
+// takes blocks q, q + 16, ...; it reaches a block's start coordinate by
+// replaying the reference's float additions (no memory traffic), so every
+// sample sees the coordinate the sequential walk would see.  The block sums
+// meet in LDS and are added in block order, as the reference adds them.
+// Used where few rays are projected at once (one angle per launch in SART).
+constexpr int kSplitRays = 16, kSplitSlots = 16, kMaxBlocks = 16384 / kBlockSamples;
+
+__global__ __launch_bounds__(256) void k_fp_ref_split(FpRefArgs p)
+{
+    __shared__ float sums[kSplitRays][kMaxBlocks];
+    const int r = threadIdx.x % kSplitRays, q = threadIdx.x / kSplitRays;
+    const int s = blockIdx.x * kSplitRays + r;
+    const int ang = p.ang0 + blockIdx.y;
+    const int N = p.n;
+    const int n_blocks = (N + kBlockSamples - 1) / kBlockSamples;
+    float pos = 0.0f, step = 0.0f;
+    int dir = 0;
+    const bool live = s < p.n_dets &&
+                      fp_ray_setup(s, ang, N, p.n_dets, p.axis, p.sin_lut, p.cos_lut, p.dir, pos, step, dir);
+    if (live) {
+        int at = 0; // samples already stepped over
+        for (int j = q; j < n_blocks; j += kSplitSlots) {
+            const int b0 = j * kBlockSamples;
+            for (; at < b0; ++at)
+                pos += step;
+            const int bend = min(N, b0 + kBlockSamples);
+            sums[r][j] = fp_block_sum(p.vol, N, dir, b0, bend, pos, step);
+            at = bend;
+        }
+    }
+    __syncthreads();
+    if (q != 0 || s >= p.n_dets)
+        return;
+    const size_t o = (size_t)ang * p.n_dets + s;
+    if (!live) {
+        fp_inactive(p, o);
+        return;
+    }
+    float acc = 0.0f;
+    for (int j = 0; j < n_blocks; ++j)
+        acc += sums[r][j];
+    fp_epilogue(p, o, acc);
 }
 
 // --------------------------------------------------------------------------
@@ -456,6 +519,20 @@ int uirk_fp_ref(const float *vol, const float *r_sino, float *w_sino, const floa
     FpRefArgs a{vol, r_sino, w_sino, ray_w, sin_lut, cos_lut, n, n_dets, ang0, n_ang, dir, axis, corr};
     dim3 block(64, 4), grid((n_dets + 63) / 64, (n_ang + 3) / 4);
     hipLaunchKernelGGL(k_fp_ref, grid, block, 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+int uirk_fp_ref_split(const float *vol, const float *r_sino, float *w_sino, const float *ray_w,
+                      const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
+                      float corr, int ang0, int n_ang, int dir, void *stream)
+{
+    if (n_ang <= 0 || n_dets <= 0)
+        return 0;
+    if (n > 16384)
+        return (int)hipErrorInvalidValue;
+    FpRefArgs a{vol, r_sino, w_sino, ray_w, sin_lut, cos_lut, n, n_dets, ang0, n_ang, dir, axis, corr};
+    dim3 grid((n_dets + kSplitRays - 1) / kSplitRays, n_ang);
+    hipLaunchKernelGGL(k_fp_ref_split, grid, dim3(256), 0, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
 

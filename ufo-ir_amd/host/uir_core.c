@@ -457,6 +457,12 @@ uir_status uir_fp_joseph(const float *volume, const float *r_sino, float *w_sino
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL pointer");
     if (part.direction != 0 && part.direction != 1)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "direction must be 0 or 1");
+    /* a subset of a few angles is few rays: spread each ray over 16 threads
+     * (bit-identical to one thread per ray) */
+    if (part.n <= 4 && dims.width <= 16384)
+        return uir_check_hip(uirk_fp_ref_split(volume, r_sino, w_sino, NULL, sin_lut, cos_lut, (int)dims.width,
+                                               (int)dims.n_dets, axis_pos, correction_scale, (int)part.offset,
+                                               (int)part.n, part.direction, stream), "FP launch");
     return uir_check_hip(uirk_fp_ref(volume, r_sino, w_sino, NULL, sin_lut, cos_lut, (int)dims.width,
                                      (int)dims.n_dets, axis_pos, correction_scale, (int)part.offset,
                                      (int)part.n, part.direction, stream), "FP launch");

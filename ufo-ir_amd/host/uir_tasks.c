@@ -67,6 +67,15 @@ struct uir_task {
     /* cached device temporaries (ufo_buffer_dup in the reference) */
     void *tmp[N_TMP];
     size_t tmp_bytes[N_TMP];
+
+    /* one SART sweep captured as a hipGraph, valid for sart_key */
+    hipGraph_t sart_graph;
+    hipGraphExec_t sart_exec;
+    struct {
+        const void *x, *b, *tmp, *ray_w, *sin, *cos;
+        uint32_t n, D, A;
+        float axis, relax, corr;
+    } sart_key;
 };
 
 typedef struct {
@@ -229,6 +238,10 @@ void uir_task_unref(uir_task *t)
     if (t->res)
         (void)hipStreamSynchronize(t->res->stream);
     task_free_tmp(t);
+    if (t->sart_exec)
+        (void)hipGraphExecDestroy(t->sart_exec);
+    if (t->sart_graph)
+        (void)hipGraphDestroy(t->sart_graph);
     uir_resources_free(t->res);
     free(t->h_sin);
     free(t->h_cos);
@@ -862,6 +875,30 @@ static uir_status sirt_run(uir_task *t, const float *b, float *x, float relax, u
 
 /* SART process, ufo-ir-sart-task.c:173-228: per angle, FP of the row with the
  * ray weighting fused (subset_fp + op_mulRows), then BP of that row into x. */
+/* One SART sweep (ufo-ir-sart-task.c:208-220): per angle in order, FP of that
+ * angle with the residual and ray weights fused (w = (b - A_a x) R_a), then
+ * its BP x = x + relax A_a^T w.  The FP spreads each ray's 64-sample blocks
+ * over 16 threads (bit-identical to the one-thread-per-ray walk): one angle
+ * is only n_dets rays. */
+static uir_status sart_sweep(uir_task *t, mctx *c, const float *b, float *x, float *sino_tmp, const float *ray_w,
+                             float axis)
+{
+    HIPTRY(hipMemcpyAsync(sino_tmp, b, c->ns * 4, hipMemcpyDeviceToDevice, c->st), "copy");
+    for (uint32_t a = 0; a < c->A; ++a) {
+        HIPTRY(uirk_fp_ref_split(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
+                                 c->p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
+        uirk_runs r;
+        memset(&r, 0, sizeof r);
+        r.count = 1;
+        r.offset[0] = a;
+        r.n[0] = 1;
+        HIPTRY(uirk_bp_ref(x, x, sino_tmp, (int)c->A, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
+                           c->p->relaxation, &r, NULL, 0, c->st), "BP launch");
+    }
+    (void)t;
+    return UIR_OK;
+}
+
 static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, mctx *c)
 {
     float *sino_tmp, *vol_tmp, *ray_w;
@@ -877,20 +914,49 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
     c->p->relaxation = relax;
     HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
     const float axis = proj_axis(c->p);
-    for (uint32_t it = 0; it < iters; ++it) {                       /* :208-220 */
-        HIPTRY(hipMemcpyAsync(sino_tmp, b, c->ns * 4, hipMemcpyDeviceToDevice, c->st), "copy");
-        for (uint32_t a = 0; a < c->A; ++a) {
-            HIPTRY(uirk_fp_ref(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
-                               c->p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
-            uirk_runs r;
-            memset(&r, 0, sizeof r);
-            r.count = 1;
-            r.offset[0] = a;
-            r.n[0] = 1;
-            HIPTRY(uirk_bp_ref(x, x, sino_tmp, (int)c->A, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
-                               c->p->relaxation, &r, NULL, 0, c->st), "BP launch");
+    if (iters == 0)
+        return UIR_OK;
+    /* 2 A launches per sweep: capture one sweep and replay it (:208-220) */
+    const int same = t->sart_exec && t->sart_key.x == x && t->sart_key.b == b && t->sart_key.tmp == sino_tmp &&
+                     t->sart_key.ray_w == ray_w && t->sart_key.sin == c->p->d_sin && t->sart_key.cos == c->p->d_cos &&
+                     t->sart_key.n == c->n && t->sart_key.D == c->D && t->sart_key.A == c->A &&
+                     t->sart_key.axis == axis && t->sart_key.relax == c->p->relaxation &&
+                     t->sart_key.corr == c->p->correction_scale;
+    if (!same && iters == 1) /* a capture pays off from the second replay on */
+        return sart_sweep(t, c, b, x, sino_tmp, ray_w, axis);
+    if (!same) {
+        if (t->sart_exec)
+            (void)hipGraphExecDestroy(t->sart_exec);
+        if (t->sart_graph)
+            (void)hipGraphDestroy(t->sart_graph);
+        t->sart_exec = NULL;
+        t->sart_graph = NULL;
+        HIPTRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        uir_status s = sart_sweep(t, c, b, x, sino_tmp, ray_w, axis);
+        hipGraph_t g = NULL;
+        const int e = hipStreamEndCapture(c->st, &g);
+        if (s != UIR_OK || e != hipSuccess) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return s != UIR_OK ? s : uir_check_hip(e, "hipStreamEndCapture");
         }
+        t->sart_graph = g;
+        HIPTRY(hipGraphInstantiate(&t->sart_exec, g, NULL, NULL, 0), "hipGraphInstantiate");
+        t->sart_key.x = x;
+        t->sart_key.b = b;
+        t->sart_key.tmp = sino_tmp;
+        t->sart_key.ray_w = ray_w;
+        t->sart_key.sin = c->p->d_sin;
+        t->sart_key.cos = c->p->d_cos;
+        t->sart_key.n = c->n;
+        t->sart_key.D = c->D;
+        t->sart_key.A = c->A;
+        t->sart_key.axis = axis;
+        t->sart_key.relax = c->p->relaxation;
+        t->sart_key.corr = c->p->correction_scale;
     }
+    for (uint32_t it = 0; it < iters; ++it)
+        HIPTRY(hipGraphLaunch(t->sart_exec, c->st), "hipGraphLaunch");
     return UIR_OK;
 }
 
