@@ -61,33 +61,50 @@ __device__ __forceinline__ bool fp_ray_setup(int s, int ang, int N, int D, float
 // by step per sample (the incremental float stepping of the reference).
 // FP_hor walks column i linear in y; FP_vert walks row i linear in x.  Samples
 // past the image edge read the zero border and do not change the sum.
+// Samples are taken in batches of kFpBatch: the batch's coordinates follow
+// the same sequential float additions, all its taps are loaded (clamped,
+// always-valid addresses, zero where the reference reads the border), and
+// only then are the samples added to the block sum in order -- the
+// arithmetic and its order are the reference's, the loads overlap.
+constexpr int kFpBatch = 8;
+
 __device__ __forceinline__ float fp_block_sum(const float *img, int N, int dir, int b0, int bend, float &pos, float step)
 {
     float inner = 0.0f;
-    if (dir == 0) {
-        for (int i = b0; i < bend; ++i) {
+    for (int i0 = b0; i0 < bend; i0 += kFpBatch) {
+        float w[kFpBatch], va[kFpBatch], vb[kFpBatch];
+        int c0[kFpBatch];
+#pragma unroll
+        for (int k = 0; k < kFpBatch; ++k) {
+            const int i = min(i0 + k, bend - 1); // lanes past bend load a valid address, unused
             const float v = pos - 0.5f;
             const float fv = floorf(v);
-            const int r0 = (int)fv;
-            const float b = v - fv;
-            const float omb = 1.0f - b;
-            const float t0 = (r0 >= 0 && r0 < N) ? img[(size_t)r0 * N + i] : 0.0f;
-            const float t1 = (r0 + 1 >= 0 && r0 + 1 < N) ? img[(size_t)(r0 + 1) * N + i] : 0.0f;
-            inner += omb * t0 + b * t1;
-            pos += step;
+            c0[k] = (int)fv;
+            w[k] = v - fv;
+            const int ca = min(max(c0[k], 0), N - 1), cb = min(max(c0[k] + 1, 0), N - 1);
+            // FP_hor walks column i linear in y; FP_vert walks row i linear in x
+            const size_t oa = dir == 0 ? (size_t)ca * N + i : (size_t)i * N + ca;
+            const size_t ob = dir == 0 ? (size_t)cb * N + i : (size_t)i * N + cb;
+            va[k] = img[oa];
+            vb[k] = img[ob];
+            if (i0 + k < bend)
+                pos += step;
         }
-    } else {
-        for (int i = b0; i < bend; ++i) {
-            const float u = pos - 0.5f;
-            const float fu = floorf(u);
-            const int c0 = (int)fu;
-            const float a = u - fu;
-            const float oma = 1.0f - a;
-            const float *row = img + (size_t)i * N;
-            const float t0 = (c0 >= 0 && c0 < N) ? row[c0] : 0.0f;
-            const float t1 = (c0 + 1 >= 0 && c0 + 1 < N) ? row[c0 + 1] : 0.0f;
-            inner += oma * t0 + a * t1;
-            pos += step;
+        // An empty statement that takes the loaded registers: every load of the
+        // batch is issued before the first one is waited for (otherwise the
+        // compiler moves a load whose value is only used under the bounds test
+        // into a branch and waits on it there).
+#pragma unroll
+        for (int k = 0; k < kFpBatch; ++k)
+            asm volatile("" : "+v"(va[k]), "+v"(vb[k]));
+#pragma unroll
+        for (int k = 0; k < kFpBatch; ++k) {
+            if (i0 + k < bend) {
+                const float t0 = (c0[k] >= 0 && c0[k] < N) ? va[k] : 0.0f;
+                const float t1 = (c0[k] + 1 >= 0 && c0[k] + 1 < N) ? vb[k] : 0.0f;
+                const float omw = 1.0f - w[k];
+                inner += omw * t0 + w[k] * t1;
+            }
         }
     }
     return inner;
@@ -153,14 +170,16 @@ __global__ __launch_bounds__(256) void k_fp_ref_split(FpRefArgs p)
     const bool live = s < p.n_dets &&
                       fp_ray_setup(s, ang, N, p.n_dets, p.axis, p.sin_lut, p.cos_lut, p.dir, pos, step, dir);
     if (live) {
-        int at = 0; // samples already stepped over
+        int at = 0; // blocks already stepped over (every block before the last is whole)
         for (int j = q; j < n_blocks; j += kSplitSlots) {
+            for (; at < j; ++at) {
+#pragma unroll
+                for (int k = 0; k < kBlockSamples; ++k)
+                    pos += step;
+            }
             const int b0 = j * kBlockSamples;
-            for (; at < b0; ++at)
-                pos += step;
-            const int bend = min(N, b0 + kBlockSamples);
-            sums[r][j] = fp_block_sum(p.vol, N, dir, b0, bend, pos, step);
-            at = bend;
+            sums[r][j] = fp_block_sum(p.vol, N, dir, b0, min(N, b0 + kBlockSamples), pos, step);
+            at = j + 1;
         }
     }
     __syncthreads();
