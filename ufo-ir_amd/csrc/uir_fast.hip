@@ -236,46 +236,48 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                 atomicOr(err, 1);
             const bool outside = overflow || pmin + W <= 0 || pmin >= N;
             if (!outside) {
-                // Every thread issues all of its loads (from clamped, always
+                // Every thread issues a batch of loads (from clamped, always
                 // valid indices, no branches) before writing LDS; texels
                 // outside the image are then replaced by the zero border.
-                // Two halves of 16 batched loads per thread keep the register
-                // footprint within four waves per SIMD.
+                // Batches cover 64 window positions each, so only the
+                // positions of the footprint (W of up to 256) are loaded.
                 constexpr int kPer = kFpLines * kFpW / 256;
-                constexpr int kHalf = kPer / 2;
+                constexpr int kQ = kPer / 4;
                 if (hor) {
                     // line = column l0 + l, position = row pmin + p; coalesced along columns
                     const int l = threadIdx.x & 31;
                     const int col = min(l0 + l, N - 1);
-                    for (int h0 = 0; h0 < kPer; h0 += kHalf) {
-                        if (8 * h0 >= W)
+                    for (int q0 = 0; q0 < kPer; q0 += kQ) {
+                        if (8 * q0 >= W)
                             break;
-                        float v[kHalf];
+                        float v[kQ];
 #pragma unroll
-                        for (int e = 0; e < kHalf; ++e) {
-                            const int p = (threadIdx.x >> 5) + 8 * (h0 + e);
+                        for (int e = 0; e < kQ; ++e) {
+                            const int p = (threadIdx.x >> 5) + 8 * (q0 + e);
                             const int row = pmin + p;
                             const int rc = min(max(row, 0), N - 1);
                             const float t = gload(img, (uint32_t)rc * (uint32_t)N + (uint32_t)col);
                             v[e] = (row == rc && l < nl) ? t : 0.0f;
                         }
 #pragma unroll
-                        for (int e = 0; e < kHalf; ++e) {
-                            const int p = (threadIdx.x >> 5) + 8 * (h0 + e);
+                        for (int e = 0; e < kQ; ++e) {
+                            const int p = (threadIdx.x >> 5) + 8 * (q0 + e);
                             if (p < W)
                                 lds[l * kFpLs + p] = v[e];
                         }
                     }
                 } else {
-                    // line = row l0 + l, position = column pmin + p
-                    for (int h0 = 0; h0 < kPer; h0 += kHalf) {
-                        if (4 * (h0 >> 2) >= nl)
+                    // line = row l0 + l, position = column pmin + p: batch q0 holds
+                    // positions [64 (q0 / kQ), + 64) of all 32 lines
+                    for (int q0 = 0; q0 < kPer; q0 += kQ) {
+                        const int pb = q0 / kQ;
+                        if (64 * pb >= W)
                             break;
-                        float v[kHalf];
+                        float v[kQ];
 #pragma unroll
-                        for (int e = 0; e < kHalf; ++e) {
-                            const int l = (threadIdx.x >> 6) + 4 * ((h0 + e) >> 2);
-                            const int p = (threadIdx.x & 63) + 64 * ((h0 + e) & 3);
+                        for (int e = 0; e < kQ; ++e) {
+                            const int l = (threadIdx.x >> 6) + 4 * e;
+                            const int p = (threadIdx.x & 63) + 64 * pb;
                             const int col = pmin + p;
                             const int cc = min(max(col, 0), N - 1);
                             const int row = min(l0 + l, N - 1);
@@ -283,9 +285,9 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                             v[e] = (col == cc && l < nl) ? t : 0.0f;
                         }
 #pragma unroll
-                        for (int e = 0; e < kHalf; ++e) {
-                            const int l = (threadIdx.x >> 6) + 4 * ((h0 + e) >> 2);
-                            const int p = (threadIdx.x & 63) + 64 * ((h0 + e) & 3);
+                        for (int e = 0; e < kQ; ++e) {
+                            const int l = (threadIdx.x >> 6) + 4 * e;
+                            const int p = (threadIdx.x & 63) + 64 * pb;
                             if (p < W)
                                 lds[l * kFpLs + p] = v[e];
                         }
@@ -359,19 +361,38 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                     q[j] = on[j] ? pos[j] - off : park - off;
                     st[j] = on[j] ? stp[j] : 0.0f;
                 }
-                for (int l = 0; l < nl; ++l) {
-                    const float *L = lds + l * kFpLs;
-                    float t0[kFpApt], t1[kFpApt], a[kFpApt];
+                // Two lines per step: the LDS reads of 2 x kFpApt samples are
+                // in flight before the first is consumed.
+                int l = 0;
+                for (; l + 2 <= nl; l += 2) {
+                    const float *L0 = lds + l * kFpLs, *L1 = L0 + kFpLs;
+                    float t0[2][kFpApt], t1[2][kFpApt], a[2][kFpApt];
 #pragma unroll
                     for (int j = 0; j < kFpApt; ++j) {
-                        const int i0 = cvt_flr(q[j]);
-                        a[j] = fract_pos(q[j]);
-                        t0[j] = L[i0];
-                        t1[j] = L[i0 + 1];
+                        const float q1 = q[j] + st[j];
+                        const int i0 = cvt_flr(q[j]), i1 = cvt_flr(q1);
+                        a[0][j] = fract_pos(q[j]);
+                        a[1][j] = fract_pos(q1);
+                        t0[0][j] = L0[i0];
+                        t1[0][j] = L0[i0 + 1];
+                        t0[1][j] = L1[i1];
+                        t1[1][j] = L1[i1 + 1];
+                        q[j] = q1 + st[j];
                     }
 #pragma unroll
                     for (int j = 0; j < kFpApt; ++j) {
-                        acc[j] = __builtin_fmaf(a[j], t1[j] - t0[j], acc[j] + t0[j]);
+                        acc[j] = __builtin_fmaf(a[0][j], t1[0][j] - t0[0][j], acc[j] + t0[0][j]);
+                        acc[j] = __builtin_fmaf(a[1][j], t1[1][j] - t0[1][j], acc[j] + t0[1][j]);
+                    }
+                }
+                if (l < nl) {
+                    const float *L = lds + l * kFpLs;
+#pragma unroll
+                    for (int j = 0; j < kFpApt; ++j) {
+                        const int i0 = cvt_flr(q[j]);
+                        const float a = fract_pos(q[j]);
+                        const float t0 = L[i0], t1 = L[i0 + 1];
+                        acc[j] = __builtin_fmaf(a, t1 - t0, acc[j] + t0);
                         q[j] += st[j];
                     }
                 }

@@ -110,8 +110,9 @@ struct uir_server {
     uint32_t n_pending;
     sreq *all;
     uint64_t next_ticket;
-    sreq **active;
+    sreq **active;           /* worker thread only */
     uint32_t n_active, cap_active;
+    uint32_t n_live;          /* admitted and not finished (under mu) */
     sgeom *geoms;
 
     /* tables (worker thread only) */
@@ -366,6 +367,7 @@ static uint32_t admit_locked(uir_server *s, sreq **out, uint32_t max_out)
         q->next_pending = NULL;
         q->state = RQ_ACTIVE;
         q->stats.t_admit = uir_clock();
+        s->n_live++;
         out[k++] = q;
     }
     return k;
@@ -692,6 +694,7 @@ static void *worker(void *arg)
                     req_release(s, q);
                     pthread_mutex_lock(&s->mu);
                     /* its geometry bytes were reserved too if the geometry was missing */
+                    s->n_live--;
                     req_fail(q, rs, msg);
                     pthread_cond_broadcast(&s->cv_done);
                     pthread_mutex_unlock(&s->mu);
@@ -750,6 +753,7 @@ static void *worker(void *arg)
                 snprintf(msg, sizeof msg, "%s", rs == UIR_OK ? "" : uir_last_error());
                 req_release(s, q);
                 pthread_mutex_lock(&s->mu);
+                s->n_live--;
                 if (rs == UIR_OK) {
                     q->state = RQ_DONE;
                     q->status = UIR_OK;
@@ -784,6 +788,7 @@ static void *worker(void *arg)
             pthread_mutex_lock(&s->mu);
             fail_all_locked(s, st, msg);
             s->n_active = 0;
+            s->n_live = 0;
         } else {
             pthread_mutex_lock(&s->mu);
         }
@@ -957,7 +962,7 @@ uir_status uir_server_get_info(uir_server *s, uir_server_info *info)
     pthread_mutex_lock(&s->mu);
     info->budget = s->budget;
     info->used = s->used;
-    info->active = s->n_active;
+    info->active = s->n_live;
     info->pending = s->n_pending;
     info->steps = s->steps;
     info->completed = s->completed;
