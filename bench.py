@@ -248,7 +248,8 @@ def main():
     sirt.process(ds, out)
     plain_img = out.numpy()
 
-    # ---- batched path
+    # ---- batched path: tile shapes tuned for this geometry first
+    tune = uir.tune_geometry(res, dict(n=n, n_dets=d, n_angles=a, step=step), args.storage, reps=3)
     specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
     bt = uir.Batch(res, specs, args.storage)
     for i in range(args.batch):
@@ -358,7 +359,8 @@ def main():
         "data": "synthetic: 20 random ellipses per slice, Joseph FP, 1%% Gaussian noise (%d distinct)" % n_distinct,
         "config": {"workload": "batched SIRT, %d slices of %dx%d, %d angles over [0, pi), %d bins, relaxation %.2f"
                                % (args.batch, n, n, a, d, relax),
-                   "batch": args.batch, "image": n, "angles": a, "bins": d, "storage": args.storage},
+                   "batch": args.batch, "image": n, "angles": a, "bins": d, "storage": args.storage,
+                   "tile": tune["best"]},
         "plain_fp32": {"value": round(plain_rate, 2), "unit": "slice-iters/s", "ms_per_iteration": round(plain_ms, 4),
                        "path": "SIRT task, one slice per process(), reference-compatible FP32"},
         "speedup_vs_plain": round(value / plain_rate, 3),

@@ -254,9 +254,44 @@ uir_status uir_batch_iterate(uir_batch *batch, uint32_t iterations, int use_grap
 /* Slice `slice`'s image as FP32 into host or device memory (synchronises). */
 uir_status uir_batch_get_image(uir_batch *batch, uint32_t slice, float *dst, int on_device);
 uir_status uir_batch_get_info(const uir_batch *batch, uir_batch_info *info);
-/* Average device duration (ms) of one FP and one BP launch over `reps`
- * launches each, timed with events on the batch stream. */
+/* Average device time (ms) of one iteration's FP launches and of its BP
+ * launches (one launch per tile shape in use), over `reps` repetitions,
+ * timed with events on the batch stream. */
 uir_status uir_batch_time_kernels(uir_batch *batch, uint32_t reps, float *fp_ms, float *bp_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Tile-shape autotuner.  No counterpart in the reference (its work-group    */
+/* shapes are the OpenCL runtime's).  The fast projector pair has variants: */
+/* FP work items of 16 or 8 angles (x 64 detectors), BP angle batches of 32 */
+/* or 16 staged angles and tiles of 64 x 64 or 64 x 32 pixels.  Tuning times */
+/* each on a probe of the geometry and keeps the fastest per geometry and    */
+/* storage dtype; batches and the server then plan each geometry's work with */
+/* its tuned shape (the default 16 / 32 / 64 when untuned).  Variants apply  */
+/* the same arithmetic; only the anchors of the FP32 ray and pixel          */
+/* coordinates move with the tile, so results agree to FP32 rounding.       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int32_t fp_angles;  /* 16 or 8 */
+    int32_t bp_angles;  /* 32 or 16 */
+    int32_t bp_rows;    /* 64 or 32 */
+} uir_tile;
+
+typedef struct {
+    uir_tile best;
+    float    fp_ms[2];        /* fp_angles 16, 8 */
+    float    bp_ms[4];        /* (bp_angles, bp_rows) = (32, 64), (32, 32), (16, 64), (16, 32) */
+    uint32_t probe_slices;    /* slices of the geometry in the probe */
+} uir_tune_result;
+
+/* Times every variant (`reps` launches each, 0 = 3) on the resources' stream
+ * and records the fastest for this geometry and dtype. */
+uir_status uir_tune_geometry(uir_resources *res, const uir_slice_spec *spec, uir_dtype storage, uint32_t reps,
+                             uir_tune_result *out);
+/* The recorded shape (UIR_ERR_NOT_FOUND, with the default in *out, if none). */
+uir_status uir_tune_get(const uir_slice_spec *spec, uir_dtype storage, uir_tile *out);
+/* Records a shape without timing (e.g. from an earlier run). */
+uir_status uir_tune_set(const uir_slice_spec *spec, uir_dtype storage, const uir_tile *tile);
+void       uir_tune_clear(void);
 
 /* ------------------------------------------------------------------------ */
 /* Reconstruction server: continuous batching.  No counterpart in the        */
@@ -290,6 +325,7 @@ typedef struct {
     uint32_t  max_active;      /* most requests in one step (0 = no limit) */
     uint64_t  hbm_budget;      /* device bytes the server may hold (0 = free HBM at creation - hbm_reserve) */
     uint64_t  hbm_reserve;     /* bytes left free when hbm_budget = 0 */
+    int32_t   autotune;        /* tune each new geometry's tile shape on first use (adds a probe) */
 } uir_server_config;
 
 typedef struct {

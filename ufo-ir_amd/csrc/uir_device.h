@@ -127,6 +127,15 @@ typedef struct {
 #define UIRK_FP_WMAX 256    /* max staged footprint width */
 #define UIRK_BP_TILE 64     /* BP tile edge */
 
+/* Tile shapes of the fast path, chosen per geometry by the autotuner
+ * (host/uir_tune.c): FP angles per work item (16 or 8), BP angles per staged
+ * batch (32 or 16) and BP tile rows (64 or 32; tiles are 64 pixels wide).
+ * The reference-compatible mode takes the default {16, 32, 64} only. */
+typedef struct {
+    int32_t fp_angles, bp_angles, bp_rows;
+} uirk_tile;
+#define UIRK_TILE_DEFAULT {UIRK_FP_ANGLES, 32, UIRK_BP_TILE}
+
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
  * FP32 / FP16 / BF16 storage).  dtype: uir_dtype of the image / sinogram
  * buffers (the reference-compatible mode takes FP32 only).  d_slices: device
@@ -134,9 +143,11 @@ typedef struct {
  * passed by value as a kernel argument; item.slice must then be 0).
  * err: device int, set non-zero if a footprint exceeded the staging window. */
 int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream);
+                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
+                  void *stream);
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream);
+                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
+                  void *stream);
 
 /* Work-table construction (host/uir_plan.c). */
 typedef struct {
@@ -151,8 +162,9 @@ typedef struct {
  * angles); angle blocks are sized so that each footprint fits the staging
  * window.  Returns 0 on success, -1 on allocation failure. */
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
-                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs);
-int uirk_plan_bp(uirk_bp_vec *out, int slice, int n);
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles);
+/* tiles of 64 x tile_rows pixels */
+int uirk_plan_bp(uirk_bp_vec *out, int slice, int n, int tile_rows);
 void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig);
 int uirk_plan_grid(int n_items, int per_cu);
 int uirk_plan_num_xcd(void);

@@ -34,14 +34,10 @@
 
 namespace {
 
-constexpr int kFpAngles = UIRK_FP_ANGLES;
-constexpr int kFpApt = kFpAngles / 4;      // angles per thread (4 waves)
 constexpr int kFpLines = UIRK_FP_LINES;
 constexpr int kFpW = UIRK_FP_WMAX;
 constexpr int kFpLs = kFpW + 1;            // LDS line stride (odd: conflict-free transposed stores)
 constexpr int kBpT = UIRK_BP_TILE;
-constexpr int kBpRows = kBpT / 4;          // rows per thread
-constexpr int kBpNa = 32;                  // angles per staged batch
 constexpr int kBpWb = 100;                 // staged bins per angle (>= 64 * sqrt(2) + margin)
 
 // Loads and stores through global-address-space pointers: the buffers come
@@ -136,7 +132,7 @@ __device__ __forceinline__ float wave_max(float v)
 // ------------------------------------------------------------------------
 // Forward projection
 // ------------------------------------------------------------------------
-template <typename T, bool REF>
+template <typename T, bool REF, int APT>
 __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_fp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
@@ -149,6 +145,11 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 
     for (int it = range.first; it < range.end; it += range.stride) {
         const uirk_fp_item item = items[it];
+        if (item.na > 4 * APT) { // planned for a wider variant: report, never drop angles silently
+            if (threadIdx.x == 0)
+                atomicOr(err, 4);
+            continue;
+        }
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
         const T *img = uniform((const T *)S->x);
@@ -165,12 +166,12 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         const bool det_on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
         const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
 
-        float pos[kFpApt], stp[kFpApt], acc[kFpApt], inner[kFpApt];
-        double p0d[kFpApt];
-        bool on[kFpApt];
+        float pos[APT], stp[APT], acc[APT], inner[APT];
+        double p0d[APT];
+        bool on[APT];
 #pragma unroll
-        for (int j = 0; j < kFpApt; ++j) {
-            const int k = wave * kFpApt + j;
+        for (int j = 0; j < APT; ++j) {
+            const int k = wave * APT + j;
             on[j] = det_on && k < item.na;
             const int ang = item.a0 + (k < item.na ? k : 0);
             const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         if (REF) {
             for (int l = 0; l < item.l_lo; ++l) {
 #pragma unroll
-                for (int j = 0; j < kFpApt; ++j)
+                for (int j = 0; j < APT; ++j)
                     pos[j] += stp[j];
             }
         }
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             // Footprint of this block's rays over the chunk.
             float lo = 3.0e38f, hi = -3.0e38f;
 #pragma unroll
-            for (int j = 0; j < kFpApt; ++j) {
+            for (int j = 0; j < APT; ++j) {
                 if (!REF)
                     pos[j] = (float)(p0d[j] + (double)l0 * (double)stp[j]);
                 if (on[j]) {
@@ -301,17 +302,17 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             const float park = lo + 1.0f;
             if (REF) {
                 if (!outside) {
-                    float pp[kFpApt], se[kFpApt];
+                    float pp[APT], se[APT];
 #pragma unroll
-                    for (int j = 0; j < kFpApt; ++j) {
+                    for (int j = 0; j < APT; ++j) {
                         pp[j] = on[j] ? pos[j] : park;
                         se[j] = on[j] ? stp[j] : 0.0f;
                     }
                     for (int l = 0; l < nl; ++l) {
                         const float *L = lds + l * kFpLs - pmin;
-                        float t0[kFpApt], t1[kFpApt], b[kFpApt];
+                        float t0[APT], t1[APT], b[APT];
 #pragma unroll
-                        for (int j = 0; j < kFpApt; ++j) {
+                        for (int j = 0; j < APT; ++j) {
                             const float v = pp[j] - 0.5f;
                             const float fv = floorf(v);
                             const int i0 = (int)fv;
@@ -320,21 +321,21 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                             t1[j] = L[i0 + 1];
                         }
 #pragma unroll
-                        for (int j = 0; j < kFpApt; ++j) {
+                        for (int j = 0; j < APT; ++j) {
                             const float omb = 1.0f - b[j];
                             inner[j] += omb * t0[j] + b[j] * t1[j];
                             pp[j] += se[j];
                         }
                         if (((l0 + l + 1) & 63) == 0) { // 64-sample blocks (:82-90)
 #pragma unroll
-                            for (int j = 0; j < kFpApt; ++j) {
+                            for (int j = 0; j < APT; ++j) {
                                 acc[j] += inner[j];
                                 inner[j] = 0.0f;
                             }
                         }
                     }
 #pragma unroll
-                    for (int j = 0; j < kFpApt; ++j)
+                    for (int j = 0; j < APT; ++j)
                         if (on[j])
                             pos[j] = pp[j];
                 } else {
@@ -342,11 +343,11 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                     // float stepping of the coordinates continues
                     for (int l = 0; l < nl; ++l) {
 #pragma unroll
-                        for (int j = 0; j < kFpApt; ++j)
+                        for (int j = 0; j < APT; ++j)
                             pos[j] += stp[j];
                         if (((l0 + l + 1) & 63) == 0) {
 #pragma unroll
-                            for (int j = 0; j < kFpApt; ++j) {
+                            for (int j = 0; j < APT; ++j) {
                                 acc[j] += inner[j];
                                 inner[j] = 0.0f;
                             }
@@ -355,20 +356,20 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                 }
             } else if (!outside) {
                 const float off = 0.5f + (float)pmin;
-                float q[kFpApt], st[kFpApt];
+                float q[APT], st[APT];
 #pragma unroll
-                for (int j = 0; j < kFpApt; ++j) {
+                for (int j = 0; j < APT; ++j) {
                     q[j] = on[j] ? pos[j] - off : park - off;
                     st[j] = on[j] ? stp[j] : 0.0f;
                 }
-                // Two lines per step: the LDS reads of 2 x kFpApt samples are
+                // Two lines per step: the LDS reads of 2 x APT samples are
                 // in flight before the first is consumed.
                 int l = 0;
                 for (; l + 2 <= nl; l += 2) {
                     const float *L0 = lds + l * kFpLs, *L1 = L0 + kFpLs;
-                    float t0[2][kFpApt], t1[2][kFpApt], a[2][kFpApt];
+                    float t0[2][APT], t1[2][APT], a[2][APT];
 #pragma unroll
-                    for (int j = 0; j < kFpApt; ++j) {
+                    for (int j = 0; j < APT; ++j) {
                         const float q1 = q[j] + st[j];
                         const int i0 = cvt_flr(q[j]), i1 = cvt_flr(q1);
                         a[0][j] = fract_pos(q[j]);
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                         q[j] = q1 + st[j];
                     }
 #pragma unroll
-                    for (int j = 0; j < kFpApt; ++j) {
+                    for (int j = 0; j < APT; ++j) {
                         acc[j] = __builtin_fmaf(a[0][j], t1[0][j] - t0[0][j], acc[j] + t0[0][j]);
                         acc[j] = __builtin_fmaf(a[1][j], t1[1][j] - t0[1][j], acc[j] + t0[1][j]);
                     }
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                 if (l < nl) {
                     const float *L = lds + l * kFpLs;
 #pragma unroll
-                    for (int j = 0; j < kFpApt; ++j) {
+                    for (int j = 0; j < APT; ++j) {
                         const int i0 = cvt_flr(q[j]);
                         const float a = fract_pos(q[j]);
                         const float t0 = L[i0], t1 = L[i0 + 1];
@@ -404,8 +405,8 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         const float *ray_w = uniform(S->ray_w);
         const float corr = S->corr;
 #pragma unroll
-        for (int j = 0; j < kFpApt; ++j) {
-            const int k = wave * kFpApt + j;
+        for (int j = 0; j < APT; ++j) {
+            const int k = wave * APT + j;
             if (!in_range || k >= item.na)
                 continue;
             const int ang = item.a0 + k;
@@ -428,16 +429,16 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
-template <typename T, bool REF>
+template <typename T, bool REF, int NA, int ROWS>
 __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
 {
     // reference-compatible path: plain staged bins; fast path: (t, dt) pairs
-    __shared__ float lds[REF ? kBpNa * kBpWb : 1];
-    __shared__ float2 ldp[REF ? 1 : kBpNa * kBpWb];
-    __shared__ int s_umin[kBpNa];
-    __shared__ float s_trig[2 * kBpNa];
+    __shared__ float lds[REF ? NA * kBpWb : 1];
+    __shared__ float2 ldp[REF ? 1 : NA * kBpWb];
+    __shared__ int s_umin[NA];
+    __shared__ float s_trig[2 * NA];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const ItemRange range = item_range(part, n_items);
@@ -471,19 +472,19 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         const float cx0 = (float)item.x0 + 0.5f - origin_shift;
         const float cx1 = cx0 + (float)(kBpT - 1);
         const float cy0 = (float)item.y0 + 0.5f - origin_shift;
-        const float cy1 = cy0 + (float)(kBpT - 1);
+        const float cy1 = cy0 + (float)(4 * ROWS - 1);
         const float fY0 = (float)(item.y0 + wave) + 0.5f - origin_shift;
 
-        float acc[kBpRows], val[kBpRows];
+        float acc[ROWS], val[ROWS];
 #pragma unroll
-        for (int k = 0; k < kBpRows; ++k) {
+        for (int k = 0; k < ROWS; ++k) {
             acc[k] = 0.0f;
             val[k] = 0.0f;
         }
         if (REF && !S->pix_w) {
             const T *xin = uniform((const T *)S->x);
 #pragma unroll
-            for (int k = 0; k < kBpRows; ++k) {
+            for (int k = 0; k < ROWS; ++k) {
                 const int py = item.y0 + wave + 4 * k;
                 acc[k] = (px < N && py < N) ? gload(xin, (uint32_t)py * (uint32_t)N + (uint32_t)px) : 0.0f;
             }
@@ -491,8 +492,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
 
         for (int r = 0; r < n_runs; ++r) {
             const int roff = (int)S->run_off[r], rn = (int)S->run_n[r];
-            for (int b0 = 0; b0 < rn; b0 += kBpNa) {
-                const int na = min(kBpNa, rn - b0);
+            for (int b0 = 0; b0 < rn; b0 += NA) {
+                const int na = min(NA, rn - b0);
                 const int abase = roff + b0;
                 __syncthreads(); // previous batch's reads are done
                 if (threadIdx.x < na) {
@@ -517,7 +518,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                 {
                     // angle k = wave + 4 * (e / 2), bin i = lane + 64 * (e % 2);
                     // all loads first, then the LDS writes
-                    constexpr int kPer = (kBpNa / 4) * 2;
+                    constexpr int kPer = (NA / 4) * 2;
                     static_assert(kBpWb <= 128, "two bins per lane");
                     float v0[kPer], v1[kPer];
 #pragma unroll
@@ -558,7 +559,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                         const float *L = lds + k * kBpWb - s_umin[k];
                         const float fXc = fX * ct;
 #pragma unroll
-                        for (int g = 0; g < kBpRows; g += 4) {
+                        for (int g = 0; g < ROWS; g += 4) {
                             float t0[4], t1[4], a[4];
 #pragma unroll
                             for (int h = 0; h < 4; ++h) {
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                         float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f));
                         const float du = -4.0f * st;
 #pragma unroll
-                        for (int g = 0; g < kBpRows; g += 8) {
+                        for (int g = 0; g < ROWS; g += 8) {
                             float2 p[8];
                             float a[8];
 #pragma unroll
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
             }
             if (REF) { // per-run accumulation (:216)
 #pragma unroll
-                for (int q = 0; q < kBpRows; ++q) {
+                for (int q = 0; q < ROWS; ++q) {
                     acc[q] = acc[q] + relax * val[q];
                     val[q] = 0.0f;
                 }
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
             const float *tv = REF ? nullptr : uniform(S->tv);
             const float tv_scale = S->tv_scale;
 #pragma unroll
-            for (int q = 0; q < kBpRows; ++q) {
+            for (int q = 0; q < ROWS; ++q) {
                 const int py = item.y0 + wave + 4 * q;
                 if (py >= N)
                     continue;
@@ -714,20 +715,51 @@ __global__ __launch_bounds__(256) void k_sb_shrink(const T *__restrict__ u, floa
     ey[o] = dy - (gy - dy);
 }
 
-template <typename T, bool REF>
+template <typename T, bool REF, int APT>
 int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_fp_tab<T, REF, APT>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
-template <typename T, bool REF>
+template <typename T, bool REF, int NA, int ROWS>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, REF>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_bp_tab<T, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
+}
+
+// fast-path FP over the tile variants: 16 or 8 angles per item (4 or 2 per thread)
+template <typename T>
+int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items,
+                   int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+{
+    if (tile.fp_angles == 8)
+        return launch_fp<T, false, 2>(sl, one, items, n, part, grid, err, st);
+    return launch_fp<T, false, 4>(sl, one, items, n, part, grid, err, st);
+}
+
+// fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
+template <typename T>
+int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items,
+                   int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+{
+    const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
+    if (na16 && r8)
+        return launch_bp<T, false, 16, 8>(sl, one, items, n, part, grid, err, st);
+    if (na16)
+        return launch_bp<T, false, 16, 16>(sl, one, items, n, part, grid, err, st);
+    if (r8)
+        return launch_bp<T, false, 32, 8>(sl, one, items, n, part, grid, err, st);
+    return launch_bp<T, false, 32, 16>(sl, one, items, n, part, grid, err, st);
+}
+
+bool tile_ok(const uirk_tile *t)
+{
+    return (t->fp_angles == 16 || t->fp_angles == 8) && (t->bp_angles == 32 || t->bp_angles == 16) &&
+           (t->bp_rows == 64 || t->bp_rows == 32);
 }
 
 } // namespace
@@ -735,7 +767,8 @@ int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *i
 extern "C" {
 
 int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream)
+                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
+                  void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -743,6 +776,9 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         single = *one;
     }
+    const uirk_tile tl = tile ? *tile : uirk_tile{UIRK_FP_ANGLES, 32, UIRK_BP_TILE};
+    if (!tile_ok(&tl))
+        return (int)hipErrorInvalidValue;
     if (n_items <= 0)
         return 0;
     if (grid <= 0 || grid > n_items)
@@ -752,20 +788,21 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
-        if (dtype != UIR_DTYPE_F32)
+        if (dtype != UIR_DTYPE_F32 || tl.fp_angles != UIRK_FP_ANGLES)
             return (int)hipErrorInvalidValue;
-        return launch_fp<float, true>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_fp<float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
     switch (dtype) {
-    case UIR_DTYPE_F16: return launch_fp<__half, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    case UIR_DTYPE_F16: return launch_fp_fast<__half>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
     case UIR_DTYPE_BF16:
-        return launch_fp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    default: return launch_fp<float, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_fp_fast<__hip_bfloat16>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    default: return launch_fp_fast<float>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
 }
 
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, int grid, int *d_err, void *stream)
+                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
+                  void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -773,6 +810,9 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         single = *one;
     }
+    const uirk_tile tl = tile ? *tile : uirk_tile{UIRK_FP_ANGLES, 32, UIRK_BP_TILE};
+    if (!tile_ok(&tl))
+        return (int)hipErrorInvalidValue;
     if (n_items <= 0)
         return 0;
     if (grid <= 0 || grid > n_items)
@@ -782,15 +822,15 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
-        if (dtype != UIR_DTYPE_F32)
+        if (dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE)
             return (int)hipErrorInvalidValue;
-        return launch_bp<float, true>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_bp<float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
     switch (dtype) {
-    case UIR_DTYPE_F16: return launch_bp<__half, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    case UIR_DTYPE_F16: return launch_bp_fast<__half>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
     case UIR_DTYPE_BF16:
-        return launch_bp<__hip_bfloat16, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    default: return launch_bp<float, false>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_bp_fast<__hip_bfloat16>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    default: return launch_bp_fast<float>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
 }
 

@@ -24,11 +24,25 @@ typedef struct {
     uint32_t n, D, A;
     float step, axis;        /* resolved axis */
     float r_min;             /* ray weight floor */
+    uirk_tile tile;          /* tile shape (autotuned per geometry, or the default) */
     float *h_sin, *h_cos;
     float *d_trig, *d_ray_w; /* carved from the batch allocation */
     uir_subset *runs;
     uint32_t n_runs;
 } geom_t;
+
+#define UIR_FP_SHAPES 2
+#define UIR_BP_SHAPES 4
+
+typedef struct {
+    uirk_tile tile;
+    void *d_items;
+    int n_items, grid;
+    uirk_xcd_part part;
+} kgroup;
+
+static int fp_shape(const uirk_tile *t) { return t->fp_angles == 8; }
+static int bp_shape(const uirk_tile *t) { return (t->bp_angles == 16) * 2 + (t->bp_rows == 32); }
 
 struct uir_batch {
     uir_resources *res;
@@ -42,10 +56,9 @@ struct uir_batch {
     size_t mem_bytes;
     void **x, **b, **r;
     uirk_slice *d_fp_sl, *d_bp_sl;
-    uirk_fp_item *d_fp_items;
-    uirk_bp_item *d_bp_items;
-    int n_fp_items, n_bp_items, fp_grid, bp_grid;
-    uirk_xcd_part fp_part, bp_part;
+    /* one launch per tile-shape group: FP by angles per item (2 shapes), BP by
+     * staged angles x tile rows (4 shapes); each group's table is slice-major */
+    kgroup fpg[UIR_FP_SHAPES], bpg[UIR_BP_SHAPES];
     int *d_err;
     double fp_samples, bp_samples;
     hipGraph_t graph;
@@ -87,8 +100,10 @@ void uir_batch_free(uir_batch *b)
     uir_dfree(b->mem);
     uir_dfree(b->d_fp_sl);
     uir_dfree(b->d_bp_sl);
-    uir_dfree(b->d_fp_items);
-    uir_dfree(b->d_bp_items);
+    for (int g = 0; g < UIR_FP_SHAPES; ++g)
+        uir_dfree(b->fpg[g].d_items);
+    for (int g = 0; g < UIR_BP_SHAPES; ++g)
+        uir_dfree(b->bpg[g].d_items);
     uir_dfree(b->d_err);
     uir_resources_free(b->res);
     free(b);
@@ -114,7 +129,7 @@ uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uin
     uir_status s = uir_dmalloc((void **)&ones, nv * 4);
     if (s == UIR_OK)
         s = uir_dmalloc((void **)&zeros, ns * 4);
-    if (s == UIR_OK && uirk_plan_fp(&fv, 0, (int)n, (int)D, axis, h_sin, h_cos, runs, n_runs))
+    if (s == UIR_OK && uirk_plan_fp(&fv, 0, (int)n, (int)D, axis, h_sin, h_cos, runs, n_runs, UIRK_FP_ANGLES))
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
     if (s == UIR_OK)
         s = upload((void **)&d_items, fv.v, fv.n * sizeof(uirk_fp_item));
@@ -135,7 +150,7 @@ uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uin
         sl.n_angles = (int32_t)A;
         sl.axis = axis;
         sl.corr = 1.0f;
-        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32,
+        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32, NULL,
                                         uirk_plan_grid((int)fv.n, 4), d_err, st), "FP launch");
     }
     if (s == UIR_OK)
@@ -188,8 +203,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     b->r = (void **)calloc(n_slices, sizeof(void *));
     uir_status s = UIR_OK;
     uirk_slice *fp_sl = NULL, *bp_sl = NULL;
-    uirk_fp_vec fv = {0};
-    uirk_bp_vec bv = {0};
+    uirk_fp_vec fv[UIR_FP_SHAPES] = {{0}};
+    uirk_bp_vec bv[UIR_BP_SHAPES] = {{0}};
     if (!b->specs || !b->geom_of || !b->geoms || !b->x || !b->b || !b->r) {
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
         goto fail;
@@ -209,6 +224,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             G->step = specs[i].step;
             G->axis = uir_geom_axis(specs[i].axis_position, specs[i].n_dets);
             G->r_min = specs[i].ray_weight_min;
+            G->tile = uir_tile_for(&specs[i], storage);
             G->h_sin = (float *)malloc(sizeof(float) * G->A);
             G->h_cos = (float *)malloc(sizeof(float) * G->A);
             G->runs = (uir_subset *)malloc(sizeof(uir_subset) * G->A);
@@ -314,8 +330,9 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         k->n_runs = 1;
         k->run_off[0] = 0;
         k->run_n[0] = G->A;
-        if (uirk_plan_fp(&fv, (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos, G->runs, G->n_runs) ||
-            uirk_plan_bp(&bv, (int)i, (int)G->n)) {
+        if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
+                         G->runs, G->n_runs, G->tile.fp_angles) ||
+            uirk_plan_bp(&bv[bp_shape(&G->tile)], (int)i, (int)G->n, G->tile.bp_rows)) {
             s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
             goto fail;
         }
@@ -328,40 +345,63 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     /* Tables stay slice-major; each XCD takes a contiguous, cost-balanced range,
      * so its L2 holds the images and sinograms of a few slices only. */
     {
-        const size_t nmax = fv.n > bv.n ? fv.n : bv.n;
-        double *cost = (double *)malloc(sizeof(double) * (nmax ? nmax : 1));
+        size_t nmax = 1;
+        for (int g = 0; g < UIR_FP_SHAPES; ++g)
+            nmax = fv[g].n > nmax ? fv[g].n : nmax;
+        for (int g = 0; g < UIR_BP_SHAPES; ++g)
+            nmax = bv[g].n > nmax ? bv[g].n : nmax;
+        double *cost = (double *)malloc(sizeof(double) * nmax);
         if (!cost) {
             s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
             goto fail;
         }
-        for (size_t i = 0; i < fv.n; ++i)
-            cost[i] = uirk_fp_item_cost(&fv.v[i]);
-        uirk_plan_xcd(cost, (int)fv.n, uirk_plan_num_xcd(), &b->fp_part);
-        for (size_t i = 0; i < bv.n; ++i)
-            cost[i] = (double)specs[bv.v[i].slice].n_angles;
-        uirk_plan_xcd(cost, (int)bv.n, uirk_plan_num_xcd(), &b->bp_part);
+        for (int g = 0; g < UIR_FP_SHAPES && s == UIR_OK; ++g) {
+            kgroup *k = &b->fpg[g];
+            k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
+            k->tile.fp_angles = g ? 8 : UIRK_FP_ANGLES;
+            for (size_t i = 0; i < fv[g].n; ++i)
+                cost[i] = uirk_fp_item_cost(&fv[g].v[i]);
+            uirk_plan_xcd(cost, (int)fv[g].n, uirk_plan_num_xcd(), &k->part);
+            k->n_items = (int)fv[g].n;
+            k->grid = uirk_plan_grid(k->n_items, 4);
+            if (k->n_items)
+                s = upload(&k->d_items, fv[g].v, fv[g].n * sizeof(uirk_fp_item));
+        }
+        for (int g = 0; g < UIR_BP_SHAPES && s == UIR_OK; ++g) {
+            kgroup *k = &b->bpg[g];
+            k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
+            k->tile.bp_angles = g >= 2 ? 16 : 32;
+            k->tile.bp_rows = (g & 1) ? 32 : UIRK_BP_TILE;
+            for (size_t i = 0; i < bv[g].n; ++i)
+                cost[i] = (double)specs[bv[g].v[i].slice].n_angles;
+            uirk_plan_xcd(cost, (int)bv[g].n, uirk_plan_num_xcd(), &k->part);
+            k->n_items = (int)bv[g].n;
+            k->grid = uirk_plan_grid(k->n_items, 8);
+            if (k->n_items)
+                s = upload(&k->d_items, bv[g].v, bv[g].n * sizeof(uirk_bp_item));
+        }
         free(cost);
+        if (s != UIR_OK)
+            goto fail;
     }
     if ((s = upload((void **)&b->d_fp_sl, fp_sl, n_slices * sizeof *fp_sl)) != UIR_OK ||
-        (s = upload((void **)&b->d_bp_sl, bp_sl, n_slices * sizeof *bp_sl)) != UIR_OK ||
-        (s = upload((void **)&b->d_fp_items, fv.v, fv.n * sizeof(uirk_fp_item))) != UIR_OK ||
-        (s = upload((void **)&b->d_bp_items, bv.v, bv.n * sizeof(uirk_bp_item))) != UIR_OK)
+        (s = upload((void **)&b->d_bp_sl, bp_sl, n_slices * sizeof *bp_sl)) != UIR_OK)
         goto fail;
-    b->n_fp_items = (int)fv.n;
-    b->n_bp_items = (int)bv.n;
-    b->fp_grid = uirk_plan_grid(b->n_fp_items, 4);
-    b->bp_grid = uirk_plan_grid(b->n_bp_items, 8);
     free(fp_sl);
     free(bp_sl);
-    free(fv.v);
-    free(bv.v);
+    for (int g = 0; g < UIR_FP_SHAPES; ++g)
+        free(fv[g].v);
+    for (int g = 0; g < UIR_BP_SHAPES; ++g)
+        free(bv[g].v);
     *out = b;
     return UIR_OK;
 fail:
     free(fp_sl);
     free(bp_sl);
-    free(fv.v);
-    free(bv.v);
+    for (int g = 0; g < UIR_FP_SHAPES; ++g)
+        free(fv[g].v);
+    for (int g = 0; g < UIR_BP_SHAPES; ++g)
+        free(bv[g].v);
     uir_batch_free(b);
     return s;
 }
@@ -396,13 +436,32 @@ uir_status uir_batch_reset(uir_batch *b)
     return UIR_OK;
 }
 
+static uir_status launch_fp_groups(uir_batch *b)
+{
+    for (int g = 0; g < UIR_FP_SHAPES; ++g) {
+        const kgroup *k = &b->fpg[g];
+        if (k->n_items)
+            TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, (const uirk_fp_item *)k->d_items, k->n_items, &k->part, 1,
+                                            b->dtype, &k->tile, k->grid, b->d_err, b->res->stream), "FP launch"));
+    }
+    return UIR_OK;
+}
+
+static uir_status launch_bp_groups(uir_batch *b)
+{
+    for (int g = 0; g < UIR_BP_SHAPES; ++g) {
+        const kgroup *k = &b->bpg[g];
+        if (k->n_items)
+            TRY(uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, (const uirk_bp_item *)k->d_items, k->n_items, &k->part, 1,
+                                            b->dtype, &k->tile, k->grid, b->d_err, b->res->stream), "BP launch"));
+    }
+    return UIR_OK;
+}
+
 static uir_status launch_iteration(uir_batch *b)
 {
-    hipStream_t st = b->res->stream;
-    TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, &b->fp_part, 1, b->dtype, b->fp_grid,
-                                    b->d_err, st), "FP launch"));
-    return uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, &b->bp_part, 1, b->dtype, b->bp_grid,
-                                       b->d_err, st), "BP launch");
+    TRY(launch_fp_groups(b));
+    return launch_bp_groups(b);
 }
 
 uir_status uir_batch_iterate(uir_batch *b, uint32_t iterations, int use_graph)
@@ -476,14 +535,21 @@ uir_status uir_batch_get_info(const uir_batch *b, uir_batch_info *info)
     if (!b || !info)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
     memset(info, 0, sizeof *info);
-    info->device_bytes = b->mem_bytes + (uint64_t)b->n_fp_items * sizeof(uirk_fp_item) +
-                         (uint64_t)b->n_bp_items * sizeof(uirk_bp_item) + 2ull * b->n_slices * sizeof(uirk_slice);
+    uint64_t nf = 0, nb = 0;
+    for (int g = 0; g < UIR_FP_SHAPES; ++g) {
+        nf += (uint64_t)b->fpg[g].n_items;
+        info->fp_grid += (uint32_t)b->fpg[g].grid * (b->fpg[g].n_items > 0);
+    }
+    for (int g = 0; g < UIR_BP_SHAPES; ++g) {
+        nb += (uint64_t)b->bpg[g].n_items;
+        info->bp_grid += (uint32_t)b->bpg[g].grid * (b->bpg[g].n_items > 0);
+    }
+    info->device_bytes = b->mem_bytes + nf * sizeof(uirk_fp_item) + nb * sizeof(uirk_bp_item) +
+                         2ull * b->n_slices * sizeof(uirk_slice);
     info->n_slices = b->n_slices;
     info->n_geometries = b->n_geoms;
-    info->fp_items = (uint32_t)b->n_fp_items;
-    info->bp_items = (uint32_t)b->n_bp_items;
-    info->fp_grid = (uint32_t)b->fp_grid;
-    info->bp_grid = (uint32_t)b->bp_grid;
+    info->fp_items = (uint32_t)nf;
+    info->bp_items = (uint32_t)nb;
     info->fp_samples = b->fp_samples;
     info->bp_samples = b->bp_samples;
     return UIR_OK;
@@ -500,13 +566,11 @@ uir_status uir_batch_time_kernels(uir_batch *b, uint32_t reps, float *fp_ms, flo
     TRY(uir_check_hip(hipEventCreate(&e2), "hipEventCreate"));
     uir_status s = uir_check_hip(hipEventRecord(e0, st), "hipEventRecord");
     for (uint32_t i = 0; i < reps && s == UIR_OK; ++i)
-        s = uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, b->d_fp_items, b->n_fp_items, &b->fp_part, 1, b->dtype, b->fp_grid,
-                                        b->d_err, st), "FP launch");
+        s = launch_fp_groups(b);
     if (s == UIR_OK)
         s = uir_check_hip(hipEventRecord(e1, st), "hipEventRecord");
     for (uint32_t i = 0; i < reps && s == UIR_OK; ++i)
-        s = uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, b->d_bp_items, b->n_bp_items, &b->bp_part, 1, b->dtype, b->bp_grid,
-                                        b->d_err, st), "BP launch");
+        s = launch_bp_groups(b);
     if (s == UIR_OK)
         s = uir_check_hip(hipEventRecord(e2, st), "hipEventRecord");
     if (s == UIR_OK)

@@ -60,6 +60,11 @@ uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uin
                            const float *h_sin, const float *h_cos, const uir_subset *runs, uint32_t n_runs,
                            const float *d_trig, float r_min, float *d_ray_w);
 
+/* autotuner (uir_tune.c): tune on a stream; the tile shape for a geometry */
+uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype storage, uint32_t reps,
+                        uir_tune_result *out);
+uirk_tile uir_tile_for(const uir_slice_spec *spec, uir_dtype storage);
+
 /* geometry helpers (ufo-ir-parallel-projector-task.c:436-498) */
 void uir_geom_lut(uint32_t n_angles, float step, float *sin_lut, float *cos_lut);
 uint32_t uir_geom_runs(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uir_subset *out);

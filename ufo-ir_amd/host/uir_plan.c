@@ -152,13 +152,15 @@ static void line_range(int n, int n_dets, double axis, const float *s, const flo
 }
 
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
-                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs)
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles)
 {
+    if (max_angles < 1 || max_angles > UIRK_FP_ANGLES)
+        max_angles = UIRK_FP_ANGLES;
     for (uint32_t si = 0; si < n_subs; ++si) {
         const int off = (int)subs[si].offset, cnt = (int)subs[si].n, dir = subs[si].direction;
         int a = off;
         while (a < off + cnt) {
-            int k = off + cnt - a < UIRK_FP_ANGLES ? off + cnt - a : UIRK_FP_ANGLES;
+            int k = off + cnt - a < max_angles ? off + cnt - a : max_angles;
             while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k))
                 k = k > 2 ? k - (k + 3) / 4 : 1;
             for (int d0 = 0; d0 < n_dets; d0 += UIRK_FP_DETS) {
@@ -179,9 +181,11 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
     return 0;
 }
 
-int uirk_plan_bp(uirk_bp_vec *out, int slice, int n)
+int uirk_plan_bp(uirk_bp_vec *out, int slice, int n, int tile_rows)
 {
-    for (int y0 = 0; y0 < n; y0 += UIRK_BP_TILE)
+    if (tile_rows != 32 && tile_rows != UIRK_BP_TILE)
+        tile_rows = UIRK_BP_TILE;
+    for (int y0 = 0; y0 < n; y0 += tile_rows)
         for (int x0 = 0; x0 < n; x0 += UIRK_BP_TILE) {
             uirk_bp_item it;
             memset(&it, 0, sizeof it);

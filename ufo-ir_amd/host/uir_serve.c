@@ -64,8 +64,21 @@ typedef struct sgeom {
     uint64_t bytes;
     int refs;
     double last_used;
+    uirk_tile tile;        /* tuned or default tile shape */
+    int fp_shape, bp_shape; /* group indices of that shape */
     struct sgeom *next;
 } sgeom;
+
+#define SRV_FP_SHAPES 2
+#define SRV_BP_SHAPES 4
+
+typedef struct {
+    uirk_tile tile;
+    void *h_items, *d_items;
+    size_t hcap, dcap;
+    int n, grid;
+    uirk_xcd_part part;
+} sgroup;
 
 typedef struct sreq {
     uint64_t ticket;
@@ -115,18 +128,15 @@ struct uir_server {
     uint32_t n_live;          /* admitted and not finished (under mu) */
     sgeom *geoms;
 
-    /* tables (worker thread only) */
+    /* tables (worker thread only): slice descriptors, and one work table per
+     * tile shape (FP: 2 shapes, BP: 4), each launched when non-empty */
     uirk_slice *h_fp_sl, *h_bp_sl;
-    uirk_fp_item *h_fp;
-    uirk_bp_item *h_bp;
     double *h_cost;
-    size_t hcap_fsl, hcap_bsl, hcap_fp, hcap_bp, hcap_cost;
+    size_t hcap_fsl, hcap_bsl, hcap_cost;
     uirk_slice *d_fp_sl, *d_bp_sl;
-    uirk_fp_item *d_fp;
-    uirk_bp_item *d_bp;
-    size_t dcap_fsl, dcap_bsl, dcap_fp, dcap_bp;
-    int n_fp, n_bp, fp_grid, bp_grid;
-    uirk_xcd_part fp_part, bp_part;
+    size_t dcap_fsl, dcap_bsl;
+    sgroup fpg[SRV_FP_SHAPES], bpg[SRV_BP_SHAPES];
+    int autotune;
 
     uint64_t steps, completed;
     double busy_s, slice_its;
@@ -256,11 +266,23 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
     uir_geom_lut(g->A, g->step, g->h_sin, g->h_cos);
     g->n_runs = uir_geom_runs(g->h_sin, g->h_cos, g->A, g->runs);
     uirk_plan_trig(g->h_sin, g->h_cos, g->A, trig);
+    if (s->autotune) {
+        uir_tile t;
+        if (uir_tune_get(sp, s->dtype, &t) == UIR_ERR_NOT_FOUND) {
+            uir_tune_result tr;
+            if ((st = uir_tune_run(s->stream, sp, s->dtype, 2, &tr)) != UIR_OK)
+                goto fail;
+        }
+    }
+    g->tile = uir_tile_for(sp, s->dtype);
+    g->fp_shape = g->tile.fp_angles == 8;
+    g->bp_shape = (g->tile.bp_angles == 16) * 2 + (g->tile.bp_rows == 32);
     {
         uirk_fp_vec fv = {0};
         uirk_bp_vec bv = {0};
-        if (uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs) ||
-            uirk_plan_bp(&bv, 0, (int)g->n)) {
+        if (uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs,
+                         g->tile.fp_angles) ||
+            uirk_plan_bp(&bv, 0, (int)g->n, g->tile.bp_rows)) {
             free(fv.v);
             free(bv.v);
             st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
@@ -485,19 +507,30 @@ static uir_status dgrow(void **p, size_t *cap, size_t need, size_t es)
 static uir_status tables_build(uir_server *s)
 {
     const uint32_t na = s->n_active;
-    size_t nfp = 0, nbp = 0;
+    size_t nf[SRV_FP_SHAPES] = {0}, nb[SRV_BP_SHAPES] = {0}, ncost = 1;
     for (uint32_t i = 0; i < na; ++i) {
-        nfp += s->active[i]->g->n_fp;
-        nbp += s->active[i]->g->n_bp;
+        const sgeom *g = s->active[i]->g;
+        nf[g->fp_shape] += g->n_fp;
+        nb[g->bp_shape] += g->n_bp;
     }
-    const size_t ncost = nfp > nbp ? nfp : nbp;
+    for (int c = 0; c < SRV_FP_SHAPES; ++c)
+        ncost = nf[c] > ncost ? nf[c] : ncost;
+    for (int c = 0; c < SRV_BP_SHAPES; ++c)
+        ncost = nb[c] > ncost ? nb[c] : ncost;
     if (grow((void **)&s->h_fp_sl, &s->hcap_fsl, na, sizeof(uirk_slice)) ||
         grow((void **)&s->h_bp_sl, &s->hcap_bsl, na, sizeof(uirk_slice)) ||
-        grow((void **)&s->h_fp, &s->hcap_fp, nfp, sizeof(uirk_fp_item)) ||
-        grow((void **)&s->h_bp, &s->hcap_bp, nbp, sizeof(uirk_bp_item)) ||
         grow((void **)&s->h_cost, &s->hcap_cost, ncost, sizeof(double)))
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
-    nfp = nbp = 0;
+    for (int c = 0; c < SRV_FP_SHAPES; ++c)
+        if (grow(&s->fpg[c].h_items, &s->fpg[c].hcap, nf[c], sizeof(uirk_fp_item)))
+            return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+    for (int c = 0; c < SRV_BP_SHAPES; ++c)
+        if (grow(&s->bpg[c].h_items, &s->bpg[c].hcap, nb[c], sizeof(uirk_bp_item)))
+            return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+    for (int c = 0; c < SRV_FP_SHAPES; ++c)
+        s->fpg[c].n = 0;
+    for (int c = 0; c < SRV_BP_SHAPES; ++c)
+        s->bpg[c].n = 0;
     for (uint32_t i = 0; i < na; ++i) {
         const sreq *q = s->active[i];
         const sgeom *g = q->g;
@@ -531,41 +564,61 @@ static uir_status tables_build(uir_server *s)
         } else {
             k->pix_scale = 1.0f / (float)g->A;
         }
+        sgroup *fg = &s->fpg[g->fp_shape], *bg = &s->bpg[g->bp_shape];
+        uirk_fp_item *fi = (uirk_fp_item *)fg->h_items;
+        uirk_bp_item *bi = (uirk_bp_item *)bg->h_items;
         for (size_t j = 0; j < g->n_fp; ++j) {
-            s->h_fp[nfp] = g->fp[j];
-            s->h_fp[nfp].slice = (int32_t)i;
-            ++nfp;
+            fi[fg->n] = g->fp[j];
+            fi[fg->n].slice = (int32_t)i;
+            ++fg->n;
         }
         for (size_t j = 0; j < g->n_bp; ++j) {
-            s->h_bp[nbp] = g->bp[j];
-            s->h_bp[nbp].slice = (int32_t)i;
-            ++nbp;
+            bi[bg->n] = g->bp[j];
+            bi[bg->n].slice = (int32_t)i;
+            ++bg->n;
         }
     }
-    {
+    /* per group: cost-balanced XCD ranges (the items stay slice-major) */
+    for (int c = 0; c < SRV_FP_SHAPES; ++c) {
+        sgroup *grp = &s->fpg[c];
         size_t o = 0;
         for (uint32_t i = 0; i < na; ++i)
-            for (size_t j = 0; j < s->active[i]->g->n_fp; ++j)
-                s->h_cost[o++] = s->active[i]->g->fp_cost[j];
-        uirk_plan_xcd(s->h_cost, (int)nfp, uirk_plan_num_xcd(), &s->fp_part);
-        o = 0;
+            if (s->active[i]->g->fp_shape == c)
+                for (size_t j = 0; j < s->active[i]->g->n_fp; ++j)
+                    s->h_cost[o++] = s->active[i]->g->fp_cost[j];
+        uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
+        grp->grid = uirk_plan_grid(grp->n, 4);
+    }
+    for (int c = 0; c < SRV_BP_SHAPES; ++c) {
+        sgroup *grp = &s->bpg[c];
+        size_t o = 0;
         for (uint32_t i = 0; i < na; ++i)
-            for (size_t j = 0; j < s->active[i]->g->n_bp; ++j)
-                s->h_cost[o++] = (double)s->active[i]->g->A;
-        uirk_plan_xcd(s->h_cost, (int)nbp, uirk_plan_num_xcd(), &s->bp_part);
+            if (s->active[i]->g->bp_shape == c)
+                for (size_t j = 0; j < s->active[i]->g->n_bp; ++j)
+                    s->h_cost[o++] = (double)s->active[i]->g->A;
+        uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
+        grp->grid = uirk_plan_grid(grp->n, 8);
     }
     TRY(dgrow((void **)&s->d_fp_sl, &s->dcap_fsl, na, sizeof(uirk_slice)));
     TRY(dgrow((void **)&s->d_bp_sl, &s->dcap_bsl, na, sizeof(uirk_slice)));
-    TRY(dgrow((void **)&s->d_fp, &s->dcap_fp, nfp, sizeof(uirk_fp_item)));
-    TRY(dgrow((void **)&s->d_bp, &s->dcap_bp, nbp, sizeof(uirk_bp_item)));
     TRY(uir_check_hip(hipMemcpy(s->d_fp_sl, s->h_fp_sl, na * sizeof(uirk_slice), hipMemcpyHostToDevice), "tables"));
     TRY(uir_check_hip(hipMemcpy(s->d_bp_sl, s->h_bp_sl, na * sizeof(uirk_slice), hipMemcpyHostToDevice), "tables"));
-    TRY(uir_check_hip(hipMemcpy(s->d_fp, s->h_fp, nfp * sizeof(uirk_fp_item), hipMemcpyHostToDevice), "tables"));
-    TRY(uir_check_hip(hipMemcpy(s->d_bp, s->h_bp, nbp * sizeof(uirk_bp_item), hipMemcpyHostToDevice), "tables"));
-    s->n_fp = (int)nfp;
-    s->n_bp = (int)nbp;
-    s->fp_grid = uirk_plan_grid(s->n_fp, 4);
-    s->bp_grid = uirk_plan_grid(s->n_bp, 8);
+    for (int c = 0; c < SRV_FP_SHAPES; ++c) {
+        sgroup *grp = &s->fpg[c];
+        if (!grp->n)
+            continue;
+        TRY(dgrow(&grp->d_items, &grp->dcap, (size_t)grp->n, sizeof(uirk_fp_item)));
+        TRY(uir_check_hip(hipMemcpy(grp->d_items, grp->h_items, (size_t)grp->n * sizeof(uirk_fp_item),
+                                    hipMemcpyHostToDevice), "tables"));
+    }
+    for (int c = 0; c < SRV_BP_SHAPES; ++c) {
+        sgroup *grp = &s->bpg[c];
+        if (!grp->n)
+            continue;
+        TRY(dgrow(&grp->d_items, &grp->dcap, (size_t)grp->n, sizeof(uirk_bp_item)));
+        TRY(uir_check_hip(hipMemcpy(grp->d_items, grp->h_items, (size_t)grp->n * sizeof(uirk_bp_item),
+                                    hipMemcpyHostToDevice), "tables"));
+    }
     return UIR_OK;
 }
 
@@ -578,10 +631,16 @@ static uir_status step_launch(uir_server *s)
         if (is_sb(&q->rq))
             TRY(uir_check_hip(uirk_sb_tv(q->x, s->dtype, q->ex, q->ey, q->tv, q->rq.slice.n, s->stream), "sb_tv"));
     }
-    TRY(uir_check_hip(uirk_fp_table(s->d_fp_sl, NULL, s->d_fp, s->n_fp, &s->fp_part, 1, s->dtype, s->fp_grid, s->d_err,
-                                    s->stream), "FP launch"));
-    TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, s->d_bp, s->n_bp, &s->bp_part, 1, s->dtype, s->bp_grid, s->d_err,
-                                    s->stream), "BP launch"));
+    for (int c = 0; c < SRV_FP_SHAPES; ++c)
+        if (s->fpg[c].n)
+            TRY(uir_check_hip(uirk_fp_table(s->d_fp_sl, NULL, (const uirk_fp_item *)s->fpg[c].d_items, s->fpg[c].n,
+                                            &s->fpg[c].part, 1, s->dtype, &s->fpg[c].tile, s->fpg[c].grid, s->d_err,
+                                            s->stream), "FP launch"));
+    for (int c = 0; c < SRV_BP_SHAPES; ++c)
+        if (s->bpg[c].n)
+            TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, (const uirk_bp_item *)s->bpg[c].d_items, s->bpg[c].n,
+                                            &s->bpg[c].part, 1, s->dtype, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
+                                            s->stream), "BP launch"));
     for (uint32_t i = 0; i < s->n_active; ++i) {
         sreq *q = s->active[i];
         q->done_steps++;
@@ -830,6 +889,16 @@ uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server *
     s->max_active = cfg->max_active;
     s->budget = budget;
     s->next_ticket = 1;
+    s->autotune = cfg->autotune;
+    for (int c = 0; c < SRV_FP_SHAPES; ++c) {
+        s->fpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
+        s->fpg[c].tile.fp_angles = c ? 8 : UIRK_FP_ANGLES;
+    }
+    for (int c = 0; c < SRV_BP_SHAPES; ++c) {
+        s->bpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
+        s->bpg[c].tile.bp_angles = c >= 2 ? 16 : 32;
+        s->bpg[c].tile.bp_rows = (c & 1) ? 32 : UIRK_BP_TILE;
+    }
     s->fatal = UIR_OK;
     pthread_mutex_init(&s->mu, NULL);
     pthread_cond_init(&s->cv_work, NULL);
@@ -1007,13 +1076,18 @@ void uir_server_free(uir_server *s)
     free(s->active);
     free(s->h_fp_sl);
     free(s->h_bp_sl);
-    free(s->h_fp);
-    free(s->h_bp);
     free(s->h_cost);
+    for (int c = 0; c < SRV_FP_SHAPES; ++c) {
+        free(s->fpg[c].h_items);
+        uir_dfree(s->fpg[c].d_items);
+    }
+    for (int c = 0; c < SRV_BP_SHAPES; ++c) {
+        free(s->bpg[c].h_items);
+        uir_dfree(s->bpg[c].d_items);
+    }
     uir_dfree(s->d_fp_sl);
     uir_dfree(s->d_bp_sl);
-    uir_dfree(s->d_fp);
-    uir_dfree(s->d_bp);
+
     uir_dfree(s->d_tmp);
     uir_dfree(s->d_err);
     if (s->ev[0])

@@ -581,8 +581,9 @@ static uir_status proj_plan(uir_task *p, uint32_t n)
     uirk_bp_vec bv = {0};
     float *trig = (float *)malloc(sizeof(float) * 2 * (p->lut_n ? p->lut_n : 1));
     uir_status s = UIR_OK;
-    if (!trig || uirk_plan_fp(&fv, 0, (int)n, (int)p->n_dets, axis, p->h_sin, p->h_cos, p->runs, p->n_runs) ||
-        uirk_plan_bp(&bv, 0, (int)n)) {
+    if (!trig || uirk_plan_fp(&fv, 0, (int)n, (int)p->n_dets, axis, p->h_sin, p->h_cos, p->runs, p->n_runs,
+                              UIRK_FP_ANGLES) ||
+        uirk_plan_bp(&bv, 0, (int)n, UIRK_BP_TILE)) {
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
         goto done;
     }
@@ -654,7 +655,7 @@ static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float
     sl.b = r_sino;
     sl.r = w_sino;
     sl.ray_w = ray_w;
-    return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_fp_items, p->n_fp_items, &p->fp_part, 0, UIR_DTYPE_F32,
+    return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_fp_items, p->n_fp_items, &p->fp_part, 0, UIR_DTYPE_F32, NULL,
                                        uirk_plan_grid(p->n_fp_items, 4), p->d_err, p->res->stream), "FP launch");
 }
 
@@ -679,7 +680,7 @@ static uir_status proj_bp_ex(uir_task *p, uint32_t n, const float *sino, float *
             sl.run_n[k] = p->runs[i].n;
         }
         sl.n_runs = (int32_t)k;
-        TRY(uir_check_hip(uirk_bp_table(NULL, &sl, p->d_bp_items, p->n_bp_items, &p->bp_part, 0, UIR_DTYPE_F32,
+        TRY(uir_check_hip(uirk_bp_table(NULL, &sl, p->d_bp_items, p->n_bp_items, &p->bp_part, 0, UIR_DTYPE_F32, NULL,
                                         uirk_plan_grid(p->n_bp_items, 8), p->d_err, p->res->stream), "BP launch"));
     }
     return UIR_OK;

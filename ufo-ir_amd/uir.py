@@ -59,7 +59,15 @@ class Request(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("storage", C.c_int), ("max_active", C.c_uint32), ("hbm_budget", C.c_uint64),
-                ("hbm_reserve", C.c_uint64)]
+                ("hbm_reserve", C.c_uint64), ("autotune", C.c_int32)]
+
+
+class Tile(C.Structure):
+    _fields_ = [("fp_angles", C.c_int32), ("bp_angles", C.c_int32), ("bp_rows", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("best", Tile), ("fp_ms", C.c_float * 2), ("bp_ms", C.c_float * 4), ("probe_slices", C.c_uint32)]
 
 
 class RequestStats(C.Structure):
@@ -171,6 +179,11 @@ def load():
     L.uir_server_free.argtypes = [vp]
     L.uir_server_free.restype = None
     L.uir_task_property_type.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+    L.uir_tune_geometry.argtypes = [vp, C.POINTER(SliceSpec), C.c_int, u32, C.POINTER(TuneResult)]
+    L.uir_tune_get.argtypes = [C.POINTER(SliceSpec), C.c_int, C.POINTER(Tile)]
+    L.uir_tune_set.argtypes = [C.POINTER(SliceSpec), C.c_int, C.POINTER(Tile)]
+    L.uir_tune_clear.argtypes = []
+    L.uir_tune_clear.restype = None
     L.uir_tiff_info.argtypes = [C.c_char_p, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     L.uir_tiff_read.argtypes = [C.c_char_p, u32, vp, sz, C.POINTER(u32), C.POINTER(u32)]
     L.uir_tiff_write.argtypes = [C.c_char_p, vp, u32, u32, C.c_int]
@@ -467,6 +480,47 @@ def header_functions(path=HEADER_PATH):
 DTYPES = {"f32": DTYPE_F32, "f16": DTYPE_F16, "bf16": DTYPE_BF16}
 
 
+def slice_spec(s):
+    """A uir_slice_spec from a dict (n, n_dets, n_angles, step, axis_position,
+    relaxation, positive_constraint, ray_weight_min)."""
+    return SliceSpec(int(s["n"]), int(s["n_dets"]), int(s["n_angles"]), float(s["step"]),
+                     float(s.get("axis_position", -1.0)), float(s.get("relaxation", 0.25)),
+                     int(bool(s.get("positive_constraint", False))), float(s.get("ray_weight_min", 0.0)))
+
+
+def _tile_dict(t):
+    return {"fp_angles": t.fp_angles, "bp_angles": t.bp_angles, "bp_rows": t.bp_rows}
+
+
+def tune_geometry(res, spec, storage="f16", reps=3):
+    """Times every tile shape on a probe of the geometry; records and returns the best."""
+    r = TuneResult()
+    sp = slice_spec(spec)
+    check(load().uir_tune_geometry(res.handle, C.byref(sp), DTYPES[storage], int(reps), C.byref(r)))
+    return {"best": _tile_dict(r.best), "fp_ms": list(r.fp_ms), "bp_ms": list(r.bp_ms),
+            "probe_slices": r.probe_slices}
+
+
+def tune_get(spec, storage="f16"):
+    """(tile dict, tuned?) for a geometry."""
+    t = Tile()
+    sp = slice_spec(spec)
+    rc = load().uir_tune_get(C.byref(sp), DTYPES[storage], C.byref(t))
+    if rc not in (UIR_OK, 6):
+        check(rc)
+    return _tile_dict(t), rc == UIR_OK
+
+
+def tune_set(spec, storage, fp_angles, bp_angles, bp_rows):
+    sp = slice_spec(spec)
+    t = Tile(int(fp_angles), int(bp_angles), int(bp_rows))
+    check(load().uir_tune_set(C.byref(sp), DTYPES[storage], C.byref(t)))
+
+
+def tune_clear():
+    load().uir_tune_clear()
+
+
 class Batch:
     """Batched SIRT over slices of uneven geometry (uir_batch_*).
 
@@ -476,9 +530,7 @@ class Batch:
     def __init__(self, res, specs, storage="f32"):
         arr = (SliceSpec * len(specs))()
         for i, s in enumerate(specs):
-            arr[i] = SliceSpec(int(s["n"]), int(s["n_dets"]), int(s["n_angles"]), float(s["step"]),
-                               float(s.get("axis_position", -1.0)), float(s.get("relaxation", 0.25)),
-                               int(bool(s.get("positive_constraint", False))), float(s.get("ray_weight_min", 0.0)))
+            arr[i] = slice_spec(s)
         h = C.c_void_p()
         check(load().uir_batch_new(res.handle, DTYPES[storage], arr, len(specs), C.byref(h)))
         self.handle = h
@@ -557,8 +609,8 @@ class Server:
     wait(ticket) returns (image, stats).  ctypes releases the GIL inside the
     calls, so submitters and waiters may run in different Python threads."""
 
-    def __init__(self, device=0, storage="f16", max_active=0, hbm_budget=0, hbm_reserve=0):
-        cfg = ServerConfig(DTYPES[storage], int(max_active), int(hbm_budget), int(hbm_reserve))
+    def __init__(self, device=0, storage="f16", max_active=0, hbm_budget=0, hbm_reserve=0, autotune=False):
+        cfg = ServerConfig(DTYPES[storage], int(max_active), int(hbm_budget), int(hbm_reserve), int(bool(autotune)))
         h = C.c_void_p()
         check(load().uir_server_new(int(device), C.byref(cfg), C.byref(h)))
         self.handle = h
