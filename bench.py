@@ -8,9 +8,10 @@ projected with the Joseph projector at 720 angles over [0, pi) onto 1448 bins,
 plus Gaussian noise of sigma = 1% of the sinogram maximum.
 
 A step is one SIRT iteration (relaxation 0.25) over a batch of --batch such
-slices resident in HBM in FP16 storage with FP32 interpolation and
-accumulation: one forward- and one back-projection launch over every slice,
-replayed from a captured hipGraph.  The plain FP32 path (the SIRT task, one
+slices resident in HBM: sinograms (measured data and residual) in FP16, images
+in FP32 (--storage f16-sino; f16 also stores the images in FP16), FP32
+interpolation and accumulation: one forward- and one back-projection launch
+over every slice, replayed from a captured hipGraph.  The plain FP32 path (the SIRT task, one
 slice per process() call, reference-compatible arithmetic) is measured beside
 it, and accuracy is reported against the FP64 evaluation of the same
 operators.
@@ -152,13 +153,124 @@ def serve_c5(args, uir):
     print(json.dumps(line), flush=True)
 
 
+def sart_c2(args, uir):
+    """Plain SART on one 1024 x 1024 slice, 720 angles, 1448 bins, 30 iterations,
+    relaxation 0.5 (the reference-compatible SART task, FP32)."""
+    res = uir.Resources(0)
+    n, a, d = 1024, 720, 1448
+    step = float(np.float32(np.pi / a))
+    proj = uir.Task("parallel-projector", step=step, is_forward=True, angles_num=a, image_size=n,
+                    detectors_num=d).setup(res)
+    img = uir.DeviceArray.from_numpy(phantom(n, 20261202))
+    s = uir.DeviceArray((a, d))
+    proj.process(img, s)
+    sino = s.numpy().astype(np.float64)
+    sino += np.random.default_rng(20261202).normal(0.0, 0.01 * sino.max(), sino.shape)
+    ds = uir.DeviceArray.from_numpy(sino.astype(np.float32))
+    out = uir.DeviceArray((n, n))
+    pj = uir.Task("parallel-projector", step=step, is_forward=False, image_size=n)
+    sart = uir.Task("sart", projector=pj, relaxation_factor=0.5, num_iterations=30).setup(res)
+    sart.process(ds, out)                      # warm-up (captures the sweep graph)
+    res.synchronize()
+    timer = uir.Timer()
+    timer.start(res.stream)
+    sart.process(ds, out)
+    timer.stop(res.stream)
+    ms = timer.elapsed_ms()
+    line = {"metric": "plain SART, 30 iterations", "value": round(ms, 2), "unit": "ms", "n_gpus": 1,
+            "higher_is_better": False, "dtype": "f32", "data": "synthetic: 20 random ellipses, 1% Gaussian noise",
+            "config": {"workload": "SART task, 1024x1024, 720 angles over [0, pi), 1448 bins, relaxation 0.5, "
+                                   "ordered single-angle subsets, 30 iterations"},
+            "ms_per_iteration": round(ms / 30, 3), "ms_per_angle": round(ms / 30 / a * 1000, 2)}
+    print(json.dumps(line), flush=True)
+
+
+def batch_c3(args, uir):
+    """Batched SIRT over 512 slices of uneven size (N in {256, 384, 512, 768,
+    1024, 2048}, angles U{180..1800}, bins ceil(N sqrt 2)), 100 iterations from
+    a captured hipGraph, FP32 and FP16 storage; accuracy against the FP64
+    oracle on the slices with N <= 384."""
+    import oracle as O
+    res = uir.Resources(0)
+    rng = np.random.default_rng(20261207)
+    sizes = [256, 384, 512, 768, 1024, 2048]
+    specs = []
+    for _ in range(args.c3_slices):
+        n = sizes[rng.integers(0, len(sizes))]
+        a = int(rng.integers(180, 1801))
+        specs.append(dict(n=n, n_dets=math.ceil(n * math.sqrt(2)), n_angles=a, step=float(np.float32(np.pi / a)),
+                          relaxation=0.25, ray_weight_min=RAY_WEIGHT_MIN))
+    # sinograms: one random-ellipse phantom per image size, projected at each
+    # slice's own angles by the reference-compatible GPU projector
+    t0 = time.time()
+    imgs = {n: uir.DeviceArray.from_numpy(phantom(n, 20261300 + n)) for n in sizes}
+    sinos = []
+    for sp in specs:
+        n, a, d = sp["n"], sp["n_angles"], sp["n_dets"]
+        pj = uir.Task("parallel-projector", step=sp["step"], is_forward=True, angles_num=a, image_size=n,
+                      detectors_num=d).setup(res)
+        s = uir.DeviceArray((a, d))
+        pj.process(imgs[n], s)
+        sinos.append(s.numpy())
+        s.free()
+    for im in imgs.values():
+        im.free()
+    t_data = time.time() - t0
+    print("c3: %d sinograms in %.1f s" % (len(specs), t_data), file=sys.stderr, flush=True)
+    # FP64 references of the first slices with N <= 384
+    t0 = time.time()
+    check = [i for i, sp in enumerate(specs) if sp["n"] <= 384][:3]
+    refs = {}
+    for i in check:
+        sp = specs[i]
+        g = O.Geometry(sp["n"], sp["n_dets"], sp["n_angles"], np.float32(sp["step"]))
+        refs[i] = O.sirt64(g, sinos[i].astype(np.float64), args.c3_iters, 0.25, ray_weight_min=RAY_WEIGHT_MIN)
+    t_ref = time.time() - t0
+    print("c3: FP64 references of %d slices in %.1f s" % (len(check), t_ref), file=sys.stderr, flush=True)
+    out = {}
+    for storage in ("f32", "f16", "f16-sino"):
+        bt = uir.Batch(res, specs, storage)
+        for i, sn in enumerate(sinos):
+            bt.set_sinogram(i, sn)
+        info = bt.info()
+        bt.reset()
+        bt.iterate(1)                          # capture
+        bt.reset()
+        res.synchronize()
+        timer = uir.Timer()
+        timer.start(res.stream)
+        for _ in range(args.c3_iters):
+            bt.iterate(1)
+        timer.stop(res.stream)
+        ms = timer.elapsed_ms()
+        errs = [O.rel_l2(bt.image(i), refs[i]) for i in check]
+        print("c3: %s %d iterations in %.1f ms" % (storage, args.c3_iters, ms), file=sys.stderr, flush=True)
+        out[storage] = {"ms_total": round(ms, 1), "ms_per_iteration": round(ms / args.c3_iters, 3),
+                        "slice_iters_per_s": round(len(specs) * args.c3_iters / (ms / 1000.0), 1),
+                        "rel_l2_vs_fp64_max": max(errs) if errs else None, "checked_slices": len(errs),
+                        "device_bytes": info["device_bytes"], "fp_items": info["fp_items"],
+                        "bp_items": info["bp_items"], "geometries": info["n_geometries"]}
+        bt.close()
+    line = {"metric": "batched SIRT over uneven slices", "value": out["f16-sino"]["slice_iters_per_s"],
+            "unit": "slice-iters/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f16 sinograms, f32 images, f32 math",
+            "data": "synthetic: 20 random ellipses per slice, Joseph FP (reference-compatible GPU projector)",
+            "config": {"workload": "%d slices, N in {256..2048}, angles U{180..1800}, bins ceil(N sqrt 2), "
+                                   "%d SIRT iterations from a captured hipGraph" % (len(specs), args.c3_iters),
+                       "slices": len(specs), "iterations": args.c3_iters},
+            "f32": out["f32"], "f16": out["f16"], "f16-sino": out["f16-sino"], "data_gen_s": round(t_data, 1),
+            "fp64_oracle_s": round(t_ref, 1)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--storage", default="f16", choices=["f16", "bf16", "f32"])
+    ap.add_argument("--storage", default="f16-sino", choices=["f16", "bf16", "f32", "f16-sino", "bf16-sino"],
+                    help="image and sinogram storage (f16-sino: FP16 sinograms, FP32 images)")
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--dets", type=int, default=1448)
@@ -171,6 +283,10 @@ def main():
     ap.add_argument("--serve-c5", action="store_true", help="run the split-Bregman TV serving workload only")
     ap.add_argument("--serve-qps-c5", type=float, default=10.0)
     ap.add_argument("--serve-requests-c5", type=int, default=200)
+    ap.add_argument("--c2", action="store_true", help="time plain SART on 1024x1024, 720 angles, 30 iterations")
+    ap.add_argument("--c3", action="store_true", help="batched SIRT over 512 slices of uneven size")
+    ap.add_argument("--c3-slices", type=int, default=512)
+    ap.add_argument("--c3-iters", type=int, default=100)
     ap.add_argument("--profile-pass", action="store_true",
                     help="for counter collection: build the batch, run 3 iterations with direct launches and one "
                          "512 MiB device-to-device copy (the calibration of the byte counters), then exit")
@@ -186,6 +302,13 @@ def main():
         sys.exit(1)
     if args.serve_c5:
         serve_c5(args, uir)
+        return
+    if args.c2:
+        sart_c2(args, uir)
+        return
+    if args.c3:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        batch_c3(args, uir)
         return
     res = uir.Resources(0)
     n, a, d = args.n, args.angles, args.dets
@@ -211,6 +334,7 @@ def main():
     t_data = time.time() - t0
 
     if args.profile_pass:
+        uir.tune_geometry(res, dict(n=n, n_dets=d, n_angles=a, step=step), args.storage, reps=3)
         specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
         bt = uir.Batch(res, specs, args.storage)
         for i in range(args.batch):
@@ -279,10 +403,11 @@ def main():
     value = args.batch * args.steps / (total_ms / 1000.0)
 
     # ---- roofline of the dominant kernel (per launch over the whole batch)
-    es = 2 if args.storage in ("f16", "bf16") else 4
+    es = 2 if args.storage in ("f16", "bf16", "f16-sino", "bf16-sino") else 4   # sinograms
+    ei = 2 if args.storage in ("f16", "bf16") else 4                             # images
     nv, ns = n * n, a * d
-    fp_bytes = args.batch * (nv * es + ns * es + ns * 4 + ns * es)     # x, b, R read; r written
-    bp_bytes = args.batch * (ns * es + nv * es + nv * es)              # r, x read; x written
+    fp_bytes = args.batch * (nv * ei + ns * es + ns * 4 + ns * es)     # x, b, R read; r written
+    bp_bytes = args.batch * (ns * es + nv * ei + nv * ei)              # r, x read; x written
     fp_flops = 4.0 * info["fp_samples"]
     bp_flops = 4.0 * info["bp_samples"]
     dom = ("fp", fp_ms, fp_bytes, fp_flops) if fp_ms >= bp_ms else ("bp", bp_ms, bp_bytes, bp_flops)
@@ -355,7 +480,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "%s-storage/f32-math" % args.storage,
+        "dtype": {"f16-sino": "f16 sinograms, f32 images, f32 math", "bf16-sino": "bf16 sinograms, f32 images, f32 math"}
+                 .get(args.storage, "%s storage, f32 math" % args.storage),
         "data": "synthetic: 20 random ellipses per slice, Joseph FP, 1%% Gaussian noise (%d distinct)" % n_distinct,
         "config": {"workload": "batched SIRT, %d slices of %dx%d, %d angles over [0, pi), %d bins, relaxation %.2f"
                                % (args.batch, n, n, a, d, relax),

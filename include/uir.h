@@ -46,7 +46,16 @@ typedef enum {
     UIR_ERR_IO = 11               /* file could not be written */
 } uir_status;
 
-typedef enum { UIR_DTYPE_F32 = 0, UIR_DTYPE_F16 = 1, UIR_DTYPE_BF16 = 2 } uir_dtype;
+/* Element types.  The two *_SINO values are storage modes of the batched
+ * engine, the server and the tuner: sinograms (measured data and residual)
+ * in FP16 / BF16, images in FP32. */
+typedef enum {
+    UIR_DTYPE_F32 = 0,
+    UIR_DTYPE_F16 = 1,
+    UIR_DTYPE_BF16 = 2,
+    UIR_DTYPE_F16_SINO = 3,
+    UIR_DTYPE_BF16_SINO = 4
+} uir_dtype;
 
 uint32_t    uir_abi_version(void);
 const char *uir_last_error(void);
@@ -262,7 +271,7 @@ uir_status uir_batch_time_kernels(uir_batch *batch, uint32_t reps, float *fp_ms,
 /* ------------------------------------------------------------------------ */
 /* Tile-shape autotuner.  No counterpart in the reference (its work-group    */
 /* shapes are the OpenCL runtime's).  The fast projector pair has variants: */
-/* FP work items of 16 or 8 angles (x 64 detectors), BP angle batches of 32 */
+/* FP work items of 16, 8 or 32 angles (x 64 detectors), BP batches of 32  */
 /* or 16 staged angles and tiles of 64 x 64 or 64 x 32 pixels.  Tuning times */
 /* each on a probe of the geometry and keeps the fastest per geometry and    */
 /* storage dtype; batches and the server then plan each geometry's work with */
@@ -271,14 +280,14 @@ uir_status uir_batch_time_kernels(uir_batch *batch, uint32_t reps, float *fp_ms,
 /* coordinates move with the tile, so results agree to FP32 rounding.       */
 /* ------------------------------------------------------------------------ */
 typedef struct {
-    int32_t fp_angles;  /* 16 or 8 */
+    int32_t fp_angles;  /* 16, 8 or 32 */
     int32_t bp_angles;  /* 32 or 16 */
     int32_t bp_rows;    /* 64 or 32 */
 } uir_tile;
 
 typedef struct {
     uir_tile best;
-    float    fp_ms[2];        /* fp_angles 16, 8 */
+    float    fp_ms[3];        /* fp_angles 16, 8, 32 */
     float    bp_ms[4];        /* (bp_angles, bp_rows) = (32, 64), (32, 32), (16, 64), (16, 32) */
     uint32_t probe_slices;    /* slices of the geometry in the probe */
 } uir_tune_result;

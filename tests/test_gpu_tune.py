@@ -13,7 +13,7 @@ from phantoms import ellipse_phantom
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = list(itertools.product((16, 8), (32, 16), (64, 32)))
+VARIANTS = list(itertools.product((16, 8, 32), (32, 16), (64, 32)))
 
 
 def _spec(n, d, a, relax=0.5):
@@ -42,7 +42,7 @@ def test_tune_geometry_records_the_fastest(res):
     r = uir.tune_geometry(res, spec, "f16", reps=2)
     print("tune 256/363/180 f16:", r)
     assert all(t > 0 for t in r["fp_ms"]) and all(t > 0 for t in r["bp_ms"])
-    fp_best = (16, 8)[int(np.argmin(r["fp_ms"]))]
+    fp_best = (16, 8, 32)[int(np.argmin(r["fp_ms"]))]
     bp_best = [(32, 64), (32, 32), (16, 64), (16, 32)][int(np.argmin(r["bp_ms"]))]
     assert r["best"] == {"fp_angles": fp_best, "bp_angles": bp_best[0], "bp_rows": bp_best[1]}
     assert uir.tune_get(spec, "f16") == (r["best"], True)
@@ -66,7 +66,7 @@ def test_every_variant_matches_fp64_and_default(res, fa, ba, br):
 
 def test_mixed_shapes_in_one_batch(res):
     geoms = [(128, 182, 90), (96, 136, 120), (160, 227, 75)]
-    shapes = [(8, 16, 32), (16, 32, 64), (8, 32, 32)]
+    shapes = [(8, 16, 32), (16, 32, 64), (32, 32, 32)]
     specs = [_spec(*gm) for gm in geoms]
     sinos = [_sino(*gm, 20 + i)[1] for i, gm in enumerate(geoms)]
     uir.tune_clear()

@@ -24,7 +24,7 @@ def test_set_get_per_geometry_and_dtype():
     assert uir.tune_get(SPEC, "f16")[1] is False
 
 
-@pytest.mark.parametrize("tile", [(12, 32, 64), (16, 8, 64), (16, 32, 48), (0, 0, 0)])
+@pytest.mark.parametrize("tile", [(12, 32, 64), (64, 32, 64), (16, 8, 64), (16, 32, 48), (0, 0, 0)])
 def test_only_built_variants_are_accepted(tile):
     with pytest.raises(uir.UirError) as e:
         uir.tune_set(SPEC, "f16", *tile)

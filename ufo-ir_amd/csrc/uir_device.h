@@ -122,13 +122,14 @@ typedef struct {
 } uirk_xcd_part;
 
 #define UIRK_FP_DETS 64     /* detectors per FP work item */
-#define UIRK_FP_ANGLES 16   /* max angles per FP work item */
+#define UIRK_FP_ANGLES 16   /* angles per FP work item (default shape) */
+#define UIRK_FP_ANGLES_MAX 32 /* widest FP shape */
 #define UIRK_FP_LINES 32    /* lines per staged chunk (divides the 64-sample block) */
 #define UIRK_FP_WMAX 256    /* max staged footprint width */
 #define UIRK_BP_TILE 64     /* BP tile edge */
 
 /* Tile shapes of the fast path, chosen per geometry by the autotuner
- * (host/uir_tune.c): FP angles per work item (16 or 8), BP angles per staged
+ * (host/uir_tune.c): FP angles per work item (16, 8 or 32), BP angles per staged
  * batch (32 or 16) and BP tile rows (64 or 32; tiles are 64 pixels wide).
  * The reference-compatible mode takes the default {16, 32, 64} only. */
 typedef struct {
@@ -137,17 +138,18 @@ typedef struct {
 #define UIRK_TILE_DEFAULT {UIRK_FP_ANGLES, 32, UIRK_BP_TILE}
 
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
- * FP32 / FP16 / BF16 storage).  dtype: uir_dtype of the image / sinogram
- * buffers (the reference-compatible mode takes FP32 only).  d_slices: device
+ * FP32 / FP16 / BF16 storage).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
+ * sinogram buffers (b, r); img_dtype: of the image buffers (x, x_out), either
+ * dtype or FP32 (the reference-compatible mode takes FP32 only).  d_slices: device
  * descriptor array indexed by item.slice, or NULL to use *one (host memory,
  * passed by value as a kernel argument; item.slice must then be 0).
  * err: device int, set non-zero if a footprint exceeded the staging window. */
 int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
-                  void *stream);
+                  const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
+                  int *d_err, void *stream);
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
-                  void *stream);
+                  const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
+                  int *d_err, void *stream);
 
 /* Work-table construction (host/uir_plan.c). */
 typedef struct {

@@ -132,7 +132,7 @@ __device__ __forceinline__ float wave_max(float v)
 // ------------------------------------------------------------------------
 // Forward projection
 // ------------------------------------------------------------------------
-template <typename T, bool REF, int APT>
+template <typename T, typename TI, bool REF, int APT>
 __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_fp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         }
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
-        const T *img = uniform((const T *)S->x);
+        const TI *img = uniform((const TI *)S->x);
         const float *trig = uniform(S->trig);
         const int s = item.d0 + lane;
         const bool hor = item.dir == 0;
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
-template <typename T, bool REF, int NA, int ROWS>
+template <typename T, typename TI, bool REF, int NA, int ROWS>
 __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
             val[k] = 0.0f;
         }
         if (REF && !S->pix_w) {
-            const T *xin = uniform((const T *)S->x);
+            const TI *xin = uniform((const TI *)S->x);
 #pragma unroll
             for (int k = 0; k < ROWS; ++k) {
                 const int py = item.y0 + wave + 4 * k;
@@ -610,8 +610,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         }
 
         if (px < N) {
-            const T *xin = uniform((const T *)S->x);
-            T *xout = uniform((T *)S->x_out);
+            const TI *xin = uniform((const TI *)S->x);
+            TI *xout = uniform((TI *)S->x_out);
             const float *pix_w = uniform(S->pix_w);
             const int posc = S->posc;
             const float gain = S->relax * S->pix_scale;
@@ -715,50 +715,79 @@ __global__ __launch_bounds__(256) void k_sb_shrink(const T *__restrict__ u, floa
     ey[o] = dy - (gy - dy);
 }
 
-template <typename T, bool REF, int APT>
+template <typename T, typename TI, bool REF, int APT>
 int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fp_tab<T, REF, APT>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_fp_tab<T, TI, REF, APT>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
-template <typename T, bool REF, int NA, int ROWS>
+template <typename T, typename TI, bool REF, int NA, int ROWS>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
-// fast-path FP over the tile variants: 16 or 8 angles per item (4 or 2 per thread)
-template <typename T>
+// fast-path FP over the tile variants: 16, 8 or 32 angles per item (4, 2 or 8 per thread)
+template <typename T, typename TI>
 int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items,
                    int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
     if (tile.fp_angles == 8)
-        return launch_fp<T, false, 2>(sl, one, items, n, part, grid, err, st);
-    return launch_fp<T, false, 4>(sl, one, items, n, part, grid, err, st);
+        return launch_fp<T, TI, false, 2>(sl, one, items, n, part, grid, err, st);
+    if (tile.fp_angles == 32)
+        return launch_fp<T, TI, false, 8>(sl, one, items, n, part, grid, err, st);
+    return launch_fp<T, TI, false, 4>(sl, one, items, n, part, grid, err, st);
 }
 
 // fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
-template <typename T>
+template <typename T, typename TI>
 int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items,
                    int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
     const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
     if (na16 && r8)
-        return launch_bp<T, false, 16, 8>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, err, st);
     if (na16)
-        return launch_bp<T, false, 16, 16>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 16, 16>(sl, one, items, n, part, grid, err, st);
     if (r8)
-        return launch_bp<T, false, 32, 8>(sl, one, items, n, part, grid, err, st);
-    return launch_bp<T, false, 32, 16>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 32, 8>(sl, one, items, n, part, grid, err, st);
+    return launch_bp<T, TI, false, 32, 16>(sl, one, items, n, part, grid, err, st);
+}
+
+// (sinogram, image) storage pairs: both FP32, both FP16, both BF16, or
+// FP16 / BF16 sinograms with an FP32 image
+template <bool FP>
+int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one,
+                const void *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+{
+#define UIRK_LAUNCH(TS, TIM)                                                                                     \
+    return FP ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)     \
+              : launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)
+    if (img_dtype == UIR_DTYPE_F32) {
+        switch (dtype) {
+        case UIR_DTYPE_F16: UIRK_LAUNCH(__half, float);
+        case UIR_DTYPE_BF16: UIRK_LAUNCH(__hip_bfloat16, float);
+        case UIR_DTYPE_F32: UIRK_LAUNCH(float, float);
+        default: return (int)hipErrorInvalidValue;
+        }
+    }
+    if (img_dtype != dtype)
+        return (int)hipErrorInvalidValue;
+    switch (dtype) {
+    case UIR_DTYPE_F16: UIRK_LAUNCH(__half, __half);
+    case UIR_DTYPE_BF16: UIRK_LAUNCH(__hip_bfloat16, __hip_bfloat16);
+    default: return (int)hipErrorInvalidValue;
+    }
+#undef UIRK_LAUNCH
 }
 
 bool tile_ok(const uirk_tile *t)
 {
-    return (t->fp_angles == 16 || t->fp_angles == 8) && (t->bp_angles == 32 || t->bp_angles == 16) &&
+    return (t->fp_angles == 16 || t->fp_angles == 8 || t->fp_angles == 32) && (t->bp_angles == 32 || t->bp_angles == 16) &&
            (t->bp_rows == 64 || t->bp_rows == 32);
 }
 
@@ -767,8 +796,8 @@ bool tile_ok(const uirk_tile *t)
 extern "C" {
 
 int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_fp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
-                  void *stream)
+                  const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
+                  int *d_err, void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -788,21 +817,16 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
-        if (dtype != UIR_DTYPE_F32 || tl.fp_angles != UIRK_FP_ANGLES)
+        if (dtype != UIR_DTYPE_F32 || img_dtype != UIR_DTYPE_F32 || tl.fp_angles != UIRK_FP_ANGLES)
             return (int)hipErrorInvalidValue;
-        return launch_fp<float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_fp<float, float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    switch (dtype) {
-    case UIR_DTYPE_F16: return launch_fp_fast<__half>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    case UIR_DTYPE_BF16:
-        return launch_fp_fast<__hip_bfloat16>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    default: return launch_fp_fast<float>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    }
+    return launch_fast<true>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
 }
 
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                  const uirk_xcd_part *part, int mode, int dtype, const uirk_tile *tile, int grid, int *d_err,
-                  void *stream)
+                  const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
+                  int *d_err, void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -822,16 +846,11 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
-        if (dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE)
+        if (dtype != UIR_DTYPE_F32 || img_dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE)
             return (int)hipErrorInvalidValue;
-        return launch_bp<float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    switch (dtype) {
-    case UIR_DTYPE_F16: return launch_bp_fast<__half>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    case UIR_DTYPE_BF16:
-        return launch_bp_fast<__hip_bfloat16>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    default: return launch_bp_fast<float>(tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
-    }
+    return launch_fast<false>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

@@ -31,7 +31,7 @@ typedef struct {
     uint32_t n_runs;
 } geom_t;
 
-#define UIR_FP_SHAPES 2
+#define UIR_FP_SHAPES 3
 #define UIR_BP_SHAPES 4
 
 typedef struct {
@@ -41,13 +41,15 @@ typedef struct {
     uirk_xcd_part part;
 } kgroup;
 
-static int fp_shape(const uirk_tile *t) { return t->fp_angles == 8; }
+static int fp_shape(const uirk_tile *t) { return t->fp_angles == 8 ? 1 : (t->fp_angles == 32 ? 2 : 0); }
+static const int k_fp_shape_angles[UIR_FP_SHAPES] = {UIRK_FP_ANGLES, 8, 32};
 static int bp_shape(const uirk_tile *t) { return (t->bp_angles == 16) * 2 + (t->bp_rows == 32); }
 
 struct uir_batch {
     uir_resources *res;
-    uir_dtype dtype;
-    size_t es; /* storage element size */
+    uir_dtype dtype;  /* storage mode */
+    int sdt, idt;     /* sinogram and image element types */
+    size_t ess, esi;  /* their sizes */
     uint32_t n_slices, n_geoms;
     uir_slice_spec *specs;
     uint32_t *geom_of;
@@ -150,7 +152,7 @@ uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uin
         sl.n_angles = (int32_t)A;
         sl.axis = axis;
         sl.corr = 1.0f;
-        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32, NULL,
+        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32, UIR_DTYPE_F32, NULL,
                                         uirk_plan_grid((int)fv.n, 4), d_err, st), "FP launch");
     }
     if (s == UIR_OK)
@@ -169,7 +171,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
 {
     if (!res || !specs || !out || n_slices == 0)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument or empty batch");
-    if (storage != UIR_DTYPE_F32 && storage != UIR_DTYPE_F16 && storage != UIR_DTYPE_BF16)
+    int sdt = 0, idt = 0;
+    if (!uir_storage_split(storage, &sdt, &idt))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
     *out = NULL;
     for (uint32_t i = 0; i < n_slices; ++i) {
@@ -193,7 +196,10 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
     b->res = uir_resources_retain(res);
     b->dtype = storage;
-    b->es = storage == UIR_DTYPE_F32 ? 4 : 2;
+    b->sdt = sdt;
+    b->idt = idt;
+    b->ess = uir_dtype_size(sdt);
+    b->esi = uir_dtype_size(idt);
     b->n_slices = n_slices;
     b->specs = (uir_slice_spec *)malloc(sizeof *specs * n_slices);
     b->geom_of = (uint32_t *)calloc(n_slices, sizeof(uint32_t));
@@ -242,7 +248,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     size_t total = 0;
     for (uint32_t i = 0; i < n_slices; ++i) {
         const size_t nv = (size_t)specs[i].n * specs[i].n, ns = (size_t)specs[i].n_angles * specs[i].n_dets;
-        total += align_up(nv * b->es) + 2 * align_up(ns * b->es);
+        total += align_up(nv * b->esi) + 2 * align_up(ns * b->ess);
     }
     for (uint32_t g = 0; g < b->n_geoms; ++g)
         total += align_up((size_t)b->geoms[g].A * b->geoms[g].D * 4) + align_up((size_t)b->geoms[g].A * 8);
@@ -263,11 +269,11 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         for (uint32_t i = 0; i < n_slices; ++i) {
             const size_t nv = (size_t)specs[i].n * specs[i].n, ns = (size_t)specs[i].n_angles * specs[i].n_dets;
             b->x[i] = p;
-            p += align_up(nv * b->es);
+            p += align_up(nv * b->esi);
             b->b[i] = p;
-            p += align_up(ns * b->es);
+            p += align_up(ns * b->ess);
             b->r[i] = p;
-            p += align_up(ns * b->es);
+            p += align_up(ns * b->ess);
         }
         for (uint32_t g = 0; g < b->n_geoms; ++g) {
             geom_t *G = &b->geoms[g];
@@ -358,7 +364,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         for (int g = 0; g < UIR_FP_SHAPES && s == UIR_OK; ++g) {
             kgroup *k = &b->fpg[g];
             k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
-            k->tile.fp_angles = g ? 8 : UIRK_FP_ANGLES;
+            k->tile.fp_angles = k_fp_shape_angles[g];
             for (size_t i = 0; i < fv[g].n; ++i)
                 cost[i] = uirk_fp_item_cost(&fv[g].v[i]);
             uirk_plan_xcd(cost, (int)fv[g].n, uirk_plan_num_xcd(), &k->part);
@@ -413,12 +419,12 @@ uir_status uir_batch_set_sinogram(uir_batch *b, uint32_t slice, const float *sin
     const size_t ns = (size_t)b->specs[slice].n_angles * b->specs[slice].n_dets;
     hipStream_t st = b->res->stream;
     if (on_device)
-        return uir_check_hip(uirk_convert(sino, UIR_DTYPE_F32, b->b[slice], b->dtype, ns, st), "convert");
+        return uir_check_hip(uirk_convert(sino, UIR_DTYPE_F32, b->b[slice], b->sdt, ns, st), "convert");
     float *tmp = NULL;
     TRY(uir_dmalloc((void **)&tmp, ns * 4));
     uir_status s = uir_check_hip(hipMemcpyAsync(tmp, sino, ns * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_convert(tmp, UIR_DTYPE_F32, b->b[slice], b->dtype, ns, st), "convert");
+        s = uir_check_hip(uirk_convert(tmp, UIR_DTYPE_F32, b->b[slice], b->sdt, ns, st), "convert");
     if (s == UIR_OK)
         s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     uir_dfree(tmp);
@@ -431,7 +437,7 @@ uir_status uir_batch_reset(uir_batch *b)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "batch is NULL");
     for (uint32_t i = 0; i < b->n_slices; ++i) {
         const size_t nv = (size_t)b->specs[i].n * b->specs[i].n;
-        TRY(uir_check_hip(hipMemsetAsync(b->x[i], 0, nv * b->es, b->res->stream), "hipMemsetAsync"));
+        TRY(uir_check_hip(hipMemsetAsync(b->x[i], 0, nv * b->esi, b->res->stream), "hipMemsetAsync"));
     }
     return UIR_OK;
 }
@@ -442,7 +448,8 @@ static uir_status launch_fp_groups(uir_batch *b)
         const kgroup *k = &b->fpg[g];
         if (k->n_items)
             TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, (const uirk_fp_item *)k->d_items, k->n_items, &k->part, 1,
-                                            b->dtype, &k->tile, k->grid, b->d_err, b->res->stream), "FP launch"));
+                                            b->sdt, b->idt, &k->tile, k->grid, b->d_err, b->res->stream),
+                              "FP launch"));
     }
     return UIR_OK;
 }
@@ -453,7 +460,8 @@ static uir_status launch_bp_groups(uir_batch *b)
         const kgroup *k = &b->bpg[g];
         if (k->n_items)
             TRY(uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, (const uirk_bp_item *)k->d_items, k->n_items, &k->part, 1,
-                                            b->dtype, &k->tile, k->grid, b->d_err, b->res->stream), "BP launch"));
+                                            b->sdt, b->idt, &k->tile, k->grid, b->d_err, b->res->stream),
+                              "BP launch"));
     }
     return UIR_OK;
 }
@@ -516,12 +524,12 @@ uir_status uir_batch_get_image(uir_batch *b, uint32_t slice, float *dst, int on_
     if (err)
         return uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window (flag %d)", err);
     if (on_device) {
-        TRY(uir_check_hip(uirk_convert(b->x[slice], b->dtype, dst, UIR_DTYPE_F32, nv, st), "convert"));
+        TRY(uir_check_hip(uirk_convert(b->x[slice], b->idt, dst, UIR_DTYPE_F32, nv, st), "convert"));
         return uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     }
     float *tmp = NULL;
     TRY(uir_dmalloc((void **)&tmp, nv * 4));
-    uir_status s = uir_check_hip(uirk_convert(b->x[slice], b->dtype, tmp, UIR_DTYPE_F32, nv, st), "convert");
+    uir_status s = uir_check_hip(uirk_convert(b->x[slice], b->idt, tmp, UIR_DTYPE_F32, nv, st), "convert");
     if (s == UIR_OK)
         s = uir_check_hip(hipMemcpyAsync(dst, tmp, nv * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (s == UIR_OK)

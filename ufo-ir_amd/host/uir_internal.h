@@ -60,6 +60,21 @@ uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uin
                            const float *h_sin, const float *h_cos, const uir_subset *runs, uint32_t n_runs,
                            const float *d_trig, float r_min, float *d_ray_w);
 
+/* A storage mode split into the sinogram and image element types; returns 0
+ * for an unknown mode. */
+static inline int uir_storage_split(int storage, int *sino, int *img)
+{
+    switch (storage) {
+    case UIR_DTYPE_F32: *sino = *img = UIR_DTYPE_F32; return 1;
+    case UIR_DTYPE_F16: *sino = *img = UIR_DTYPE_F16; return 1;
+    case UIR_DTYPE_BF16: *sino = *img = UIR_DTYPE_BF16; return 1;
+    case UIR_DTYPE_F16_SINO: *sino = UIR_DTYPE_F16; *img = UIR_DTYPE_F32; return 1;
+    case UIR_DTYPE_BF16_SINO: *sino = UIR_DTYPE_BF16; *img = UIR_DTYPE_F32; return 1;
+    default: return 0;
+    }
+}
+static inline size_t uir_dtype_size(int dtype) { return dtype == UIR_DTYPE_F32 ? 4 : 2; }
+
 /* autotuner (uir_tune.c): tune on a stream; the tile shape for a geometry */
 uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype storage, uint32_t reps,
                         uir_tune_result *out);

@@ -154,7 +154,7 @@ static void line_range(int n, int n_dets, double axis, const float *s, const flo
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
                  const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles)
 {
-    if (max_angles < 1 || max_angles > UIRK_FP_ANGLES)
+    if (max_angles < 1 || max_angles > UIRK_FP_ANGLES_MAX)
         max_angles = UIRK_FP_ANGLES;
     for (uint32_t si = 0; si < n_subs; ++si) {
         const int off = (int)subs[si].offset, cnt = (int)subs[si].n, dir = subs[si].direction;

@@ -69,7 +69,7 @@ typedef struct sgeom {
     struct sgeom *next;
 } sgeom;
 
-#define SRV_FP_SHAPES 2
+#define SRV_FP_SHAPES 3
 #define SRV_BP_SHAPES 4
 
 typedef struct {
@@ -101,8 +101,9 @@ typedef struct sreq {
 
 struct uir_server {
     int device;
-    uir_dtype dtype;
-    size_t es;
+    uir_dtype dtype;  /* storage mode */
+    int sdt, idt;     /* sinogram and image element types */
+    size_t ess, esi;
     uint32_t max_active;
     uint64_t budget, used;
     hipStream_t stream;
@@ -157,10 +158,10 @@ double uir_clock(void)
 
 static int is_sb(const uir_request *r) { return r->method == UIR_METHOD_SBTV_SIRT; }
 
-static uint64_t req_bytes(const uir_request *r, size_t es)
+static uint64_t req_bytes(const uir_request *r, size_t ess, size_t esi)
 {
     const uint64_t nv = (uint64_t)r->slice.n * r->slice.n, ns = (uint64_t)r->slice.n_angles * r->slice.n_dets;
-    uint64_t b = align_up(nv * es) + 2 * align_up(ns * es);
+    uint64_t b = align_up(nv * esi) + 2 * align_up(ns * ess);
     if (is_sb(r))
         b += 5 * align_up(nv * 4);
     return b + REQ_SLACK;
@@ -172,8 +173,11 @@ uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint
 {
     if (!req || !bytes)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
-    const size_t es = storage == UIR_DTYPE_F32 ? 4 : 2;
-    *bytes = req_bytes(req, es) + geom_bytes(req->slice.n_dets, req->slice.n_angles);
+    int sdt = 0, idt = 0;
+    if (!uir_storage_split(storage, &sdt, &idt))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
+    *bytes = req_bytes(req, uir_dtype_size(sdt), uir_dtype_size(idt)) +
+             geom_bytes(req->slice.n_dets, req->slice.n_angles);
     return UIR_OK;
 }
 
@@ -275,7 +279,7 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
         }
     }
     g->tile = uir_tile_for(sp, s->dtype);
-    g->fp_shape = g->tile.fp_angles == 8;
+    g->fp_shape = g->tile.fp_angles == 8 ? 1 : (g->tile.fp_angles == 32 ? 2 : 0);
     g->bp_shape = (g->tile.bp_angles == 16) * 2 + (g->tile.bp_rows == 32);
     {
         uirk_fp_vec fv = {0};
@@ -423,17 +427,17 @@ static uir_status req_setup(uir_server *s, sreq *q)
         q->g = g;
     }
     q->g->last_used = uir_clock();
-    size_t total = align_up(nv * s->es) + 2 * align_up(ns * s->es);
+    size_t total = align_up(nv * s->esi) + 2 * align_up(ns * s->ess);
     if (is_sb(&q->rq))
         total += 5 * align_up(nv * 4);
     TRY(uir_dmalloc(&q->mem, total));
     char *p = (char *)q->mem;
     q->x = p;
-    p += align_up(nv * s->es);
+    p += align_up(nv * s->esi);
     q->b = p;
-    p += align_up(ns * s->es);
+    p += align_up(ns * s->ess);
     q->r = p;
-    p += align_up(ns * s->es);
+    p += align_up(ns * s->ess);
     if (is_sb(&q->rq)) {
         float **arr[5] = {&q->bx, &q->by, &q->ex, &q->ey, &q->tv};
         for (int i = 0; i < 5; ++i) {
@@ -445,7 +449,7 @@ static uir_status req_setup(uir_server *s, sreq *q)
     TRY(uir_check_hip(uirk_set((float *)q->mem, total / 4, 0.0f, s->stream), "set"));
     TRY(tmp_reserve(s, ns > nv ? ns : nv));
     TRY(uir_check_hip(hipMemcpyAsync(s->d_tmp, q->h_sino, ns * 4, hipMemcpyHostToDevice, s->stream), "upload"));
-    TRY(uir_check_hip(uirk_convert(s->d_tmp, UIR_DTYPE_F32, q->b, s->dtype, ns, s->stream), "convert"));
+    TRY(uir_check_hip(uirk_convert(s->d_tmp, UIR_DTYPE_F32, q->b, s->sdt, ns, s->stream), "convert"));
     TRY(uir_check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize"));
     free(q->h_sino);
     q->h_sino = NULL;
@@ -629,17 +633,17 @@ static uir_status step_launch(uir_server *s)
     for (uint32_t i = 0; i < s->n_active; ++i) {
         sreq *q = s->active[i];
         if (is_sb(&q->rq))
-            TRY(uir_check_hip(uirk_sb_tv(q->x, s->dtype, q->ex, q->ey, q->tv, q->rq.slice.n, s->stream), "sb_tv"));
+            TRY(uir_check_hip(uirk_sb_tv(q->x, s->idt, q->ex, q->ey, q->tv, q->rq.slice.n, s->stream), "sb_tv"));
     }
     for (int c = 0; c < SRV_FP_SHAPES; ++c)
         if (s->fpg[c].n)
             TRY(uir_check_hip(uirk_fp_table(s->d_fp_sl, NULL, (const uirk_fp_item *)s->fpg[c].d_items, s->fpg[c].n,
-                                            &s->fpg[c].part, 1, s->dtype, &s->fpg[c].tile, s->fpg[c].grid, s->d_err,
+                                            &s->fpg[c].part, 1, s->sdt, s->idt, &s->fpg[c].tile, s->fpg[c].grid, s->d_err,
                                             s->stream), "FP launch"));
     for (int c = 0; c < SRV_BP_SHAPES; ++c)
         if (s->bpg[c].n)
             TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, (const uirk_bp_item *)s->bpg[c].d_items, s->bpg[c].n,
-                                            &s->bpg[c].part, 1, s->dtype, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
+                                            &s->bpg[c].part, 1, s->sdt, s->idt, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
                                             s->stream), "BP launch"));
     for (uint32_t i = 0; i < s->n_active; ++i) {
         sreq *q = s->active[i];
@@ -651,7 +655,7 @@ static uir_status step_launch(uir_server *s)
             q->inner = 0;
             /* the last outer iteration's Bregman update does not change u */
             if (q->done_steps < q->total_steps)
-                TRY(uir_check_hip(uirk_sb_shrink(q->x, s->dtype, q->bx, q->by, q->ex, q->ey, q->rq.slice.n,
+                TRY(uir_check_hip(uirk_sb_shrink(q->x, s->idt, q->bx, q->by, q->ex, q->ey, q->rq.slice.n,
                                                  q->rq.lambda, s->stream), "sb_shrink"));
         }
     }
@@ -665,7 +669,7 @@ static uir_status req_finish(uir_server *s, sreq *q)
     int err = 0;
     TRY(uir_check_hip(hipMemcpyAsync(&err, s->d_err, sizeof err, hipMemcpyDeviceToHost, s->stream), "flag read"));
     TRY(tmp_reserve(s, nv));
-    TRY(uir_check_hip(uirk_convert(q->x, s->dtype, s->d_tmp, UIR_DTYPE_F32, nv, s->stream), "convert"));
+    TRY(uir_check_hip(uirk_convert(q->x, s->idt, s->d_tmp, UIR_DTYPE_F32, nv, s->stream), "convert"));
     q->h_image = (float *)malloc(nv * 4);
     if (!q->h_image)
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
@@ -864,7 +868,8 @@ uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server *
     if (!cfg || !out)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = NULL;
-    if (cfg->storage != UIR_DTYPE_F32 && cfg->storage != UIR_DTYPE_F16 && cfg->storage != UIR_DTYPE_BF16)
+    int sdt = 0, idt = 0;
+    if (!uir_storage_split(cfg->storage, &sdt, &idt))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
     int nd = 0;
     TRY(uir_device_count(&nd));
@@ -885,14 +890,17 @@ uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server *
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
     s->device = device;
     s->dtype = cfg->storage;
-    s->es = cfg->storage == UIR_DTYPE_F32 ? 4 : 2;
+    s->sdt = sdt;
+    s->idt = idt;
+    s->ess = uir_dtype_size(sdt);
+    s->esi = uir_dtype_size(idt);
     s->max_active = cfg->max_active;
     s->budget = budget;
     s->next_ticket = 1;
     s->autotune = cfg->autotune;
     for (int c = 0; c < SRV_FP_SHAPES; ++c) {
         s->fpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
-        s->fpg[c].tile.fp_angles = c ? 8 : UIRK_FP_ANGLES;
+        s->fpg[c].tile.fp_angles = c == 1 ? 8 : (c == 2 ? 32 : UIRK_FP_ANGLES);
     }
     for (int c = 0; c < SRV_BP_SHAPES; ++c) {
         s->bpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
@@ -931,7 +939,7 @@ uir_status uir_server_submit(uir_server *s, const uir_request *req, const float 
     if (!s || !req || !sinogram || !ticket)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
     TRY(check_request(req));
-    const uint64_t need = req_bytes(req, s->es) + geom_bytes(req->slice.n_dets, req->slice.n_angles);
+    const uint64_t need = req_bytes(req, s->ess, s->esi) + geom_bytes(req->slice.n_dets, req->slice.n_angles);
     if (need > s->budget)
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "request needs %llu device bytes, the budget is %llu",
                              (unsigned long long)need, (unsigned long long)s->budget);
@@ -946,7 +954,7 @@ uir_status uir_server_submit(uir_server *s, const uir_request *req, const float 
     }
     memcpy(q->h_sino, sinogram, ns * 4);
     q->rq = *req;
-    q->bytes = req_bytes(req, s->es);
+    q->bytes = req_bytes(req, s->ess, s->esi);
     q->stats.device_bytes = q->bytes;
     q->stats.t_submit = uir_clock();
     q->state = RQ_PENDING;

@@ -3,7 +3,7 @@
  *
  * For one geometry and storage dtype it builds a probe of a few slices on
  * scratch device memory (the work a batch of that geometry does per
- * iteration), times every FP variant (16 or 8 angles per work item) and every
+ * iteration), times every FP variant (16, 8 or 32 angles per work item) and every
  * BP variant (32 or 16 staged angles x tiles of 64 or 32 rows) with events on
  * the given stream, and keeps the fastest of each in a process-wide cache
  * keyed by geometry and dtype.  Batches and the server plan their work tables
@@ -31,7 +31,7 @@ static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
 static tune_entry *g_cache;
 static size_t g_n, g_cap;
 
-static const uir_tile k_fp_variants[2] = {{16, 32, 64}, {8, 32, 64}};
+static const uir_tile k_fp_variants[3] = {{16, 32, 64}, {8, 32, 64}, {32, 32, 64}};
 static const uir_tile k_bp_variants[4] = {{16, 32, 64}, {16, 32, 32}, {16, 16, 64}, {16, 16, 32}};
 
 static int key_eq(const tune_entry *e, const uir_slice_spec *s, int dtype)
@@ -62,7 +62,7 @@ uir_status uir_tune_set(const uir_slice_spec *spec, uir_dtype storage, const uir
     if (!spec || !tile)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
     const uirk_tile t = {tile->fp_angles, tile->bp_angles, tile->bp_rows};
-    if (!((t.fp_angles == 16 || t.fp_angles == 8) && (t.bp_angles == 32 || t.bp_angles == 16) &&
+    if (!((t.fp_angles == 16 || t.fp_angles == 8 || t.fp_angles == 32) && (t.bp_angles == 32 || t.bp_angles == 16) &&
           (t.bp_rows == 64 || t.bp_rows == 32)))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "tile %d/%d/%d is not a variant", t.fp_angles, t.bp_angles,
                              t.bp_rows);
@@ -116,8 +116,8 @@ static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 /* Average ms of `reps` launches of one variant (after one untimed launch). */
 static uir_status time_variant(hipStream_t st, int fp, const uirk_slice *d_sl, const void *d_items, int n_items,
-                               const uirk_xcd_part *part, int dtype, const uirk_tile *tile, int grid, int *d_err,
-                               uint32_t reps, float *ms)
+                               const uirk_xcd_part *part, int sdt, int idt, const uirk_tile *tile, int grid,
+                               int *d_err, uint32_t reps, float *ms)
 {
     hipEvent_t e0, e1;
     uir_status s = uir_check_hip(hipEventCreate(&e0), "hipEventCreate");
@@ -132,10 +132,10 @@ static uir_status time_variant(hipStream_t st, int fp, const uirk_slice *d_sl, c
             s = uir_check_hip(hipEventRecord(e0, st), "hipEventRecord");
         if (s != UIR_OK)
             break;
-        const int e = fp ? uirk_fp_table(d_sl, NULL, (const uirk_fp_item *)d_items, n_items, part, 1, dtype, tile, grid,
-                                         d_err, st)
-                         : uirk_bp_table(d_sl, NULL, (const uirk_bp_item *)d_items, n_items, part, 1, dtype, tile, grid,
-                                         d_err, st);
+        const int e = fp ? uirk_fp_table(d_sl, NULL, (const uirk_fp_item *)d_items, n_items, part, 1, sdt, idt, tile,
+                                         grid, d_err, st)
+                         : uirk_bp_table(d_sl, NULL, (const uirk_bp_item *)d_items, n_items, part, 1, sdt, idt, tile,
+                                         grid, d_err, st);
         s = uir_check_hip(e, fp ? "FP launch" : "BP launch");
     }
     if (s == UIR_OK)
@@ -157,7 +157,8 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
 {
     if (!spec || !out)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (storage != UIR_DTYPE_F32 && storage != UIR_DTYPE_F16 && storage != UIR_DTYPE_BF16)
+    int sdt = 0, idt = 0;
+    if (!uir_storage_split(storage, &sdt, &idt))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
     const uint32_t n = spec->n, D = spec->n_dets, A = spec->n_angles;
     if (!n || !D || !A || n > 16384 || (uint64_t)n * n * 4 >= (1ull << 32) || (uint64_t)A * D * 4 >= (1ull << 32))
@@ -166,7 +167,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
         reps = 3;
     memset(out, 0, sizeof *out);
     hipStream_t st = (hipStream_t)stream;
-    const size_t es = storage == UIR_DTYPE_F32 ? 4 : 2;
+    const size_t ess = uir_dtype_size(sdt), esi = uir_dtype_size(idt);
     const size_t nv = (size_t)n * n, ns = (size_t)A * D;
     /* about 8 M pixels of probe, at least one slice, at most 16 */
     uint32_t B = (uint32_t)((8u << 20) / nv);
@@ -191,7 +192,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
     const uint32_t n_runs = uir_geom_runs(h_sin, h_cos, A, runs);
     uirk_plan_trig(h_sin, h_cos, A, trig);
     /* per slice: x (storage), b and r (storage); shared: ray weights (FP32, ones), trig */
-    const size_t per = align_up(nv * es) + 2 * align_up(ns * es);
+    const size_t per = align_up(nv * esi) + 2 * align_up(ns * ess);
     const size_t total = (size_t)B * per + align_up(ns * 4) + align_up((size_t)A * 8);
     if ((s = uir_dmalloc(&mem, total)) != UIR_OK || (s = uir_dmalloc((void **)&d_err, sizeof(int))) != UIR_OK)
         goto out;
@@ -211,8 +212,8 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
         char *p = base + (size_t)i * per;
         f->x = p;
         f->x_out = p;
-        f->b = p + align_up(nv * es);
-        f->r = p + align_up(nv * es) + align_up(ns * es);
+        f->b = p + align_up(nv * esi);
+        f->r = p + align_up(nv * esi) + align_up(ns * ess);
         f->ray_w = d_rw;
         f->trig = d_trig;
         f->n = (int32_t)n;
@@ -234,7 +235,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
     const int n_xcd = uirk_plan_num_xcd();
     float best_fp = 3.0e38f, best_bp = 3.0e38f;
     out->best = (uir_tile){UIRK_FP_ANGLES, 32, UIRK_BP_TILE};
-    for (int v = 0; v < 2 && s == UIR_OK; ++v) {
+    for (int v = 0; v < 3 && s == UIR_OK; ++v) {
         const uirk_tile tl = {k_fp_variants[v].fp_angles, 32, UIRK_BP_TILE};
         uirk_fp_vec fv = {0};
         for (uint32_t i = 0; i < B && s == UIR_OK; ++i)
@@ -256,7 +257,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, fv.v, fv.n * sizeof(uirk_fp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 1, d_sl, d_items, (int)fv.n, &part, storage, &tl, uirk_plan_grid((int)fv.n, 4), d_err,
+            s = time_variant(st, 1, d_sl, d_items, (int)fv.n, &part, sdt, idt, &tl, uirk_plan_grid((int)fv.n, 4), d_err,
                              reps, &out->fp_ms[v]);
         if (s == UIR_OK && out->fp_ms[v] < best_fp) {
             best_fp = out->fp_ms[v];
@@ -278,7 +279,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, bv.v, bv.n * sizeof(uirk_bp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 0, d_sl, d_items, (int)bv.n, &part, storage, &tl, uirk_plan_grid((int)bv.n, 8), d_err,
+            s = time_variant(st, 0, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl, uirk_plan_grid((int)bv.n, 8), d_err,
                              reps, &out->bp_ms[v]);
         if (s == UIR_OK && out->bp_ms[v] < best_bp) {
             best_bp = out->bp_ms[v];

@@ -67,7 +67,7 @@ class Tile(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("best", Tile), ("fp_ms", C.c_float * 2), ("bp_ms", C.c_float * 4), ("probe_slices", C.c_uint32)]
+    _fields_ = [("best", Tile), ("fp_ms", C.c_float * 3), ("bp_ms", C.c_float * 4), ("probe_slices", C.c_uint32)]
 
 
 class RequestStats(C.Structure):
@@ -477,7 +477,8 @@ def header_functions(path=HEADER_PATH):
     return sorted(set(names))
 
 
-DTYPES = {"f32": DTYPE_F32, "f16": DTYPE_F16, "bf16": DTYPE_BF16}
+DTYPES = {"f32": DTYPE_F32, "f16": DTYPE_F16, "bf16": DTYPE_BF16,
+          "f16-sino": 3, "bf16-sino": 4}   # sinograms FP16 / BF16, images FP32
 
 
 def slice_spec(s):
