@@ -287,6 +287,8 @@ def main():
     ap.add_argument("--c3", action="store_true", help="batched SIRT over 512 slices of uneven size")
     ap.add_argument("--c3-slices", type=int, default=512)
     ap.add_argument("--c3-iters", type=int, default=100)
+    ap.add_argument("--tile", default=None,
+                    help="fp_angles,bp_angles,bp_rows: use this tile shape instead of tuning (no probe launches)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for counter collection: build the batch, run 3 iterations with direct launches and one "
                          "512 MiB device-to-device copy (the calibration of the byte counters), then exit")
@@ -334,7 +336,11 @@ def main():
     t_data = time.time() - t0
 
     if args.profile_pass:
-        uir.tune_geometry(res, dict(n=n, n_dets=d, n_angles=a, step=step), args.storage, reps=3)
+        if args.tile:
+            uir.tune_set(dict(n=n, n_dets=d, n_angles=a, step=step), args.storage,
+                         *(int(v) for v in args.tile.split(",")))
+        else:
+            uir.tune_geometry(res, dict(n=n, n_dets=d, n_angles=a, step=step), args.storage, reps=3)
         specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
         bt = uir.Batch(res, specs, args.storage)
         for i in range(args.batch):
@@ -372,8 +378,14 @@ def main():
     sirt.process(ds, out)
     plain_img = out.numpy()
 
-    # ---- batched path: tile shapes tuned for this geometry first
-    tune = uir.tune_geometry(res, dict(n=n, n_dets=d, n_angles=a, step=step), args.storage, reps=3)
+    # ---- batched path: tile shapes tuned for this geometry first (or given)
+    geo = dict(n=n, n_dets=d, n_angles=a, step=step)
+    if args.tile:
+        fa, ba, br = (int(v) for v in args.tile.split(","))
+        uir.tune_set(geo, args.storage, fa, ba, br)
+        tune = {"best": {"fp_angles": fa, "bp_angles": ba, "bp_rows": br}}
+    else:
+        tune = uir.tune_geometry(res, geo, args.storage, reps=3)
     specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
     bt = uir.Batch(res, specs, args.storage)
     for i in range(args.batch):
