@@ -33,7 +33,8 @@ class Geom(C.Structure):
 class AsdPocsParams(C.Structure):
     _fields_ = [("beta", C.c_float), ("beta_red", C.c_float), ("alpha", C.c_float),
                 ("alpha_red", C.c_float), ("r_max", C.c_float), ("positive_constraint", C.c_int),
-                ("df_is_sirt", C.c_int), ("df_iters", C.c_uint32), ("df_relax_initial", C.c_float)]
+                ("df_is_sirt", C.c_int), ("df_iters", C.c_uint32), ("df_relax_initial", C.c_float),
+                ("ng", C.c_uint32), ("textbook", C.c_int)]
 
 
 def build():
@@ -91,6 +92,8 @@ def lib():
         L.orc64_bp.argtypes = [G, f64p, f64p]
         L.orc64_sirt.argtypes = [G, f64p, f64p, C.c_uint32, C.c_double, C.c_int, C.c_double]
         L.orc64_sart.argtypes = [G, f64p, f64p, C.c_uint32, C.c_double]
+        L.orc64_l1_grad.argtypes = [f64p, f64p, C.c_int]
+        L.orc64_asdpocs.argtypes = [G, f64p, f64p, C.c_uint32, C.POINTER(AsdPocsParams), C.c_double, C.c_void_p]
         L.orc64_sb_tv.argtypes = [f64p, f64p, f64p, f64p, C.c_int]
         L.orc64_sb_shrink.argtypes = [f64p, f64p, f64p, f64p, f64p, C.c_int, C.c_double]
         L.orc64_sbsirt.argtypes = [G, f64p, f64p, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -239,14 +242,44 @@ def sb_shrink64(u, bx, by, lam):
     return bx, by, ex, ey
 
 
+def _asd_params(beta, beta_red, alpha, alpha_red, r_max, positive_constraint, df, df_iters, df_relax_initial, ng,
+                textbook):
+    return AsdPocsParams(beta, beta_red, alpha, alpha_red, r_max, int(bool(positive_constraint)),
+                         1 if df == "sirt" else 0, int(df_iters), df_relax_initial, int(ng), int(bool(textbook)))
+
+
 def asdpocs(geom, b, iters, beta=1.0, beta_red=0.995, alpha=0.2, alpha_red=0.95, r_max=0.95,
-            positive_constraint=True, df="sart", df_iters=10, df_relax_initial=0.25, diag=False):
-    p = AsdPocsParams(beta, beta_red, alpha, alpha_red, r_max, int(bool(positive_constraint)),
-                      1 if df == "sirt" else 0, int(df_iters), df_relax_initial)
+            positive_constraint=True, df="sart", df_iters=10, df_relax_initial=0.25, diag=False, ng=20,
+            textbook=False):
+    """Ref-compat FP32 ASD-POCS; textbook=True is the published variant
+    (warm-started data step, L2 distances, normalised TV steps, ng of them,
+    output after TV)."""
+    p = _asd_params(beta, beta_red, alpha, alpha_red, r_max, positive_constraint, df, df_iters, df_relax_initial,
+                    ng, textbook)
     out = np.zeros(geom.vol_shape(), np.float32)
     d = np.zeros(4 * max(int(iters), 1), np.float32)
     lib().orc_asdpocs(geom.ptr, _f32(b), out, int(iters), C.byref(p), d.ctypes.data if diag else None)
     return (out, d.reshape(-1, 4)[:iters]) if diag else out
+
+
+def asdpocs64(geom, b, iters, beta=1.0, beta_red=0.995, alpha=0.2, alpha_red=0.95, r_max=0.95,
+              positive_constraint=True, df="sart", df_iters=10, df_relax_initial=0.25, ng=20, textbook=False,
+              ray_weight_min=0.0, diag=False):
+    """FP64 ASD-POCS with exact coordinates (both variants of asdpocs)."""
+    p = _asd_params(beta, beta_red, alpha, alpha_red, r_max, positive_constraint, df, df_iters, df_relax_initial,
+                    ng, textbook)
+    out = np.zeros(geom.vol_shape(), np.float64)
+    d = np.zeros(4 * max(int(iters), 1), np.float64)
+    lib().orc64_asdpocs(geom.ptr, np.ascontiguousarray(b, np.float64), out, int(iters), C.byref(p),
+                        float(ray_weight_min), d.ctypes.data if diag else None)
+    return (out, d.reshape(-1, 4)[:iters]) if diag else out
+
+
+def l1_grad64(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    lib().orc64_l1_grad(x, out, x.shape[0])
+    return out
 
 
 def sbtv(geom, b, iters, lam=0.1, mu=0.5, dot_mode=0):

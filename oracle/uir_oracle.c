@@ -331,13 +331,17 @@ float orc_l2_compat(const float *a, size_t w, size_t h)
     return (float)sqrt(orc_dot_compat(a, a, w, h));
 }
 
+/* The inner product of the textbook variants (compat = false): every element
+ * once, accumulated in double and rounded to float once. */
 static float dot_plain(const float *a, const float *b, size_t n)
 {
-    float s = 0;
+    double s = 0;
     for (size_t i = 0; i < n; ++i)
-        s += a[i] * b[i];
-    return s;
+        s += (double)a[i] * (double)b[i];
+    return (float)s;
 }
+
+static float l2_plain(const float *a, size_t n) { return (float)sqrt((double)dot_plain(a, a, n)); }
 
 void orc_mul_scalar(float *v, size_t n, float m) /* :324-340 */
 {
@@ -470,7 +474,8 @@ static run_list make_runs(const orc_geom *g)
 
 /* SIRT, ufo-ir-sirt-task.c:163-222.  The reference has no positivity clamp;
  * posc != 0 composes POSC (ufo-ir-basic-ops.cl:255-273) after each update. */
-void orc_sirt(const orc_geom *g, const float *b, float *x, uint32_t iters, float relax, int posc)
+static void sirt_core(const orc_geom *g, const float *b, float *x, uint32_t iters, float relax, int posc,
+                      int warm)
 {
     const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
     run_list rl = make_runs(g);
@@ -487,7 +492,8 @@ void orc_sirt(const orc_geom *g, const float *b, float *x, uint32_t iters, float
     orc_op_set(pix_w, nv, 0.0f);
     orc_bp(g, pix_w, sino_tmp, rl.runs, rl.n_runs, 1.0f);
     orc_op_inv(pix_w, nv);
-    orc_op_set(x, nv, 0.0f);                            /* :195-197 */
+    if (!warm)
+        orc_op_set(x, nv, 0.0f);                        /* :195-197 */
 
     for (uint32_t it = 0; it < iters; ++it) {           /* :202-215 */
         memcpy(sino_tmp, b, ns * sizeof(float));
@@ -503,8 +509,14 @@ void orc_sirt(const orc_geom *g, const float *b, float *x, uint32_t iters, float
     free(vol_tmp); free(ray_w); free(sino_tmp); free(pix_w); free(rl.runs);
 }
 
-/* SART, ufo-ir-sart-task.c:173-228: per-angle ordered subsets. */
-void orc_sart(const orc_geom *g, const float *b, float *x, uint32_t iters, float relax)
+void orc_sirt(const orc_geom *g, const float *b, float *x, uint32_t iters, float relax, int posc)
+{
+    sirt_core(g, b, x, iters, relax, posc, 0);
+}
+
+/* SART, ufo-ir-sart-task.c:173-228: per-angle ordered subsets.  warm != 0
+ * starts from x as given instead of zero (the textbook ASD-POCS data step). */
+static void sart_core(const orc_geom *g, const float *b, float *x, uint32_t iters, float relax, int warm)
 {
     const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
     orc_subset *subs = (orc_subset *)malloc(sizeof(orc_subset) * (size_t)(g->n_angles > 0 ? g->n_angles : 1));
@@ -518,7 +530,8 @@ void orc_sart(const orc_geom *g, const float *b, float *x, uint32_t iters, float
     for (int i = 0; i < n_sub; ++i)
         orc_fp_subset(g, vol_tmp, ray_w, subs[i], 1.0f);
     orc_op_inv(ray_w, ns);
-    orc_op_set(x, nv, 0.0f);                            /* :205-207 */
+    if (!warm)
+        orc_op_set(x, nv, 0.0f);                        /* :205-207 */
     for (uint32_t it = 0; it < iters; ++it) {           /* :208-220 */
         memcpy(sino_tmp, b, ns * sizeof(float));
         for (int i = 0; i < n_sub; ++i) {
@@ -530,11 +543,30 @@ void orc_sart(const orc_geom *g, const float *b, float *x, uint32_t iters, float
     free(subs); free(sino_tmp); free(vol_tmp); free(ray_w);
 }
 
-/* ASD-POCS, ufo-ir-asdpocs-task.c:411-509 with tvstd :550-580. */
+void orc_sart(const orc_geom *g, const float *b, float *x, uint32_t iters, float relax)
+{
+    sart_core(g, b, x, iters, relax, 0);
+}
+
+/* ASD-POCS, ufo-ir-asdpocs-task.c:411-509 with tvstd :550-580 (textbook = 0).
+ *
+ * textbook != 0 is ASD-POCS as published (Sidky and Pan, Phys. Med. Biol. 53
+ * (2008) 4777, Sec. 4): the data step continues from the current image
+ * instead of zero, the distances dd, dp, dg are L2 norms, dp is measured from
+ * the image at the start of the iteration, each of the ng TV steps moves
+ * dtgv along the L2-normalised gradient (a zero gradient ends the steps), and
+ * the output is the image after the TV steps.  The gradient is the
+ * reference's smoothed TV gradient (ufo-math-tvstd-method.cl:5-57). */
+static float l1_or_l2(const float *v, size_t w, size_t h, int textbook)
+{
+    return textbook ? l2_plain(v, w * h) : orc_l1_norm(v, w, h);
+}
+
 void orc_asdpocs(const orc_geom *g, const float *b, float *out, uint32_t iters,
                  const orc_asdpocs_params *p, float *diag)
 {
     const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
+    const int tb = p->textbook;
     run_list rl = make_runs(g);
     float *x = (float *)calloc(nv, sizeof(float));          /* :441-442 */
     float *x_prev = (float *)calloc(nv, sizeof(float));     /* :444-445 */
@@ -544,36 +576,42 @@ void orc_asdpocs(const orc_geom *g, const float *b, float *out, uint32_t iters,
     float dp = 1.0f, dd = 1.0f, dg = 1.0f, dtgv = 1.0f;     /* :439 */
     float beta = p->beta;                                   /* :453 */
     float df_relax = p->df_relax_initial;
+    const uint32_t ng = tb ? p->ng : 20;                    /* tvstd :573: 20 steps, ng unused */
 
     for (uint32_t it = 0; it < iters; ++it) {
         /* :459 g_object_set on a [0,1] float property: out-of-range values are
          * rejected by GLib and the previous value stays. */
         if (beta >= 0.0f && beta <= 1.0f)
             df_relax = beta;
+        if (tb)
+            memcpy(x_prev, x, nv * sizeof(float));
         if (p->df_is_sirt)                                  /* :460 */
-            orc_sirt(g, b, x, p->df_iters, df_relax, 0);
+            sirt_core(g, b, x, p->df_iters, df_relax, 0, tb);
         else
-            orc_sart(g, b, x, p->df_iters, df_relax);
+            sart_core(g, b, x, p->df_iters, df_relax, tb);
         if (p->positive_constraint)                         /* :463-465 */
             orc_op_posc(x, x, nv);
-        memcpy(out, x, nv * sizeof(float));                 /* :468 */
+        if (!tb)
+            memcpy(out, x, nv * sizeof(float));             /* :468 */
         memcpy(b_res, b, ns * sizeof(float));               /* :471-475 */
         orc_fp(g, x, b_res, rl.runs, rl.n_runs, -1.0f);
-        dd = orc_l1_norm(b_res, (size_t)g->n_dets, (size_t)g->n_angles); /* :478 */
+        dd = l1_or_l2(b_res, (size_t)g->n_dets, (size_t)g->n_angles, tb); /* :478 */
         orc_op_sub(x, x_prev, x_res, nv);                   /* :481-482 */
-        dp = orc_l1_norm(x_res, (size_t)g->n, (size_t)g->n);
+        dp = l1_or_l2(x_res, (size_t)g->n, (size_t)g->n, tb);
         if (it == 0)                                        /* :485-487 */
             dtgv = p->alpha * dp;
         memcpy(x_prev, x, nv * sizeof(float));              /* :490 */
         orc_op_set(grad, nv, 0.0f);                         /* tvstd :559-579 */
-        for (int k = 0; k < 20; ++k) {
+        for (uint32_t k = 0; k < ng; ++k) {
             orc_l1_grad(x, grad, g->n, g->n);
-            float l1 = orc_l1_norm(grad, (size_t)g->n, (size_t)g->n);
-            float factor = dtgv / l1;
+            float norm = l1_or_l2(grad, (size_t)g->n, (size_t)g->n, tb);
+            if (tb && !(norm > 0.0f))
+                break;
+            float factor = dtgv / norm;
             orc_op_sub_scaled(x, grad, factor, x, nv);
         }
         orc_op_sub(x, x_prev, x_res, nv);                   /* :496-497 */
-        dg = orc_l1_norm(x_res, (size_t)g->n, (size_t)g->n);
+        dg = l1_or_l2(x_res, (size_t)g->n, (size_t)g->n, tb);
         beta *= p->beta_red;                                /* :498 */
         if (diag) {
             diag[4 * it + 0] = dd;
@@ -584,6 +622,8 @@ void orc_asdpocs(const orc_geom *g, const float *b, float *out, uint32_t iters,
         const float epsilon = 0.001f;                       /* :495,501-503 */
         if (dg > p->r_max * dp && dd > epsilon)
             dtgv *= p->alpha_red;
+        if (tb)
+            memcpy(out, x, nv * sizeof(float));
     }
     free(x); free(x_prev); free(x_res); free(b_res); free(grad); free(rl.runs);
 }
@@ -944,8 +984,8 @@ void orc64_bp(const orc_geom *g, double *vol, const double *sino)
     }
 }
 
-void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc,
-                double r_min)
+static void sirt64_core(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc,
+                        double r_min, int warm)
 {
     const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
     double *R = (double *)malloc(ns * sizeof(double));
@@ -960,7 +1000,8 @@ void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, d
     for (size_t i = 0; i < ns; ++i) R[i] = (R[i] != 0 && R[i] >= r_min) ? 1.0 / R[i] : 0.0;
     orc64_bp(g, C, ones_s);
     for (size_t i = 0; i < nv; ++i) C[i] = C[i] != 0 ? 1.0 / C[i] : 0.0;
-    for (size_t i = 0; i < nv; ++i) x[i] = 0.0;
+    if (!warm)
+        for (size_t i = 0; i < nv; ++i) x[i] = 0.0;
     for (uint32_t it = 0; it < iters; ++it) {
         orc64_fp(g, x, r);
         for (size_t i = 0; i < ns; ++i) r[i] = (b[i] - r[i]) * R[i];
@@ -971,6 +1012,12 @@ void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, d
         }
     }
     free(R); free(C); free(ones_v); free(ones_s); free(r); free(bp);
+}
+
+void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc,
+                double r_min)
+{
+    sirt64_core(g, b, x, iters, relax, posc, r_min, 0);
 }
 
 /* single-angle FP/BP helpers for SART64 */
@@ -994,7 +1041,8 @@ static void bp64_angle_add(const orc_geom *g, double *vol, double *tmp, const do
     for (size_t i = 0; i < nv; ++i) vol[i] += relax * tmp[i];
 }
 
-void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax)
+static void sart64_core(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, double r_min,
+                        int warm)
 {
     const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
     const int D = g->n_dets;
@@ -1004,8 +1052,9 @@ void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, d
     double *tmp = (double *)malloc(nv * sizeof(double));
     for (size_t i = 0; i < nv; ++i) ones_v[i] = 1.0;
     orc64_fp(g, ones_v, R);
-    for (size_t i = 0; i < ns; ++i) R[i] = R[i] != 0 ? 1.0 / R[i] : 0.0;
-    for (size_t i = 0; i < nv; ++i) x[i] = 0.0;
+    for (size_t i = 0; i < ns; ++i) R[i] = (R[i] != 0 && R[i] >= r_min) ? 1.0 / R[i] : 0.0;
+    if (!warm)
+        for (size_t i = 0; i < nv; ++i) x[i] = 0.0;
     for (uint32_t it = 0; it < iters; ++it) {
         for (int a = 0; a < g->n_angles; ++a) {
             fp64_angle(g, x, row, a);
@@ -1015,6 +1064,99 @@ void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, d
         }
     }
     free(R); free(ones_v); free(row); free(tmp);
+}
+
+void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax)
+{
+    sart64_core(g, b, x, iters, relax, 0.0, 0);
+}
+
+/* The smoothed TV gradient of ufo-math-tvstd-method.cl:5-57 (eps 1e-3, zero
+ * outside the image) in FP64: the gradient of
+ *   TV(f) = sum_{x,y} sqrt(eps + (f[x,y] - f[x-1,y])^2 + (f[x,y] - f[x,y+1])^2)
+ * over the image extended by one zero pixel on each side. */
+void orc64_l1_grad(const double *in, double *out, int n)
+{
+    const double eps = 0.001;
+#define V(xx, yy) (((xx) < 0 || (yy) < 0 || (xx) >= n || (yy) >= n) ? 0.0 : in[(size_t)(yy) * n + (xx)])
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            const double f3 = V(x, y), f1 = V(x, y - 1), f0 = V(x - 1, y - 1), f2 = V(x - 1, y);
+            const double f4 = V(x + 1, y), f6 = V(x + 1, y + 1), f5 = V(x, y + 1);
+            out[(size_t)y * n + x] = (2.0 * f3 - f2 - f5) / sqrt(eps + (f3 - f2) * (f3 - f2) + (f3 - f5) * (f3 - f5))
+                                   - (f4 - f3) / sqrt(eps + (f4 - f3) * (f4 - f3) + (f4 - f6) * (f4 - f6))
+                                   - (f1 - f3) / sqrt(eps + (f1 - f3) * (f1 - f3) + (f1 - f0) * (f1 - f0));
+        }
+#undef V
+}
+
+static double norm64(const double *v, size_t n, int l2)
+{
+    double s = 0;
+    for (size_t i = 0; i < n; ++i)
+        s += l2 ? v[i] * v[i] : fabs(v[i]);
+    return l2 ? sqrt(s) : s;
+}
+
+/* ASD-POCS in FP64 with exact coordinates, both variants of orc_asdpocs.
+ * Ray weights below r_min are zeroed as in orc64_sirt. */
+void orc64_asdpocs(const orc_geom *g, const double *b, double *out, uint32_t iters, const orc_asdpocs_params *p,
+                   double r_min, double *diag)
+{
+    const size_t nv = (size_t)g->n * g->n, ns = (size_t)g->n_angles * g->n_dets;
+    const int tb = p->textbook;
+    double *x = (double *)calloc(nv, sizeof(double));
+    double *x_prev = (double *)calloc(nv, sizeof(double));
+    double *x_res = (double *)malloc(nv * sizeof(double));
+    double *b_res = (double *)malloc(ns * sizeof(double));
+    double *grad = (double *)malloc(nv * sizeof(double));
+    double dp = 1, dd = 1, dg = 1, dtgv = 1;
+    double beta = p->beta, df_relax = p->df_relax_initial;
+    const uint32_t ng = tb ? p->ng : 20;
+    for (uint32_t it = 0; it < iters; ++it) {
+        if (beta >= 0.0 && beta <= 1.0)
+            df_relax = beta;
+        if (tb)
+            memcpy(x_prev, x, nv * sizeof(double));
+        if (p->df_is_sirt)
+            sirt64_core(g, b, x, p->df_iters, df_relax, 0, r_min, tb);
+        else
+            sart64_core(g, b, x, p->df_iters, df_relax, r_min, tb);
+        if (p->positive_constraint)
+            for (size_t i = 0; i < nv; ++i) x[i] = x[i] > 0 ? x[i] : 0.0;
+        if (!tb)
+            memcpy(out, x, nv * sizeof(double));
+        orc64_fp(g, x, b_res);
+        for (size_t i = 0; i < ns; ++i) b_res[i] = b[i] - b_res[i];
+        dd = norm64(b_res, ns, tb);
+        for (size_t i = 0; i < nv; ++i) x_res[i] = x[i] - x_prev[i];
+        dp = norm64(x_res, nv, tb);
+        if (it == 0)
+            dtgv = p->alpha * dp;
+        memcpy(x_prev, x, nv * sizeof(double));
+        for (uint32_t k = 0; k < ng; ++k) {
+            orc64_l1_grad(x, grad, g->n);
+            const double norm = norm64(grad, nv, tb);
+            if (tb && !(norm > 0))
+                break;
+            const double factor = dtgv / norm;
+            for (size_t i = 0; i < nv; ++i) x[i] -= factor * grad[i];
+        }
+        for (size_t i = 0; i < nv; ++i) x_res[i] = x[i] - x_prev[i];
+        dg = norm64(x_res, nv, tb);
+        beta *= p->beta_red;
+        if (diag) {
+            diag[4 * it + 0] = dd;
+            diag[4 * it + 1] = dp;
+            diag[4 * it + 2] = dg;
+            diag[4 * it + 3] = dtgv;
+        }
+        if (dg > p->r_max * dp && dd > 0.001)
+            dtgv *= p->alpha_red;
+        if (tb)
+            memcpy(out, x, nv * sizeof(double));
+    }
+    free(x); free(x_prev); free(x_res); free(b_res); free(grad);
 }
 
 /* ---- split-Bregman TV with SIRT-preconditioned sweeps (FP64) ----
@@ -1063,7 +1205,7 @@ void orc64_sbsirt(const orc_geom *g, const double *f, double *u, uint32_t outer,
     const int n = g->n;
     const size_t nv = (size_t)n * n, ns = (size_t)g->n_angles * g->n_dets;
     double *R = (double *)calloc(ns, sizeof(double));
-    double *ones_v = (double *)malloc(nv * sizeof(double));
+    double *ones_v = (double *)calloc(nv, sizeof(double));
     double *r = (double *)calloc(ns, sizeof(double));
     double *bp = (double *)calloc(nv, sizeof(double));
     double *tv = (double *)calloc(nv, sizeof(double));

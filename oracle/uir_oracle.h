@@ -105,6 +105,8 @@ typedef struct {
     int df_is_sirt;          /* 0: SART df-minimizer, 1: SIRT */
     uint32_t df_iters;       /* df-minimizer num-iterations */
     float df_relax_initial;  /* df-minimizer relaxation_factor before the first set */
+    uint32_t ng;             /* TV steps per iteration (textbook only; the reference takes 20) */
+    int textbook;            /* 0: the reference; 1: ASD-POCS as published (see uir_oracle.c) */
 } orc_asdpocs_params;
 /* diag (optional, 4 floats per outer iteration): dd, dp, dg, dtgv */
 void orc_asdpocs(const orc_geom *g, const float *b, float *out, uint32_t iters,
@@ -128,6 +130,10 @@ void orc64_bp(const orc_geom *g, double *vol, const double *sino);
 void orc64_sirt(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax, int posc,
                 double r_min);
 void orc64_sart(const orc_geom *g, const double *b, double *x, uint32_t iters, double relax);
+void orc64_l1_grad(const double *in, double *out, int n);
+/* ASD-POCS (both variants of orc_asdpocs); diag as orc_asdpocs */
+void orc64_asdpocs(const orc_geom *g, const double *b, double *out, uint32_t iters, const orc_asdpocs_params *p,
+                   double r_min, double *diag);
 
 /* split-Bregman TV with SIRT-preconditioned sweeps for the u-subproblem (FP64):
  * outer Bregman iterations x inner sweeps, relaxation w, TV weight lambda,

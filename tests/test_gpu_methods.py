@@ -82,3 +82,42 @@ def test_sbtv(res):
     got, ds = _run(m, b, n)
     assert O.rel_l2(ds.numpy(), bnorm) <= 1e-6   # input normalised in place
     assert O.rel_l2(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("df", ["sart", "sirt"])
+def test_asdpocs_textbook(res, df):
+    """compat = false: ASD-POCS as published (warm-started data step, L2
+    distances, ng normalised TV steps, output after TV) against the oracle's
+    textbook variant."""
+    n = d = 96
+    a, step = 45, np.float32(np.pi / 45)
+    g = O.Geometry(n, d, a, step)
+    rng = np.random.default_rng(31)
+    b = O.fp(g, ellipse_phantom(n, seed=4)) + rng.normal(0, 0.5, (a, d)).astype(np.float32)
+    ref, diag = O.asdpocs(g, b, 5, df=df, df_iters=1, diag=True, ng=10, textbook=True, beta=0.9, beta_red=0.9)
+    proj = uir.Task("parallel-projector", step=float(step), is_forward=False)
+    dfm = uir.Task(df, num_iterations=1)
+    m = uir.Task("asdpocs", projector=proj, df_minimizer=dfm, num_iterations=5, compat=False, ng=10, beta=0.9,
+                 beta_red=0.9).setup(res)
+    got, _ = _run(m, b, n)
+    assert O.rel_l2(got, ref) <= TOL
+    assert np.allclose(m.asdpocs_diagnostics(), diag, rtol=1e-4), (m.asdpocs_diagnostics(), diag)
+    # and it is not the reference's result
+    m2 = uir.Task("asdpocs", projector=proj, df_minimizer=uir.Task(df, num_iterations=1), num_iterations=5,
+                  ng=10, beta=0.9, beta_red=0.9).setup(res)
+    got2, _ = _run(m2, b, n)
+    assert O.rel_l2(got2, ref) > 1e-2
+
+
+def test_sbtv_true_dot(res):
+    """compat = false: CGS with an inner product over every element."""
+    n = d = 64
+    a, step = 45, np.float32(np.pi / 45)
+    g = O.Geometry(n, d, a, step)
+    b = O.fp(g, ellipse_phantom(n, seed=21))
+    ref, _ = O.sbtv(g, b, 2, lam=0.1, mu=0.5, dot_mode=1)
+    m, _ = _setup(res, "sbtv", n, d, a, step, num_iterations=2, compat=False, **{"lambda": 0.1, "mu": 0.5})
+    got, _ = _run(m, b, n)
+    assert O.rel_l2(got, ref) <= TOL
+    compat_ref, _ = O.sbtv(g, b, 2, lam=0.1, mu=0.5)
+    assert O.rel_l2(got, compat_ref) > 10 * TOL

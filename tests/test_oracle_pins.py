@@ -237,35 +237,43 @@ def test_difference_operators_are_mutual_adjoints():
     assert np.allclose(O.dy(u), u - np.roll(u, 1, axis=0), atol=1e-6)
 
 
-def test_l1_grad_is_gradient_of_smoothed_tv():
-    """l1_grad (ufo-math-tvstd-method.cl) is the gradient of
-    TV(f) = sum sqrt(eps + (f(x,y) - f(x-1,y))^2 + (f(x,y) - f(x,y+1))^2),
-    eps = 1e-3, over the image grown by one zero row above and one zero column
-    to the right; checked against central finite differences in FP64."""
-    rng = np.random.default_rng(6)
-    h, w = 7, 9
-    f = rng.standard_normal((h, w))
-    eps = 1e-3
+def smoothed_tv(img, eps=1e-3):
+    """TV(f) = sum sqrt(eps + (f(x,y) - f(x-1,y))^2 + (f(x,y) - f(x,y+1))^2)
+    over the image grown by one zero row above and one zero column to the
+    right (FP64)."""
+    h, w = img.shape
+    p = np.zeros((h + 2, w + 2))
+    p[1:-1, 1:-1] = img
+    c = p[0:h + 1, 1:w + 2]                       # rows y = -1 .. h-1, columns x = 0 .. w
+    return float(np.sum(np.sqrt(eps + (c - p[0:h + 1, 0:w + 1]) ** 2 + (c - p[1:h + 2, 1:w + 2]) ** 2)))
 
-    def tv(img):
-        p = np.zeros((h + 2, w + 2))
-        p[1:-1, 1:-1] = img
-        total = 0.0
-        for y in range(0, h + 1):          # rows y = -1 .. h-1 of the image
-            for x in range(1, w + 2):      # columns x = 0 .. w
-                c = p[y, x]
-                total += np.sqrt(eps + (c - p[y, x - 1]) ** 2 + (c - p[y + 1, x]) ** 2)
-        return total
 
-    g = O.l1_grad(f.astype(np.float32)).astype(np.float64)
+def tv_gradient_fd(f, hstep=1e-6):
+    """Central finite differences of smoothed_tv."""
     num = np.zeros_like(f)
-    hstep = 1e-6
-    for y in range(h):
-        for x in range(w):
+    for y in range(f.shape[0]):
+        for x in range(f.shape[1]):
             e = np.zeros_like(f)
             e[y, x] = hstep
-            num[y, x] = (tv(f + e) - tv(f - e)) / (2 * hstep)
-    assert np.allclose(g, num, atol=2e-4)
+            num[y, x] = (smoothed_tv(f + e) - smoothed_tv(f - e)) / (2 * hstep)
+    return num
+
+
+@pytest.mark.parametrize("which", ["fp32", "fp64"])
+def test_l1_grad_is_gradient_of_smoothed_tv(which):
+    """l1_grad (ufo-math-tvstd-method.cl) and its FP64 restatement are the
+    gradient of the smoothed TV (eps = 1e-3) with a zero border; checked
+    against central finite differences in FP64 (catches a swapped neighbour
+    or a wrong sign in any of the three terms)."""
+    rng = np.random.default_rng(6)
+    f = rng.standard_normal((7, 9)) if which == "fp32" else rng.standard_normal((9, 9))
+    if which == "fp32":
+        g = O.l1_grad(f.astype(np.float32)).astype(np.float64)
+        f = f.astype(np.float32).astype(np.float64)
+    else:
+        g = O.l1_grad64(f)
+    num = tv_gradient_fd(f)
+    assert np.allclose(g, num, atol=2e-4 if which == "fp32" else 1e-6)
 
 
 def test_sbtv_shrink_is_isotropic_prox():
@@ -350,6 +358,94 @@ def test_asdpocs_output_is_posc_of_last_df_minimizer(df):
         relax = np.float32(relax * np.float32(red))
     last = O.sart(g, b, 3, relax) if df == "sart" else O.sirt(g, b, 3, relax)
     assert np.array_equal(out, np.maximum(last, 0))
+
+
+def asd_problem(n=24, d=34, a=20, seed=11, noise=0.3):
+    g = geom(n, d, a)
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, n))
+    x[n // 4:3 * n // 4, n // 4:3 * n // 4] = 1.0
+    x[n // 3:n // 2, n // 3:2 * n // 3] += 0.5
+    b = O.fp(g, x.astype(np.float32)) + rng.normal(0, noise, (a, d)).astype(np.float32)
+    return g, b
+
+
+def test_asdpocs_textbook_without_tv_is_warm_sart_chain(tiny):
+    """Textbook ASD-POCS with alpha = 0 takes TV steps of length 0, so it is
+    the warm-started chain x <- POSC(SART_{beta_k}(x)) with beta_k = beta
+    red^k, one matrix-form SART pass per iteration (catches a data step that
+    restarts from zero, or a relaxation schedule off by one)."""
+    g, Afp, Abp, b = tiny
+    d, nn = g.n_dets, g.n * g.n
+    rng = np.random.default_rng(12)
+    bn = b + rng.normal(0, 0.5, b.shape)
+    bs = bn.reshape(g.n_angles, d)
+
+    def chain(Afp, Abp):
+        R = 1 / np.where(Afp.sum(1) != 0, Afp.sum(1), np.inf)
+        x = np.zeros(Afp.shape[1])
+        relax = np.float32(0.9)
+        for _ in range(4):
+            for i in range(g.n_angles):
+                rows = slice(i * d, (i + 1) * d)
+                x = x + float(relax) * (Abp[:, rows] @ (R[rows] * (bn[rows] - Afp[rows] @ x)))
+            x = np.maximum(x, 0)
+            relax = np.float32(relax * np.float32(0.7))
+        return x
+
+    x = chain(Afp, Abp)
+    got = O.asdpocs(g, bs.astype(np.float32), 4, beta=0.9, beta_red=0.7, alpha=0.0, df_iters=1, textbook=True)
+    assert O.rel_l2(got.ravel(), x) < 2e-5
+    # the FP64 restatement, against the matrices of its own (exact-coordinate) operators
+    A64 = np.stack([O.fp64(g, np.eye(nn)[k].reshape(g.n, g.n)).ravel() for k in range(nn)], 1)
+    B64 = np.stack([O.bp64(g, np.eye(g.n_angles * d)[k].reshape(g.n_angles, d)).ravel()
+                    for k in range(g.n_angles * d)], 1)
+    got64 = O.asdpocs64(g, bs, 4, beta=0.9, beta_red=0.7, alpha=0.0, df_iters=1, textbook=True)
+    assert O.rel_l2(got64.ravel(), chain(A64, B64)) < 1e-7   # beta_k: float vs double products
+    # the reference variant restarts from zero every iteration: a different image
+    ref = O.asdpocs(g, bs.astype(np.float32), 4, beta=0.9, beta_red=0.7, alpha=0.0, df_iters=1)
+    assert O.rel_l2(ref.ravel(), x) > 1e-3
+
+
+def test_asdpocs_textbook_one_step_closed_form():
+    """K = 1, ng = 1: x = POSC(SART(0)), dtgv = alpha |x|_2 and the output is
+    x - dtgv g / |g|_2 with g the finite-difference gradient of the smoothed
+    TV (catches an L1 normalisation, output before the TV step, or dp taken
+    from the wrong image)."""
+    g, b = asd_problem()
+    alpha = 0.3
+
+    def one_step(x):
+        grad = tv_gradient_fd(x)
+        return x - alpha * np.linalg.norm(x) * grad / np.linalg.norm(grad)
+
+    x = np.maximum(O.sart64(g, b.astype(np.float64), 1, 1.0), 0)
+    got64, diag = O.asdpocs64(g, b.astype(np.float64), 1, beta=1.0, alpha=alpha, ng=1, df_iters=1, textbook=True,
+                              diag=True)
+    assert O.rel_l2(got64.ravel(), one_step(x).ravel()) < 1e-6
+    assert abs(diag[0, 1] - np.linalg.norm(x)) < 1e-9 * np.linalg.norm(x)
+    x32 = np.maximum(O.sart(g, b, 1, 1.0), 0).astype(np.float64)
+    got = O.asdpocs(g, b, 1, beta=1.0, alpha=alpha, ng=1, df_iters=1, textbook=True)
+    assert O.rel_l2(got.ravel(), one_step(x32).ravel()) < 1e-4
+
+
+def test_asdpocs_textbook_tv_steps_have_length_dtgv():
+    """Each normalised TV step moves the image by exactly dtgv in L2, so with
+    ng = 1 the TV distance dg equals dtgv in every iteration, and with ng = 3
+    it is at most 3 dtgv (triangle inequality); dtgv shrinks by alpha_red exactly when dg >
+    r_max dp and dd > 1e-3 (catches an unnormalised or L1-normalised step)."""
+    g, b = asd_problem(seed=13)
+    for ng in (1, 3):
+        out, d = O.asdpocs64(g, b.astype(np.float64), 6, beta=0.8, beta_red=0.9, alpha=0.5, ng=ng, df_iters=1,
+                             textbook=True, diag=True)
+        dd, dp, dg, dtgv = d.T
+        if ng == 1:
+            assert np.allclose(dg, dtgv, rtol=1e-9)
+        else:
+            assert np.all(dg <= 3 * dtgv * (1 + 1e-9))
+        red = float(np.float32(0.95))          # r_max and alpha_red are float parameters
+        shrink = (dg[:-1] > red * dp[:-1]) & (dd[:-1] > 1e-3)
+        assert shrink.any() and np.allclose(dtgv[1:], np.where(shrink, dtgv[:-1] * red, dtgv[:-1]), rtol=1e-12)
 
 
 def test_sbtv_operator_matrix(tiny):

@@ -63,6 +63,7 @@ struct uir_task {
     float *diag;
     uint32_t diag_rows;
     float lambda, mu;
+    int compat; /* asdpocs / sbtv: 1 = the reference's arithmetic, 0 = the textbook variant */
 
     /* cached device temporaries (ufo_buffer_dup in the reference) */
     void *tmp[N_TMP];
@@ -124,12 +125,14 @@ static const prop_def g_asdpocs_props[] = {
     {"r-max", P_FLOAT, OFF(r_max), 0.0, FLT_MAX},                     /* :139-144 */
     {"positive-constraint", P_BOOL, OFF(positive_constraint), 0, 1},  /* :146-151 */
     {"df-minimizer", P_OBJECT, OFF(df_minimizer), 0, 0},              /* :153-158 */
+    {"compat", P_BOOL, OFF(compat), 0, 1},                            /* extension: 0 = ASD-POCS as published */
     {NULL, 0, 0, 0, 0}};
 
 static const prop_def g_sbtv_props[] = {
     METHOD_PROPS,
     {"lambda", P_FLOAT, OFF(lambda), 0.0, FLT_MAX},                   /* ufo-ir-sbtv-task.c:90-95 */
     {"mu", P_FLOAT, OFF(mu), 0.0, FLT_MAX},                           /* :96-101 */
+    {"compat", P_BOOL, OFF(compat), 0, 1},                            /* extension: 0 = CGS with a true inner product */
     {NULL, 0, 0, 0, 0}};
 
 static const char *g_plugin_names[] = {"parallel-projector", "sirt", "sart", "asdpocs", "sbtv"};
@@ -181,6 +184,7 @@ static uir_task *task_alloc(task_kind k)
     t->r_max = 0.95f;
     t->lambda = 0.1f;                /* ufo-ir-sbtv-task.c (class_init defaults) */
     t->mu = 0.5f;
+    t->compat = 1;
     return t;
 }
 
@@ -859,14 +863,15 @@ static uir_status sirt_weights(uir_task *t, mctx *c, float **ray_w, float **pix_
  * FP with the residual and ray weighting fused (b - A x) * R, and BP with the
  * pixel weighting and update fused x + C * relax * A^T r (+ POSC if enabled). */
 static uir_status sirt_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, int posc,
-                           mctx *c)
+                           int warm, mctx *c)
 {
     float *ray_w, *pix_w, *sino_tmp;
     TRY(sirt_weights(t, c, &ray_w, &pix_w));
     TRY(tmp_get(t, 2, c->ns * 4, &sino_tmp));
     c->p->relaxation = relax;            /* :195-197 */
     c->p->correction_scale = -1.0f;
-    HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
+    if (!warm)
+        HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
     for (uint32_t it = 0; it < iters; ++it) { /* :202-215 */
         TRY(proj_fp(c->p, c->n, x, b, sino_tmp, ray_w));
         TRY(proj_bp_ex(c->p, c->n, sino_tmp, x, pix_w, posc));
@@ -900,7 +905,8 @@ static uir_status sart_sweep(uir_task *t, mctx *c, const float *b, float *x, flo
     return UIR_OK;
 }
 
-static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, mctx *c)
+static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, int warm,
+                           mctx *c)
 {
     float *sino_tmp, *vol_tmp, *ray_w;
     TRY(tmp_get(t, 4, c->ns * 4, &sino_tmp));
@@ -913,7 +919,8 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
     HIPTRY(uirk_inv(ray_w, ray_w, c->ns, c->st), "inv");
     c->p->correction_scale = -1.0f;                                 /* :205-207 */
     c->p->relaxation = relax;
-    HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
+    if (!warm)
+        HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
     const float axis = proj_axis(c->p);
     if (iters == 0)
         return UIR_OK;
@@ -961,14 +968,29 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
     return UIR_OK;
 }
 
-static uir_status method_process_df(uir_task *t, const float *b, float *x, mctx *c, float relax)
+/* warm != 0: the data step continues from x (textbook ASD-POCS) instead of zero. */
+static uir_status method_process_df(uir_task *t, const float *b, float *x, mctx *c, float relax, int warm)
 {
     if (t->kind == K_SIRT)
-        return sirt_run(t, b, x, relax, t->num_iterations, t->positive_constraint, c);
-    return sart_run(t, b, x, relax, t->num_iterations, c);
+        return sirt_run(t, b, x, relax, t->num_iterations, t->positive_constraint, warm, c);
+    return sart_run(t, b, x, relax, t->num_iterations, warm, c);
 }
 
-/* ASD-POCS process, ufo-ir-asdpocs-task.c:411-509 (tvstd :550-580). */
+/* The textbook variants' distances: L2 norms, each element once, in FP64. */
+static uir_status ws_l2(uir_workspace *ws, const float *v, size_t n, void *st, double *out)
+{
+    double d;
+    TRY(uir_ws_dot_f64(ws, v, v, n, st, &d));
+    *out = sqrt(d);
+    return UIR_OK;
+}
+
+/* ASD-POCS process, ufo-ir-asdpocs-task.c:411-509 (tvstd :550-580).
+ * compat = 0 runs ASD-POCS as published (Sidky and Pan, Phys. Med. Biol. 53
+ * (2008) 4777): the data step continues from the current image, dd, dp and dg
+ * are L2 norms with dp measured from the image at the start of the
+ * iteration, the ng TV steps move dtgv along the L2-normalised gradient (a
+ * zero gradient ends them), and the output is the image after TV. */
 static uir_status asdpocs_run(uir_task *t, const float *b, float *out, mctx *c)
 {
     uir_task *df = t->df_minimizer;
@@ -978,6 +1000,7 @@ static uir_status asdpocs_run(uir_task *t, const float *b, float *out, mctx *c)
         TRY(uir_task_set_object(df, "projector", t->projector)); /* :434 */
     if (!df->res)
         df->res = uir_resources_retain(t->res);
+    const int tb = !t->compat;
     float *x, *x_prev, *x_res, *b_res, *grad;
     TRY(tmp_get(t, 7, c->nv * 4, &x));
     TRY(tmp_get(t, 8, c->nv * 4, &x_prev));
@@ -991,35 +1014,42 @@ static uir_status asdpocs_run(uir_task *t, const float *b, float *out, mctx *c)
     t->diag_rows = 0;
     float dp = 1.0f, dd = 1.0f, dg = 1.0f, dtgv = 1.0f;
     float beta = t->beta;
+    const uint32_t ng = tb ? t->ng : 20; /* tvstd :573 takes 20 steps; ng is unused there */
     uir_workspace *ws = &t->res->ws;
+#define NORM(v, n, out) (tb ? ws_l2(ws, (v), (n), c->st, (out)) : uir_ws_l1(ws, (v), (n), c->st, (out)))
     for (uint32_t it = 0; it < t->num_iterations; ++it) {
         /* g_object_set(df, "relaxation_factor", beta): GLib rejects values outside [0,1]. */
         if (beta >= 0.0f && beta <= 1.0f)
             df->relaxation_factor = beta;
-        TRY(method_process_df(df, b, x, c, df->relaxation_factor));        /* :460 */
+        if (tb)
+            HIPTRY(hipMemcpyAsync(x_prev, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy");
+        TRY(method_process_df(df, b, x, c, df->relaxation_factor, tb));    /* :460 */
         if (t->positive_constraint)                                         /* :463-465 */
             HIPTRY(uirk_unary(UIRK_POSC, x, x, c->nv, 0, c->st), "POSC");
-        HIPTRY(hipMemcpyAsync(out, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy"); /* :468 */
+        if (!tb)
+            HIPTRY(hipMemcpyAsync(out, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy"); /* :468 */
         c->p->correction_scale = -1.0f;                                     /* :471-475 */
         TRY(proj_fp(c->p, c->n, x, b, b_res, NULL));
         double d;
-        TRY(uir_ws_l1(ws, b_res, c->ns, c->st, &d));                       /* :478 */
+        TRY(NORM(b_res, c->ns, &d));                                        /* :478 */
         dd = (float)d;
         HIPTRY(uirk_binary(UIRK_SUB, x, x_prev, 0, x_res, c->nv, c->st), "deduction"); /* :481-482 */
-        TRY(uir_ws_l1(ws, x_res, c->nv, c->st, &d));
+        TRY(NORM(x_res, c->nv, &d));
         dp = (float)d;
         if (it == 0)                                                        /* :485-487 */
             dtgv = t->alpha * dp;
         HIPTRY(hipMemcpyAsync(x_prev, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy"); /* :490 */
         HIPTRY(uirk_set(grad, c->nv, 0.0f, c->st), "set");                 /* tvstd :559 */
-        for (int k = 0; k < 20; ++k) {                                      /* :573-579 */
+        for (uint32_t k = 0; k < ng; ++k) {                                 /* :573-579 */
             HIPTRY(uirk_l1_grad(x, grad, c->n, c->n, c->st), "l1_grad");
-            TRY(uir_ws_l1(ws, grad, c->nv, c->st, &d));
+            TRY(NORM(grad, c->nv, &d));
+            if (tb && !(d > 0))
+                break;
             const float factor = dtgv / (float)d;
             HIPTRY(uirk_binary(UIRK_SUB_SCALED, x, grad, factor, x, c->nv, c->st), "deduction2");
         }
         HIPTRY(uirk_binary(UIRK_SUB, x, x_prev, 0, x_res, c->nv, c->st), "deduction"); /* :496-497 */
-        TRY(uir_ws_l1(ws, x_res, c->nv, c->st, &d));
+        TRY(NORM(x_res, c->nv, &d));
         dg = (float)d;
         beta *= t->beta_red;                                                /* :498 */
         t->diag[4 * it + 0] = dd;
@@ -1030,7 +1060,10 @@ static uir_status asdpocs_run(uir_task *t, const float *b, float *out, mctx *c)
         const float epsilon = 0.001f;                                       /* :495,501-503 */
         if (dg > t->r_max * dp && dd > epsilon)
             dtgv *= t->alpha_red;
+        if (tb)
+            HIPTRY(hipMemcpyAsync(out, x, c->nv * 4, hipMemcpyDeviceToDevice, c->st), "copy");
     }
+#undef NORM
     return UIR_OK;
 }
 
@@ -1051,17 +1084,25 @@ typedef struct {
     float *tempA, *tempAt, *tempD, *tempDxt;
 } sbctx;
 
-static uir_status sb_l2(sbctx *s, const float *v, float *out)
+/* The CGS inner product: the reference's strided "dot"
+ * (ufo-ir-basic-ops-processor.c:148-196) when compat, else every element once,
+ * accumulated in FP64 and rounded to float. */
+static uir_status sb_dot(sbctx *s, const float *a, const float *b, float *out)
 {
-    float d;
-    TRY(uir_ws_dot_compat(&s->t->res->ws, v, v, s->c->n, s->c->n, s->c->st, &d));
-    *out = (float)sqrt(d);
+    if (s->t->compat)
+        return uir_ws_dot_compat(&s->t->res->ws, a, b, s->c->n, s->c->n, s->c->st, out);
+    double d;
+    TRY(uir_ws_dot_f64(&s->t->res->ws, a, b, s->c->nv, s->c->st, &d));
+    *out = (float)d;
     return UIR_OK;
 }
 
-static uir_status sb_dot(sbctx *s, const float *a, const float *b, float *out)
+static uir_status sb_l2(sbctx *s, const float *v, float *out)
 {
-    return uir_ws_dot_compat(&s->t->res->ws, a, b, s->c->n, s->c->n, s->c->st, out);
+    float d;
+    TRY(sb_dot(s, v, v, &d));
+    *out = (float)sqrt(d);
+    return UIR_OK;
 }
 
 /* processA, :587-635 */
@@ -1322,9 +1363,9 @@ uir_status uir_task_process(uir_task *t, const uir_buffer *inputs, uir_buffer *o
     float *x = (float *)output->data;
     switch (t->kind) {
     case K_SIRT:
-        return sirt_run(t, b, x, t->relaxation_factor, t->num_iterations, t->positive_constraint, &c);
+        return sirt_run(t, b, x, t->relaxation_factor, t->num_iterations, t->positive_constraint, 0, &c);
     case K_SART:
-        return sart_run(t, b, x, t->relaxation_factor, t->num_iterations, &c);
+        return sart_run(t, b, x, t->relaxation_factor, t->num_iterations, 0, &c);
     case K_ASDPOCS:
         return asdpocs_run(t, b, x, &c);
     case K_SBTV:
