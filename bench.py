@@ -264,6 +264,145 @@ def batch_c3(args, uir):
     print(json.dumps(line), flush=True)
 
 
+def ellipsoid_slices(n, n_slices, seed, k=32):
+    """A piecewise-constant 3-D phantom of k random ellipsoids in [-1, 1]^3
+    (centres U(-0.6, 0.6), semi-axes U(0.08, 0.5), rotation about z U(0, pi),
+    additive attenuations U(-0.2, 0.5), clamped >= 0), sampled on n x n pixel
+    centres at n_slices z planes; yields one (n, n) FP32 slice at a time."""
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(-0.6, 0.6, (k, 3))
+    ax = rng.uniform(0.08, 0.5, (k, 3))
+    rot = rng.uniform(0.0, np.pi, k)
+    val = rng.uniform(-0.2, 0.5, k).astype(np.float32)
+    c = (np.arange(n) + 0.5 - n / 2.0) / (n / 2.0)
+    for zi in range(n_slices):
+        z = (zi + 0.5 - n_slices / 2.0) / (n_slices / 2.0)
+        img = np.zeros((n, n), np.float32)
+        for e in range(k):
+            h = 1.0 - ((z - cen[e, 2]) / ax[e, 2]) ** 2
+            if h <= 0:
+                continue
+            a, b = ax[e, 0] * np.sqrt(h), ax[e, 1] * np.sqrt(h)
+            r = max(a, b)
+            lo = lambda v: max(0, int((v - r + 1) * n / 2) - 1)
+            hi = lambda v: min(n, int((v + r + 1) * n / 2) + 2)
+            x0, x1, y0, y1 = lo(cen[e, 0]), hi(cen[e, 0]), lo(-cen[e, 1]), hi(-cen[e, 1])
+            if x0 >= x1 or y0 >= y1:
+                continue
+            X, Y = np.meshgrid(c[x0:x1], -c[y0:y1])
+            xr = (X - cen[e, 0]) * np.cos(rot[e]) + (Y - cen[e, 1]) * np.sin(rot[e])
+            yr = -(X - cen[e, 0]) * np.sin(rot[e]) + (Y - cen[e, 1]) * np.cos(rot[e])
+            img[y0:y1, x0:x1][(xr / a) ** 2 + (yr / b) ** 2 <= 1.0] += val[e]
+        np.maximum(img, 0, out=img)
+        yield img
+
+
+def asdpocs_c4(args, uir):
+    """Batched ASD-POCS on a stack of independent 1024 x 1024 slices of a 3-D
+    random-ellipsoid phantom, sparse view (120 angles over [0, pi), 1448
+    bins), Poisson noise at I0 = 5e3 with 0.01 attenuation per pixel unit,
+    40 iterations of one SART pass and 20 TV steps, FP64 norm reductions on
+    the device.  Beside it: the plain per-slice task path on a few slices,
+    accuracy against the FP64 oracle on one slice, and the FP32 oracle on
+    the host cores as the CPU baseline."""
+    import oracle as O
+    res = uir.Resources(0)
+    n, a, d = 1024, 120, 1448
+    S, iters = args.c4_slices, args.c4_iters
+    step = float(np.float32(np.pi / a))
+    spec = dict(n=n, n_dets=d, n_angles=a, step=step, axis_position=-1.0, relaxation=0.25,
+                ray_weight_min=RAY_WEIGHT_MIN)
+    params = dict(beta=1.0, beta_red=0.995, alpha=0.2, alpha_red=0.95, r_max=0.95, ng=20, df_iterations=1,
+                  positive_constraint=True, compat=bool(args.c4_compat))
+    i0, mu = 5e3, 0.01
+    rng = np.random.default_rng(20261204)
+    t0 = time.time()
+    proj = uir.Task("parallel-projector", step=step, is_forward=True, angles_num=a, image_size=n,
+                    detectors_num=d).setup(res)
+    dimg, dsino = uir.DeviceArray((n, n)), uir.DeviceArray((a, d))
+    sinos = []
+    for k, img in enumerate(ellipsoid_slices(n, S, 20261204)):
+        dimg.upload(img, res.stream)
+        proj.process(dimg, dsino)
+        p = dsino.numpy(res.stream).astype(np.float64)
+        counts = np.maximum(rng.poisson(i0 * np.exp(-mu * p)), 1)
+        sinos.append((-np.log(counts / i0) / mu).astype(np.float32))
+        if k % 128 == 127:
+            print("c4: %d / %d sinograms" % (k + 1, S), file=sys.stderr, flush=True)
+    dimg.free()
+    dsino.free()
+    t_data = time.time() - t0
+    out = {}
+    for storage in args.c4_storage.split(","):
+        bt = uir.Batch(res, [spec] * S, storage)
+        for k, sn in enumerate(sinos):
+            bt.set_sinogram(k, sn)
+        bt.asdpocs(1, **params)                # allocates the method's buffers, captures the iteration graph
+        res.synchronize()
+        timer = uir.Timer()
+        timer.start(res.stream)
+        bt.asdpocs(iters, **params)
+        timer.stop(res.stream)
+        ms = timer.elapsed_ms()
+        print("c4: %s %d slices x %d iterations in %.1f ms" % (storage, S, iters, ms), file=sys.stderr, flush=True)
+        out[storage] = {"ms_total": round(ms, 1), "ms_per_iteration": round(ms / iters, 2),
+                        "slice_iters_per_s": round(S * iters / (ms / 1000.0), 1),
+                        "device_bytes": bt.info()["device_bytes"]}
+        if storage == args.c4_storage.split(",")[0]:
+            diag0 = bt.asdpocs_diagnostics(0)
+        out[storage]["image0"] = bt.image(0)
+        bt.close()
+    # the plain path: the asdpocs task, one slice per process() call
+    proj_b = uir.Task("parallel-projector", step=step, is_forward=False)
+    sart = uir.Task("sart", num_iterations=1)
+    task = uir.Task("asdpocs", projector=proj_b, df_minimizer=sart, num_iterations=iters,
+                    compat=bool(args.c4_compat)).setup(res)
+    n_plain = min(S, args.c4_plain_slices)
+    dout = uir.DeviceArray((n, n))
+    ds = [uir.DeviceArray.from_numpy(sinos[k]) for k in range(n_plain)]
+    task.process(ds[0], dout)
+    res.synchronize()
+    t1 = time.perf_counter()
+    for k in range(n_plain):
+        task.process(ds[k], dout)
+    res.synchronize()
+    plain_s = (time.perf_counter() - t1) / n_plain
+    # accuracy of slice 0 against the FP64 oracle, every storage mode
+    g = O.Geometry(n, d, a, np.float32(step))
+    t2 = time.time()
+    kw = {k: v for k, v in params.items() if k not in ("df_iterations", "compat")}
+    ref = O.asdpocs64(g, sinos[0].astype(np.float64), iters, df_iters=1, textbook=not args.c4_compat,
+                      ray_weight_min=RAY_WEIGHT_MIN, **kw)
+    t_ref = time.time() - t2
+    for storage in out:
+        img0 = out[storage].pop("image0")
+        out[storage]["rel_l2_vs_fp64_slice0"] = O.rel_l2(img0, ref)
+        out[storage]["psnr_vs_fp64_slice0_db"] = round(O.psnr(img0, ref), 2)
+    # CPU baseline: the FP32 oracle (C, OpenMP) on slice 0, two iterations
+    t3 = time.perf_counter()
+    O.asdpocs(g, sinos[0], 2, df_iters=1, textbook=not args.c4_compat, **kw)
+    cpu_s_per_it = (time.perf_counter() - t3) / 2
+    main_storage = args.c4_storage.split(",")[0]
+    line = {"metric": "batched ASD-POCS, sparse-view stack", "value": out[main_storage]["slice_iters_per_s"],
+            "unit": "slice-iters/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": {"f32": "f32", "f16-sino": "f16 sinograms, f32 images, f32 math"}.get(main_storage, main_storage),
+            "data": "synthetic: 32 random ellipsoids, Joseph FP, Poisson counts at I0 = 5e3, 0.01 per pixel unit",
+            "config": {"workload": "%d slices of 1024x1024, 120 angles over [0, pi), 1448 bins; %d ASD-POCS "
+                                   "iterations of 1 SART pass + 20 TV steps (%s), FP64 device reductions"
+                                   % (S, iters, "reference arithmetic" if args.c4_compat else "as published"),
+                       "slices": S, "iterations": iters, "compat": bool(args.c4_compat)},
+            "storage": out,
+            "plain": {"s_per_slice": round(plain_s, 4), "slices_timed": n_plain,
+                      "slice_iters_per_s": round(iters / plain_s, 1)},
+            "speedup_vs_plain": round(out[main_storage]["slice_iters_per_s"] / (iters / plain_s), 2),
+            "diagnostics_slice0_last": [float(v) for v in diag0[-1]] if len(diag0) else None,
+            "cpu_baseline": {"value": round(1.0 / cpu_s_per_it, 3), "unit": "slice-iters/s",
+                             "cores": O.num_threads(), "kind": "port",
+                             "sample": "FP32 oracle ASD-POCS, slice 0, 2 iterations"},
+            "data_gen_s": round(t_data, 1), "fp64_oracle_s": round(t_ref, 1)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,6 +428,12 @@ def main():
     ap.add_argument("--c3", action="store_true", help="batched SIRT over 512 slices of uneven size")
     ap.add_argument("--c3-slices", type=int, default=512)
     ap.add_argument("--c3-iters", type=int, default=100)
+    ap.add_argument("--c4", action="store_true", help="batched ASD-POCS on a sparse-view stack of 1024^2 slices")
+    ap.add_argument("--c4-slices", type=int, default=1024)
+    ap.add_argument("--c4-iters", type=int, default=40)
+    ap.add_argument("--c4-storage", default="f32,f16-sino", help="comma-separated storage modes (first is `value`)")
+    ap.add_argument("--c4-compat", action="store_true", help="the reference's ASD-POCS arithmetic")
+    ap.add_argument("--c4-plain-slices", type=int, default=4)
     ap.add_argument("--tile", default=None,
                     help="fp_angles,bp_angles,bp_rows: use this tile shape instead of tuning (no probe launches)")
     ap.add_argument("--profile-pass", action="store_true",
@@ -310,6 +455,8 @@ def main():
     if args.c2:
         sart_c2(args, uir)
         return
+    if args.c4:
+        return asdpocs_c4(args, uir)
     if args.c3:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         batch_c3(args, uir)

@@ -268,6 +268,39 @@ uir_status uir_batch_get_info(const uir_batch *batch, uir_batch_info *info);
  * timed with events on the batch stream. */
 uir_status uir_batch_time_kernels(uir_batch *batch, uint32_t reps, float *fp_ms, float *bp_ms);
 
+/* Batched ASD-POCS: the asdpocs task (ufo-ir-asdpocs-task.c:411-509) with a
+ * SART df-minimizer (ufo-ir-sart-task.c:173-228) on every slice of a batch at
+ * once, fast-path projectors.  Per SART angle one FP and one BP launch cover
+ * every slice that has the angle; the TV steps and the FP64 norm reductions
+ * run on the device, and one iteration replays from a captured hipGraph.
+ * compat = 1 is the reference's arithmetic (the df-minimizer restarts from
+ * zero each iteration, L1 distances, 20 TV steps of length dtgv / |g|_1, the
+ * output is the image before the TV steps); compat = 0 is ASD-POCS as
+ * published (warm-started data step, L2 distances, ng steps along the
+ * L2-normalised gradient, output after TV).  The slice spec's relaxation is
+ * the df-minimizer's relaxation_factor until beta sets it; its
+ * positive_constraint is not used (see positive_constraint below). */
+typedef struct {
+    float    beta, beta_red;      /* ufo-ir-asdpocs-task.c:104-116: 1.0, 0.995 */
+    float    alpha, alpha_red;    /* :125-137: 0.2, 0.95 */
+    float    r_max;               /* :139-144: 0.95 */
+    uint32_t ng;                  /* TV steps per iteration, <= 1000 (compat: 20, as tvstd takes) */
+    uint32_t df_iterations;       /* SART passes per iteration (the df-minimizer's num-iterations) */
+    int32_t  positive_constraint; /* :146-151: 1 */
+    int32_t  compat;              /* 1: the reference; 0: as published */
+} uir_asdpocs_params;
+
+void       uir_asdpocs_params_default(uir_asdpocs_params *p);
+/* `iterations` ASD-POCS iterations of every slice from x = 0; the result
+ * replaces the slice images (uir_batch_get_image).  Asynchronous on the batch
+ * stream.  The first call allocates the method's own buffers (a second image
+ * and an FP32 gradient per slice). */
+uir_status uir_batch_asdpocs(uir_batch *batch, uint32_t iterations, const uir_asdpocs_params *p, int use_graph);
+/* Rows (dd, dp, dg, dtgv) of slice `slice` from the last uir_batch_asdpocs,
+ * one per iteration (as uir_asdpocs_get_diagnostics); synchronises. */
+uir_status uir_batch_asdpocs_diagnostics(uir_batch *batch, uint32_t slice, float *rows4, uint32_t max_rows,
+                                         uint32_t *n_rows);
+
 /* ------------------------------------------------------------------------ */
 /* Tile-shape autotuner.  No counterpart in the reference (its work-group    */
 /* shapes are the OpenCL runtime's).  The fast projector pair has variants: */

@@ -1,0 +1,115 @@
+"""Batched ASD-POCS (uir_batch_asdpocs): both variants against the FP64
+oracle (ufo-ir-asdpocs-task.c semantics for compat, the published method
+otherwise), per-slice diagnostics, independence of a slice's result from the
+batch mix, graph replay equal to direct launches, and the single-angle
+projector launches against the FP64 operators."""
+import numpy as np
+import pytest
+
+import oracle as O
+import uir
+from phantoms import ellipse_phantom
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [  # (n, n_dets, n_angles): uneven sizes and angle counts in one batch
+    (96, 136, 40),
+    (64, 91, 24),
+    (128, 182, 30),
+]
+RMIN = 0.1
+
+
+def _spec(n, d, a, relax=0.25):
+    return dict(n=n, n_dets=d, n_angles=a, step=float(np.float32(np.pi / a)), axis_position=-1.0,
+                relaxation=relax, positive_constraint=False, ray_weight_min=RMIN)
+
+
+def _data(n, d, a, seed, noise=0.02):
+    g = O.Geometry(n, d, a, np.float32(np.pi / a))
+    b = O.fp64(g, ellipse_phantom(n, seed=seed).astype(np.float64))
+    b = b + np.random.default_rng(seed).normal(0, noise * b.max(), b.shape)
+    return g, b.astype(np.float32)
+
+
+def _batch(res, storage, geoms=GEOMS, seed0=500):
+    bt = uir.Batch(res, [_spec(*gm) for gm in geoms], storage)
+    data = []
+    for i, gm in enumerate(geoms):
+        g, b = _data(*gm, seed0 + i)
+        bt.set_sinogram(i, b)
+        data.append((g, b))
+    return bt, data
+
+
+KW = dict(beta=0.9, beta_red=0.9, alpha=0.2, alpha_red=0.95, r_max=0.95)
+
+
+@pytest.mark.parametrize("compat", [True, False])
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-4), ("f16-sino", 1e-3)])
+def test_batch_asdpocs_matches_fp64(res, compat, storage, tol):
+    """The oracle runs on the sinogram the batch holds (rounded to FP16 for
+    f16-sino): what is measured is the arithmetic, not the data's rounding."""
+    iters, ng = 4, 10
+    bt, data = _batch(res, storage)
+    bt.asdpocs(iters, ng=ng, compat=compat, **KW)
+    for i, (g, b) in enumerate(data):
+        held = b.astype(np.float16) if storage == "f16-sino" else b
+        ref, diag = O.asdpocs64(g, held.astype(np.float64), iters, ng=ng, df_iters=1, textbook=not compat,
+                                ray_weight_min=RMIN, df_relax_initial=0.25, diag=True, **KW)
+        got = bt.image(i)
+        err = O.rel_l2(got, ref)
+        print("compat" if compat else "textbook", storage, GEOMS[i], "rel-L2 vs FP64", err, flush=True)
+        assert err <= tol
+        gd = bt.asdpocs_diagnostics(i)
+        assert gd.shape == (iters, 4)
+        assert np.allclose(gd, diag, rtol=20 * tol, atol=1e-6), (gd, diag)
+
+
+def test_batch_asdpocs_slice_independent_of_mix(res):
+    """A slice's image is the same alone and in a batch of other geometries."""
+    bt, _ = _batch(res, "f32")
+    bt.asdpocs(3, compat=False, ng=8, **KW)
+    together = [bt.image(i) for i in range(len(GEOMS))]
+    for i, gm in enumerate(GEOMS):
+        alone, _ = _batch(res, "f32", geoms=[gm], seed0=500 + i)
+        alone.asdpocs(3, compat=False, ng=8, **KW)
+        assert np.array_equal(alone.image(0), together[i]), GEOMS[i]
+
+
+def test_batch_asdpocs_graph_equals_direct(res):
+    bt, _ = _batch(res, "f32")
+    bt.asdpocs(3, use_graph=False, compat=False, ng=6, **KW)
+    direct = [bt.image(i) for i in range(len(GEOMS))]
+    bt.asdpocs(3, use_graph=True, compat=False, ng=6, **KW)
+    for i in range(len(GEOMS)):
+        assert np.array_equal(bt.image(i), direct[i])
+    # a second replay of the same graph starts from zero again
+    bt.asdpocs(3, use_graph=True, compat=False, ng=6, **KW)
+    for i in range(len(GEOMS)):
+        assert np.array_equal(bt.image(i), direct[i])
+
+
+def test_batch_asdpocs_without_tv_is_sart(res):
+    """alpha = 0 and no positivity: the textbook loop is warm-started SART
+    with the beta schedule, i.e. plain SART when beta_red = 1."""
+    bt, data = _batch(res, "f32")
+    bt.asdpocs(3, compat=False, alpha=0.0, beta=0.5, beta_red=1.0, positive_constraint=False)
+    for i, (g, b) in enumerate(data):
+        ref = O.asdpocs64(g, b.astype(np.float64), 3, alpha=0.0, beta=0.5, beta_red=1.0, positive_constraint=False,
+                          df_iters=1, textbook=True, ray_weight_min=RMIN)
+        assert O.rel_l2(bt.image(i), ref) <= 2e-5
+
+
+@pytest.mark.parametrize("n,d,a", [(1024, 1448, 120)])
+def test_batch_asdpocs_sparse_view_stack_shape(res, n, d, a):
+    """The sparse-view stack geometry (1024^2, 120 angles, 1448 bins), two
+    iterations of the textbook variant against FP64."""
+    g, b = _data(n, d, a, 77, noise=0.01)
+    bt = uir.Batch(res, [_spec(n, d, a)], "f32")
+    bt.set_sinogram(0, b)
+    bt.asdpocs(2, compat=False, **KW)
+    ref = O.asdpocs64(g, b.astype(np.float64), 2, df_iters=1, textbook=True, ray_weight_min=RMIN, **KW)
+    err = O.rel_l2(bt.image(0), ref)
+    print("1024^2 x 120 rel-L2 vs FP64", err, flush=True)
+    assert err <= 1e-4

@@ -150,6 +150,16 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
                   const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
                   int *d_err, void *stream);
+/* The same BP restricted to angle a_sel (fast mode; a_sel < 0: all angles).
+ * Items of slices with fewer angles are left alone. */
+int uirk_bp_table_angle(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
+                        const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
+                        int a_sel, int *d_err, void *stream);
+/* Fast-path FP of one angle over every slice that has it (the SART data step):
+ * r[angle] = (b[angle] + corr * FP(x)[angle]) * ray_w[angle] (ray_w NULL: no
+ * factor).  Items: {slice, d0} per 128 detectors (uirk_plan_fp_angle). */
+int uirk_fp_angle(const uirk_slice *d_slices, const uirk_fp_item *d_items, int n_items, const uirk_xcd_part *part,
+                  int dtype, int img_dtype, int angle, int grid, int *d_err, void *stream);
 
 /* Work-table construction (host/uir_plan.c). */
 typedef struct {
@@ -187,6 +197,42 @@ int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream);
 int uirk_sb_shrink(const void *u, int dtype, float *bx, float *by, float *ex, float *ey, uint32_t n, float lambda,
                    void *stream);
+
+/* ---- Batched ASD-POCS steps between the projections (uir_asd.hip) ----
+ * Grids are (nblk, n_slices): block (c, s) covers elements [c, c + 1) *
+ * UIRK_ASD_CHUNK of slice s (nblk must cover the largest image; sinograms
+ * are strided over the nblk blocks) and writes partials[s * nblk + c]. */
+#define UIRK_ASD_CHUNK 4096
+typedef struct {
+    void *x, *x_prev;    /* image and its checkpoint (image dtype) */
+    float *g;            /* TV gradient (FP32) */
+    const void *r;       /* residual sinogram b - A x (sinogram dtype) */
+    uint32_t n, n_sino;  /* image edge; n_angles * n_dets */
+} uirk_asd_slice;
+typedef struct {
+    float beta, relax, dtgv, dd, dp, dg, factor;
+    int32_t it, stop, pad;
+} uirk_asd_state;
+typedef struct {
+    float beta_red, alpha, alpha_red, r_max;
+    int32_t textbook;
+} uirk_asd_params;
+/* kind: BEGIN (flag 0: x = 0, 1: x_prev = x, 2: x = x_prev), POSC_DP (flag = posc:
+ * x = max(x, 0), partial |x - x_prev|, x_prev = x), DIFF (partial |x - x_prev|),
+ * TVGRAD (g = l1_grad(x), partial |g|), TVSTEP (x -= factor g); l2 selects
+ * sums of squares over sums of magnitudes. */
+enum { UIRK_ASD_BEGIN = 0, UIRK_ASD_POSC_DP = 1, UIRK_ASD_DIFF = 2, UIRK_ASD_TVGRAD = 3, UIRK_ASD_TVSTEP = 4 };
+int uirk_asd_image(int kind, int img_dtype, const uirk_asd_slice *d_sl, int n_slices, int nblk,
+                   const uirk_asd_state *d_state, int flag, int l2, double *d_partials, void *stream);
+/* partial |r| over the residual sinograms */
+int uirk_asd_sino_norm(int dtype, const uirk_asd_slice *d_sl, int n_slices, int nblk, int l2, double *d_partials,
+                       void *stream);
+/* what: DD, DP (sets dtgv = alpha dp in iteration 0), G (factor = dtgv / |g|;
+ * textbook: a zero gradient ends the TV steps), DG (diagnostics row, dtgv
+ * and beta updates, next relaxation into d_bp_sl[s].relax, iteration count). */
+enum { UIRK_ASD_DD = 0, UIRK_ASD_DP = 1, UIRK_ASD_G = 2, UIRK_ASD_DG = 3 };
+int uirk_asd_finish(uirk_asd_state *d_state, const double *d_partials, int n_slices, int nblk, int what,
+                    const uirk_asd_params *p, uirk_slice *d_bp_sl, float *d_diag, int max_iters, void *stream);
 
 /* Finite differences (ufo-ir-gradient-processor.cl) and l1_grad. */
 int uirk_diff(int kind, const float *in, float *out, uint32_t w, uint32_t h, void *stream);

@@ -129,6 +129,87 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+// Workgroup-wide (4 waves) min of lo and max of hi.  Its first barrier also
+// ends the previous chunk's reads of the staging area.
+__device__ __forceinline__ void block_lohi(float &lo, float &hi, float (*red)[4], int lane, int wave)
+{
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    __syncthreads();
+    if (lane == 0) {
+        red[0][wave] = lo;
+        red[1][wave] = hi;
+    }
+    __syncthreads();
+    lo = fminf(fminf(red[0][0], red[0][1]), fminf(red[0][2], red[0][3]));
+    hi = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
+}
+
+// Stages one FP chunk: lines [l0, l0 + nl) (rows for the vertical family,
+// columns for the horizontal one) x positions [pmin, pmin + W) go to
+// lds[line * kFpLs + position - pmin], W <= kFpW.  Every thread issues a batch
+// of loads (from clamped, always valid indices, no branches) before writing
+// LDS; texels outside the image are then replaced by the zero border.
+// Batches cover 64 window positions each, so only the footprint's positions
+// are loaded.
+template <typename TI>
+__device__ __forceinline__ void fp_stage(float (&lds)[kFpLines * kFpLs], const TI *img, int N, bool hor, int l0, int nl,
+                                         int pmin, int W)
+{
+    constexpr int kPer = kFpLines * kFpW / 256;
+    constexpr int kQ = kPer / 4;
+    if (hor) {
+        // line = column l0 + l, position = row pmin + p; coalesced along columns
+        const int l = threadIdx.x & 31;
+        const int col = min(l0 + l, N - 1);
+        for (int q0 = 0; q0 < kPer; q0 += kQ) {
+            if (8 * q0 >= W)
+                break;
+            float v[kQ];
+#pragma unroll
+            for (int e = 0; e < kQ; ++e) {
+                const int p = (threadIdx.x >> 5) + 8 * (q0 + e);
+                const int row = pmin + p;
+                const int rc = min(max(row, 0), N - 1);
+                const float t = gload(img, (uint32_t)rc * (uint32_t)N + (uint32_t)col);
+                v[e] = (row == rc && l < nl) ? t : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < kQ; ++e) {
+                const int p = (threadIdx.x >> 5) + 8 * (q0 + e);
+                if (p < W)
+                    lds[l * kFpLs + p] = v[e];
+            }
+        }
+    } else {
+        // line = row l0 + l, position = column pmin + p: batch q0 holds
+        // positions [64 (q0 / kQ), + 64) of all 32 lines
+        for (int q0 = 0; q0 < kPer; q0 += kQ) {
+            const int pb = q0 / kQ;
+            if (64 * pb >= W)
+                break;
+            float v[kQ];
+#pragma unroll
+            for (int e = 0; e < kQ; ++e) {
+                const int l = (threadIdx.x >> 6) + 4 * e;
+                const int p = (threadIdx.x & 63) + 64 * pb;
+                const int col = pmin + p;
+                const int cc = min(max(col, 0), N - 1);
+                const int row = min(l0 + l, N - 1);
+                const float t = gload(img, (uint32_t)row * (uint32_t)N + (uint32_t)cc);
+                v[e] = (col == cc && l < nl) ? t : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < kQ; ++e) {
+                const int l = (threadIdx.x >> 6) + 4 * e;
+                const int p = (threadIdx.x & 63) + 64 * pb;
+                if (p < W)
+                    lds[l * kFpLs + p] = v[e];
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // Forward projection
 // ------------------------------------------------------------------------
@@ -215,16 +296,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
                     hi = fmaxf(hi, fmaxf(a, b));
                 }
             }
-            lo = wave_min(lo);
-            hi = wave_max(hi);
-            __syncthreads(); // previous chunk's LDS reads are done
-            if (lane == 0) {
-                red[0][wave] = lo;
-                red[1][wave] = hi;
-            }
-            __syncthreads();
-            lo = fminf(fminf(red[0][0], red[0][1]), fminf(red[0][2], red[0][3]));
-            hi = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
+            block_lohi(lo, hi, red, lane, wave);
             if (!(lo <= hi))
                 break; // no active ray in this workgroup
             const int pmin = (int)floorf(lo - 0.5f) - 1;
@@ -236,65 +308,8 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             if (overflow && threadIdx.x == 0)
                 atomicOr(err, 1);
             const bool outside = overflow || pmin + W <= 0 || pmin >= N;
-            if (!outside) {
-                // Every thread issues a batch of loads (from clamped, always
-                // valid indices, no branches) before writing LDS; texels
-                // outside the image are then replaced by the zero border.
-                // Batches cover 64 window positions each, so only the
-                // positions of the footprint (W of up to 256) are loaded.
-                constexpr int kPer = kFpLines * kFpW / 256;
-                constexpr int kQ = kPer / 4;
-                if (hor) {
-                    // line = column l0 + l, position = row pmin + p; coalesced along columns
-                    const int l = threadIdx.x & 31;
-                    const int col = min(l0 + l, N - 1);
-                    for (int q0 = 0; q0 < kPer; q0 += kQ) {
-                        if (8 * q0 >= W)
-                            break;
-                        float v[kQ];
-#pragma unroll
-                        for (int e = 0; e < kQ; ++e) {
-                            const int p = (threadIdx.x >> 5) + 8 * (q0 + e);
-                            const int row = pmin + p;
-                            const int rc = min(max(row, 0), N - 1);
-                            const float t = gload(img, (uint32_t)rc * (uint32_t)N + (uint32_t)col);
-                            v[e] = (row == rc && l < nl) ? t : 0.0f;
-                        }
-#pragma unroll
-                        for (int e = 0; e < kQ; ++e) {
-                            const int p = (threadIdx.x >> 5) + 8 * (q0 + e);
-                            if (p < W)
-                                lds[l * kFpLs + p] = v[e];
-                        }
-                    }
-                } else {
-                    // line = row l0 + l, position = column pmin + p: batch q0 holds
-                    // positions [64 (q0 / kQ), + 64) of all 32 lines
-                    for (int q0 = 0; q0 < kPer; q0 += kQ) {
-                        const int pb = q0 / kQ;
-                        if (64 * pb >= W)
-                            break;
-                        float v[kQ];
-#pragma unroll
-                        for (int e = 0; e < kQ; ++e) {
-                            const int l = (threadIdx.x >> 6) + 4 * e;
-                            const int p = (threadIdx.x & 63) + 64 * pb;
-                            const int col = pmin + p;
-                            const int cc = min(max(col, 0), N - 1);
-                            const int row = min(l0 + l, N - 1);
-                            const float t = gload(img, (uint32_t)row * (uint32_t)N + (uint32_t)cc);
-                            v[e] = (col == cc && l < nl) ? t : 0.0f;
-                        }
-#pragma unroll
-                        for (int e = 0; e < kQ; ++e) {
-                            const int l = (threadIdx.x >> 6) + 4 * e;
-                            const int p = (threadIdx.x & 63) + 64 * pb;
-                            if (p < W)
-                                lds[l * kFpLs + p] = v[e];
-                        }
-                    }
-                }
-            }
+            if (!outside)
+                fp_stage(lds, img, N, hor, l0, nl, pmin, W);
             __syncthreads();
 
             // Inactive rays sample a fixed in-window position with zero step;
@@ -426,13 +441,134 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
     }
 }
 
+// Forward projection of one angle (the SART data step): the residual row
+// r[angle] = (b[angle] - FP(x)[angle]) * R[angle] of every slice that has the
+// angle, fast-path arithmetic.  A work item is 128 detectors of one slice (two
+// groups of 64, one wave pair each); the two waves of a pair sample
+// alternate 16-line halves of each staged 32-line chunk, and their partial
+// sums are added in LDS.  The footprint of 128 rays over 32 lines is at most
+// 128 sqrt(2) + 35 < 256 positions wide.
+template <typename T, typename TI>
+__global__ __launch_bounds__(256) void k_fp_angle(const uirk_slice *__restrict__ slices,
+                                                  const uirk_fp_item *__restrict__ items, int n_items,
+                                                  uirk_xcd_part part, int angle, int *__restrict__ err)
+{
+    __shared__ float lds[kFpLines * kFpLs];
+    __shared__ float red[2][4];
+    __shared__ float half_sum[2][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int grp = wave >> 1, half = wave & 1;
+    const ItemRange range = item_range(part, n_items);
+
+    for (int it = range.first; it < range.end; it += range.stride) {
+        const uirk_fp_item item = items[it];
+        const uirk_slice *S = &slices[item.slice];
+        if (angle >= S->n_angles)
+            continue; // uniform over the workgroup
+        const int N = S->n, D = S->n_dets;
+        const TI *img = uniform((const TI *)S->x);
+        const float *trig = uniform(S->trig);
+        const int s = item.d0 + 64 * grp + lane;
+        const float st = gload(trig, 2 * angle), ct = gload(trig, 2 * angle + 1);
+        // direction of ufo-ir-parallel-projector-task.c:462-498: vertical iff |sin| <= |cos|
+        const bool hor = !(fabsf(st) <= fabsf(ct));
+
+        // projector-parallel-joseph.cl:51-66 (detector quantities use n_dets)
+        const float required_width = S->axis * 2;
+        const float diff = (float)D - required_width;
+        const float rot = diff / 2.0f;
+        const float origin_shift = (float)N / 2.0f;
+        const bool in_range = s < D;
+        const bool on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
+        const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
+        float det_step, step;
+        if (hor) {
+            det_step = -1.0f / st;
+            step = ct / st;
+        } else {
+            det_step = 1.0f / ct;
+            step = st / ct;
+        }
+        const double p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
+
+        float acc = 0.0f;
+        for (int l0 = 0; l0 < N; l0 += kFpLines) {
+            const int nl = min(kFpLines, N - l0);
+            const float pos = (float)(p0 + (double)l0 * (double)step);
+            float lo = 3.0e38f, hi = -3.0e38f;
+            if (on) {
+                const float a = pos, b = pos + (float)(nl - 1) * step;
+                lo = fminf(a, b);
+                hi = fmaxf(a, b);
+            }
+            block_lohi(lo, hi, red, lane, wave);
+            if (!(lo <= hi))
+                break; // no active ray in this workgroup
+            const int pmin = (int)floorf(lo - 0.5f) - 1;
+            const int W = (int)floorf(hi - 0.5f) + 3 - pmin;
+            const bool overflow = W > kFpW;
+            if (overflow && threadIdx.x == 0)
+                atomicOr(err, 1);
+            const bool outside = overflow || pmin + W <= 0 || pmin >= N;
+            if (!outside)
+                fp_stage(lds, img, N, hor, l0, nl, pmin, W);
+            __syncthreads();
+            if (outside)
+                continue;
+            // this wave's lines [lb, le) of the chunk; inactive rays sample a
+            // fixed in-window position with zero step
+            const int lb = 16 * half, le = min(lb + 16, nl);
+            const float off = 0.5f + (float)pmin;
+            float q = on ? (float)(p0 + (double)(l0 + lb) * (double)step) - off : lo + 1.0f - off;
+            const float se = on ? step : 0.0f;
+            int l = lb;
+            for (; l + 2 <= le; l += 2) {
+                const float *L0 = lds + l * kFpLs, *L1 = L0 + kFpLs;
+                const float q1 = q + se;
+                const int i0 = cvt_flr(q), i1 = cvt_flr(q1);
+                const float a0 = fract_pos(q), a1 = fract_pos(q1);
+                const float t00 = L0[i0], t01 = L0[i0 + 1], t10 = L1[i1], t11 = L1[i1 + 1];
+                q = q1 + se;
+                acc = __builtin_fmaf(a0, t01 - t00, acc + t00);
+                acc = __builtin_fmaf(a1, t11 - t10, acc + t10);
+            }
+            if (l < le) {
+                const float *L = lds + l * kFpLs;
+                const int i0 = cvt_flr(q);
+                const float a = fract_pos(q);
+                const float t0 = L[i0], t1 = L[i0 + 1];
+                acc = __builtin_fmaf(a, t1 - t0, acc + t0);
+            }
+        }
+
+        if (half)
+            half_sum[grp][lane] = acc;
+        __syncthreads();
+        if (!half && in_range) {
+            const T *bsrc = uniform((const T *)S->b);
+            T *rdst = uniform((T *)S->r);
+            const float *ray_w = uniform(S->ray_w);
+            const uint32_t o = (uint32_t)angle * (uint32_t)D + (uint32_t)s;
+            const float bval = gload(bsrc, o);
+            float w = on ? bval + (acc + half_sum[grp][lane]) * S->corr : bval;
+            if (ray_w)
+                w = w * gload(ray_w, o);
+            gstore(rdst, o, w);
+        }
+        __syncthreads(); // half_sum is free for the next item
+    }
+}
+
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
+// a_sel >= 0 backprojects angle a_sel only (the SART update; items of slices
+// without that angle are left alone), else all of the slice's direction runs.
 template <typename T, typename TI, bool REF, int NA, int ROWS>
 __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
-                                                uirk_xcd_part part, int *__restrict__ err)
+                                                uirk_xcd_part part, int a_sel, int *__restrict__ err)
 {
     // reference-compatible path: plain staged bins; fast path: (t, dt) pairs
     __shared__ float lds[REF ? NA * kBpWb : 1];
@@ -449,8 +585,10 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         const int N = S->n, D = S->n_dets;
         const T *sino = uniform((const T *)S->b);
         const float *trig = uniform(S->trig);
+        if (a_sel >= S->n_angles)
+            continue; // uniform over the workgroup
         const float relax = S->relax;
-        const int n_runs = S->n_runs;
+        const int n_runs = a_sel >= 0 ? 1 : S->n_runs;
         const int px = item.x0 + lane;
 
         // projector-parallel-joseph.cl:174-190 (detector quantities use n_dets)
@@ -491,7 +629,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         }
 
         for (int r = 0; r < n_runs; ++r) {
-            const int roff = (int)S->run_off[r], rn = (int)S->run_n[r];
+            const int roff = a_sel >= 0 ? a_sel : (int)S->run_off[r];
+            const int rn = a_sel >= 0 ? 1 : (int)S->run_n[r];
             for (int b0 = 0; b0 < rn; b0 += NA) {
                 const int na = min(NA, rn - b0);
                 const int abase = roff + b0;
@@ -725,9 +864,10 @@ int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *i
 
 template <typename T, typename TI, bool REF, int NA, int ROWS>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
-              const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+              const uirk_xcd_part &part, int grid, int a_sel, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, a_sel,
+                       err);
     return (int)hipGetLastError();
 }
 
@@ -746,27 +886,38 @@ int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice
 // fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
 template <typename T, typename TI>
 int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items,
-                   int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+                   int n, const uirk_xcd_part &part, int grid, int a_sel, int *err, hipStream_t st)
 {
     const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
     if (na16 && r8)
-        return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, a_sel, err, st);
     if (na16)
-        return launch_bp<T, TI, false, 16, 16>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 16, 16>(sl, one, items, n, part, grid, a_sel, err, st);
     if (r8)
-        return launch_bp<T, TI, false, 32, 8>(sl, one, items, n, part, grid, err, st);
-    return launch_bp<T, TI, false, 32, 16>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 32, 8>(sl, one, items, n, part, grid, a_sel, err, st);
+    return launch_bp<T, TI, false, 32, 16>(sl, one, items, n, part, grid, a_sel, err, st);
+}
+
+template <typename T, typename TI>
+int launch_fp_angle(const uirk_slice *sl, const uirk_fp_item *items, int n, const uirk_xcd_part &part, int grid,
+                    int angle, int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_fp_angle<T, TI>), dim3(grid), dim3(256), 0, st, sl, items, n, part, angle, err);
+    return (int)hipGetLastError();
 }
 
 // (sinogram, image) storage pairs: both FP32, both FP16, both BF16, or
-// FP16 / BF16 sinograms with an FP32 image
-template <bool FP>
+// FP16 / BF16 sinograms with an FP32 image.  K: 0 = FP over all angles, 1 = BP
+// (a_sel as k_bp_tab), 2 = FP of angle a_sel (k_fp_angle).
+template <int K>
 int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one,
-                const void *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+                const void *items, int n, const uirk_xcd_part &part, int grid, int a_sel, int *err, hipStream_t st)
 {
-#define UIRK_LAUNCH(TS, TIM)                                                                                     \
-    return FP ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)     \
-              : launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)
+#define UIRK_LAUNCH(TS, TIM)                                                                                      \
+    return K == 0 ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)  \
+         : K == 1 ? launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, a_sel, err, \
+                                            st)                                                                    \
+                  : launch_fp_angle<TS, TIM>(sl, (const uirk_fp_item *)items, n, part, grid, a_sel, err, st)
     if (img_dtype == UIR_DTYPE_F32) {
         switch (dtype) {
         case UIR_DTYPE_F16: UIRK_LAUNCH(__half, float);
@@ -821,12 +972,37 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_fp<float, float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    return launch_fast<true>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    return launch_fast<0>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
+}
+
+int uirk_fp_angle(const uirk_slice *d_slices, const uirk_fp_item *d_items, int n_items, const uirk_xcd_part *part,
+                  int dtype, int img_dtype, int angle, int grid, int *d_err, void *stream)
+{
+    if (!d_slices || angle < 0)
+        return (int)hipErrorInvalidValue;
+    if (n_items <= 0)
+        return 0;
+    if (grid <= 0 || grid > n_items)
+        grid = n_items;
+    uirk_xcd_part pt{};
+    if (part)
+        pt = *part;
+    const uirk_tile tl = UIRK_TILE_DEFAULT;
+    return launch_fast<2>(dtype, img_dtype, tl, d_slices, uirk_slice{}, d_items, n_items, pt, grid, angle, d_err,
+                          (hipStream_t)stream);
 }
 
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
                   const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
                   int *d_err, void *stream)
+{
+    return uirk_bp_table_angle(d_slices, one, d_items, n_items, part, mode, dtype, img_dtype, tile, grid, -1, d_err,
+                               stream);
+}
+
+int uirk_bp_table_angle(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
+                        const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
+                        int a_sel, int *d_err, void *stream)
 {
     uirk_slice single{};
     if (!d_slices) {
@@ -846,11 +1022,12 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
-        if (dtype != UIR_DTYPE_F32 || img_dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE)
+        if (dtype != UIR_DTYPE_F32 || img_dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE ||
+            a_sel >= 0)
             return (int)hipErrorInvalidValue;
-        return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
+        return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
     }
-    return launch_fast<false>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    return launch_fast<1>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, a_sel, d_err, st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

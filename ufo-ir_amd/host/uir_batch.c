@@ -18,55 +18,11 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "uir_internal.h"
-
-typedef struct {
-    uint32_t n, D, A;
-    float step, axis;        /* resolved axis */
-    float r_min;             /* ray weight floor */
-    uirk_tile tile;          /* tile shape (autotuned per geometry, or the default) */
-    float *h_sin, *h_cos;
-    float *d_trig, *d_ray_w; /* carved from the batch allocation */
-    uir_subset *runs;
-    uint32_t n_runs;
-} geom_t;
-
-#define UIR_FP_SHAPES 3
-#define UIR_BP_SHAPES 4
-
-typedef struct {
-    uirk_tile tile;
-    void *d_items;
-    int n_items, grid;
-    uirk_xcd_part part;
-} kgroup;
+#include "uir_batch_impl.h"
 
 static int fp_shape(const uirk_tile *t) { return t->fp_angles == 8 ? 1 : (t->fp_angles == 32 ? 2 : 0); }
 static const int k_fp_shape_angles[UIR_FP_SHAPES] = {UIRK_FP_ANGLES, 8, 32};
 static int bp_shape(const uirk_tile *t) { return (t->bp_angles == 16) * 2 + (t->bp_rows == 32); }
-
-struct uir_batch {
-    uir_resources *res;
-    uir_dtype dtype;  /* storage mode */
-    int sdt, idt;     /* sinogram and image element types */
-    size_t ess, esi;  /* their sizes */
-    uint32_t n_slices, n_geoms;
-    uir_slice_spec *specs;
-    uint32_t *geom_of;
-    geom_t *geoms;
-    void *mem;
-    size_t mem_bytes;
-    void **x, **b, **r;
-    uirk_slice *d_fp_sl, *d_bp_sl;
-    /* one launch per tile-shape group: FP by angles per item (2 shapes), BP by
-     * staged angles x tile rows (4 shapes); each group's table is slice-major */
-    kgroup fpg[UIR_FP_SHAPES], bpg[UIR_BP_SHAPES];
-    int *d_err;
-    double fp_samples, bp_samples;
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    uint32_t graph_iters;
-};
 
 #define TRY(x) do { uir_status s_ = (x); if (s_ != UIR_OK) return s_; } while (0)
 
@@ -107,6 +63,7 @@ void uir_batch_free(uir_batch *b)
     for (int g = 0; g < UIR_BP_SHAPES; ++g)
         uir_dfree(b->bpg[g].d_items);
     uir_dfree(b->d_err);
+    uir_asd_free(b->asd);
     uir_resources_free(b->res);
     free(b);
 }

@@ -91,6 +91,12 @@ class BatchInfo(C.Structure):
                 ("bp_grid", C.c_uint32), ("fp_samples", C.c_double), ("bp_samples", C.c_double)]
 
 
+class AsdPocsParams(C.Structure):
+    _fields_ = [("beta", C.c_float), ("beta_red", C.c_float), ("alpha", C.c_float), ("alpha_red", C.c_float),
+                ("r_max", C.c_float), ("ng", C.c_uint32), ("df_iterations", C.c_uint32),
+                ("positive_constraint", C.c_int32), ("compat", C.c_int32)]
+
+
 _lib = None
 
 
@@ -166,6 +172,10 @@ def load():
     L.uir_batch_set_sinogram.argtypes = [vp, u32, vp, C.c_int]
     L.uir_batch_reset.argtypes = [vp]
     L.uir_batch_iterate.argtypes = [vp, u32, C.c_int]
+    L.uir_asdpocs_params_default.argtypes = [C.POINTER(AsdPocsParams)]
+    L.uir_asdpocs_params_default.restype = None
+    L.uir_batch_asdpocs.argtypes = [vp, u32, C.POINTER(AsdPocsParams), C.c_int]
+    L.uir_batch_asdpocs_diagnostics.argtypes = [vp, u32, vp, u32, C.POINTER(u32)]
     L.uir_batch_get_image.argtypes = [vp, u32, vp, C.c_int]
     L.uir_batch_get_info.argtypes = [vp, C.POINTER(BatchInfo)]
     L.uir_batch_time_kernels.argtypes = [vp, u32, C.POINTER(f32), C.POINTER(f32)]
@@ -550,6 +560,27 @@ class Batch:
 
     def iterate(self, iterations, use_graph=True):
         check(load().uir_batch_iterate(self.handle, int(iterations), int(bool(use_graph))))
+
+    def asdpocs(self, iterations, use_graph=True, **params):
+        """Batched ASD-POCS (uir_batch_asdpocs); params: beta, beta_red,
+        alpha, alpha_red, r_max, ng, df_iterations, positive_constraint,
+        compat (defaults of uir_asdpocs_params_default)."""
+        p = AsdPocsParams()
+        load().uir_asdpocs_params_default(C.byref(p))
+        for k, v in params.items():
+            if k not in dict(AsdPocsParams._fields_):
+                raise TypeError("unknown ASD-POCS parameter %r" % k)
+            setattr(p, k, int(v) if isinstance(v, bool) else v)
+        check(load().uir_batch_asdpocs(self.handle, int(iterations), C.byref(p), int(bool(use_graph))))
+
+    def asdpocs_diagnostics(self, i, max_rows=100000):
+        """(iterations, 4) rows dd, dp, dg, dtgv of slice i from the last asdpocs()."""
+        n = C.c_uint32()
+        check(load().uir_batch_asdpocs_diagnostics(self.handle, i, None, 0, C.byref(n)))
+        rows = np.zeros((min(n.value, max_rows), 4), np.float32)
+        if len(rows):
+            check(load().uir_batch_asdpocs_diagnostics(self.handle, i, rows.ctypes.data, len(rows), C.byref(n)))
+        return rows
 
     def image(self, i):
         n = int(self.specs[i]["n"])
