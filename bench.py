@@ -353,7 +353,7 @@ def asdpocs_c4(args, uir):
         out[storage]["image0"] = bt.image(0)
         bt.close()
     # the plain path: the asdpocs task, one slice per process() call
-    proj_b = uir.Task("parallel-projector", step=step, is_forward=False)
+    proj_b = uir.Task("parallel-projector", step=step, is_forward=False, image_size=n)
     sart = uir.Task("sart", num_iterations=1)
     task = uir.Task("asdpocs", projector=proj_b, df_minimizer=sart, num_iterations=iters,
                     compat=bool(args.c4_compat)).setup(res)
@@ -456,7 +456,9 @@ def main():
         sart_c2(args, uir)
         return
     if args.c4:
-        return asdpocs_c4(args, uir)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        asdpocs_c4(args, uir)
+        return
     if args.c3:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         batch_c3(args, uir)

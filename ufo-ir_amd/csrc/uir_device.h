@@ -155,6 +155,10 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
 int uirk_bp_table_angle(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
                         const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
                         int a_sel, int *d_err, void *stream);
+/* Ray weights of the fast path: ray_w = 1 / FP(1) with FP(1) evaluated in FP64
+ * (exact coordinates, (sin, cos) from trig); 0 where FP(1) = 0 or < r_min. */
+int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
+                       void *stream);
 /* Fast-path FP of one angle over every slice that has it (the SART data step):
  * r[angle] = (b[angle] + corr * FP(x)[angle]) * ray_w[angle] (ray_w NULL: no
  * factor).  Items: {slice, d0} per 128 detectors (uirk_plan_fp_angle). */

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
 
         float pos[APT], stp[APT], acc[APT], inner[APT];
-        double p0d[APT];
+        double p0d[APT], sd[APT];
         bool on[APT];
 #pragma unroll
         for (int j = 0; j < APT; ++j) {
@@ -256,19 +256,28 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             on[j] = det_on && k < item.na;
             const int ang = item.a0 + (k < item.na ? k : 0);
             const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
-            float det_step, step;
-            if (hor) {
-                det_step = -1.0f / st;
-                step = ct / st;
+            double p0;
+            if (REF) {
+                float det_step, step;
+                if (hor) {
+                    det_step = -1.0f / st;
+                    step = ct / st;
+                } else {
+                    det_step = 1.0f / ct;
+                    step = st / ct;
+                }
+                // start coordinate: double expression rounded to float (:73, :134)
+                p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
+                sd[j] = (double)step;
             } else {
-                det_step = 1.0f / ct;
-                step = st / ct;
+                // exact coordinates: steps and chunk anchors in double
+                const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+                sd[j] = hor ? (double)ct / (double)st : (double)st / (double)ct;
+                p0 = tdet * det_step - 0.5 * (double)N * sd[j] + 0.5 * (double)N;
             }
-            // start coordinate: double expression rounded to float (:73, :134)
-            const double p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
             p0d[j] = p0;
             pos[j] = (float)p0;
-            stp[j] = step;
+            stp[j] = (float)sd[j];
             acc[j] = 0.0f;
             inner[j] = 0.0f;
         }
@@ -289,7 +298,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 #pragma unroll
             for (int j = 0; j < APT; ++j) {
                 if (!REF)
-                    pos[j] = (float)(p0d[j] + (double)l0 * (double)stp[j]);
+                    pos[j] = (float)(p0d[j] + (double)l0 * sd[j]);
                 if (on[j]) {
                     const float a = pos[j], b = pos[j] + (float)(nl - 1) * stp[j];
                     lo = fminf(lo, fminf(a, b));
@@ -478,24 +487,19 @@ __global__ __launch_bounds__(256) void k_fp_angle(const uirk_slice *__restrict__
         const float required_width = S->axis * 2;
         const float diff = (float)D - required_width;
         const float rot = diff / 2.0f;
-        const float origin_shift = (float)N / 2.0f;
         const bool in_range = s < D;
         const bool on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
         const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
-        float det_step, step;
-        if (hor) {
-            det_step = -1.0f / st;
-            step = ct / st;
-        } else {
-            det_step = 1.0f / ct;
-            step = st / ct;
-        }
-        const double p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
+        // exact coordinates: steps and anchors in double
+        const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+        const double step_d = hor ? (double)ct / (double)st : (double)st / (double)ct;
+        const double p0 = tdet * det_step - 0.5 * (double)N * step_d + 0.5 * (double)N;
+        const float step = (float)step_d;
 
         float acc = 0.0f;
         for (int l0 = 0; l0 < N; l0 += kFpLines) {
             const int nl = min(kFpLines, N - l0);
-            const float pos = (float)(p0 + (double)l0 * (double)step);
+            const float pos = (float)(p0 + (double)l0 * step_d);
             float lo = 3.0e38f, hi = -3.0e38f;
             if (on) {
                 const float a = pos, b = pos + (float)(nl - 1) * step;
@@ -520,7 +524,7 @@ __global__ __launch_bounds__(256) void k_fp_angle(const uirk_slice *__restrict__
             // fixed in-window position with zero step
             const int lb = 16 * half, le = min(lb + 16, nl);
             const float off = 0.5f + (float)pmin;
-            float q = on ? (float)(p0 + (double)(l0 + lb) * (double)step) - off : lo + 1.0f - off;
+            float q = on ? (float)(p0 + (double)(l0 + lb) * step_d) - off : lo + 1.0f - off;
             const float se = on ? step : 0.0f;
             int l = lb;
             for (; l + 2 <= le; l += 2) {
@@ -854,6 +858,45 @@ __global__ __launch_bounds__(256) void k_sb_shrink(const T *__restrict__ u, floa
     ey[o] = dy - (gy - dy);
 }
 
+// Ray weights R = 1 / FP(1) of the fast path's geometry, FP(1) in FP64 with
+// exact coordinates: one thread per ray, every line, the operations of the
+// FP64 evaluation of the projector (samples outside the image add exact
+// zeros).  R = 0 where FP(1) = 0 or FP(1) < r_min.  FP64 keeps the floor's
+// decision from depending on FP32 rounding: rays that clip a corner have
+// FP(1) made of one or two fractional weights, which float coordinates
+// near 1000 shift by ~1e-4.
+__global__ __launch_bounds__(256) void k_ray_weights64(const float *__restrict__ trig, int N, int D, int A,
+                                                       float axis_f, float r_min, float *__restrict__ ray_w)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x, ang = blockIdx.y;
+    if (s >= D || ang >= A)
+        return;
+    const double axis = (double)axis_f;
+    const double required_width = 2.0 * axis;
+    const double diff = (double)D - required_width;
+    float w = 0.0f;
+    if (!((diff < 0 && (double)s < fabs(diff)) || (diff > 0 && (double)s > required_width))) {
+        const float stf = trig[2 * ang], ctf = trig[2 * ang + 1];
+        const double st = (double)stf, ct = (double)ctf;
+        const bool vertical = fabsf(stf) <= fabsf(ctf);
+        const double t = (double)s + 0.5 - axis;
+        const double det_step = vertical ? 1.0 / ct : -1.0 / st;
+        const double step = vertical ? st / ct : ct / st;
+        const double v0 = t * det_step - 0.5 * N * step + 0.5 * N;
+        double acc = 0.0;
+        for (int i = 0; i < N; ++i) {
+            const double v = v0 + i * step - 0.5;
+            const double fv = floor(v);
+            const int j0 = (int)fv;
+            const double b = v - fv;
+            const double t0 = (j0 >= 0 && j0 < N) ? 1.0 : 0.0, t1 = (j0 + 1 >= 0 && j0 + 1 < N) ? 1.0 : 0.0;
+            acc += (1.0 - b) * t0 + b * t1;
+        }
+        w = (acc != 0.0 && acc >= (double)r_min) ? (float)(1.0 / acc) : 0.0f;
+    }
+    ray_w[(size_t)ang * D + s] = w;
+}
+
 template <typename T, typename TI, bool REF, int APT>
 int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
@@ -973,6 +1016,18 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
         return launch_fp<float, float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
     return launch_fast<0>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
+}
+
+int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
+                       void *stream)
+{
+    if (n <= 0 || n_dets <= 0 || n_angles <= 0)
+        return 0;
+    if (n_angles > 65535)
+        return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ray_weights64, dim3((n_dets + 255) / 256, n_angles), dim3(256), 0, (hipStream_t)stream,
+                       d_trig, n, n_dets, n_angles, axis, r_min, d_ray_w);
+    return (int)hipGetLastError();
 }
 
 int uirk_fp_angle(const uirk_slice *d_slices, const uirk_fp_item *d_items, int n_items, const uirk_xcd_part *part,

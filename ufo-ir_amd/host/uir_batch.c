@@ -74,53 +74,15 @@ static uir_status upload(void **dst, const void *src, size_t bytes)
     return uir_check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
 }
 
-/* R = 1 / FP(1) for one geometry with the fast FP32 projector (inv(0) = 0),
- * written to d_ray_w (A x D FP32); d_trig holds the geometry's (sin, cos)
- * table.  Synchronises the stream. */
-uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uint32_t A, float axis,
-                           const float *h_sin, const float *h_cos, const uir_subset *runs, uint32_t n_runs,
-                           const float *d_trig, float r_min, float *d_ray_w)
+/* R = 1 / FP(1) for one geometry, FP(1) in FP64 (uirk_ray_weights64), into
+ * d_ray_w (A x D FP32); d_trig holds the geometry's (sin, cos) table.
+ * Synchronises the stream. */
+uir_status uir_ray_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, float axis, const float *d_trig,
+                           float r_min, float *d_ray_w)
 {
-    const size_t nv = (size_t)n * n, ns = (size_t)A * D;
-    float *ones = NULL, *zeros = NULL;
-    uirk_fp_item *d_items = NULL;
-    uirk_fp_vec fv = {0};
-    uir_status s = uir_dmalloc((void **)&ones, nv * 4);
-    if (s == UIR_OK)
-        s = uir_dmalloc((void **)&zeros, ns * 4);
-    if (s == UIR_OK && uirk_plan_fp(&fv, 0, (int)n, (int)D, axis, h_sin, h_cos, runs, n_runs, UIRK_FP_ANGLES))
-        s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
-    if (s == UIR_OK)
-        s = upload((void **)&d_items, fv.v, fv.n * sizeof(uirk_fp_item));
-    hipStream_t st = (hipStream_t)stream;
-    if (s == UIR_OK)
-        s = uir_check_hip(uirk_set(ones, nv, 1.0f, st), "set");
-    if (s == UIR_OK)
-        s = uir_check_hip(uirk_set(zeros, ns, 0.0f, st), "set");
-    if (s == UIR_OK) {
-        uirk_slice sl;
-        memset(&sl, 0, sizeof sl);
-        sl.x = ones;
-        sl.b = zeros;
-        sl.r = d_ray_w;
-        sl.trig = d_trig;
-        sl.n = (int32_t)n;
-        sl.n_dets = (int32_t)D;
-        sl.n_angles = (int32_t)A;
-        sl.axis = axis;
-        sl.corr = 1.0f;
-        s = uir_check_hip(uirk_fp_table(NULL, &sl, d_items, (int)fv.n, NULL, 1, UIR_DTYPE_F32, UIR_DTYPE_F32, NULL,
-                                        uirk_plan_grid((int)fv.n, 4), d_err, st), "FP launch");
-    }
-    if (s == UIR_OK)
-        s = uir_check_hip(uirk_inv_min(d_ray_w, d_ray_w, ns, r_min, st), "inv");
-    if (s == UIR_OK)
-        s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
-    uir_dfree(ones);
-    uir_dfree(zeros);
-    uir_dfree(d_items);
-    free(fv.v);
-    return s;
+    TRY(uir_check_hip(uirk_ray_weights64(d_trig, (int)n, (int)D, (int)A, axis, r_min, d_ray_w, stream),
+                      "ray weights"));
+    return uir_check_hip(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
 }
 
 uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_spec *specs, uint32_t n_slices,
@@ -256,8 +218,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         s = uir_check_hip(hipMemcpy(G->d_trig, trig, sizeof(float) * 2 * G->A, hipMemcpyHostToDevice), "hipMemcpy");
         free(trig);
         if (s != UIR_OK ||
-            (s = uir_ray_weights(b->res->stream, b->d_err, G->n, G->D, G->A, G->axis, G->h_sin, G->h_cos, G->runs,
-                                 G->n_runs, G->d_trig, G->r_min, G->d_ray_w)) != UIR_OK)
+            (s = uir_ray_weights(b->res->stream, G->n, G->D, G->A, G->axis, G->d_trig, G->r_min, G->d_ray_w)) != UIR_OK)
             goto fail;
     }
 

@@ -53,12 +53,10 @@ uir_status uir_ws_normalize(uir_workspace *ws, float *v, size_t n, void *stream)
 uir_status uir_dmalloc(void **ptr, size_t bytes);
 void uir_dfree(void *ptr);
 
-/* R = 1 / FP(1) of one geometry with the fast FP32 projector into d_ray_w
- * (A x D), 0 where FP(1) = 0 or FP(1) < r_min; synchronises the stream
- * (uir_batch.c). */
-uir_status uir_ray_weights(void *stream, int *d_err, uint32_t n, uint32_t D, uint32_t A, float axis,
-                           const float *h_sin, const float *h_cos, const uir_subset *runs, uint32_t n_runs,
-                           const float *d_trig, float r_min, float *d_ray_w);
+/* R = 1 / FP(1) of one geometry (FP(1) in FP64) into d_ray_w (A x D), 0 where
+ * FP(1) = 0 or FP(1) < r_min; synchronises the stream (uir_batch.c). */
+uir_status uir_ray_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, float axis, const float *d_trig,
+                           float r_min, float *d_ray_w);
 
 /* A storage mode split into the sinogram and image element types; returns 0
  * for an unknown mode. */

@@ -311,8 +311,7 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
     if ((st = uir_check_hip(hipMemcpyAsync(g->d_trig, trig, sizeof(float) * 2 * g->A, hipMemcpyHostToDevice,
                                            s->stream), "hipMemcpyAsync")) != UIR_OK)
         goto fail;
-    if ((st = uir_ray_weights(s->stream, s->d_err, g->n, g->D, g->A, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs,
-                              g->d_trig, g->r_min, g->d_ray_w)) != UIR_OK)
+    if ((st = uir_ray_weights(s->stream, g->n, g->D, g->A, g->axis, g->d_trig, g->r_min, g->d_ray_w)) != UIR_OK)
         goto fail;
     free(trig);
     *out = g;
