@@ -144,7 +144,12 @@ template <typename TI> __device__ __forceinline__ float nb(const TI *img, int n,
 
 // g = l1_grad(x) (ufo-math-tvstd-method.cl:5-57: smoothed TV gradient, eps
 // 1e-3, zero outside the image) and the partial of |g| for this TV step;
-// nothing once the slice's TV steps have ended.
+// nothing once the slice's TV steps have ended.  Block c of a slice covers
+// columns [256 (c % cx), +256) x rows [kTvRows (c / cx), +kTvRows): each thread
+// walks down one column with the 3 x 3 neighbourhood in registers, loading
+// one row (x - 1, x, x + 1) per pixel, all coalesced.  The three quotients
+// v / sqrt(w) are formed as v * rsqrt(w) (hardware reciprocal square root,
+// ~1 ulp).
 template <typename TI>
 __global__ __launch_bounds__(kThreads) void k_asd_tvgrad(const uirk_asd_slice *__restrict__ sl,
                                                          const uirk_asd_state *__restrict__ state, int l2,
@@ -152,24 +157,34 @@ __global__ __launch_bounds__(kThreads) void k_asd_tvgrad(const uirk_asd_slice *_
 {
     const uirk_asd_slice S = sl[blockIdx.y];
     const int n = (int)S.n;
-    const uint32_t nv = S.n * S.n, base = blockIdx.x * kChunk;
+    const int cx = (n + kThreads - 1) / kThreads, cy = (n + UIRK_ASD_TV_ROWS - 1) / UIRK_ASD_TV_ROWS;
     double acc = 0;
-    if (base < nv && !state[blockIdx.y].stop) {
+    if ((int)blockIdx.x < cx * cy && !state[blockIdx.y].stop) {
         const TI *img = (const TI *)S.x;
-        const float eps = 0.001f;
-        for (int k = 0; k < kPer; ++k) {
-            const uint32_t i = base + k * kThreads + threadIdx.x;
-            if (i >= nv)
-                continue;
-            const int x = (int)(i % S.n), y = (int)(i / S.n);
-            const float f3 = nb(img, n, x, y), f1 = nb(img, n, x, y - 1), f0 = nb(img, n, x - 1, y - 1);
-            const float f2 = nb(img, n, x - 1, y), f4 = nb(img, n, x + 1, y);
-            const float f6 = nb(img, n, x + 1, y + 1), f5 = nb(img, n, x, y + 1);
-            const float df = (2.0f * f3 - f2 - f5) / sqrtf(eps + (f3 - f2) * (f3 - f2) + (f3 - f5) * (f3 - f5))
-                           - (f4 - f3) / sqrtf(eps + (f4 - f3) * (f4 - f3) + (f4 - f6) * (f4 - f6))
-                           - (f1 - f3) / sqrtf(eps + (f1 - f3) * (f1 - f3) + (f1 - f0) * (f1 - f0));
-            S.g[i] = df;
-            acc += measure(df, l2);
+        const int x = (int)(blockIdx.x % cx) * kThreads + threadIdx.x;
+        const int y0 = (int)(blockIdx.x / cx) * UIRK_ASD_TV_ROWS;
+        const int y1 = min(y0 + UIRK_ASD_TV_ROWS, n);
+        if (x < n) {
+            const float eps = 0.001f;
+            // rows y - 1 (a), y (b); columns x - 1 (m), x (0), x + 1 (p)
+            float am = nb(img, n, x - 1, y0 - 1), a0 = nb(img, n, x, y0 - 1);
+            float bm = nb(img, n, x - 1, y0), b0 = nb(img, n, x, y0), bp = nb(img, n, x + 1, y0);
+            for (int y = y0; y < y1; ++y) {
+                const float cm = nb(img, n, x - 1, y + 1), c0 = nb(img, n, x, y + 1), cp = nb(img, n, x + 1, y + 1);
+                const float f0 = am, f1 = a0, f2 = bm, f3 = b0, f4 = bp, f5 = c0, f6 = cp;
+                const float w0 = eps + (f3 - f2) * (f3 - f2) + (f3 - f5) * (f3 - f5);
+                const float w1 = eps + (f4 - f3) * (f4 - f3) + (f4 - f6) * (f4 - f6);
+                const float w2 = eps + (f1 - f3) * (f1 - f3) + (f1 - f0) * (f1 - f0);
+                const float df = (2.0f * f3 - f2 - f5) * __builtin_amdgcn_rsqf(w0) -
+                                 (f4 - f3) * __builtin_amdgcn_rsqf(w1) - (f1 - f3) * __builtin_amdgcn_rsqf(w2);
+                S.g[(uint32_t)y * (uint32_t)n + (uint32_t)x] = df;
+                acc += measure(df, l2);
+                am = bm;
+                a0 = b0;
+                bm = cm;
+                b0 = c0;
+                bp = cp;
+            }
         }
     }
     block_sum(acc, &partials[(size_t)blockIdx.y * nblk + blockIdx.x]);

@@ -150,15 +150,15 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
                   const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
                   int *d_err, void *stream);
-/* The same BP restricted to angle a_sel (fast mode; a_sel < 0: all angles).
- * Items of slices with fewer angles are left alone. */
-int uirk_bp_table_angle(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                        const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
-                        int a_sel, int *d_err, void *stream);
 /* Ray weights of the fast path: ray_w = 1 / FP(1) with FP(1) evaluated in FP64
  * (exact coordinates, (sin, cos) from trig); 0 where FP(1) = 0 or < r_min. */
 int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
                        void *stream);
+/* Fast-path BP of one angle over every slice that has it (the SART update):
+ * x_out = x + relax * pix_scale * A_angle^T b[angle] (posc: clamped at 0),
+ * a streaming pass; slices of up to max_n x max_n pixels. */
+int uirk_bp_angle(const uirk_slice *d_slices, int n_slices, int max_n, int dtype, int img_dtype, int angle,
+                  void *stream);
 /* Fast-path FP of one angle over every slice that has it (the SART data step):
  * r[angle] = (b[angle] + corr * FP(x)[angle]) * ray_w[angle] (ray_w NULL: no
  * factor).  Items: {slice, d0} per 128 detectors (uirk_plan_fp_angle). */
@@ -204,9 +204,11 @@ int uirk_sb_shrink(const void *u, int dtype, float *bx, float *by, float *ex, fl
 
 /* ---- Batched ASD-POCS steps between the projections (uir_asd.hip) ----
  * Grids are (nblk, n_slices): block (c, s) covers elements [c, c + 1) *
- * UIRK_ASD_CHUNK of slice s (nblk must cover the largest image; sinograms
- * are strided over the nblk blocks) and writes partials[s * nblk + c]. */
+ * UIRK_ASD_CHUNK of slice s (TVGRAD: a 256 x UIRK_ASD_TV_ROWS tile; nblk
+ * must cover the largest image both ways; sinograms are strided over the
+ * nblk blocks) and writes partials[s * nblk + c]. */
 #define UIRK_ASD_CHUNK 4096
+#define UIRK_ASD_TV_ROWS 16 /* TVGRAD blocks: 256 columns x 16 rows */
 typedef struct {
     void *x, *x_prev;    /* image and its checkpoint (image dtype) */
     float *g;            /* TV gradient (FP32) */

@@ -567,12 +567,10 @@ __global__ __launch_bounds__(256) void k_fp_angle(const uirk_slice *__restrict__
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
-// a_sel >= 0 backprojects angle a_sel only (the SART update; items of slices
-// without that angle are left alone), else all of the slice's direction runs.
 template <typename T, typename TI, bool REF, int NA, int ROWS>
 __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
-                                                uirk_xcd_part part, int a_sel, int *__restrict__ err)
+                                                uirk_xcd_part part, int *__restrict__ err)
 {
     // reference-compatible path: plain staged bins; fast path: (t, dt) pairs
     __shared__ float lds[REF ? NA * kBpWb : 1];
@@ -589,10 +587,8 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         const int N = S->n, D = S->n_dets;
         const T *sino = uniform((const T *)S->b);
         const float *trig = uniform(S->trig);
-        if (a_sel >= S->n_angles)
-            continue; // uniform over the workgroup
         const float relax = S->relax;
-        const int n_runs = a_sel >= 0 ? 1 : S->n_runs;
+        const int n_runs = S->n_runs;
         const int px = item.x0 + lane;
 
         // projector-parallel-joseph.cl:174-190 (detector quantities use n_dets)
@@ -633,8 +629,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
         }
 
         for (int r = 0; r < n_runs; ++r) {
-            const int roff = a_sel >= 0 ? a_sel : (int)S->run_off[r];
-            const int rn = a_sel >= 0 ? 1 : (int)S->run_n[r];
+            const int roff = (int)S->run_off[r], rn = (int)S->run_n[r];
             for (int b0 = 0; b0 < rn; b0 += NA) {
                 const int na = min(NA, rn - b0);
                 const int abase = roff + b0;
@@ -786,6 +781,78 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
     }
 }
 
+// BP of one angle (the SART update x += relax * A_k^T r_k), fast-path
+// arithmetic: a streaming pass over the images.  Block (t, s) is the t-th
+// 64 x 64 tile of slice s (one column and 16 rows per thread); the residual
+// row r[angle] (a few KB per slice) is read straight from global memory,
+// where every tile of the slice finds it in cache.  Exact per-pixel detector
+// coordinates, clamp-to-edge bins as k_bp_tab's fast path.
+template <typename T, typename TI>
+__global__ __launch_bounds__(256) void k_bp_angle(const uirk_slice *__restrict__ slices, int angle)
+{
+    const uirk_slice *S = &slices[blockIdx.y];
+    const int N = S->n, D = S->n_dets;
+    const int tiles_x = (N + kBpT - 1) / kBpT;
+    if (angle >= S->n_angles || (int)blockIdx.x >= tiles_x * tiles_x)
+        return;
+    const int x0 = (blockIdx.x % tiles_x) * kBpT, y0 = (blockIdx.x / tiles_x) * kBpT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = x0 + lane;
+    if (px >= N)
+        return;
+    const T *row = uniform((const T *)S->b) + (size_t)angle * D;
+    const float *trig = uniform(S->trig);
+    const float st = gload(trig, 2 * angle), ct = gload(trig, 2 * angle + 1);
+    // projector-parallel-joseph.cl:174-190 (detector quantities use n_dets)
+    const float axis = S->axis;
+    const float required_width = axis * 2;
+    const float diff = (float)D - required_width;
+    const float half_active = diff < 0 ? (float)D - axis : axis;
+    const int bin_lo = (int)floorf(axis - half_active + 0.5f - 0.5f);
+    const int bin_hi = (int)floorf(axis + half_active - 0.5f - 0.5f);
+    const float shift = (float)D / 2.0f - diff / 2.0f;
+    const float origin_shift = (float)N / 2.0f;
+    const float fX = (float)px + 0.5f - origin_shift;
+    const float ux = __builtin_fmaf(fX, ct, shift - 0.5f);
+    const TI *xin = uniform((const TI *)S->x);
+    TI *xout = uniform((TI *)S->x_out);
+    const float gain = S->relax * S->pix_scale;
+    const int posc = S->posc;
+    constexpr int kRows = kBpT / 4;
+    float xv[kRows], t0[kRows], t1[kRows], a[kRows];
+    // all loads first: the pixels, then both taps of every row
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const int py = min(y0 + wave + 4 * k, N - 1);
+        xv[k] = gload(xin, (uint32_t)py * (uint32_t)N + (uint32_t)px);
+        const float fY = (float)py + 0.5f - origin_shift;
+        const float u = __builtin_fmaf(-fY, st, ux);
+        const int bin = cvt_flr(u);
+        a[k] = u - floorf(u);
+        const int b0 = min(max(bin, bin_lo), bin_hi), b1 = min(max(bin + 1, bin_lo), bin_hi);
+        t0[k] = gload(row, (uint32_t)b0);
+        t1[k] = gload(row, (uint32_t)b1);
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const int py = y0 + wave + 4 * k;
+        if (py >= N)
+            continue;
+        float out = __builtin_fmaf(__builtin_fmaf(a[k], t1[k] - t0[k], t0[k]), gain, xv[k]);
+        if (posc)
+            out = out > 0 ? out : 0.0f;
+        gstore(xout, (uint32_t)py * (uint32_t)N + (uint32_t)px, out);
+    }
+}
+
+template <typename T, typename TI>
+int launch_bp_angle(const uirk_slice *sl, int n_slices, int max_n, int angle, hipStream_t st)
+{
+    const int tx = (max_n + kBpT - 1) / kBpT;
+    hipLaunchKernelGGL((k_bp_angle<T, TI>), dim3(tx * tx, n_slices), dim3(256), 0, st, sl, angle);
+    return (int)hipGetLastError();
+}
+
 __global__ void k_convert_f32_to(const float *src, void *dst, int dtype, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -907,10 +974,9 @@ int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *i
 
 template <typename T, typename TI, bool REF, int NA, int ROWS>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
-              const uirk_xcd_part &part, int grid, int a_sel, int *err, hipStream_t st)
+              const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, a_sel,
-                       err);
+    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
@@ -929,16 +995,16 @@ int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice
 // fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
 template <typename T, typename TI>
 int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items,
-                   int n, const uirk_xcd_part &part, int grid, int a_sel, int *err, hipStream_t st)
+                   int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
     const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
     if (na16 && r8)
-        return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, a_sel, err, st);
+        return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, err, st);
     if (na16)
-        return launch_bp<T, TI, false, 16, 16>(sl, one, items, n, part, grid, a_sel, err, st);
+        return launch_bp<T, TI, false, 16, 16>(sl, one, items, n, part, grid, err, st);
     if (r8)
-        return launch_bp<T, TI, false, 32, 8>(sl, one, items, n, part, grid, a_sel, err, st);
-    return launch_bp<T, TI, false, 32, 16>(sl, one, items, n, part, grid, a_sel, err, st);
+        return launch_bp<T, TI, false, 32, 8>(sl, one, items, n, part, grid, err, st);
+    return launch_bp<T, TI, false, 32, 16>(sl, one, items, n, part, grid, err, st);
 }
 
 template <typename T, typename TI>
@@ -951,16 +1017,15 @@ int launch_fp_angle(const uirk_slice *sl, const uirk_fp_item *items, int n, cons
 
 // (sinogram, image) storage pairs: both FP32, both FP16, both BF16, or
 // FP16 / BF16 sinograms with an FP32 image.  K: 0 = FP over all angles, 1 = BP
-// (a_sel as k_bp_tab), 2 = FP of angle a_sel (k_fp_angle).
+// over all angles, 2 = FP of angle `angle` (k_fp_angle).
 template <int K>
 int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one,
-                const void *items, int n, const uirk_xcd_part &part, int grid, int a_sel, int *err, hipStream_t st)
+                const void *items, int n, const uirk_xcd_part &part, int grid, int angle, int *err, hipStream_t st)
 {
 #define UIRK_LAUNCH(TS, TIM)                                                                                      \
     return K == 0 ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)  \
-         : K == 1 ? launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, a_sel, err, \
-                                            st)                                                                    \
-                  : launch_fp_angle<TS, TIM>(sl, (const uirk_fp_item *)items, n, part, grid, a_sel, err, st)
+         : K == 1 ? launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)  \
+                  : launch_fp_angle<TS, TIM>(sl, (const uirk_fp_item *)items, n, part, grid, angle, err, st)
     if (img_dtype == UIR_DTYPE_F32) {
         switch (dtype) {
         case UIR_DTYPE_F16: UIRK_LAUNCH(__half, float);
@@ -1030,6 +1095,31 @@ int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, flo
     return (int)hipGetLastError();
 }
 
+int uirk_bp_angle(const uirk_slice *d_slices, int n_slices, int max_n, int dtype, int img_dtype, int angle,
+                  void *stream)
+{
+    if (!d_slices || angle < 0 || n_slices > 65535 || max_n > 16384)
+        return (int)hipErrorInvalidValue;
+    if (n_slices <= 0 || max_n <= 0)
+        return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (img_dtype == UIR_DTYPE_F32) {
+        switch (dtype) {
+        case UIR_DTYPE_F16: return launch_bp_angle<__half, float>(d_slices, n_slices, max_n, angle, st);
+        case UIR_DTYPE_BF16: return launch_bp_angle<__hip_bfloat16, float>(d_slices, n_slices, max_n, angle, st);
+        case UIR_DTYPE_F32: return launch_bp_angle<float, float>(d_slices, n_slices, max_n, angle, st);
+        default: return (int)hipErrorInvalidValue;
+        }
+    }
+    if (img_dtype != dtype)
+        return (int)hipErrorInvalidValue;
+    switch (dtype) {
+    case UIR_DTYPE_F16: return launch_bp_angle<__half, __half>(d_slices, n_slices, max_n, angle, st);
+    case UIR_DTYPE_BF16: return launch_bp_angle<__hip_bfloat16, __hip_bfloat16>(d_slices, n_slices, max_n, angle, st);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
 int uirk_fp_angle(const uirk_slice *d_slices, const uirk_fp_item *d_items, int n_items, const uirk_xcd_part *part,
                   int dtype, int img_dtype, int angle, int grid, int *d_err, void *stream)
 {
@@ -1051,14 +1141,6 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
                   const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
                   int *d_err, void *stream)
 {
-    return uirk_bp_table_angle(d_slices, one, d_items, n_items, part, mode, dtype, img_dtype, tile, grid, -1, d_err,
-                               stream);
-}
-
-int uirk_bp_table_angle(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
-                        const uirk_xcd_part *part, int mode, int dtype, int img_dtype, const uirk_tile *tile, int grid,
-                        int a_sel, int *d_err, void *stream)
-{
     uirk_slice single{};
     if (!d_slices) {
         if (!one)
@@ -1077,12 +1159,11 @@ int uirk_bp_table_angle(const uirk_slice *d_slices, const uirk_slice *one, const
         pt = *part;
     hipStream_t st = (hipStream_t)stream;
     if (mode == 0) {
-        if (dtype != UIR_DTYPE_F32 || img_dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE ||
-            a_sel >= 0)
+        if (dtype != UIR_DTYPE_F32 || img_dtype != UIR_DTYPE_F32 || tl.bp_angles != 32 || tl.bp_rows != UIRK_BP_TILE)
             return (int)hipErrorInvalidValue;
-        return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
+        return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    return launch_fast<1>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, a_sel, d_err, st);
+    return launch_fast<1>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

@@ -6,8 +6,9 @@
  * One iteration, for every slice at once:
  *   data step   x = 0 (reference) or x_prev = x (textbook); df_iterations SART
  *               passes: per angle k, one FP launch writing the residual rows
- *               r[k] = (b[k] - A_k x) R[k] of every slice that has angle k and
- *               one BP launch adding relax A_k^T r[k] to the images;
+ *               r[k] = (b[k] - A_k x) R[k] of every slice that has angle k
+ *               (uirk_fp_angle) and one streaming BP launch adding
+ *               relax A_k^T r[k] to the images (uirk_bp_angle);
  *   POSC + dp   x = max(x, 0), dp = |x - x_prev|, x_prev = x;
  *   dd          r = b - A x over all angles (the batch's FP tables), |r|;
  *   TV          ng times: g = l1_grad(x), |g|, x -= dtgv / |g| g;
@@ -35,10 +36,10 @@ struct uir_asd {
     uirk_fp_item *d_items;     /* per-angle FP items: {slice, d0} per 128 detectors */
     int n_items, grid;
     uirk_xcd_part part;
+    uint32_t n_max, a_max;
     uirk_asd_state *d_state;
     double *d_partials;
     int nblk;
-    uint32_t a_max;
     float *d_diag;             /* diag_cap rows of (dd, dp, dg, dtgv) per slice */
     uint32_t diag_cap, diag_rows;
     hipGraph_t graph;
@@ -101,13 +102,16 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
     uirk_asd_slice *sl = (uirk_asd_slice *)calloc(S, sizeof *sl);
     uirk_slice *fp = (uirk_slice *)calloc(S, sizeof *fp), *dd = (uirk_slice *)calloc(S, sizeof *dd);
     a->h_bp_sl = (uirk_slice *)calloc(S, sizeof *a->h_bp_sl);
-    size_t n_items = 0, total = 0, max_nv = 1;
+    size_t n_items = 0, total = 0, max_nv = 1, max_tv = 1;
     for (uint32_t i = 0; i < S; ++i) {
         const uir_slice_spec *sp = &b->specs[i];
         const size_t nv = (size_t)sp->n * sp->n;
         n_items += (sp->n_dets + 127) / 128;
         total += align_up(nv * b->esi) + align_up(nv * 4);
+        const size_t tiles = (size_t)((sp->n + 255) / 256) * ((sp->n + UIRK_ASD_TV_ROWS - 1) / UIRK_ASD_TV_ROWS);
         max_nv = nv > max_nv ? nv : max_nv;
+        max_tv = tiles > max_tv ? tiles : max_tv;
+        a->n_max = sp->n > a->n_max ? sp->n : a->n_max;
         a->a_max = sp->n_angles > a->a_max ? sp->n_angles : a->a_max;
     }
     uirk_fp_item *items = (uirk_fp_item *)calloc(n_items ? n_items : 1, sizeof *items);
@@ -128,6 +132,8 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
     if ((s = uir_dmalloc(&a->mem, total)) != UIR_OK)
         goto done;
     a->nblk = (int)((max_nv + UIRK_ASD_CHUNK - 1) / UIRK_ASD_CHUNK);
+    if ((size_t)a->nblk < max_tv)
+        a->nblk = (int)max_tv;
     {
         char *p = (char *)a->mem;
         size_t k = 0;
@@ -221,13 +227,7 @@ static uir_status launch_asd_iteration(uir_batch *b, const uir_asdpocs_params *p
         for (uint32_t k = 0; k < a->a_max; ++k) {
             HIP(uirk_fp_angle(a->d_fp_sl, a->d_items, a->n_items, &a->part, b->sdt, b->idt, (int)k, a->grid, b->d_err,
                               st), "SART FP launch");
-            for (int g = 0; g < UIR_BP_SHAPES; ++g) {
-                const kgroup *kg = &b->bpg[g];
-                if (kg->n_items)
-                    HIP(uirk_bp_table_angle(a->d_bp_sl, NULL, (const uirk_bp_item *)kg->d_items, kg->n_items, &kg->part,
-                                            1, b->sdt, b->idt, &kg->tile, kg->grid, (int)k, b->d_err, st),
-                        "SART BP launch");
-            }
+            HIP(uirk_bp_angle(a->d_bp_sl, S, (int)a->n_max, b->sdt, b->idt, (int)k, st), "SART BP launch");
         }
     HIP(uirk_asd_image(UIRK_ASD_POSC_DP, b->idt, a->d_sl, S, nblk, a->d_state, p->positive_constraint ? 1 : 0, tb,
                        a->d_partials, st), "ASD-POCS POSC");
