@@ -334,6 +334,7 @@ def asdpocs_c4(args, uir):
     t_data = time.time() - t0
     out = {}
     for storage in args.c4_storage.split(","):
+        tile = uir.tune_geometry(res, spec, storage, reps=3)["best"]   # the residual FP's tile shape
         bt = uir.Batch(res, [spec] * S, storage)
         for k, sn in enumerate(sinos):
             bt.set_sinogram(k, sn)
@@ -347,7 +348,7 @@ def asdpocs_c4(args, uir):
         print("c4: %s %d slices x %d iterations in %.1f ms" % (storage, S, iters, ms), file=sys.stderr, flush=True)
         out[storage] = {"ms_total": round(ms, 1), "ms_per_iteration": round(ms / iters, 2),
                         "slice_iters_per_s": round(S * iters / (ms / 1000.0), 1),
-                        "device_bytes": bt.info()["device_bytes"]}
+                        "device_bytes": bt.info()["device_bytes"], "tile": tile}
         if storage == args.c4_storage.split(",")[0]:
             diag0 = bt.asdpocs_diagnostics(0)
         out[storage]["image0"] = bt.image(0)

@@ -48,7 +48,8 @@ __device__ __forceinline__ double measure(float d, int l2) { return l2 ? (double
 // mode 0: x = 0 (the reference restarts its df-minimizer from zero,
 // ufo-ir-sart-task.c:205-207); 1: x_prev = x (textbook: warm start, dp
 // measured from the image before the data step); 2: x = x_prev (the
-// reference's output, the image before the TV steps, :468).
+// reference's output, the image before the TV steps, :468); 3: x = g read as
+// image-dtype data (the SART step's second buffer).
 template <typename TI>
 __global__ __launch_bounds__(kThreads) void k_asd_begin(const uirk_asd_slice *__restrict__ sl, int mode)
 {
@@ -64,6 +65,8 @@ __global__ __launch_bounds__(kThreads) void k_asd_begin(const uirk_asd_slice *__
                 xp[i] = x[i];
             else if (mode == 2)
                 x[i] = xp[i];
+            else if (mode == 3)
+                x[i] = ((const TI *)S.g)[i];
             else
                 st(x, i, 0.0f);
         }
@@ -213,8 +216,8 @@ __global__ __launch_bounds__(kThreads) void k_asd_tvstep(const uirk_asd_slice *_
 // step's scalar update (ufo-ir-asdpocs-task.c:478-503, tvstd :573-579).
 __global__ __launch_bounds__(kThreads) void k_asd_finish(uirk_asd_state *__restrict__ state,
                                                          const double *__restrict__ partials, int nblk, int what,
-                                                         uirk_asd_params p, uirk_slice *__restrict__ bp_sl,
-                                                         float *__restrict__ diag, int n_slices, int max_iters)
+                                                         uirk_asd_params p, float *__restrict__ diag, int n_slices,
+                                                         int max_iters)
 {
     const int s = blockIdx.x;
     double v = 0;
@@ -260,7 +263,6 @@ __global__ __launch_bounds__(kThreads) void k_asd_finish(uirk_asd_state *__restr
             S.dtgv *= p.alpha_red;
         if (S.beta >= 0.0f && S.beta <= 1.0f)        // :459, a [0, 1] property
             S.relax = S.beta;
-        bp_sl[s].relax = S.relax;
         S.it += 1;
         S.stop = 0;
     }
@@ -324,12 +326,12 @@ int uirk_asd_sino_norm(int dtype, const uirk_asd_slice *d_sl, int n_slices, int 
 }
 
 int uirk_asd_finish(uirk_asd_state *d_state, const double *d_partials, int n_slices, int nblk, int what,
-                    const uirk_asd_params *p, uirk_slice *d_bp_sl, float *d_diag, int max_iters, void *stream)
+                    const uirk_asd_params *p, float *d_diag, int max_iters, void *stream)
 {
     if (n_slices <= 0)
         return 0;
     hipLaunchKernelGGL(k_asd_finish, dim3(n_slices), dim3(kThreads), 0, (hipStream_t)stream, d_state, d_partials, nblk,
-                       what, *p, d_bp_sl, d_diag, n_slices, max_iters);
+                       what, *p, d_diag, n_slices, max_iters);
     return (int)hipGetLastError();
 }
 

@@ -154,17 +154,6 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
  * (exact coordinates, (sin, cos) from trig); 0 where FP(1) = 0 or < r_min. */
 int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
                        void *stream);
-/* Fast-path BP of one angle over every slice that has it (the SART update):
- * x_out = x + relax * pix_scale * A_angle^T b[angle] (posc: clamped at 0),
- * a streaming pass; slices of up to max_n x max_n pixels. */
-int uirk_bp_angle(const uirk_slice *d_slices, int n_slices, int max_n, int dtype, int img_dtype, int angle,
-                  void *stream);
-/* Fast-path FP of one angle over every slice that has it (the SART data step):
- * r[angle] = (b[angle] + corr * FP(x)[angle]) * ray_w[angle] (ray_w NULL: no
- * factor).  Items: {slice, d0} per 128 detectors (uirk_plan_fp_angle). */
-int uirk_fp_angle(const uirk_slice *d_slices, const uirk_fp_item *d_items, int n_items, const uirk_xcd_part *part,
-                  int dtype, int img_dtype, int angle, int grid, int *d_err, void *stream);
-
 /* Work-table construction (host/uir_plan.c). */
 typedef struct {
     uirk_fp_item *v;
@@ -223,7 +212,8 @@ typedef struct {
     float beta_red, alpha, alpha_red, r_max;
     int32_t textbook;
 } uirk_asd_params;
-/* kind: BEGIN (flag 0: x = 0, 1: x_prev = x, 2: x = x_prev), POSC_DP (flag = posc:
+/* kind: BEGIN (flag 0: x = 0, 1: x_prev = x, 2: x = x_prev, 3: x = g as image
+ * dtype), POSC_DP (flag = posc:
  * x = max(x, 0), partial |x - x_prev|, x_prev = x), DIFF (partial |x - x_prev|),
  * TVGRAD (g = l1_grad(x), partial |g|), TVSTEP (x -= factor g); l2 selects
  * sums of squares over sums of magnitudes. */
@@ -235,10 +225,33 @@ int uirk_asd_sino_norm(int dtype, const uirk_asd_slice *d_sl, int n_slices, int 
                        void *stream);
 /* what: DD, DP (sets dtgv = alpha dp in iteration 0), G (factor = dtgv / |g|;
  * textbook: a zero gradient ends the TV steps), DG (diagnostics row, dtgv
- * and beta updates, next relaxation into d_bp_sl[s].relax, iteration count). */
+ * and beta updates, the next data step's relaxation, iteration count). */
 enum { UIRK_ASD_DD = 0, UIRK_ASD_DP = 1, UIRK_ASD_G = 2, UIRK_ASD_DG = 3 };
 int uirk_asd_finish(uirk_asd_state *d_state, const double *d_partials, int n_slices, int nblk, int what,
-                    const uirk_asd_params *p, uirk_slice *d_bp_sl, float *d_diag, int max_iters, void *stream);
+                    const uirk_asd_params *p, float *d_diag, int max_iters, void *stream);
+
+/* Fused SART step (uir_fast.hip): one pass over the images applies the BP of
+ * bp_angle (x_out = x_in + relax A^T r[bp_angle], relax from the slice's
+ * state; bp_angle < 0 or >= n_angles: x_out = x_in) and writes the partial
+ * FP sums of fp_angle (< 0: none) per (band of 64 lines, tile % 3, ray) to
+ * part; uirk_sart_rows then forms r[fp_angle] = (b + corr FP) * ray_w.
+ * x_in and x_out are distinct buffers (tiles read a halo of their
+ * neighbours).  Tiles are 64 x 64; part holds ceil(n / 64) * 3 * n_dets
+ * floats per slice. */
+typedef struct {
+    const void *x_in;    /* image dtype */
+    void *x_out;
+    void *r;             /* residual rows (sinogram dtype) */
+    const void *b;       /* measured sinogram (sinogram dtype) */
+    const float *ray_w;
+    const float *trig;
+    float *part;
+    int32_t n, n_dets, n_angles, pad;
+    float axis, corr;
+} uirk_sart_slice;
+int uirk_sart_step(const uirk_sart_slice *d_slices, const uirk_asd_state *d_state, int n_slices, int max_n,
+                   int dtype, int img_dtype, int bp_angle, int fp_angle, int *d_err, void *stream);
+int uirk_sart_rows(const uirk_sart_slice *d_slices, int n_slices, int max_d, int dtype, int angle, void *stream);
 
 /* Finite differences (ufo-ir-gradient-processor.cl) and l1_grad. */
 int uirk_diff(int kind, const float *in, float *out, uint32_t w, uint32_t h, void *stream);

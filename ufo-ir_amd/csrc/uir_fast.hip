@@ -450,120 +450,6 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
     }
 }
 
-// Forward projection of one angle (the SART data step): the residual row
-// r[angle] = (b[angle] - FP(x)[angle]) * R[angle] of every slice that has the
-// angle, fast-path arithmetic.  A work item is 128 detectors of one slice (two
-// groups of 64, one wave pair each); the two waves of a pair sample
-// alternate 16-line halves of each staged 32-line chunk, and their partial
-// sums are added in LDS.  The footprint of 128 rays over 32 lines is at most
-// 128 sqrt(2) + 35 < 256 positions wide.
-template <typename T, typename TI>
-__global__ __launch_bounds__(256) void k_fp_angle(const uirk_slice *__restrict__ slices,
-                                                  const uirk_fp_item *__restrict__ items, int n_items,
-                                                  uirk_xcd_part part, int angle, int *__restrict__ err)
-{
-    __shared__ float lds[kFpLines * kFpLs];
-    __shared__ float red[2][4];
-    __shared__ float half_sum[2][64];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int grp = wave >> 1, half = wave & 1;
-    const ItemRange range = item_range(part, n_items);
-
-    for (int it = range.first; it < range.end; it += range.stride) {
-        const uirk_fp_item item = items[it];
-        const uirk_slice *S = &slices[item.slice];
-        if (angle >= S->n_angles)
-            continue; // uniform over the workgroup
-        const int N = S->n, D = S->n_dets;
-        const TI *img = uniform((const TI *)S->x);
-        const float *trig = uniform(S->trig);
-        const int s = item.d0 + 64 * grp + lane;
-        const float st = gload(trig, 2 * angle), ct = gload(trig, 2 * angle + 1);
-        // direction of ufo-ir-parallel-projector-task.c:462-498: vertical iff |sin| <= |cos|
-        const bool hor = !(fabsf(st) <= fabsf(ct));
-
-        // projector-parallel-joseph.cl:51-66 (detector quantities use n_dets)
-        const float required_width = S->axis * 2;
-        const float diff = (float)D - required_width;
-        const float rot = diff / 2.0f;
-        const bool in_range = s < D;
-        const bool on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
-        const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
-        // exact coordinates: steps and anchors in double
-        const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
-        const double step_d = hor ? (double)ct / (double)st : (double)st / (double)ct;
-        const double p0 = tdet * det_step - 0.5 * (double)N * step_d + 0.5 * (double)N;
-        const float step = (float)step_d;
-
-        float acc = 0.0f;
-        for (int l0 = 0; l0 < N; l0 += kFpLines) {
-            const int nl = min(kFpLines, N - l0);
-            const float pos = (float)(p0 + (double)l0 * step_d);
-            float lo = 3.0e38f, hi = -3.0e38f;
-            if (on) {
-                const float a = pos, b = pos + (float)(nl - 1) * step;
-                lo = fminf(a, b);
-                hi = fmaxf(a, b);
-            }
-            block_lohi(lo, hi, red, lane, wave);
-            if (!(lo <= hi))
-                break; // no active ray in this workgroup
-            const int pmin = (int)floorf(lo - 0.5f) - 1;
-            const int W = (int)floorf(hi - 0.5f) + 3 - pmin;
-            const bool overflow = W > kFpW;
-            if (overflow && threadIdx.x == 0)
-                atomicOr(err, 1);
-            const bool outside = overflow || pmin + W <= 0 || pmin >= N;
-            if (!outside)
-                fp_stage(lds, img, N, hor, l0, nl, pmin, W);
-            __syncthreads();
-            if (outside)
-                continue;
-            // this wave's lines [lb, le) of the chunk; inactive rays sample a
-            // fixed in-window position with zero step
-            const int lb = 16 * half, le = min(lb + 16, nl);
-            const float off = 0.5f + (float)pmin;
-            float q = on ? (float)(p0 + (double)(l0 + lb) * step_d) - off : lo + 1.0f - off;
-            const float se = on ? step : 0.0f;
-            int l = lb;
-            for (; l + 2 <= le; l += 2) {
-                const float *L0 = lds + l * kFpLs, *L1 = L0 + kFpLs;
-                const float q1 = q + se;
-                const int i0 = cvt_flr(q), i1 = cvt_flr(q1);
-                const float a0 = fract_pos(q), a1 = fract_pos(q1);
-                const float t00 = L0[i0], t01 = L0[i0 + 1], t10 = L1[i1], t11 = L1[i1 + 1];
-                q = q1 + se;
-                acc = __builtin_fmaf(a0, t01 - t00, acc + t00);
-                acc = __builtin_fmaf(a1, t11 - t10, acc + t10);
-            }
-            if (l < le) {
-                const float *L = lds + l * kFpLs;
-                const int i0 = cvt_flr(q);
-                const float a = fract_pos(q);
-                const float t0 = L[i0], t1 = L[i0 + 1];
-                acc = __builtin_fmaf(a, t1 - t0, acc + t0);
-            }
-        }
-
-        if (half)
-            half_sum[grp][lane] = acc;
-        __syncthreads();
-        if (!half && in_range) {
-            const T *bsrc = uniform((const T *)S->b);
-            T *rdst = uniform((T *)S->r);
-            const float *ray_w = uniform(S->ray_w);
-            const uint32_t o = (uint32_t)angle * (uint32_t)D + (uint32_t)s;
-            const float bval = gload(bsrc, o);
-            float w = on ? bval + (acc + half_sum[grp][lane]) * S->corr : bval;
-            if (ray_w)
-                w = w * gload(ray_w, o);
-            gstore(rdst, o, w);
-        }
-        __syncthreads(); // half_sum is free for the next item
-    }
-}
-
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
@@ -781,75 +667,270 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
     }
 }
 
-// BP of one angle (the SART update x += relax * A_k^T r_k), fast-path
-// arithmetic: a streaming pass over the images.  Block (t, s) is the t-th
-// 64 x 64 tile of slice s (one column and 16 rows per thread); the residual
-// row r[angle] (a few KB per slice) is read straight from global memory,
-// where every tile of the slice finds it in cache.  Exact per-pixel detector
-// coordinates, clamp-to-edge bins as k_bp_tab's fast path.
-template <typename T, typename TI>
-__global__ __launch_bounds__(256) void k_bp_angle(const uirk_slice *__restrict__ slices, int angle)
+// ------------------------------------------------------------------------
+// Fused SART step: the BP of angle k - 1 and the FP of angle k in one pass
+// ------------------------------------------------------------------------
+// Block (t, s) owns the 64 x 64 tile t of slice s.  It stages the tile with a
+// one-pixel halo (66 x 66) from x_in, applies the BP of angle bp_angle to
+// every staged pixel (x += relax * A^T r[bp_angle], fast-path arithmetic:
+// exact per-pixel detector coordinate, clamp-to-edge bins as k_bp_tab),
+// writes the 64 x 64 interior to x_out (a different buffer: the
+// halo belongs to the neighbours) and keeps the updated tile in LDS.  It then
+// sums, for angle fp_angle, every Joseph sample whose left (lower) tap c0
+// lies in the tile's positions (tile 0 also owns c0 = -1) over the tile's 64
+// lines, ray by ray.  Each ray's partial goes to part[band][tile % 3][ray]:
+// a ray's samples in one band of lines touch at most three consecutive tiles
+// (even with the margins below), so tiles sharing a slot never share a ray.
+// k_sart_rows then adds the partials of the tiles each ray crosses.
+//
+// The sample-to-tile assignment must not depend on the tile: every tile
+// evaluates the tap position of (ray, line) as fma(line, step, q0) in float,
+// q0 the ray's position at line 0 rounded once from double, and visits only
+// the lines whose taps can fall in its positions (with a one-position margin).
+constexpr int kSt = 64;          // tile edge
+constexpr int kSl = kSt + 2;     // staged edge
+constexpr int kSartRays = 128;   // candidate rays per tile (at most 98 by construction)
+
+// Candidate rays of a tile: those whose tap position c = pos - 0.5 falls in
+// [p_base - 3, p_base + 67) on a line of [la, lb]; pos(s, l) = (s + 0.5 - axis)
+// det_step + C + l step is affine in (s, l), so the set is the interval between
+// the corner solutions (one more ray of margin each side).
+__device__ __forceinline__ void sart_candidates(double det_step, double step, double C, double axis, int la, int lb,
+                                                int p_base, int D, int &s_lo, int &s_hi)
 {
-    const uirk_slice *S = &slices[blockIdx.y];
-    const int N = S->n, D = S->n_dets;
-    const int tiles_x = (N + kBpT - 1) / kBpT;
-    if (angle >= S->n_angles || (int)blockIdx.x >= tiles_x * tiles_x)
-        return;
-    const int x0 = (blockIdx.x % tiles_x) * kBpT, y0 = (blockIdx.x / tiles_x) * kBpT;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = x0 + lane;
-    if (px >= N)
-        return;
-    const T *row = uniform((const T *)S->b) + (size_t)angle * D;
-    const float *trig = uniform(S->trig);
-    const float st = gload(trig, 2 * angle), ct = gload(trig, 2 * angle + 1);
-    // projector-parallel-joseph.cl:174-190 (detector quantities use n_dets)
-    const float axis = S->axis;
-    const float required_width = axis * 2;
-    const float diff = (float)D - required_width;
-    const float half_active = diff < 0 ? (float)D - axis : axis;
-    const int bin_lo = (int)floorf(axis - half_active + 0.5f - 0.5f);
-    const int bin_hi = (int)floorf(axis + half_active - 0.5f - 0.5f);
-    const float shift = (float)D / 2.0f - diff / 2.0f;
-    const float origin_shift = (float)N / 2.0f;
-    const float fX = (float)px + 0.5f - origin_shift;
-    const float ux = __builtin_fmaf(fX, ct, shift - 0.5f);
-    const TI *xin = uniform((const TI *)S->x);
-    TI *xout = uniform((TI *)S->x_out);
-    const float gain = S->relax * S->pix_scale;
-    const int posc = S->posc;
-    constexpr int kRows = kBpT / 4;
-    float xv[kRows], t0[kRows], t1[kRows], a[kRows];
-    // all loads first: the pixels, then both taps of every row
+    double smin = 1e300, smax = -1e300;
+    const int ls[2] = {la, lb};
+    const double cs[2] = {(double)p_base - 3.0, (double)p_base + 67.0};
 #pragma unroll
-    for (int k = 0; k < kRows; ++k) {
-        const int py = min(y0 + wave + 4 * k, N - 1);
-        xv[k] = gload(xin, (uint32_t)py * (uint32_t)N + (uint32_t)px);
-        const float fY = (float)py + 0.5f - origin_shift;
-        const float u = __builtin_fmaf(-fY, st, ux);
-        const int bin = cvt_flr(u);
-        a[k] = u - floorf(u);
-        const int b0 = min(max(bin, bin_lo), bin_hi), b1 = min(max(bin + 1, bin_lo), bin_hi);
-        t0[k] = gload(row, (uint32_t)b0);
-        t1[k] = gload(row, (uint32_t)b1);
-    }
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int k = 0; k < kRows; ++k) {
-        const int py = y0 + wave + 4 * k;
-        if (py >= N)
-            continue;
-        float out = __builtin_fmaf(__builtin_fmaf(a[k], t1[k] - t0[k], t0[k]), gain, xv[k]);
-        if (posc)
-            out = out > 0 ? out : 0.0f;
-        gstore(xout, (uint32_t)py * (uint32_t)N + (uint32_t)px, out);
-    }
+        for (int j = 0; j < 2; ++j) {
+            const double s = (cs[j] + 0.5 - C - (double)ls[i] * step) / det_step - 0.5 + axis;
+            smin = fmin(smin, s);
+            smax = fmax(smax, s);
+        }
+    s_lo = max((int)floor(smin) - 1, 0);
+    s_hi = min((int)ceil(smax) + 1, D - 1);
 }
 
 template <typename T, typename TI>
-int launch_bp_angle(const uirk_slice *sl, int n_slices, int max_n, int angle, hipStream_t st)
+__global__ __launch_bounds__(256) void k_sart_step(const uirk_sart_slice *__restrict__ slices,
+                                                   const uirk_asd_state *__restrict__ state, int bp_angle,
+                                                   int fp_angle, int *__restrict__ err)
 {
-    const int tx = (max_n + kBpT - 1) / kBpT;
-    hipLaunchKernelGGL((k_bp_angle<T, TI>), dim3(tx * tx, n_slices), dim3(256), 0, st, sl, angle);
+    __shared__ float tile[kSl * kSl];
+    __shared__ float half_sum[kSartRays];
+    const uirk_sart_slice *S = &slices[blockIdx.y];
+    const int N = S->n, D = S->n_dets;
+    const int nt = (N + kSt - 1) / kSt;
+    if ((int)blockIdx.x >= nt * nt)
+        return; // uniform over the block
+    const int tx = (int)blockIdx.x % nt, ty = (int)blockIdx.x / nt;
+    const int x0 = tx * kSt, y0 = ty * kSt;
+    const TI *xin = uniform((const TI *)S->x_in);
+    TI *xout = uniform((TI *)S->x_out);
+    const float *trig = uniform(S->trig);
+    const float axis_f = S->axis;
+    const float required_width = axis_f * 2;
+    const float diff = (float)D - required_width;
+    const float origin_shift = (float)N / 2.0f;
+
+    // BP of bp_angle (projector-parallel-joseph.cl:174-190 geometry, clamp-to-edge bins)
+    const bool do_bp = bp_angle >= 0 && bp_angle < S->n_angles;
+    float bst = 0.0f, bct = 0.0f, gain = 0.0f;
+    int bin_lo = 0, bin_hi = 0;
+    const T *row = nullptr;
+    if (do_bp) {
+        bst = gload(trig, 2 * bp_angle);
+        bct = gload(trig, 2 * bp_angle + 1);
+        const float half_active = diff < 0 ? (float)D - axis_f : axis_f;
+        bin_lo = (int)floorf(axis_f - half_active + 0.5f - 0.5f);
+        bin_hi = (int)floorf(axis_f + half_active - 0.5f - 0.5f);
+        gain = state[blockIdx.y].relax;
+        row = uniform((const T *)S->r) + (size_t)bp_angle * D;
+    }
+    const float shift = (float)D / 2.0f - diff / 2.0f;
+    // The BP taps of one pixel: u = X cos - Y sin + axis - 0.5.
+    auto bp_taps = [&](int px, int py, float &a, float &t0, float &t1) {
+        const float fX = (float)px + 0.5f - origin_shift, fY = (float)py + 0.5f - origin_shift;
+        const float u = __builtin_fmaf(-fY, bst, __builtin_fmaf(fX, bct, shift - 0.5f));
+        const int bin = cvt_flr(u);
+        a = u - floorf(u);
+        const int b0 = min(max(bin, bin_lo), bin_hi), b1 = min(max(bin + 1, bin_lo), bin_hi);
+        t0 = gload(row, (uint32_t)b0);
+        t1 = gload(row, (uint32_t)b1);
+    };
+    // Staged (i, j) is pixel (y0 - 1 + i, x0 - 1 + j); out-of-image entries are 0.
+    // Interior: one column and 16 rows per thread, in two batches of 8 rows
+    // whose loads are all issued first.
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int px = x0 + lane;
+        const bool col_in = px < N;
+        constexpr int kRows = 8;
+        for (int k0 = 0; k0 < kSt / 4; k0 += kRows) {
+            float xv[kRows], t0[kRows], t1[kRows], a[kRows];
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) {
+                const int py = min(y0 + wave + 4 * (k0 + k), N - 1), pxc = min(px, N - 1);
+                xv[k] = gload(xin, (uint32_t)py * (uint32_t)N + (uint32_t)pxc);
+                a[k] = t0[k] = t1[k] = 0.0f;
+                if (do_bp)
+                    bp_taps(pxc, py, a[k], t0[k], t1[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kRows; ++k) {
+                const int py = y0 + wave + 4 * (k0 + k);
+                float v = 0.0f;
+                if (col_in && py < N) {
+                    v = do_bp ? __builtin_fmaf(__builtin_fmaf(a[k], t1[k] - t0[k], t0[k]), gain, xv[k]) : xv[k];
+                    gstore(xout, (uint32_t)py * (uint32_t)N + (uint32_t)px, v);
+                }
+                tile[(wave + 4 * (k0 + k) + 1) * kSl + lane + 1] = v;
+            }
+        }
+    }
+    // Halo ring (4 * 65 entries): updated here, written by the neighbours.
+    for (int e = threadIdx.x; e < 4 * (kSl - 1); e += 256) {
+        int i, j;
+        if (e < kSl) {
+            i = 0;
+            j = e;
+        } else if (e < 2 * kSl) {
+            i = kSl - 1;
+            j = e - kSl;
+        } else if (e < 2 * kSl + kSt) {
+            i = e - 2 * kSl + 1;
+            j = 0;
+        } else {
+            i = e - 2 * kSl - kSt + 1;
+            j = kSl - 1;
+        }
+        const int py = y0 - 1 + i, px = x0 - 1 + j;
+        float v = 0.0f;
+        if (py >= 0 && py < N && px >= 0 && px < N) {
+            v = gload(xin, (uint32_t)py * (uint32_t)N + (uint32_t)px);
+            if (do_bp) {
+                float a, t0, t1;
+                bp_taps(px, py, a, t0, t1);
+                v = __builtin_fmaf(__builtin_fmaf(a, t1 - t0, t0), gain, v);
+            }
+        }
+        tile[i * kSl + j] = v;
+    }
+    if (fp_angle < 0 || fp_angle >= S->n_angles)
+        return; // uniform over the block
+    __syncthreads();
+
+    // FP of fp_angle over the tile (the fast FP's geometry, exact anchors)
+    const float st = gload(trig, 2 * fp_angle), ct = gload(trig, 2 * fp_angle + 1);
+    const bool hor = !(fabsf(st) <= fabsf(ct));
+    const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+    const double step = hor ? (double)ct / (double)st : (double)st / (double)ct;
+    const double C = -0.5 * (double)N * step + 0.5 * (double)N;
+    const int l0 = hor ? x0 : y0, p_base = hor ? y0 : x0;
+    const int nl = min(kSt, N - l0);
+    int s_lo, s_hi;
+    sart_candidates(det_step, step, C, (double)axis_f, l0, l0 + nl - 1, p_base, D, s_lo, s_hi);
+    if (s_hi - s_lo + 1 > kSartRays && threadIdx.x == 0)
+        atomicOr(err, 8); // cannot happen (at most 98): reported, never silent
+    const int r = threadIdx.x & (kSartRays - 1), h = threadIdx.x / kSartRays;
+    const int s = s_lo + r;
+    const float rot = diff / 2.0f;
+    const bool cand = s <= s_hi;
+    const bool on = cand && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
+    float acc = 0.0f;
+    if (on) {
+        const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
+        const double q0 = tdet * det_step + C - 0.5; // tap coordinate at line 0
+        const float q0f = (float)q0, se = (float)step;
+        const int c_min = p_base == 0 ? -1 : p_base, c_max = p_base + kSt - 1;
+        int lb = l0 + 32 * h, le = min(lb + 32, l0 + nl);
+        if (fabs(step) > 1e-9) { // lines whose tap is within one position of [c_min, c_max + 1)
+            const double la = ((double)c_min - 1.0 - q0) / step, lz = ((double)c_max + 2.0 - q0) / step;
+            lb = max(lb, (int)floor(fmin(la, lz)) - 1);
+            le = min(le, (int)ceil(fmax(la, lz)) + 2);
+        }
+        for (int l = lb; l < le; ++l) {
+            const float qa = __builtin_fmaf((float)l, se, q0f);
+            const int c0 = cvt_flr(qa);
+            if (c0 >= c_min && c0 <= c_max) {
+                const float a = __builtin_amdgcn_fractf(qa);
+                const int i0 = c0 - p_base + 1, li = l - l0 + 1; // staged indices, i0 in [0, 64]
+                const float t0 = hor ? tile[i0 * kSl + li] : tile[li * kSl + i0];
+                const float t1 = hor ? tile[(i0 + 1) * kSl + li] : tile[li * kSl + i0 + 1];
+                acc = __builtin_fmaf(a, t1 - t0, acc + t0);
+            }
+        }
+    }
+    if (h == 1)
+        half_sum[r] = acc;
+    __syncthreads();
+    if (h == 0 && cand) {
+        const int band = hor ? tx : ty, slot = (hor ? ty : tx) % 3;
+        S->part[((size_t)band * 3 + (size_t)slot) * (size_t)D + (size_t)s] = acc + half_sum[r];
+    }
+}
+
+// Residual rows of the fused step: r[angle] = (b[angle] + corr * FP) * R over
+// every slice that has the angle; a ray's FP is the sum, band by band in
+// order, of the partials of the tiles its taps cross (with one position of
+// margin; every such tile had the ray among its candidates).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sart_rows(const uirk_sart_slice *__restrict__ slices, int angle)
+{
+    const uirk_sart_slice *S = &slices[blockIdx.y];
+    const int N = S->n, D = S->n_dets;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= D || angle >= S->n_angles)
+        return;
+    const float *trig = uniform(S->trig);
+    const float st = gload(trig, 2 * angle), ct = gload(trig, 2 * angle + 1);
+    const bool hor = !(fabsf(st) <= fabsf(ct));
+    const float axis_f = S->axis;
+    const float required_width = axis_f * 2;
+    const float diff = (float)D - required_width;
+    const float rot = diff / 2.0f;
+    const bool on = !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
+    const uint32_t o = (uint32_t)angle * (uint32_t)D + (uint32_t)s;
+    const float bval = gload(uniform((const T *)S->b), o);
+    float w = bval;
+    if (on) {
+        const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+        const double step = hor ? (double)ct / (double)st : (double)st / (double)ct;
+        const double C = -0.5 * (double)N * step + 0.5 * (double)N;
+        const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
+        const double p0 = tdet * det_step + C;
+        const int nt = (N + kSt - 1) / kSt;
+        const float *part = uniform(S->part);
+        float sum = 0.0f;
+        for (int band = 0; band < nt; ++band) {
+            const int la = band * kSt, lb = min(la + kSt - 1, N - 1);
+            const double ca = p0 + (double)la * step - 0.5, cb = p0 + (double)lb * step - 0.5;
+            int lo = (int)floor(fmin(ca, cb)) - 1, hi = (int)floor(fmax(ca, cb)) + 1;
+            if (hi < -1 || lo > N - 1)
+                continue;
+            lo = max(lo, -1);
+            hi = min(hi, N - 1);
+            const int t_lo = lo < 0 ? 0 : lo / kSt, t_hi = hi < 0 ? 0 : hi / kSt;
+            for (int t = t_lo; t <= t_hi; ++t)
+                sum += part[((size_t)band * 3 + (size_t)(t % 3)) * (size_t)D + (size_t)s];
+        }
+        w = bval + sum * S->corr;
+    }
+    const float *ray_w = uniform(S->ray_w);
+    if (ray_w)
+        w = w * gload(ray_w, o);
+    gstore(uniform((T *)S->r), o, w);
+}
+
+template <typename T, typename TI>
+int launch_sart(const uirk_sart_slice *sl, const uirk_asd_state *state, int n_slices, int max_n, int bp_angle,
+                int fp_angle, int *err, hipStream_t st)
+{
+    const int t = (max_n + kSt - 1) / kSt;
+    hipLaunchKernelGGL((k_sart_step<T, TI>), dim3(t * t, n_slices), dim3(256), 0, st, sl, state, bp_angle, fp_angle,
+                       err);
     return (int)hipGetLastError();
 }
 
@@ -1007,25 +1088,15 @@ int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice
     return launch_bp<T, TI, false, 32, 16>(sl, one, items, n, part, grid, err, st);
 }
 
-template <typename T, typename TI>
-int launch_fp_angle(const uirk_slice *sl, const uirk_fp_item *items, int n, const uirk_xcd_part &part, int grid,
-                    int angle, int *err, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_fp_angle<T, TI>), dim3(grid), dim3(256), 0, st, sl, items, n, part, angle, err);
-    return (int)hipGetLastError();
-}
-
 // (sinogram, image) storage pairs: both FP32, both FP16, both BF16, or
-// FP16 / BF16 sinograms with an FP32 image.  K: 0 = FP over all angles, 1 = BP
-// over all angles, 2 = FP of angle `angle` (k_fp_angle).
-template <int K>
+// FP16 / BF16 sinograms with an FP32 image
+template <bool FP>
 int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one,
-                const void *items, int n, const uirk_xcd_part &part, int grid, int angle, int *err, hipStream_t st)
+                const void *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-#define UIRK_LAUNCH(TS, TIM)                                                                                      \
-    return K == 0 ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)  \
-         : K == 1 ? launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)  \
-                  : launch_fp_angle<TS, TIM>(sl, (const uirk_fp_item *)items, n, part, grid, angle, err, st)
+#define UIRK_LAUNCH(TS, TIM)                                                                                     \
+    return FP ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)     \
+              : launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)
     if (img_dtype == UIR_DTYPE_F32) {
         switch (dtype) {
         case UIR_DTYPE_F16: UIRK_LAUNCH(__half, float);
@@ -1080,7 +1151,7 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_fp<float, float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    return launch_fast<0>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
+    return launch_fast<true>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
 }
 
 int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
@@ -1095,46 +1166,55 @@ int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, flo
     return (int)hipGetLastError();
 }
 
-int uirk_bp_angle(const uirk_slice *d_slices, int n_slices, int max_n, int dtype, int img_dtype, int angle,
-                  void *stream)
+int uirk_sart_step(const uirk_sart_slice *d_slices, const uirk_asd_state *d_state, int n_slices, int max_n,
+                   int dtype, int img_dtype, int bp_angle, int fp_angle, int *d_err, void *stream)
 {
-    if (!d_slices || angle < 0 || n_slices > 65535 || max_n > 16384)
+    if (!d_slices || !d_state || n_slices > 65535 || max_n > 16384)
         return (int)hipErrorInvalidValue;
     if (n_slices <= 0 || max_n <= 0)
         return 0;
     hipStream_t st = (hipStream_t)stream;
     if (img_dtype == UIR_DTYPE_F32) {
         switch (dtype) {
-        case UIR_DTYPE_F16: return launch_bp_angle<__half, float>(d_slices, n_slices, max_n, angle, st);
-        case UIR_DTYPE_BF16: return launch_bp_angle<__hip_bfloat16, float>(d_slices, n_slices, max_n, angle, st);
-        case UIR_DTYPE_F32: return launch_bp_angle<float, float>(d_slices, n_slices, max_n, angle, st);
+        case UIR_DTYPE_F16:
+            return launch_sart<__half, float>(d_slices, d_state, n_slices, max_n, bp_angle, fp_angle, d_err, st);
+        case UIR_DTYPE_BF16:
+            return launch_sart<__hip_bfloat16, float>(d_slices, d_state, n_slices, max_n, bp_angle, fp_angle, d_err,
+                                                      st);
+        case UIR_DTYPE_F32:
+            return launch_sart<float, float>(d_slices, d_state, n_slices, max_n, bp_angle, fp_angle, d_err, st);
         default: return (int)hipErrorInvalidValue;
         }
     }
     if (img_dtype != dtype)
         return (int)hipErrorInvalidValue;
     switch (dtype) {
-    case UIR_DTYPE_F16: return launch_bp_angle<__half, __half>(d_slices, n_slices, max_n, angle, st);
-    case UIR_DTYPE_BF16: return launch_bp_angle<__hip_bfloat16, __hip_bfloat16>(d_slices, n_slices, max_n, angle, st);
+    case UIR_DTYPE_F16:
+        return launch_sart<__half, __half>(d_slices, d_state, n_slices, max_n, bp_angle, fp_angle, d_err, st);
+    case UIR_DTYPE_BF16:
+        return launch_sart<__hip_bfloat16, __hip_bfloat16>(d_slices, d_state, n_slices, max_n, bp_angle, fp_angle,
+                                                           d_err, st);
     default: return (int)hipErrorInvalidValue;
     }
 }
 
-int uirk_fp_angle(const uirk_slice *d_slices, const uirk_fp_item *d_items, int n_items, const uirk_xcd_part *part,
-                  int dtype, int img_dtype, int angle, int grid, int *d_err, void *stream)
+int uirk_sart_rows(const uirk_sart_slice *d_slices, int n_slices, int max_d, int dtype, int angle, void *stream)
 {
-    if (!d_slices || angle < 0)
+    if (!d_slices || angle < 0 || n_slices > 65535)
         return (int)hipErrorInvalidValue;
-    if (n_items <= 0)
+    if (n_slices <= 0 || max_d <= 0)
         return 0;
-    if (grid <= 0 || grid > n_items)
-        grid = n_items;
-    uirk_xcd_part pt{};
-    if (part)
-        pt = *part;
-    const uirk_tile tl = UIRK_TILE_DEFAULT;
-    return launch_fast<2>(dtype, img_dtype, tl, d_slices, uirk_slice{}, d_items, n_items, pt, grid, angle, d_err,
-                          (hipStream_t)stream);
+    const dim3 grid((max_d + 255) / 256, n_slices);
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case UIR_DTYPE_F16: hipLaunchKernelGGL(k_sart_rows<__half>, grid, dim3(256), 0, st, d_slices, angle); break;
+    case UIR_DTYPE_BF16:
+        hipLaunchKernelGGL(k_sart_rows<__hip_bfloat16>, grid, dim3(256), 0, st, d_slices, angle);
+        break;
+    case UIR_DTYPE_F32: hipLaunchKernelGGL(k_sart_rows<float>, grid, dim3(256), 0, st, d_slices, angle); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
 }
 
 int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_bp_item *d_items, int n_items,
@@ -1163,7 +1243,7 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    return launch_fast<1>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, -1, d_err, st);
+    return launch_fast<false>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

@@ -5,10 +5,11 @@
  *
  * One iteration, for every slice at once:
  *   data step   x = 0 (reference) or x_prev = x (textbook); df_iterations SART
- *               passes: per angle k, one FP launch writing the residual rows
+ *               passes: per angle k, one pass over the images that adds
+ *               relax A_{k-1}^T r[k-1] and sums the FP of angle k tile by
+ *               tile (uirk_sart_step), then the residual rows
  *               r[k] = (b[k] - A_k x) R[k] of every slice that has angle k
- *               (uirk_fp_angle) and one streaming BP launch adding
- *               relax A_k^T r[k] to the images (uirk_bp_angle);
+ *               (uirk_sart_rows);
  *   POSC + dp   x = max(x, 0), dp = |x - x_prev|, x_prev = x;
  *   dd          r = b - A x over all angles (the batch's FP tables), |r|;
  *   TV          ng times: g = l1_grad(x), |g|, x -= dtgv / |g| g;
@@ -29,14 +30,11 @@
 #define HIP(x, what) TRY(uir_check_hip((x), (what)))
 
 struct uir_asd {
-    void *mem;                 /* x_prev (image dtype) and g (FP32) of every slice */
+    void *mem;                 /* per slice: x_prev (image dtype), g (FP32), SART partial sums (FP32) */
     uirk_asd_slice *d_sl;
-    uirk_slice *h_bp_sl;       /* SART BP descriptors (relax reset at each call) */
-    uirk_slice *d_fp_sl, *d_dd_sl, *d_bp_sl;
-    uirk_fp_item *d_items;     /* per-angle FP items: {slice, d0} per 128 detectors */
-    int n_items, grid;
-    uirk_xcd_part part;
-    uint32_t n_max, a_max;
+    uirk_sart_slice *d_sart[2]; /* fused SART step: x -> g, g -> x (g is free during the data step) */
+    uirk_slice *d_dd_sl;       /* residual FP r = b - A x for dd */
+    uint32_t n_max, d_max, a_max;
     uirk_asd_state *d_state;
     double *d_partials;
     int nblk;
@@ -57,14 +55,12 @@ void uir_asd_free(struct uir_asd *a)
         (void)hipGraphDestroy(a->graph);
     uir_dfree(a->mem);
     uir_dfree(a->d_sl);
-    uir_dfree(a->d_fp_sl);
+    uir_dfree(a->d_sart[0]);
+    uir_dfree(a->d_sart[1]);
     uir_dfree(a->d_dd_sl);
-    uir_dfree(a->d_bp_sl);
-    uir_dfree(a->d_items);
     uir_dfree(a->d_state);
     uir_dfree(a->d_partials);
     uir_dfree(a->d_diag);
-    free(a->h_bp_sl);
     free(a);
 }
 
@@ -100,23 +96,22 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
     const uint32_t S = b->n_slices;
     uir_status s = UIR_OK;
     uirk_asd_slice *sl = (uirk_asd_slice *)calloc(S, sizeof *sl);
-    uirk_slice *fp = (uirk_slice *)calloc(S, sizeof *fp), *dd = (uirk_slice *)calloc(S, sizeof *dd);
-    a->h_bp_sl = (uirk_slice *)calloc(S, sizeof *a->h_bp_sl);
-    size_t n_items = 0, total = 0, max_nv = 1, max_tv = 1;
+    uirk_sart_slice *sa = (uirk_sart_slice *)calloc(2 * (size_t)S, sizeof *sa);
+    uirk_slice *dd = (uirk_slice *)calloc(S, sizeof *dd);
+    size_t total = 0, max_nv = 1, max_tv = 1;
     for (uint32_t i = 0; i < S; ++i) {
         const uir_slice_spec *sp = &b->specs[i];
         const size_t nv = (size_t)sp->n * sp->n;
-        n_items += (sp->n_dets + 127) / 128;
-        total += align_up(nv * b->esi) + align_up(nv * 4);
+        const size_t part = (size_t)((sp->n + 63) / 64) * 3 * sp->n_dets;
+        total += align_up(nv * b->esi) + align_up(nv * 4) + align_up(part * 4);
         const size_t tiles = (size_t)((sp->n + 255) / 256) * ((sp->n + UIRK_ASD_TV_ROWS - 1) / UIRK_ASD_TV_ROWS);
         max_nv = nv > max_nv ? nv : max_nv;
         max_tv = tiles > max_tv ? tiles : max_tv;
         a->n_max = sp->n > a->n_max ? sp->n : a->n_max;
+        a->d_max = sp->n_dets > a->d_max ? sp->n_dets : a->d_max;
         a->a_max = sp->n_angles > a->a_max ? sp->n_angles : a->a_max;
     }
-    uirk_fp_item *items = (uirk_fp_item *)calloc(n_items ? n_items : 1, sizeof *items);
-    double *cost = (double *)calloc(n_items ? n_items : 1, sizeof *cost);
-    if (!sl || !fp || !dd || !a->h_bp_sl || !items || !cost) {
+    if (!sl || !sa || !dd) {
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
         goto done;
     }
@@ -136,66 +131,62 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
         a->nblk = (int)max_tv;
     {
         char *p = (char *)a->mem;
-        size_t k = 0;
         for (uint32_t i = 0; i < S; ++i) {
             const uir_slice_spec *sp = &b->specs[i];
             const geom_t *G = &b->geoms[b->geom_of[i]];
             const size_t nv = (size_t)sp->n * sp->n;
+            const size_t part = (size_t)((sp->n + 63) / 64) * 3 * sp->n_dets;
             sl[i].x = b->x[i];
             sl[i].x_prev = p;
             p += align_up(nv * b->esi);
             sl[i].g = (float *)p;
             p += align_up(nv * 4);
+            float *d_part = (float *)p;
+            p += align_up(part * 4);
             sl[i].r = b->r[i];
             sl[i].n = sp->n;
             sl[i].n_sino = sp->n_angles * sp->n_dets;
-            /* SART FP: r[k] = (b[k] - A_k x) R[k]; dd FP: r = b - A x */
-            memset(&fp[i], 0, sizeof fp[i]);
-            fp[i].x = b->x[i];
-            fp[i].b = b->b[i];
-            fp[i].r = b->r[i];
-            fp[i].ray_w = G->d_ray_w;
-            fp[i].trig = G->d_trig;
-            fp[i].n = (int32_t)G->n;
-            fp[i].n_dets = (int32_t)G->D;
-            fp[i].n_angles = (int32_t)G->A;
-            fp[i].axis = G->axis;
-            fp[i].corr = -1.0f;
-            dd[i] = fp[i];
-            dd[i].ray_w = NULL;
-            /* SART BP: x += relax A_k^T r[k] (pixel weight 1, no clamp) */
-            uirk_slice *k2 = &a->h_bp_sl[i];
-            *k2 = fp[i];
-            k2->x_out = b->x[i];
-            k2->b = b->r[i];
-            k2->r = NULL;
-            k2->ray_w = NULL;
-            k2->pix_scale = 1.0f;
-            k2->posc = 0;
-            for (uint32_t d0 = 0; d0 < sp->n_dets; d0 += 128, ++k) {
-                items[k].slice = (int32_t)i;
-                items[k].d0 = (int32_t)d0;
-                cost[k] = (double)sp->n;
-            }
+            /* fused SART step: r[k] = (b[k] - A_k x) R[k], x += relax A_k^T r[k] */
+            uirk_sart_slice *f = &sa[i];
+            f->x_in = b->x[i];
+            f->x_out = sl[i].g; /* holds image-dtype data while the data step runs */
+            f->r = b->r[i];
+            f->b = b->b[i];
+            f->ray_w = G->d_ray_w;
+            f->trig = G->d_trig;
+            f->part = d_part;
+            f->n = (int32_t)G->n;
+            f->n_dets = (int32_t)G->D;
+            f->n_angles = (int32_t)G->A;
+            f->axis = G->axis;
+            f->corr = -1.0f;
+            sa[S + i] = *f;
+            sa[S + i].x_in = sl[i].g;
+            sa[S + i].x_out = b->x[i];
+            /* dd: r = b - A x over all angles */
+            memset(&dd[i], 0, sizeof dd[i]);
+            dd[i].x = b->x[i];
+            dd[i].b = b->b[i];
+            dd[i].r = b->r[i];
+            dd[i].trig = G->d_trig;
+            dd[i].n = (int32_t)G->n;
+            dd[i].n_dets = (int32_t)G->D;
+            dd[i].n_angles = (int32_t)G->A;
+            dd[i].axis = G->axis;
+            dd[i].corr = -1.0f;
         }
     }
-    a->n_items = (int)n_items;
-    a->grid = uirk_plan_grid(a->n_items, 4);
-    uirk_plan_xcd(cost, a->n_items, uirk_plan_num_xcd(), &a->part);
     if ((s = upload((void **)&a->d_sl, sl, S * sizeof *sl)) != UIR_OK ||
-        (s = upload((void **)&a->d_fp_sl, fp, S * sizeof *fp)) != UIR_OK ||
+        (s = upload((void **)&a->d_sart[0], sa, S * sizeof *sa)) != UIR_OK ||
+        (s = upload((void **)&a->d_sart[1], sa + S, S * sizeof *sa)) != UIR_OK ||
         (s = upload((void **)&a->d_dd_sl, dd, S * sizeof *dd)) != UIR_OK ||
-        (s = upload((void **)&a->d_bp_sl, a->h_bp_sl, S * sizeof *a->h_bp_sl)) != UIR_OK ||
-        (s = upload((void **)&a->d_items, items, n_items * sizeof *items)) != UIR_OK ||
         (s = uir_dmalloc((void **)&a->d_state, S * sizeof(uirk_asd_state))) != UIR_OK ||
         (s = uir_dmalloc((void **)&a->d_partials, (size_t)S * a->nblk * sizeof(double))) != UIR_OK)
         goto done;
 done:
     free(sl);
-    free(fp);
+    free(sa);
     free(dd);
-    free(items);
-    free(cost);
     if (s != UIR_OK) {
         uir_asd_free(a);
         return s;
@@ -219,16 +210,26 @@ static uir_status launch_asd_iteration(uir_batch *b, const uir_asdpocs_params *p
     const int S = (int)b->n_slices, tb = !p->compat, nblk = a->nblk;
     const uirk_asd_params kp = {p->beta_red, p->alpha, p->alpha_red, p->r_max, tb};
 #define FINISH(what)                                                                                              \
-    HIP(uirk_asd_finish(a->d_state, a->d_partials, S, nblk, (what), &kp, a->d_bp_sl, a->d_diag, (int)diag_iters, \
-                        st), "ASD-POCS reduction")
+    HIP(uirk_asd_finish(a->d_state, a->d_partials, S, nblk, (what), &kp, a->d_diag, (int)diag_iters, st),     \
+        "ASD-POCS reduction")
     HIP(uirk_asd_image(UIRK_ASD_BEGIN, b->idt, a->d_sl, S, nblk, a->d_state, tb ? 1 : 0, tb, a->d_partials, st),
         "ASD-POCS start");
-    for (uint32_t pass = 0; pass < p->df_iterations; ++pass)
-        for (uint32_t k = 0; k < a->a_max; ++k) {
-            HIP(uirk_fp_angle(a->d_fp_sl, a->d_items, a->n_items, &a->part, b->sdt, b->idt, (int)k, a->grid, b->d_err,
-                              st), "SART FP launch");
-            HIP(uirk_bp_angle(a->d_bp_sl, S, (int)a->n_max, b->sdt, b->idt, (int)k, st), "SART BP launch");
+    /* SART passes: launch j applies the BP of angle j - 1 and sums the FP of
+     * angle j, from buffer j % 2 to (j + 1) % 2 (x, g); the last launch only
+     * backprojects.  An odd launch count leaves the image in g. */
+    const uint32_t A = a->a_max;
+    for (uint32_t pass = 0; pass < p->df_iterations; ++pass) {
+        for (uint32_t j = 0; j <= A; ++j) {
+            const int fp = j < A ? (int)j : -1;
+            HIP(uirk_sart_step(a->d_sart[j % 2], a->d_state, S, (int)a->n_max, b->sdt, b->idt, (int)j - 1, fp, b->d_err,
+                               st), "SART step launch");
+            if (fp >= 0)
+                HIP(uirk_sart_rows(a->d_sart[j % 2], S, (int)a->d_max, b->sdt, fp, st), "SART residual launch");
         }
+        if ((A + 1) % 2)
+            HIP(uirk_asd_image(UIRK_ASD_BEGIN, b->idt, a->d_sl, S, nblk, a->d_state, 3, 0, a->d_partials, st),
+                "SART copy");
+    }
     HIP(uirk_asd_image(UIRK_ASD_POSC_DP, b->idt, a->d_sl, S, nblk, a->d_state, p->positive_constraint ? 1 : 0, tb,
                        a->d_partials, st), "ASD-POCS POSC");
     FINISH(UIRK_ASD_DP);
@@ -296,13 +297,9 @@ uir_status uir_batch_asdpocs(uir_batch *b, uint32_t iterations, const uir_asdpoc
         h[i].beta = p->beta;
         h[i].relax = (p->beta >= 0.0f && p->beta <= 1.0f) ? p->beta : b->specs[i].relaxation;
         h[i].dtgv = h[i].dd = h[i].dp = h[i].dg = 1.0f;
-        a->h_bp_sl[i].relax = h[i].relax;
     }
     uir_status s = uir_check_hip(hipMemcpyAsync(a->d_state, h, S * sizeof *h, hipMemcpyHostToDevice, st), "state");
-    if (s == UIR_OK)
-        s = uir_check_hip(hipMemcpyAsync(a->d_bp_sl, a->h_bp_sl, S * sizeof *a->h_bp_sl, hipMemcpyHostToDevice, st),
-                          "descriptors");
-    if (s == UIR_OK) /* the host arrays must outlive the copies */
+    if (s == UIR_OK) /* the host array must outlive the copy */
         s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     free(h);
     TRY(s);
