@@ -603,8 +603,10 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                             }
                         }
                     } else {
-                        const float2 *P = ldp + k * kBpWb - s_umin[k];
-                        float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f));
+                        // coordinates relative to the staged window: the address of a
+                        // sample is then P + 8 floor(uu), one shift-add
+                        const float2 *P = ldp + k * kBpWb;
+                        float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f)) - (float)s_umin[k];
                         const float du = -4.0f * st;
 #pragma unroll
                         for (int g = 0; g < ROWS; g += 8) {
