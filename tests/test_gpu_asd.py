@@ -121,3 +121,19 @@ def test_batch_asdpocs_sparse_view_stack_shape(res, n, d, a):
     err = O.rel_l2(bt.image(0), ref)
     print("1024^2 x 120 rel-L2 vs FP64", err, flush=True)
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 2e-2), ("bf16", 1e-1)])
+def test_batch_asdpocs_half_images(res, storage, tol):
+    """FP16 / BF16 image storage runs the same kernels on half-precision
+    images (the second SART buffer then holds half data too); the iterate's
+    rounding at every SART angle random-walks, hence the wide bar."""
+    bt, data = _batch(res, storage)
+    bt.asdpocs(3, compat=False, ng=6, **KW)
+    for i, (g, b) in enumerate(data):
+        held = b.astype(np.float16) if storage == "f16" else b
+        ref = O.asdpocs64(g, held.astype(np.float64), 3, ng=6, df_iters=1, textbook=True, ray_weight_min=RMIN,
+                          df_relax_initial=0.25, **KW)
+        err = O.rel_l2(bt.image(i), ref)
+        print(storage, GEOMS[i], "rel-L2 vs FP64", err, flush=True)
+        assert err <= tol
