@@ -137,3 +137,22 @@ def test_batch_asdpocs_half_images(res, storage, tol):
         err = O.rel_l2(bt.image(i), ref)
         print(storage, GEOMS[i], "rel-L2 vs FP64", err, flush=True)
         assert err <= tol
+
+
+@pytest.mark.parametrize("shift", [4.5, -6.0])
+def test_batch_asdpocs_shifted_axis(res, shift):
+    """The fused SART step and the residual rows with axis_position off the
+    detector centre (inactive detectors, shrunken BP window)."""
+    n, d, a = GEOMS[0]
+    axis = d / 2.0 + shift
+    g = O.Geometry(n, d, a, np.float32(np.pi / a), axis)
+    b = O.fp64(g, ellipse_phantom(n, seed=801).astype(np.float64))
+    b = (b + np.random.default_rng(5).normal(0, 0.02 * b.max(), b.shape)).astype(np.float32)
+    bt = uir.Batch(res, [dict(_spec(n, d, a), axis_position=axis)], "f32")
+    bt.set_sinogram(0, b)
+    bt.asdpocs(3, compat=False, ng=8, **KW)
+    ref = O.asdpocs64(g, b.astype(np.float64), 3, ng=8, df_iters=1, textbook=True, ray_weight_min=RMIN,
+                      df_relax_initial=0.25, **KW)
+    err = O.rel_l2(bt.image(0), ref)
+    print("axis shift", shift, "rel-L2 vs FP64", err, flush=True)
+    assert err <= 1e-4
