@@ -440,7 +440,8 @@ uir_status uir_batch_get_image(uir_batch *b, uint32_t slice, float *dst, int on_
     TRY(uir_check_hip(hipMemcpyAsync(&err, b->d_err, sizeof err, hipMemcpyDeviceToHost, st), "flag read"));
     TRY(uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize"));
     if (err)
-        return uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window (flag %d)", err);
+        return uir_set_error(UIR_ERR_DEVICE, "projector work exceeded a staging limit (flag %d: 1 FP window, 2 BP "
+                             "window, 4 FP item angles, 8 SART candidate rays)", err);
     if (on_device) {
         TRY(uir_check_hip(uirk_convert(b->x[slice], b->idt, dst, UIR_DTYPE_F32, nv, st), "convert"));
         return uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
