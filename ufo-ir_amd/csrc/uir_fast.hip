@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                                                 const uirk_bp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
 {
-    // reference-compatible path: plain staged bins; fast path: (t, dt) pairs
+    // reference-compatible path: plain staged bins; fast path: (e, d) pairs
     __shared__ float lds[REF ? NA * kBpWb : 1];
     __shared__ float2 ldp[REF ? 1 : NA * kBpWb];
     __shared__ int s_umin[NA];
@@ -569,10 +569,14 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
                         const int k = wave + 4 * (e >> 1);
                         const int i = lane + 64 * (e & 1);
                         if (k < na && i < kBpWb) {
-                            if constexpr (REF)
+                            if constexpr (REF) {
                                 lds[k * kBpWb + i] = v0[e];
-                            else
-                                ldp[k * kBpWb + i] = make_float2(v0[e], v1[e] - v0[e]);
+                            } else {
+                                // (e, d) with d = t[i+1] - t[i], e = t[i] - i d: the value at
+                                // window coordinate w in [i, i+1) is e + w d
+                                const float d = v1[e] - v0[e];
+                                ldp[k * kBpWb + i] = make_float2(__builtin_fmaf(-(float)i, d, v0[e]), d);
+                            }
                         }
                     }
                 }
@@ -611,16 +615,15 @@ __global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ s
 #pragma unroll
                         for (int g = 0; g < ROWS; g += 8) {
                             float2 p[8];
-                            float a[8];
+                            float w[8];
 #pragma unroll
                             for (int h = 0; h < 8; ++h) {
-                                const float uu = __builtin_fmaf((float)h, du, u);
-                                a[h] = __builtin_amdgcn_fractf(uu);
-                                p[h] = P[cvt_flr(uu)];
+                                w[h] = __builtin_fmaf((float)h, du, u);
+                                p[h] = P[cvt_flr(w[h])];
                             }
 #pragma unroll
                             for (int h = 0; h < 8; ++h)
-                                acc[g + h] = __builtin_fmaf(a[h], p[h].y, acc[g + h] + p[h].x);
+                                acc[g + h] = __builtin_fmaf(w[h], p[h].y, acc[g + h] + p[h].x);
                             u = __builtin_fmaf(8.0f, du, u);
                         }
                     }
