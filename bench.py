@@ -586,13 +586,13 @@ def main():
     if tfl / FP32_PEAK_TFLOPS >= gbs / HBM_PEAK_GBS:
         roof = {"bound": "mfma", "achieved": round(tfl, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tfl / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_fp_tab" if kname == "fp" else "k_bp_tab",
+                "kernel": "k_fp_pairs" if kname == "fp" else "k_bp_tab",
                 "note": "compute-bound on the FP32 vector ALU (no MFMA); 4 flop per interpolation sample; "
                         "hbm frac %.4f at %.1f GB/s algorithmic" % (gbs / HBM_PEAK_GBS, gbs)}
     else:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_fp_tab" if kname == "fp" else "k_bp_tab"}
+                "kernel": "k_fp_pairs" if kname == "fp" else "k_bp_tab"}
 
     # ---- CPU baseline and FP64 accuracy (the oracle, on this host's cores)
     cpu = None

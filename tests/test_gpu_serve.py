@@ -84,8 +84,15 @@ def test_serve_result_same_alone_and_in_mix():
             mix_sinos += [bb.astype(np.float32)] * 2
         mix_reqs += [_sirt(n, d, a, 9), _sb(n, d, a, 4, 3)]
         mix_sinos += [b.astype(np.float32)] * 2
+        # A long request runs first, so the mix arrives while steps are being
+        # taken and joins them (without it, the mix's sharing of steps would
+        # depend on how fast the first request is admitted and run).
+        nb, db, ab = 256, 363, 180
+        _, bb = _data(nb, db, ab, 40)
+        blocker = srv.submit(_sirt(nb, db, ab, 200), bb.astype(np.float32))
         out = _run(srv, mix_reqs, mix_sinos)
-    assert max(o[1]["max_batch"] for o in out) > 1
+        srv.wait(blocker, timeout=120)
+    assert min(o[1]["max_batch"] for o in out) > 1
     assert np.array_equal(out[-2][0], alone_sirt)
     assert np.array_equal(out[-1][0], alone_sb)
 

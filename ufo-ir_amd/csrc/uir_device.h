@@ -125,7 +125,7 @@ typedef struct {
 #define UIRK_FP_ANGLES 16   /* angles per FP work item (default shape) */
 #define UIRK_FP_ANGLES_MAX 32 /* widest FP shape */
 #define UIRK_FP_LINES 32    /* lines per staged chunk (divides the 64-sample block) */
-#define UIRK_FP_WMAX 256    /* max staged footprint width */
+#define UIRK_FP_WMAX 192    /* max staged footprint width (a multiple of 64) */
 #define UIRK_BP_TILE 64     /* BP tile edge */
 
 /* Tile shapes of the fast path, chosen per geometry by the autotuner

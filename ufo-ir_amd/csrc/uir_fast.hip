@@ -8,8 +8,9 @@
 //                 restatement of projector-parallel-joseph.cl.
 //   REF  = false: the batched fast path.  FP32 interpolation and accumulation
 //                 on FP32 / FP16 / BF16 storage, exact (double-anchored) ray
-//                 coordinates, fract / floor-convert sampling, fused SIRT
-//                 epilogues.
+//                 coordinates, sampling from staged (e, d) pairs with one
+//                 floor-convert and one FMA, fused SIRT epilogues.  Its FP is
+//                 k_fp_pairs; k_fp_tab is the reference-compatible FP.
 // Work is described by tables (uirk_fp_item / uirk_bp_item) over any number
 // of slices of any geometry; the kernels are persistent (grid-stride over the
 // table) and every item writes disjoint outputs, so results do not depend on
@@ -20,7 +21,7 @@
 //     chunks of 32 lines (rows for the vertical family, columns for the
 //     horizontal one); each chunk's footprint is staged into LDS line-major
 //     (transposed for the horizontal family), so both families read their two
-//     taps as adjacent LDS words.
+//     taps as adjacent LDS words (one 8-byte pair in the fast path).
 // BP: a workgroup owns a 64 x 64 pixel tile (one column and 16 rows per
 //     thread); sinogram rows and their sin/cos are staged 32 angles at a time
 //     over the tile's detector footprint.
@@ -129,6 +130,25 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+// Wave-wide min / max returned as a uniform (scalar) value: a DPP inclusive
+// scan (row shifts, then row broadcasts 15 and 31) leaves the result in lane
+// 63.  Lanes a step does not write keep their value, which min / max ignore.
+template <bool MAX> __device__ __forceinline__ float dpp_step(float v, float t) { return MAX ? fmaxf(v, t) : fminf(v, t); }
+template <bool MAX> __device__ __forceinline__ float wave_reduce_uniform(float v)
+{
+#define UIRK_DPP(ctrl, rm, bm)                                                                                     \
+    v = dpp_step<MAX>(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), ctrl, rm, bm, false)))
+    UIRK_DPP(0x111, 0xf, 0xf); // row_shr:1
+    UIRK_DPP(0x112, 0xf, 0xf); // row_shr:2
+    UIRK_DPP(0x113, 0xf, 0xf); // row_shr:3
+    UIRK_DPP(0x114, 0xf, 0xe); // row_shr:4
+    UIRK_DPP(0x118, 0xf, 0xc); // row_shr:8
+    UIRK_DPP(0x142, 0xa, 0xf); // row_bcast:15
+    UIRK_DPP(0x143, 0xc, 0xf); // row_bcast:31
+#undef UIRK_DPP
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Workgroup-wide (4 waves) min of lo and max of hi.  Its first barrier also
 // ends the previous chunk's reads of the staging area.
 __device__ __forceinline__ void block_lohi(float &lo, float &hi, float (*red)[4], int lane, int wave)
@@ -156,8 +176,9 @@ template <typename TI>
 __device__ __forceinline__ void fp_stage(float (&lds)[kFpLines * kFpLs], const TI *img, int N, bool hor, int l0, int nl,
                                          int pmin, int W)
 {
-    constexpr int kPer = kFpLines * kFpW / 256;
-    constexpr int kQ = kPer / 4;
+    static_assert(kFpLines == 32 && kFpW % 64 == 0, "staging layout: 32 lines, 64-position batches");
+    constexpr int kQ = 8;                  // loads per batch: 64 positions x 32 lines / 256 threads
+    constexpr int kPer = kQ * (kFpW / 64); // loads per thread over the widest window
     if (hor) {
         // line = column l0 + l, position = row pmin + p; coalesced along columns
         const int l = threadIdx.x & 31;
@@ -210,14 +231,129 @@ __device__ __forceinline__ void fp_stage(float (&lds)[kFpLines * kFpLs], const T
     }
 }
 
+// Buffer resources for the fast FP's staging.  A raw buffer load whose byte
+// offset is at or beyond the resource's size returns zero and touches no
+// memory, which gives the zero border outside the image without clamping or
+// selecting: negative indices become offsets above 2^32 - 2^24, beyond any
+// resource here (the batch keeps N <= 16384, so an image is at most 1 GiB).
+// The immediate and scalar offsets of the loads are always zero, so the range
+// check sees the whole offset.
+constexpr int kBufDword3 = 0x00020000; // gfx9 raw-buffer format word (32-bit data, no swizzle)
+
+template <typename TI> __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const TI *base, uint32_t elems)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(elems * (uint32_t)sizeof(TI)), kBufDword3);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const float *, int i)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (uint32_t)i * 4u, 0, 0));
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __half *, int i)
+{
+    return __half2float(__ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, (uint32_t)i * 2u, 0, 0)));
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfloat16 *, int i)
+{
+    return __uint_as_float((unsigned)__builtin_amdgcn_raw_buffer_load_b16(r, (uint32_t)i * 2u, 0, 0) << 16);
+}
+
+// Fast FP staging as (e, d) pairs.  For line l and window position p (texel
+// t_p at position pmin + p, zero outside the image):
+//   d = t_{p+1} - t_p,  e = t_p - (p - kFpPH) d,
+// so the interpolated value at window coordinate w in [p, p + 1) is
+// e + (w - kFpPH) d: one FMA per sample, no fract and no second tap.  The
+// origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpW / 2.
+constexpr int kFpPs = kFpW + 1; // pair line stride (odd)
+constexpr int kFpPH = kFpW / 2;
+
+template <typename TI>
+__device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpLines * kFpPs], const TI *img, int N, bool hor, int l0,
+                                               int nl, int pmin, int W)
+{
+    static_assert(kFpLines == 32 && kFpW % 64 == 0 && kFpPs > kFpW, "staging layout");
+    // Stores are unconditional: positions up to the end of the last 64-wide
+    // batch (< kFpW) and lines up to 31 lie inside the staging area, and
+    // entries at positions >= W or lines >= nl are never sampled.  (Guarding
+    // them would let the compiler sink each load into its store's branch and
+    // wait for it there.)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (!hor) {
+        // line = row l0 + l, position = column pmin + p.  A wave instruction
+        // covers 64 consecutive positions of one row; a thread loads its
+        // position in the 8 rows wave, wave + 4, ... before using any.  One
+        // resource per row: columns outside [0, N) read as zero; rows
+        // l >= nl get an empty resource.
+        for (int pb = 0; pb < W; pb += 64) {
+            const int p = pb + lane;
+            const int c = pmin + p;
+            float t0[kFpLines / 4], t1[kFpLines / 4];
+#pragma unroll
+            for (int i = 0; i < kFpLines / 4; ++i) {
+                const int l = wave + 4 * i;
+                const __amdgpu_buffer_rsrc_t r = buf_rsrc(img + (uint32_t)min(l0 + l, N - 1) * (uint32_t)N,
+                                                          l < nl ? (uint32_t)N : 0u);
+                t0[i] = bload(r, img, c);
+                t1[i] = bload(r, img, c + 1);
+            }
+            const float h = (float)(p - kFpPH);
+#pragma unroll
+            for (int i = 0; i < kFpLines / 4; ++i) {
+                const float d = t1[i] - t0[i];
+                lds[(wave + 4 * i) * kFpPs + p] = make_float2(__builtin_fmaf(-h, d, t0[i]), d);
+            }
+        }
+    } else {
+        // line = column l0 + l (l = lane mod 32), position = row pmin + p.  A
+        // wave instruction covers one row of 32 columns in each half-wave
+        // (coalesced); a thread loads 9 consecutive rows for 8 pairs.  One
+        // resource over the image: rows outside [0, N) read as zero; columns
+        // at or beyond N occur only for lines l >= nl.
+        const int l = lane & 31;
+        const int col = l0 + l;
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(img, (uint32_t)N * (uint32_t)N);
+        for (int pb = 0; pb < W; pb += 64) {
+            if (pb + 16 * wave >= W)
+                break;
+            const int p0 = pb + 16 * wave + 8 * (lane >> 5);
+            const int i0 = (pmin + p0) * N + col;
+            float t[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+                t[k] = bload(r, img, i0 + k * N);
+            const float h0 = (float)(p0 - kFpPH);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float d = t[k + 1] - t[k];
+                lds[l * kFpPs + p0 + k] = make_float2(__builtin_fmaf(-(h0 + (float)k), d, t[k]), d);
+            }
+        }
+    }
+}
+
+// Start coordinate (line 0) and per-line step of the fast path's ray of
+// detector coordinate tdet at angle ang, in double (projector-parallel-joseph.cl:51-73
+// with exact arithmetic).  Every lane that evaluates the same ray gets the
+// same bits.
+__device__ __forceinline__ void fast_ray(const float *trig, int ang, bool hor, double tdet, int N, double &p0,
+                                         double &sd)
+{
+    const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
+    const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+    sd = hor ? (double)ct / (double)st : (double)st / (double)ct;
+    p0 = tdet * det_step - 0.5 * (double)N * sd + 0.5 * (double)N;
+}
+
 // ------------------------------------------------------------------------
 // Forward projection
 // ------------------------------------------------------------------------
+// Reference-compatible FP over a work table (the fast path is k_fp_pairs).
 template <typename T, typename TI, bool REF, int APT>
 __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_fp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
 {
+    static_assert(REF, "the fast path is k_fp_pairs");
     __shared__ float lds[kFpLines * kFpLs];
     __shared__ float red[2][4];
     const int lane = threadIdx.x & 63;
@@ -248,7 +384,6 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
 
         float pos[APT], stp[APT], acc[APT], inner[APT];
-        double p0d[APT], sd[APT];
         bool on[APT];
 #pragma unroll
         for (int j = 0; j < APT; ++j) {
@@ -256,40 +391,27 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             on[j] = det_on && k < item.na;
             const int ang = item.a0 + (k < item.na ? k : 0);
             const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
-            double p0;
-            if (REF) {
-                float det_step, step;
-                if (hor) {
-                    det_step = -1.0f / st;
-                    step = ct / st;
-                } else {
-                    det_step = 1.0f / ct;
-                    step = st / ct;
-                }
-                // start coordinate: double expression rounded to float (:73, :134)
-                p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
-                sd[j] = (double)step;
+            float det_step, step;
+            if (hor) {
+                det_step = -1.0f / st;
+                step = ct / st;
             } else {
-                // exact coordinates: steps and chunk anchors in double
-                const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
-                sd[j] = hor ? (double)ct / (double)st : (double)st / (double)ct;
-                p0 = tdet * det_step - 0.5 * (double)N * sd[j] + 0.5 * (double)N;
+                det_step = 1.0f / ct;
+                step = st / ct;
             }
-            p0d[j] = p0;
-            pos[j] = (float)p0;
-            stp[j] = (float)sd[j];
+            // start coordinate: double expression rounded to float (:73, :134)
+            pos[j] = (float)(tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N));
+            stp[j] = step;
             acc[j] = 0.0f;
             inner[j] = 0.0f;
         }
 
-        // Lines before l_lo read only the zero border: the reference-compatible
-        // path replays their float steps exactly, the fast path re-anchors.
-        if (REF) {
-            for (int l = 0; l < item.l_lo; ++l) {
+        // Lines before l_lo read only the zero border: their float steps are
+        // replayed exactly.
+        for (int l = 0; l < item.l_lo; ++l) {
 #pragma unroll
-                for (int j = 0; j < APT; ++j)
-                    pos[j] += stp[j];
-            }
+            for (int j = 0; j < APT; ++j)
+                pos[j] += stp[j];
         }
         for (int l0 = item.l_lo; l0 < item.l_hi; l0 += kFpLines) {
             const int nl = min(kFpLines, item.l_hi - l0);
@@ -297,8 +419,6 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             float lo = 3.0e38f, hi = -3.0e38f;
 #pragma unroll
             for (int j = 0; j < APT; ++j) {
-                if (!REF)
-                    pos[j] = (float)(p0d[j] + (double)l0 * sd[j]);
                 if (on[j]) {
                     const float a = pos[j], b = pos[j] + (float)(nl - 1) * stp[j];
                     lo = fminf(lo, fminf(a, b));
@@ -324,101 +444,56 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             // Inactive rays sample a fixed in-window position with zero step;
             // their sums are never written.
             const float park = lo + 1.0f;
-            if (REF) {
-                if (!outside) {
-                    float pp[APT], se[APT];
-#pragma unroll
-                    for (int j = 0; j < APT; ++j) {
-                        pp[j] = on[j] ? pos[j] : park;
-                        se[j] = on[j] ? stp[j] : 0.0f;
-                    }
-                    for (int l = 0; l < nl; ++l) {
-                        const float *L = lds + l * kFpLs - pmin;
-                        float t0[APT], t1[APT], b[APT];
-#pragma unroll
-                        for (int j = 0; j < APT; ++j) {
-                            const float v = pp[j] - 0.5f;
-                            const float fv = floorf(v);
-                            const int i0 = (int)fv;
-                            b[j] = v - fv;
-                            t0[j] = L[i0];
-                            t1[j] = L[i0 + 1];
-                        }
-#pragma unroll
-                        for (int j = 0; j < APT; ++j) {
-                            const float omb = 1.0f - b[j];
-                            inner[j] += omb * t0[j] + b[j] * t1[j];
-                            pp[j] += se[j];
-                        }
-                        if (((l0 + l + 1) & 63) == 0) { // 64-sample blocks (:82-90)
-#pragma unroll
-                            for (int j = 0; j < APT; ++j) {
-                                acc[j] += inner[j];
-                                inner[j] = 0.0f;
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int j = 0; j < APT; ++j)
-                        if (on[j])
-                            pos[j] = pp[j];
-                } else {
-                    // outside the image: the samples are zero, the exact
-                    // float stepping of the coordinates continues
-                    for (int l = 0; l < nl; ++l) {
-#pragma unroll
-                        for (int j = 0; j < APT; ++j)
-                            pos[j] += stp[j];
-                        if (((l0 + l + 1) & 63) == 0) {
-#pragma unroll
-                            for (int j = 0; j < APT; ++j) {
-                                acc[j] += inner[j];
-                                inner[j] = 0.0f;
-                            }
-                        }
-                    }
-                }
-            } else if (!outside) {
-                const float off = 0.5f + (float)pmin;
-                float q[APT], st[APT];
+            if (!outside) {
+                float pp[APT], se[APT];
 #pragma unroll
                 for (int j = 0; j < APT; ++j) {
-                    q[j] = on[j] ? pos[j] - off : park - off;
-                    st[j] = on[j] ? stp[j] : 0.0f;
+                    pp[j] = on[j] ? pos[j] : park;
+                    se[j] = on[j] ? stp[j] : 0.0f;
                 }
-                // Two lines per step: the LDS reads of 2 x APT samples are
-                // in flight before the first is consumed.
-                int l = 0;
-                for (; l + 2 <= nl; l += 2) {
-                    const float *L0 = lds + l * kFpLs, *L1 = L0 + kFpLs;
-                    float t0[2][APT], t1[2][APT], a[2][APT];
+                for (int l = 0; l < nl; ++l) {
+                    const float *L = lds + l * kFpLs - pmin;
+                    float t0[APT], t1[APT], b[APT];
 #pragma unroll
                     for (int j = 0; j < APT; ++j) {
-                        const float q1 = q[j] + st[j];
-                        const int i0 = cvt_flr(q[j]), i1 = cvt_flr(q1);
-                        a[0][j] = fract_pos(q[j]);
-                        a[1][j] = fract_pos(q1);
-                        t0[0][j] = L0[i0];
-                        t1[0][j] = L0[i0 + 1];
-                        t0[1][j] = L1[i1];
-                        t1[1][j] = L1[i1 + 1];
-                        q[j] = q1 + st[j];
+                        const float v = pp[j] - 0.5f;
+                        const float fv = floorf(v);
+                        const int i0 = (int)fv;
+                        b[j] = v - fv;
+                        t0[j] = L[i0];
+                        t1[j] = L[i0 + 1];
                     }
 #pragma unroll
                     for (int j = 0; j < APT; ++j) {
-                        acc[j] = __builtin_fmaf(a[0][j], t1[0][j] - t0[0][j], acc[j] + t0[0][j]);
-                        acc[j] = __builtin_fmaf(a[1][j], t1[1][j] - t0[1][j], acc[j] + t0[1][j]);
+                        const float omb = 1.0f - b[j];
+                        inner[j] += omb * t0[j] + b[j] * t1[j];
+                        pp[j] += se[j];
+                    }
+                    if (((l0 + l + 1) & 63) == 0) { // 64-sample blocks (:82-90)
+#pragma unroll
+                        for (int j = 0; j < APT; ++j) {
+                            acc[j] += inner[j];
+                            inner[j] = 0.0f;
+                        }
                     }
                 }
-                if (l < nl) {
-                    const float *L = lds + l * kFpLs;
 #pragma unroll
-                    for (int j = 0; j < APT; ++j) {
-                        const int i0 = cvt_flr(q[j]);
-                        const float a = fract_pos(q[j]);
-                        const float t0 = L[i0], t1 = L[i0 + 1];
-                        acc[j] = __builtin_fmaf(a, t1 - t0, acc[j] + t0);
-                        q[j] += st[j];
+                for (int j = 0; j < APT; ++j)
+                    if (on[j])
+                        pos[j] = pp[j];
+            } else {
+                // outside the image: the samples are zero, the exact
+                // float stepping of the coordinates continues
+                for (int l = 0; l < nl; ++l) {
+#pragma unroll
+                    for (int j = 0; j < APT; ++j)
+                        pos[j] += stp[j];
+                    if (((l0 + l + 1) & 63) == 0) {
+#pragma unroll
+                        for (int j = 0; j < APT; ++j) {
+                            acc[j] += inner[j];
+                            inner[j] = 0.0f;
+                        }
                     }
                 }
             }
@@ -438,11 +513,186 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             const float bval = gload(bsrc, o);
             float w;
             if (on[j]) {
-                const float sum = REF ? acc[j] + inner[j] : acc[j];
+                const float sum = acc[j] + inner[j];
                 w = bval + sum * corr; // :92-94
             } else {
                 w = bval; // inactive detector: the reference leaves the value as read
             }
+            if (ray_w)
+                w = w * gload(ray_w, o);
+            gstore(rdst, o, w);
+        }
+    }
+}
+
+// Fast-path FP: the work table of k_fp_tab, exact (double-anchored) ray
+// coordinates per chunk, the chunk's footprint staged as (e, d) pairs.
+//  - The footprint is the hull of the extreme rays: for one angle a ray's
+//    chunk positions are monotone in the detector index, so the first and
+//    last active detectors bound it.  Each wave evaluates those
+//    4 angles-per-item x 2 detectors x 2 chunk ends itself (no workgroup
+//    reduction, no barrier) with the same operations as the ray lanes.
+//  - A sample costs an add (coordinate), a floor-convert, an address, one
+//    8-byte LDS read and two FMA-class operations.
+template <typename T, typename TI, int APT>
+__global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
+                                                  const uirk_fp_item *__restrict__ items, int n_items,
+                                                  uirk_xcd_part part, int *__restrict__ err)
+{
+    __shared__ float2 lds[kFpLines * kFpPs];
+    constexpr int kA = 4 * APT;               // angles per item
+    constexpr int kC = (4 * kA + 63) / 64;    // footprint corners per lane
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const ItemRange range = item_range(part, n_items);
+
+    for (int it = range.first; it < range.end; it += range.stride) {
+        const uirk_fp_item item = items[it];
+        const uirk_slice *S = slices ? &slices[item.slice] : &one;
+        const int N = S->n, D = S->n_dets;
+        if (item.na > kA || N > 16384) { // planned for a wider variant / beyond the staging's offsets
+            if (threadIdx.x == 0)
+                atomicOr(err, 4);
+            continue;
+        }
+        const TI *img = uniform((const TI *)S->x);
+        const float *trig = uniform(S->trig);
+        const int s = item.d0 + lane;
+        const bool hor = item.dir == 0;
+
+        // projector-parallel-joseph.cl:51-66 (detector quantities use n_dets)
+        const float required_width = S->axis * 2;
+        const float diff = (float)D - required_width;
+        const float rot = diff / 2.0f;
+        const bool in_range = s < D;
+        const bool det_on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
+        const double tbase = 0.5 + (double)rot;
+        const double thalf = (double)(0.5f * (float)D);
+
+        float stp[APT], acc[APT];
+        double p0d[APT], sd[APT];
+        bool on[APT];
+#pragma unroll
+        for (int j = 0; j < APT; ++j) {
+            const int k = wave * APT + j;
+            on[j] = det_on && k < item.na;
+            fast_ray(trig, item.a0 + (k < item.na ? k : 0), hor, (tbase + (double)s) - thalf, N, p0d[j], sd[j]);
+            stp[j] = (float)sd[j];
+            acc[j] = 0.0f;
+        }
+        // footprint corners: (angle, first / last active detector, chunk start / end)
+        const unsigned long long m = __ballot(det_on);
+        double cp[kC], cs[kC];
+        bool con[kC], cend[kC];
+        if (m != 0) {
+            const int s_first = item.d0 + __builtin_ctzll(m), s_last = item.d0 + 63 - __builtin_clzll(m);
+#pragma unroll
+            for (int e = 0; e < kC; ++e) {
+                const int idx = lane + 64 * e;
+                const int k = idx >> 2;
+                con[e] = k < item.na;
+                cend[e] = idx & 1;
+                const int sx = (idx & 2) ? s_last : s_first;
+                fast_ray(trig, item.a0 + (con[e] ? k : 0), hor, (tbase + (double)sx) - thalf, N, cp[e], cs[e]);
+            }
+        }
+
+        for (int l0 = item.l_lo; m != 0 && l0 < item.l_hi; l0 += kFpLines) {
+            const int nl = min(kFpLines, item.l_hi - l0);
+            float lo = 3.0e38f, hi = -3.0e38f;
+#pragma unroll
+            for (int e = 0; e < kC; ++e) {
+                const float a = (float)(cp[e] + (double)l0 * cs[e]);
+                const float v = cend[e] ? a + (float)(nl - 1) * (float)cs[e] : a;
+                if (con[e]) {
+                    lo = fminf(lo, v);
+                    hi = fmaxf(hi, v);
+                }
+            }
+            lo = wave_reduce_uniform<false>(lo);
+            hi = wave_reduce_uniform<true>(hi);
+            if (!(lo <= hi))
+                break; // no active ray (the same decision in every wave)
+            const int pmin = (int)floorf(lo - 0.5f) - 1;
+            const int W = (int)floorf(hi - 0.5f) + 3 - pmin;
+            // A footprint wider than the staging window is reported and the
+            // chunk's samples are skipped (the host sizes angle blocks so that
+            // this does not happen); no sample ever reads outside the window.
+            const bool overflow = W > kFpW;
+            if (overflow && threadIdx.x == 0)
+                atomicOr(err, 1);
+            const bool outside = overflow || pmin + W <= 0 || pmin >= N;
+            if (outside)
+                continue;
+            __syncthreads(); // the previous chunk's reads are done
+            fp_stage_pairs(lds, img, N, hor, l0, nl, pmin, W);
+            __syncthreads();
+
+            // Window coordinate relative to kFpPH; inactive rays sample a
+            // fixed in-window position with zero step and are never written.
+            const float off = 0.5f + (float)pmin + (float)kFpPH;
+            const float park = lo + 1.0f - off;
+            float u[APT], st[APT];
+#pragma unroll
+            for (int j = 0; j < APT; ++j) {
+                const float pos = (float)(p0d[j] + (double)l0 * sd[j]);
+                u[j] = on[j] ? pos - off : park;
+                st[j] = on[j] ? stp[j] : 0.0f;
+            }
+            const float2 *P = lds + kFpPH;
+            int l = 0;
+            // kQ lines per step (16 or 32 samples per thread), then single lines:
+            // the LDS reads of a step are all in flight before the first is
+            // consumed.  The coordinates advance by the same float additions
+            // whatever the grouping.
+            constexpr int kQ = APT >= 8 ? 2 : 8;
+            for (; l + kQ <= nl; l += kQ) {
+                const float2 *P0 = P + l * kFpPs;
+                float2 v[kQ][APT];
+                float w[kQ][APT];
+#pragma unroll
+                for (int j = 0; j < APT; ++j) {
+                    w[0][j] = u[j];
+#pragma unroll
+                    for (int q = 1; q < kQ; ++q)
+                        w[q][j] = w[q - 1][j] + st[j];
+#pragma unroll
+                    for (int q = 0; q < kQ; ++q)
+                        v[q][j] = P0[q * kFpPs + cvt_flr(w[q][j])];
+                }
+#pragma unroll
+                for (int j = 0; j < APT; ++j) {
+#pragma unroll
+                    for (int q = 0; q < kQ; ++q)
+                        acc[j] = __builtin_fmaf(w[q][j], v[q][j].y, acc[j] + v[q][j].x);
+                    u[j] = w[kQ - 1][j] + st[j];
+                }
+            }
+            for (; l < nl; ++l) {
+                const float2 *P0 = P + l * kFpPs;
+#pragma unroll
+                for (int j = 0; j < APT; ++j) {
+                    const float2 v = P0[cvt_flr(u[j])];
+                    acc[j] = __builtin_fmaf(u[j], v.y, acc[j] + v.x);
+                    u[j] += st[j];
+                }
+            }
+        }
+
+        const T *bsrc = uniform((const T *)S->b);
+        T *rdst = uniform((T *)S->r);
+        const float *ray_w = uniform(S->ray_w);
+        const float corr = S->corr;
+#pragma unroll
+        for (int j = 0; j < APT; ++j) {
+            const int k = wave * APT + j;
+            if (!in_range || k >= item.na)
+                continue;
+            const uint32_t o = (uint32_t)(item.a0 + k) * (uint32_t)D + (uint32_t)s;
+            // :92-94; an inactive detector keeps the value as read
+            float w = gload(bsrc, o);
+            if (on[j])
+                w = w + acc[j] * corr;
             if (ray_w)
                 w = w * gload(ray_w, o);
             gstore(rdst, o, w);
@@ -1058,6 +1308,14 @@ int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *i
     return (int)hipGetLastError();
 }
 
+template <typename T, typename TI, int APT>
+int launch_fp_pairs(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n,
+                    const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_fp_pairs<T, TI, APT>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    return (int)hipGetLastError();
+}
+
 template <typename T, typename TI, bool REF, int NA, int ROWS>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
@@ -1072,10 +1330,10 @@ int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice
                    int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
     if (tile.fp_angles == 8)
-        return launch_fp<T, TI, false, 2>(sl, one, items, n, part, grid, err, st);
+        return launch_fp_pairs<T, TI, 2>(sl, one, items, n, part, grid, err, st);
     if (tile.fp_angles == 32)
-        return launch_fp<T, TI, false, 8>(sl, one, items, n, part, grid, err, st);
-    return launch_fp<T, TI, false, 4>(sl, one, items, n, part, grid, err, st);
+        return launch_fp_pairs<T, TI, 8>(sl, one, items, n, part, grid, err, st);
+    return launch_fp_pairs<T, TI, 4>(sl, one, items, n, part, grid, err, st);
 }
 
 // fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
