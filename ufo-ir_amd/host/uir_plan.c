@@ -1,14 +1,14 @@
 /*
  * uir_plan.c -- work-table construction for the table-driven projectors.
  *
- * FP items: for every subset (a direction run, or one angle), angle blocks of
- * up to UIRK_FP_ANGLES angles crossed with detector blocks of UIRK_FP_DETS.
- * An angle block is accepted only if the footprint of its rays over any chunk
- * of UIRK_FP_LINES lines fits the LDS staging window.  Ray positions are
- * affine in the detector index and the line index, so the footprint width of
- * a (detector block, chunk) window is a convex function of the window's
- * position and attains its maximum at one of the four extreme windows; those
- * four are evaluated exactly here (with a margin for float stepping).
+ * FP items: for every subset (a direction run, or one angle) and every
+ * detector block of UIRK_FP_DETS, angle blocks of up to the tile's angle
+ * count.  An angle block is accepted only if the footprint of its rays over
+ * any chunk of UIRK_FP_LINES lines fits the LDS staging window.  Ray
+ * positions are affine in the detector index and the line index, so the
+ * footprint width over a chunk is a convex function of the chunk's position
+ * and attains its maximum at the first or the last chunk; both are evaluated
+ * exactly here (with a margin for float stepping).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -77,17 +77,14 @@ static int window_width(int n, int n_dets, double axis, const float *s, const fl
     return (int)(floor(hi - 0.5) - floor(lo - 0.5)) + 4 + 2;
 }
 
-static int block_fits(int n, int n_dets, float axis, const float *s, const float *c, int dir, int a0, int k)
+/* Whether the footprint of angles [a0, a0 + k) over detector block d0 fits
+ * the staging window in every chunk: the width is convex in the chunk's
+ * first line, so the first and the last chunk bound it. */
+static int block_fits(int n, int n_dets, float axis, const float *s, const float *c, int dir, int a0, int k, int d0)
 {
-    const int dlast = ((n_dets - 1) / UIRK_FP_DETS) * UIRK_FP_DETS;
     const int llast = ((n - 1) / UIRK_FP_LINES) * UIRK_FP_LINES;
-    const int dd[2] = {0, dlast};
-    const int ll[2] = {0, llast};
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            if (window_width(n, n_dets, axis, s, c, dir, a0, k, dd[i], ll[j]) > UIRK_FP_WMAX)
-                return 0;
-    return 1;
+    return window_width(n, n_dets, axis, s, c, dir, a0, k, d0, 0) <= UIRK_FP_WMAX &&
+           window_width(n, n_dets, axis, s, c, dir, a0, k, d0, llast) <= UIRK_FP_WMAX;
 }
 
 /* Lines [*lo, *hi) outside which no ray of the block (angles [a0, a0 + k),
@@ -158,12 +155,15 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
         max_angles = UIRK_FP_ANGLES;
     for (uint32_t si = 0; si < n_subs; ++si) {
         const int off = (int)subs[si].offset, cnt = (int)subs[si].n, dir = subs[si].direction;
-        int a = off;
-        while (a < off + cnt) {
-            int k = off + cnt - a < max_angles ? off + cnt - a : max_angles;
-            while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k))
-                k = k > 2 ? k - (k + 3) / 4 : 1;
-            for (int d0 = 0; d0 < n_dets; d0 += UIRK_FP_DETS) {
+        /* Angle blocks are cut per detector block: rays far from the axis
+         * spread over more positions per angle, so only the outer blocks
+         * need fewer angles to fit the window. */
+        for (int d0 = 0; d0 < n_dets; d0 += UIRK_FP_DETS) {
+            int a = off;
+            while (a < off + cnt) {
+                int k = off + cnt - a < max_angles ? off + cnt - a : max_angles;
+                while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0))
+                    k = k > 2 ? k - (k + 3) / 4 : 1;
                 uirk_fp_item it;
                 memset(&it, 0, sizeof it);
                 it.slice = slice;
@@ -174,8 +174,8 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
                 line_range(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, &it.l_lo, &it.l_hi);
                 if (fp_push(out, it))
                     return -1;
+                a += k;
             }
-            a += k;
         }
     }
     return 0;
