@@ -543,7 +543,7 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
     constexpr int kA = 4 * APT;               // angles per item
     constexpr int kC = (4 * kA + 63) / 64;    // footprint corners per lane
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const ItemRange range = item_range(part, n_items);
 
     for (int it = range.first; it < range.end; it += range.stride) {
@@ -627,6 +627,8 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
             __syncthreads(); // the previous chunk's reads are done
             fp_stage_pairs(lds, img, N, hor, l0, nl, pmin, W);
             __syncthreads();
+            if (wave * APT >= item.na)
+                continue; // this wave has no angle of the item: it only stages
 
             // Window coordinate relative to kFpPH; inactive rays sample a
             // fixed in-window position with zero step and are never written.

@@ -161,9 +161,13 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
         for (int d0 = 0; d0 < n_dets; d0 += UIRK_FP_DETS) {
             int a = off;
             while (a < off + cnt) {
+                /* Shrink in steps of a wave's angle count (a quarter of the
+                 * tile's), so that the item's last wave is full: a wave with
+                 * no active angle skips the sampling. */
+                const int per_wave = max_angles >= 4 ? max_angles / 4 : 1;
                 int k = off + cnt - a < max_angles ? off + cnt - a : max_angles;
                 while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0))
-                    k = k > 2 ? k - (k + 3) / 4 : 1;
+                    k = k > per_wave ? ((k - 1) / per_wave) * per_wave : k - 1;
                 uirk_fp_item it;
                 memset(&it, 0, sizeof it);
                 it.slice = slice;
