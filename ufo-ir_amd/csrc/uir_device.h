@@ -125,7 +125,8 @@ typedef struct {
 #define UIRK_FP_ANGLES 16   /* angles per FP work item (default shape) */
 #define UIRK_FP_ANGLES_MAX 32 /* widest FP shape */
 #define UIRK_FP_LINES 32    /* lines per staged chunk (divides the 64-sample block) */
-#define UIRK_FP_WMAX 192    /* max staged footprint width (a multiple of 64) */
+#define UIRK_FP_WMAX 192    /* max footprint width of the pair-staged FP (a multiple of 64) */
+#define UIRK_FP_WMAX_SINGLE 256 /* max footprint width of the single-value FP (a multiple of 64) */
 #define UIRK_BP_TILE 64     /* BP tile edge */
 
 /* Tile shapes of the fast path, chosen per geometry by the autotuner
@@ -134,8 +135,14 @@ typedef struct {
  * The reference-compatible mode takes the default {16, 32, 64} only. */
 typedef struct {
     int32_t fp_angles, bp_angles, bp_rows;
+    /* FP staging: 0 = (e, d) pairs (k_fp_pairs, windows up to UIRK_FP_WMAX),
+     * 1 = single values (k_fp_tab, windows up to UIRK_FP_WMAX_SINGLE), for
+     * geometries whose angle blocks the narrower window would cut short */
+    int32_t fp_single;
 } uirk_tile;
-#define UIRK_TILE_DEFAULT {UIRK_FP_ANGLES, 32, UIRK_BP_TILE}
+#define UIRK_TILE_DEFAULT {UIRK_FP_ANGLES, 32, UIRK_BP_TILE, 0}
+/* Footprint width limit of a tile's FP. */
+#define UIRK_TILE_FP_WMAX(t) ((t).fp_single ? UIRK_FP_WMAX_SINGLE : UIRK_FP_WMAX)
 
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
  * FP32 / FP16 / BF16 storage).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
@@ -165,9 +172,10 @@ typedef struct {
 } uirk_bp_vec;
 /* Appends the FP items of one slice for the given subsets (runs or single
  * angles); angle blocks are sized so that each footprint fits the staging
- * window.  Returns 0 on success, -1 on allocation failure. */
+ * window of wmax positions (UIRK_TILE_FP_WMAX of the tile).  Returns 0 on
+ * success, -1 on allocation failure. */
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
-                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles);
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax);
 /* tiles of 64 x tile_rows pixels */
 int uirk_plan_bp(uirk_bp_vec *out, int slice, int n, int tile_rows);
 void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig);

@@ -8,9 +8,11 @@
 //                 restatement of projector-parallel-joseph.cl.
 //   REF  = false: the batched fast path.  FP32 interpolation and accumulation
 //                 on FP32 / FP16 / BF16 storage, exact (double-anchored) ray
-//                 coordinates, sampling from staged (e, d) pairs with one
-//                 floor-convert and one FMA, fused SIRT epilogues.  Its FP is
-//                 k_fp_pairs; k_fp_tab is the reference-compatible FP.
+//                 coordinates, fused SIRT epilogues.  Its FP is k_fp_pairs
+//                 (staged (e, d) pairs, one floor-convert and one FMA per
+//                 sample) or, for geometries whose angle blocks need windows
+//                 wider than the pairs' (sparse angles on large images),
+//                 k_fp_tab<..., false, ...> (single staged values, fract).
 // Work is described by tables (uirk_fp_item / uirk_bp_item) over any number
 // of slices of any geometry; the kernels are persistent (grid-stride over the
 // table) and every item writes disjoint outputs, so results do not depend on
@@ -36,7 +38,7 @@
 namespace {
 
 constexpr int kFpLines = UIRK_FP_LINES;
-constexpr int kFpW = UIRK_FP_WMAX;
+constexpr int kFpW = UIRK_FP_WMAX_SINGLE; // single-value staging (k_fp_tab)
 constexpr int kFpLs = kFpW + 1;            // LDS line stride (odd: conflict-free transposed stores)
 constexpr int kBpT = UIRK_BP_TILE;
 constexpr int kBpWb = 100;                 // staged bins per angle (>= 64 * sqrt(2) + margin)
@@ -262,17 +264,18 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfl
 //   d = t_{p+1} - t_p,  e = t_p - (p - kFpPH) d,
 // so the interpolated value at window coordinate w in [p, p + 1) is
 // e + (w - kFpPH) d: one FMA per sample, no fract and no second tap.  The
-// origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpW / 2.
-constexpr int kFpPs = kFpW + 1; // pair line stride (odd)
-constexpr int kFpPH = kFpW / 2;
+// origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpPW / 2.
+constexpr int kFpPW = UIRK_FP_WMAX; // widest pair window
+constexpr int kFpPs = kFpPW + 1;    // pair line stride (odd)
+constexpr int kFpPH = kFpPW / 2;
 
 template <typename TI>
 __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpLines * kFpPs], const TI *img, int N, bool hor, int l0,
                                                int nl, int pmin, int W)
 {
-    static_assert(kFpLines == 32 && kFpW % 64 == 0 && kFpPs > kFpW, "staging layout");
+    static_assert(kFpLines == 32 && kFpPW % 64 == 0 && kFpPs > kFpPW, "staging layout");
     // Stores are unconditional: positions up to the end of the last 64-wide
-    // batch (< kFpW) and lines up to 31 lie inside the staging area, and
+    // batch (< kFpPW) and lines up to 31 lie inside the staging area, and
     // entries at positions >= W or lines >= nl are never sampled.  (Guarding
     // them would let the compiler sink each load into its store's branch and
     // wait for it there.)
@@ -347,13 +350,14 @@ __device__ __forceinline__ void fast_ray(const float *trig, int ang, bool hor, d
 // ------------------------------------------------------------------------
 // Forward projection
 // ------------------------------------------------------------------------
-// Reference-compatible FP over a work table (the fast path is k_fp_pairs).
+// FP over a work table with single-value staging: the reference-compatible
+// FP (REF) and the fast path for geometries whose angle blocks need windows
+// wider than the pair staging holds (fp_single tiles).
 template <typename T, typename TI, bool REF, int APT>
 __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_fp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
 {
-    static_assert(REF, "the fast path is k_fp_pairs");
     __shared__ float lds[kFpLines * kFpLs];
     __shared__ float red[2][4];
     const int lane = threadIdx.x & 63;
@@ -384,6 +388,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
         const double tdet = ((0.5 + (double)rot) + (double)s) - (double)(0.5f * (float)D);
 
         float pos[APT], stp[APT], acc[APT], inner[APT];
+        double p0d[APT], sd[APT];
         bool on[APT];
 #pragma unroll
         for (int j = 0; j < APT; ++j) {
@@ -391,27 +396,40 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             on[j] = det_on && k < item.na;
             const int ang = item.a0 + (k < item.na ? k : 0);
             const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
-            float det_step, step;
-            if (hor) {
-                det_step = -1.0f / st;
-                step = ct / st;
+            double p0;
+            if (REF) {
+                float det_step, step;
+                if (hor) {
+                    det_step = -1.0f / st;
+                    step = ct / st;
+                } else {
+                    det_step = 1.0f / ct;
+                    step = st / ct;
+                }
+                // start coordinate: double expression rounded to float (:73, :134)
+                p0 = tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N);
+                sd[j] = (double)step;
             } else {
-                det_step = 1.0f / ct;
-                step = st / ct;
+                // exact coordinates: steps and chunk anchors in double
+                const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+                sd[j] = hor ? (double)ct / (double)st : (double)st / (double)ct;
+                p0 = tdet * det_step - 0.5 * (double)N * sd[j] + 0.5 * (double)N;
             }
-            // start coordinate: double expression rounded to float (:73, :134)
-            pos[j] = (float)(tdet * (double)det_step + (double)((-origin_shift) * step) + (double)(0.5f * (float)N));
-            stp[j] = step;
+            p0d[j] = p0;
+            pos[j] = (float)p0;
+            stp[j] = (float)sd[j];
             acc[j] = 0.0f;
             inner[j] = 0.0f;
         }
 
-        // Lines before l_lo read only the zero border: their float steps are
-        // replayed exactly.
-        for (int l = 0; l < item.l_lo; ++l) {
+        // Lines before l_lo read only the zero border: the reference-compatible
+        // path replays their float steps exactly, the fast path re-anchors.
+        if (REF) {
+            for (int l = 0; l < item.l_lo; ++l) {
 #pragma unroll
-            for (int j = 0; j < APT; ++j)
-                pos[j] += stp[j];
+                for (int j = 0; j < APT; ++j)
+                    pos[j] += stp[j];
+            }
         }
         for (int l0 = item.l_lo; l0 < item.l_hi; l0 += kFpLines) {
             const int nl = min(kFpLines, item.l_hi - l0);
@@ -419,6 +437,8 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             float lo = 3.0e38f, hi = -3.0e38f;
 #pragma unroll
             for (int j = 0; j < APT; ++j) {
+                if (!REF)
+                    pos[j] = (float)(p0d[j] + (double)l0 * sd[j]);
                 if (on[j]) {
                     const float a = pos[j], b = pos[j] + (float)(nl - 1) * stp[j];
                     lo = fminf(lo, fminf(a, b));
@@ -444,56 +464,101 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             // Inactive rays sample a fixed in-window position with zero step;
             // their sums are never written.
             const float park = lo + 1.0f;
-            if (!outside) {
-                float pp[APT], se[APT];
-#pragma unroll
-                for (int j = 0; j < APT; ++j) {
-                    pp[j] = on[j] ? pos[j] : park;
-                    se[j] = on[j] ? stp[j] : 0.0f;
-                }
-                for (int l = 0; l < nl; ++l) {
-                    const float *L = lds + l * kFpLs - pmin;
-                    float t0[APT], t1[APT], b[APT];
+            if (REF) {
+                if (!outside) {
+                    float pp[APT], se[APT];
 #pragma unroll
                     for (int j = 0; j < APT; ++j) {
-                        const float v = pp[j] - 0.5f;
-                        const float fv = floorf(v);
-                        const int i0 = (int)fv;
-                        b[j] = v - fv;
-                        t0[j] = L[i0];
-                        t1[j] = L[i0 + 1];
+                        pp[j] = on[j] ? pos[j] : park;
+                        se[j] = on[j] ? stp[j] : 0.0f;
                     }
-#pragma unroll
-                    for (int j = 0; j < APT; ++j) {
-                        const float omb = 1.0f - b[j];
-                        inner[j] += omb * t0[j] + b[j] * t1[j];
-                        pp[j] += se[j];
-                    }
-                    if (((l0 + l + 1) & 63) == 0) { // 64-sample blocks (:82-90)
+                    for (int l = 0; l < nl; ++l) {
+                        const float *L = lds + l * kFpLs - pmin;
+                        float t0[APT], t1[APT], b[APT];
 #pragma unroll
                         for (int j = 0; j < APT; ++j) {
-                            acc[j] += inner[j];
-                            inner[j] = 0.0f;
+                            const float v = pp[j] - 0.5f;
+                            const float fv = floorf(v);
+                            const int i0 = (int)fv;
+                            b[j] = v - fv;
+                            t0[j] = L[i0];
+                            t1[j] = L[i0 + 1];
+                        }
+#pragma unroll
+                        for (int j = 0; j < APT; ++j) {
+                            const float omb = 1.0f - b[j];
+                            inner[j] += omb * t0[j] + b[j] * t1[j];
+                            pp[j] += se[j];
+                        }
+                        if (((l0 + l + 1) & 63) == 0) { // 64-sample blocks (:82-90)
+#pragma unroll
+                            for (int j = 0; j < APT; ++j) {
+                                acc[j] += inner[j];
+                                inner[j] = 0.0f;
+                            }
                         }
                     }
-                }
-#pragma unroll
-                for (int j = 0; j < APT; ++j)
-                    if (on[j])
-                        pos[j] = pp[j];
-            } else {
-                // outside the image: the samples are zero, the exact
-                // float stepping of the coordinates continues
-                for (int l = 0; l < nl; ++l) {
 #pragma unroll
                     for (int j = 0; j < APT; ++j)
-                        pos[j] += stp[j];
-                    if (((l0 + l + 1) & 63) == 0) {
+                        if (on[j])
+                            pos[j] = pp[j];
+                } else {
+                    // outside the image: the samples are zero, the exact
+                    // float stepping of the coordinates continues
+                    for (int l = 0; l < nl; ++l) {
 #pragma unroll
-                        for (int j = 0; j < APT; ++j) {
-                            acc[j] += inner[j];
-                            inner[j] = 0.0f;
+                        for (int j = 0; j < APT; ++j)
+                            pos[j] += stp[j];
+                        if (((l0 + l + 1) & 63) == 0) {
+#pragma unroll
+                            for (int j = 0; j < APT; ++j) {
+                                acc[j] += inner[j];
+                                inner[j] = 0.0f;
+                            }
                         }
+                    }
+                }
+            } else if (!outside) {
+                const float off = 0.5f + (float)pmin;
+                float q[APT], st[APT];
+#pragma unroll
+                for (int j = 0; j < APT; ++j) {
+                    q[j] = on[j] ? pos[j] - off : park - off;
+                    st[j] = on[j] ? stp[j] : 0.0f;
+                }
+                // Two lines per step: the LDS reads of 2 x APT samples are
+                // in flight before the first is consumed.
+                int l = 0;
+                for (; l + 2 <= nl; l += 2) {
+                    const float *L0 = lds + l * kFpLs, *L1 = L0 + kFpLs;
+                    float t0[2][APT], t1[2][APT], a[2][APT];
+#pragma unroll
+                    for (int j = 0; j < APT; ++j) {
+                        const float q1 = q[j] + st[j];
+                        const int i0 = cvt_flr(q[j]), i1 = cvt_flr(q1);
+                        a[0][j] = fract_pos(q[j]);
+                        a[1][j] = fract_pos(q1);
+                        t0[0][j] = L0[i0];
+                        t1[0][j] = L0[i0 + 1];
+                        t0[1][j] = L1[i1];
+                        t1[1][j] = L1[i1 + 1];
+                        q[j] = q1 + st[j];
+                    }
+#pragma unroll
+                    for (int j = 0; j < APT; ++j) {
+                        acc[j] = __builtin_fmaf(a[0][j], t1[0][j] - t0[0][j], acc[j] + t0[0][j]);
+                        acc[j] = __builtin_fmaf(a[1][j], t1[1][j] - t0[1][j], acc[j] + t0[1][j]);
+                    }
+                }
+                if (l < nl) {
+                    const float *L = lds + l * kFpLs;
+#pragma unroll
+                    for (int j = 0; j < APT; ++j) {
+                        const int i0 = cvt_flr(q[j]);
+                        const float a = fract_pos(q[j]);
+                        const float t0 = L[i0], t1 = L[i0 + 1];
+                        acc[j] = __builtin_fmaf(a, t1 - t0, acc[j] + t0);
+                        q[j] += st[j];
                     }
                 }
             }
@@ -513,7 +578,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
             const float bval = gload(bsrc, o);
             float w;
             if (on[j]) {
-                const float sum = acc[j] + inner[j];
+                const float sum = REF ? acc[j] + inner[j] : acc[j];
                 w = bval + sum * corr; // :92-94
             } else {
                 w = bval; // inactive detector: the reference leaves the value as read
@@ -618,7 +683,7 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
             // A footprint wider than the staging window is reported and the
             // chunk's samples are skipped (the host sizes angle blocks so that
             // this does not happen); no sample ever reads outside the window.
-            const bool overflow = W > kFpW;
+            const bool overflow = W > kFpPW;
             if (overflow && threadIdx.x == 0)
                 atomicOr(err, 1);
             const bool outside = overflow || pmin + W <= 0 || pmin >= N;
@@ -1331,6 +1396,13 @@ template <typename T, typename TI>
 int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items,
                    int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
+    if (tile.fp_single) {
+        if (tile.fp_angles == 8)
+            return launch_fp<T, TI, false, 2>(sl, one, items, n, part, grid, err, st);
+        if (tile.fp_angles == 32)
+            return launch_fp<T, TI, false, 8>(sl, one, items, n, part, grid, err, st);
+        return launch_fp<T, TI, false, 4>(sl, one, items, n, part, grid, err, st);
+    }
     if (tile.fp_angles == 8)
         return launch_fp_pairs<T, TI, 2>(sl, one, items, n, part, grid, err, st);
     if (tile.fp_angles == 32)
@@ -1383,7 +1455,7 @@ int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, const uirk_slic
 bool tile_ok(const uirk_tile *t)
 {
     return (t->fp_angles == 16 || t->fp_angles == 8 || t->fp_angles == 32) && (t->bp_angles == 32 || t->bp_angles == 16) &&
-           (t->bp_rows == 64 || t->bp_rows == 32);
+           (t->bp_rows == 64 || t->bp_rows == 32) && (t->fp_single == 0 || t->fp_single == 1);
 }
 
 } // namespace
@@ -1400,7 +1472,7 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         single = *one;
     }
-    const uirk_tile tl = tile ? *tile : uirk_tile{UIRK_FP_ANGLES, 32, UIRK_BP_TILE};
+    const uirk_tile tl = tile ? *tile : uirk_tile UIRK_TILE_DEFAULT;
     if (!tile_ok(&tl))
         return (int)hipErrorInvalidValue;
     if (n_items <= 0)
@@ -1492,7 +1564,7 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         single = *one;
     }
-    const uirk_tile tl = tile ? *tile : uirk_tile{UIRK_FP_ANGLES, 32, UIRK_BP_TILE};
+    const uirk_tile tl = tile ? *tile : uirk_tile UIRK_TILE_DEFAULT;
     if (!tile_ok(&tl))
         return (int)hipErrorInvalidValue;
     if (n_items <= 0)

@@ -20,8 +20,11 @@
 
 #include "uir_batch_impl.h"
 
-static int fp_shape(const uirk_tile *t) { return t->fp_angles == 8 ? 1 : (t->fp_angles == 32 ? 2 : 0); }
-static const int k_fp_shape_angles[UIR_FP_SHAPES] = {UIRK_FP_ANGLES, 8, 32};
+static int fp_shape(const uirk_tile *t)
+{
+    return (t->fp_angles == 8 ? 1 : (t->fp_angles == 32 ? 2 : 0)) + 3 * (t->fp_single != 0);
+}
+static const int k_fp_shape_angles[3] = {UIRK_FP_ANGLES, 8, 32};
 static int bp_shape(const uirk_tile *t) { return (t->bp_angles == 16) * 2 + (t->bp_rows == 32); }
 
 #define TRY(x) do { uir_status s_ = (x); if (s_ != UIR_OK) return s_; } while (0)
@@ -255,7 +258,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         k->run_off[0] = 0;
         k->run_n[0] = G->A;
         if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
-                         G->runs, G->n_runs, G->tile.fp_angles) ||
+                         G->runs, G->n_runs, G->tile.fp_angles, UIRK_TILE_FP_WMAX(G->tile)) ||
             uirk_plan_bp(&bv[bp_shape(&G->tile)], (int)i, (int)G->n, G->tile.bp_rows)) {
             s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
             goto fail;
@@ -282,7 +285,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         for (int g = 0; g < UIR_FP_SHAPES && s == UIR_OK; ++g) {
             kgroup *k = &b->fpg[g];
             k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
-            k->tile.fp_angles = k_fp_shape_angles[g];
+            k->tile.fp_angles = k_fp_shape_angles[g % 3];
+            k->tile.fp_single = g / 3;
             for (size_t i = 0; i < fv[g].n; ++i)
                 cost[i] = uirk_fp_item_cost(&fv[g].v[i]);
             uirk_plan_xcd(cost, (int)fv[g].n, uirk_plan_num_xcd(), &k->part);

@@ -18,7 +18,7 @@ typedef struct {
     uint32_t n_runs;
 } geom_t;
 
-#define UIR_FP_SHAPES 3
+#define UIR_FP_SHAPES 6 /* FP angles 16, 8, 32, with pair staging, then with single values */
 #define UIR_BP_SHAPES 4
 
 typedef struct {

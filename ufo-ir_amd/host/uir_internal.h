@@ -77,6 +77,10 @@ static inline size_t uir_dtype_size(int dtype) { return dtype == UIR_DTYPE_F32 ?
 uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype storage, uint32_t reps,
                         uir_tune_result *out);
 uirk_tile uir_tile_for(const uir_slice_spec *spec, uir_dtype storage);
+/* Whether the geometry's fast FP uses single-value staging (wider windows):
+ * when one angle step moves rays at the image edge by more than 15 pixels,
+ * the pair window would cut its angle blocks to a few angles. */
+int uir_fp_single_for(const uir_slice_spec *spec);
 
 /* geometry helpers (ufo-ir-parallel-projector-task.c:436-498) */
 void uir_geom_lut(uint32_t n_angles, float step, float *sin_lut, float *cos_lut);

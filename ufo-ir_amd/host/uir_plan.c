@@ -80,11 +80,12 @@ static int window_width(int n, int n_dets, double axis, const float *s, const fl
 /* Whether the footprint of angles [a0, a0 + k) over detector block d0 fits
  * the staging window in every chunk: the width is convex in the chunk's
  * first line, so the first and the last chunk bound it. */
-static int block_fits(int n, int n_dets, float axis, const float *s, const float *c, int dir, int a0, int k, int d0)
+static int block_fits(int n, int n_dets, float axis, const float *s, const float *c, int dir, int a0, int k, int d0,
+                      int wmax)
 {
     const int llast = ((n - 1) / UIRK_FP_LINES) * UIRK_FP_LINES;
-    return window_width(n, n_dets, axis, s, c, dir, a0, k, d0, 0) <= UIRK_FP_WMAX &&
-           window_width(n, n_dets, axis, s, c, dir, a0, k, d0, llast) <= UIRK_FP_WMAX;
+    return window_width(n, n_dets, axis, s, c, dir, a0, k, d0, 0) <= wmax &&
+           window_width(n, n_dets, axis, s, c, dir, a0, k, d0, llast) <= wmax;
 }
 
 /* Lines [*lo, *hi) outside which no ray of the block (angles [a0, a0 + k),
@@ -149,7 +150,7 @@ static void line_range(int n, int n_dets, double axis, const float *s, const flo
 }
 
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
-                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles)
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax)
 {
     if (max_angles < 1 || max_angles > UIRK_FP_ANGLES_MAX)
         max_angles = UIRK_FP_ANGLES;
@@ -166,7 +167,7 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
                  * no active angle skips the sampling. */
                 const int per_wave = max_angles >= 4 ? max_angles / 4 : 1;
                 int k = off + cnt - a < max_angles ? off + cnt - a : max_angles;
-                while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0))
+                while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, wmax))
                     k = k > per_wave ? ((k - 1) / per_wave) * per_wave : k - 1;
                 uirk_fp_item it;
                 memset(&it, 0, sizeof it);

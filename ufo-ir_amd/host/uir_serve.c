@@ -69,7 +69,7 @@ typedef struct sgeom {
     struct sgeom *next;
 } sgeom;
 
-#define SRV_FP_SHAPES 3
+#define SRV_FP_SHAPES 6
 #define SRV_BP_SHAPES 4
 
 typedef struct {
@@ -130,7 +130,7 @@ struct uir_server {
     sgeom *geoms;
 
     /* tables (worker thread only): slice descriptors, and one work table per
-     * tile shape (FP: 2 shapes, BP: 4), each launched when non-empty */
+     * tile shape (FP: 6 shapes, BP: 4), each launched when non-empty */
     uirk_slice *h_fp_sl, *h_bp_sl;
     double *h_cost;
     size_t hcap_fsl, hcap_bsl, hcap_cost;
@@ -279,13 +279,13 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
         }
     }
     g->tile = uir_tile_for(sp, s->dtype);
-    g->fp_shape = g->tile.fp_angles == 8 ? 1 : (g->tile.fp_angles == 32 ? 2 : 0);
+    g->fp_shape = (g->tile.fp_angles == 8 ? 1 : (g->tile.fp_angles == 32 ? 2 : 0)) + 3 * (g->tile.fp_single != 0);
     g->bp_shape = (g->tile.bp_angles == 16) * 2 + (g->tile.bp_rows == 32);
     {
         uirk_fp_vec fv = {0};
         uirk_bp_vec bv = {0};
         if (uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs,
-                         g->tile.fp_angles) ||
+                         g->tile.fp_angles, UIRK_TILE_FP_WMAX(g->tile)) ||
             uirk_plan_bp(&bv, 0, (int)g->n, g->tile.bp_rows)) {
             free(fv.v);
             free(bv.v);
@@ -899,7 +899,8 @@ uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server *
     s->autotune = cfg->autotune;
     for (int c = 0; c < SRV_FP_SHAPES; ++c) {
         s->fpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
-        s->fpg[c].tile.fp_angles = c == 1 ? 8 : (c == 2 ? 32 : UIRK_FP_ANGLES);
+        s->fpg[c].tile.fp_angles = c % 3 == 1 ? 8 : (c % 3 == 2 ? 32 : UIRK_FP_ANGLES);
+        s->fpg[c].tile.fp_single = c / 3;
     }
     for (int c = 0; c < SRV_BP_SHAPES; ++c) {
         s->bpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;

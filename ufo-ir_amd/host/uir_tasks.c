@@ -586,7 +586,7 @@ static uir_status proj_plan(uir_task *p, uint32_t n)
     float *trig = (float *)malloc(sizeof(float) * 2 * (p->lut_n ? p->lut_n : 1));
     uir_status s = UIR_OK;
     if (!trig || uirk_plan_fp(&fv, 0, (int)n, (int)p->n_dets, axis, p->h_sin, p->h_cos, p->runs, p->n_runs,
-                              UIRK_FP_ANGLES) ||
+                              UIRK_FP_ANGLES, UIRK_FP_WMAX_SINGLE) ||
         uirk_plan_bp(&bv, 0, (int)n, UIRK_BP_TILE)) {
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
         goto done;

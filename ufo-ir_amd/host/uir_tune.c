@@ -61,7 +61,7 @@ uir_status uir_tune_set(const uir_slice_spec *spec, uir_dtype storage, const uir
 {
     if (!spec || !tile)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
-    const uirk_tile t = {tile->fp_angles, tile->bp_angles, tile->bp_rows};
+    const uirk_tile t = {tile->fp_angles, tile->bp_angles, tile->bp_rows, 0};
     if (!((t.fp_angles == 16 || t.fp_angles == 8 || t.fp_angles == 32) && (t.bp_angles == 32 || t.bp_angles == 16) &&
           (t.bp_rows == 64 || t.bp_rows == 32)))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "tile %d/%d/%d is not a variant", t.fp_angles, t.bp_angles,
@@ -104,11 +104,19 @@ void uir_tune_clear(void)
     pthread_mutex_unlock(&g_mu);
 }
 
+int uir_fp_single_for(const uir_slice_spec *spec)
+{
+    /* Measured at 1024^2 / 720 angles (step x N = 4.5) and 2048^2 / 900
+     * (7.1) the pair FP is 4-15 % faster; at 1024^2 / 120 (27) and
+     * 2048^2 / 90 (71) the single-value FP is 15-33 % faster. */
+    return fabs((double)spec->step) * (double)spec->n > 15.0;
+}
+
 uirk_tile uir_tile_for(const uir_slice_spec *spec, uir_dtype storage)
 {
     uir_tile t;
     (void)uir_tune_get(spec, storage, &t);
-    const uirk_tile k = {t.fp_angles, t.bp_angles, t.bp_rows};
+    const uirk_tile k = {t.fp_angles, t.bp_angles, t.bp_rows, uir_fp_single_for(spec)};
     return k;
 }
 
@@ -236,10 +244,11 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
     float best_fp = 3.0e38f, best_bp = 3.0e38f;
     out->best = (uir_tile){UIRK_FP_ANGLES, 32, UIRK_BP_TILE};
     for (int v = 0; v < 3 && s == UIR_OK; ++v) {
-        const uirk_tile tl = {k_fp_variants[v].fp_angles, 32, UIRK_BP_TILE};
+        const uirk_tile tl = {k_fp_variants[v].fp_angles, 32, UIRK_BP_TILE, uir_fp_single_for(spec)};
         uirk_fp_vec fv = {0};
         for (uint32_t i = 0; i < B && s == UIR_OK; ++i)
-            if (uirk_plan_fp(&fv, (int)i, (int)n, (int)D, axis, h_sin, h_cos, runs, n_runs, tl.fp_angles))
+            if (uirk_plan_fp(&fv, (int)i, (int)n, (int)D, axis, h_sin, h_cos, runs, n_runs, tl.fp_angles,
+                             UIRK_TILE_FP_WMAX(tl)))
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
         uirk_xcd_part part;
         free(cost);
@@ -266,7 +275,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
         free(fv.v);
     }
     for (int v = 0; v < 4 && s == UIR_OK; ++v) {
-        const uirk_tile tl = {UIRK_FP_ANGLES, k_bp_variants[v].bp_angles, k_bp_variants[v].bp_rows};
+        const uirk_tile tl = {UIRK_FP_ANGLES, k_bp_variants[v].bp_angles, k_bp_variants[v].bp_rows, 0};
         uirk_bp_vec bv = {0};
         for (uint32_t i = 0; i < B && s == UIR_OK; ++i)
             if (uirk_plan_bp(&bv, (int)i, (int)n, tl.bp_rows))
