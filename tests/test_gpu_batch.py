@@ -166,3 +166,26 @@ def test_batch_shifted_axis_matches_fp64(res, shift):
     err = O.rel_l2(bt.image(0), ref)
     print("axis shift", shift, "rel-L2 vs FP64", err)
     assert err <= 2e-5
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16-sino", 1e-3)])
+def test_batch_sparse_and_dense_angles_in_one_batch(res, storage, tol):
+    """A sparse-angle slice (one angle step moves edge rays by ~34 pixels: the
+    FP stages single values with wider windows) next to dense-angle slices
+    (staged (e, d) pairs): both FP kinds run in one iteration and each slice
+    matches FP64."""
+    geoms = [(256, 363, 24), (128, 182, 90), (192, 272, 16)]
+    specs = [_spec(*gm) for gm in geoms]
+    assert specs[0]["step"] * 256 > 15 and specs[1]["step"] * 128 < 15
+    bt = uir.Batch(res, specs, storage)
+    refs = []
+    for i, (n, d, a) in enumerate(geoms):
+        g, b = _data(n, d, a, 700 + i)
+        bt.set_sinogram(i, b.astype(np.float32))
+        refs.append(O.sirt64(g, b, 10, 0.5))
+    bt.reset()
+    bt.iterate(10)
+    for i in range(len(geoms)):
+        err = O.rel_l2(bt.image(i), refs[i])
+        print(storage, geoms[i], "rel-L2 vs FP64", err)
+        assert err <= tol
