@@ -1,0 +1,148 @@
+"""FP work-table planner (uirk_plan_fp, host code, no device needed).
+
+The planner cuts each direction run into angle blocks per 64-detector block
+so that every 32-line chunk's footprint fits the staging window.  These tests
+recompute, in float64 and independently of the planner's own corner
+argument, what the FP kernels do with each item:
+  - every (angle, detector block) is planned exactly once, inside one run;
+  - the window the kernel stages for each chunk (hull of all active rays over
+    the chunk's lines, floor(hi - 0.5) - floor(lo - 0.5) + 4 positions) fits
+    the limit the planner was given, for chunks starting at the item's l_lo;
+  - every sample of every active ray that can touch the image lies in the
+    item's line range [l_lo, l_hi)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import uir
+
+LINES, DETS = 32, 64
+
+
+class Item(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("slice", "dir", "a0", "na", "d0", "l_lo", "l_hi", "pad")]
+
+
+class Vec(C.Structure):
+    _fields_ = [("v", C.POINTER(Item)), ("n", C.c_size_t), ("cap", C.c_size_t)]
+
+
+class Subset(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("n", C.c_uint32), ("direction", C.c_int32)]
+
+
+def _plan(n, d, a, step, axis_prop, max_angles, wmax):
+    L = uir.load()
+    s = np.zeros(a, np.float32)
+    c = np.zeros(a, np.float32)
+    fp = C.POINTER(C.c_float)
+    L.uir_geom_lut.argtypes = [C.c_uint32, C.c_float, fp, fp]
+    L.uir_geom_lut(a, step, s.ctypes.data_as(fp), c.ctypes.data_as(fp))
+    L.uir_geom_axis.restype = C.c_float
+    L.uir_geom_axis.argtypes = [C.c_float, C.c_uint32]
+    axis = L.uir_geom_axis(axis_prop, d)
+    # direction runs, computed here: 1 = vertical (|sin| <= |cos|), 0 = horizontal
+    dirs = (np.abs(s) <= np.abs(c)).astype(int)
+    subs = []
+    i = 0
+    while i < a:
+        j = i
+        while j < a and dirs[j] == dirs[i]:
+            j += 1
+        subs.append((i, j - i, int(dirs[i])))
+        i = j
+    arr = (Subset * len(subs))(*[Subset(o, k, dr) for o, k, dr in subs])
+    vec = Vec()
+    L.uirk_plan_fp.restype = C.c_int
+    L.uirk_plan_fp.argtypes = [C.POINTER(Vec), C.c_int, C.c_int, C.c_int, C.c_float, fp, fp, C.POINTER(Subset),
+                               C.c_uint32, C.c_int, C.c_int]
+    assert L.uirk_plan_fp(C.byref(vec), 0, n, d, axis, s.ctypes.data_as(fp), c.ctypes.data_as(fp), arr,
+                          len(subs), max_angles, wmax) == 0
+    items = [(it.dir, it.a0, it.na, it.d0, it.l_lo, it.l_hi) for it in vec.v[:vec.n]]
+    C.CDLL(None).free(C.cast(vec.v, C.c_void_p))
+    return items, subs, s.astype(np.float64), c.astype(np.float64), float(axis)
+
+
+def _positions(dirn, s, c, dets, axis, n, lines):
+    """Ray positions p[angle, det, line] of the fast path (exact coordinates)."""
+    if dirn == 0:
+        det_step, step = -1.0 / s, c / s
+    else:
+        det_step, step = 1.0 / c, s / c
+    t = dets[None, :] + 0.5 - axis
+    p0 = t * det_step[:, None] - 0.5 * n * step[:, None] + 0.5 * n
+    return p0[:, :, None] + lines[None, None, :] * step[:, None, None]
+
+
+def _active(dets, d, axis):
+    req = 2.0 * np.float32(axis)
+    diff = np.float32(d) - req
+    on = dets < d
+    if diff < 0:
+        on &= ~(dets.astype(np.float32) < abs(diff))
+    if diff > 0:
+        on &= ~(dets.astype(np.float32) > req)
+    return on
+
+
+GEOMS = [  # (n, n_dets, n_angles, step scale, axis property)
+    (96, 136, 40, 1.0, -1.0),
+    (128, 182, 90, 1.0, -1.0),
+    (160, 227, 75, 1.0, -1.0),
+    (256, 363, 24, 1.0, -1.0),
+    (192, 272, 180, 1.0, 140.5),   # axis shifted right of the detector centre
+    (192, 272, 180, 1.0, 128.0),   # and left
+    (200, 283, 17, 0.37, -1.0),    # angles over part of [0, pi)
+]
+
+
+@pytest.mark.parametrize("wmax", [192, 256])
+@pytest.mark.parametrize("max_angles", [8, 16, 32])
+@pytest.mark.parametrize("geom", GEOMS)
+def test_plan_covers_each_ray_once_and_windows_fit(geom, max_angles, wmax):
+    n, d, a, scale, axis_prop = geom
+    step = float(np.float32(scale * np.pi / a))
+    items, subs, s, c, axis = _plan(n, d, a, step, axis_prop, max_angles, wmax)
+    seen = np.zeros((a, (d + DETS - 1) // DETS), int)
+    run_of = np.zeros(a, int)
+    for r, (o, k, _) in enumerate(subs):
+        run_of[o:o + k] = r
+    lines_all = np.arange(n, dtype=np.float64)
+    for dirn, a0, na, d0, l_lo, l_hi in items:
+        assert 1 <= na <= max_angles and d0 % DETS == 0 and 0 <= l_lo <= l_hi <= n
+        assert run_of[a0] == run_of[a0 + na - 1] and subs[run_of[a0]][2] == dirn
+        seen[a0:a0 + na, d0 // DETS] += 1
+        dets = np.arange(d0, min(d0 + DETS, d))
+        dets = dets[_active(dets, d, axis)]
+        if dets.size == 0:
+            continue
+        ang = np.arange(a0, a0 + na)
+        p = _positions(dirn, s[ang], c[ang], dets, axis, n, lines_all)
+        touch = (p >= -0.5) & (p < n + 0.5)
+        ls = np.nonzero(touch.any(axis=(0, 1)))[0]
+        if ls.size:
+            assert l_lo <= ls.min() and ls.max() < l_hi, (dirn, a0, na, d0, l_lo, l_hi, ls.min(), ls.max())
+        for l0 in range(l_lo, l_hi, LINES):
+            nl = min(LINES, l_hi - l0)
+            ends = p[:, :, [l0, l0 + nl - 1]]
+            lo, hi = ends.min(), ends.max()
+            if hi < -1.0 or lo > n + 1.0:
+                continue  # the kernel skips chunks outside the image
+            w = np.floor(hi - 0.5) - np.floor(lo - 0.5) + 4
+            assert w <= wmax, (dirn, a0, na, d0, l0, w)
+    assert (seen == 1).all()
+
+
+def test_wide_detector_blocks_cut_only_where_needed():
+    """At 1024^2 / 720 angles / 1448 bins with 16-angle items and the pair
+    window, only the outer detector blocks get fewer than 16 angles."""
+    n, d, a = 1024, 1448, 720
+    items, _, _, _, _ = _plan(n, d, a, float(np.float32(np.pi / a)), -1.0, 16, 192)
+    full = {}
+    for _, _, na, d0, _, _ in items:
+        full.setdefault(d0, []).append(na)
+    mean = {d0: np.mean(v) for d0, v in full.items()}
+    centre = [mean[d0] for d0 in mean if abs(d0 + 32 - d / 2) < 256]
+    edge = [mean[d0] for d0 in mean if abs(d0 + 32 - d / 2) > 600]
+    assert min(centre) > 14.0 and min(edge) < min(centre)
