@@ -634,16 +634,35 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
         const double tbase = 0.5 + (double)rot;
         const double thalf = (double)(0.5f * (float)D);
 
+        // Rays of this thread: one angle, APT detectors kDL apart.  A wave
+        // instruction (fixed j) then covers kDL adjacent detectors of each of
+        // the wave's APT angles, whose positions lie closer together than those
+        // of 64 adjacent detectors of one angle (fewer distinct LDS pairs per
+        // read), and a thread divides once per item instead of APT times.
+        constexpr int kDL = 64 / APT;
+        const int k_ang = wave * APT + lane / kDL; // angle within the item
+        const bool ang_on = k_ang < item.na;
         float stp[APT], acc[APT];
         double p0d[APT], sd[APT];
-        bool on[APT];
+        bool on[APT], ray_in[APT];
+        {
+            // fast_ray, with the angle's quantities computed once
+            const int ang = item.a0 + (ang_on ? k_ang : 0);
+            const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
+            const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
+            const double sdv = hor ? (double)ct / (double)st : (double)st / (double)ct;
 #pragma unroll
-        for (int j = 0; j < APT; ++j) {
-            const int k = wave * APT + j;
-            on[j] = det_on && k < item.na;
-            fast_ray(trig, item.a0 + (k < item.na ? k : 0), hor, (tbase + (double)s) - thalf, N, p0d[j], sd[j]);
-            stp[j] = (float)sd[j];
-            acc[j] = 0.0f;
+            for (int j = 0; j < APT; ++j) {
+                const int sj = item.d0 + lane % kDL + kDL * j;
+                ray_in[j] = sj < D;
+                const bool dj_on =
+                    ray_in[j] && !((diff < 0 && (float)sj < fabsf(diff)) || (diff > 0 && (float)sj > required_width));
+                on[j] = dj_on && ang_on;
+                p0d[j] = ((tbase + (double)sj) - thalf) * det_step - 0.5 * (double)N * sdv + 0.5 * (double)N;
+                sd[j] = sdv;
+                stp[j] = (float)sdv;
+                acc[j] = 0.0f;
+            }
         }
         // footprint corners: (angle, first / last active detector, chunk start / end)
         const unsigned long long m = __ballot(det_on);
@@ -752,10 +771,9 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
         const float corr = S->corr;
 #pragma unroll
         for (int j = 0; j < APT; ++j) {
-            const int k = wave * APT + j;
-            if (!in_range || k >= item.na)
+            if (!ray_in[j] || !ang_on)
                 continue;
-            const uint32_t o = (uint32_t)(item.a0 + k) * (uint32_t)D + (uint32_t)s;
+            const uint32_t o = (uint32_t)(item.a0 + k_ang) * (uint32_t)D + (uint32_t)(item.d0 + lane % kDL + kDL * j);
             // :92-94; an inactive detector keeps the value as read
             float w = gload(bsrc, o);
             if (on[j])
