@@ -143,6 +143,10 @@ typedef struct {
 #define UIRK_TILE_DEFAULT {UIRK_FP_ANGLES, 32, UIRK_BP_TILE, 0}
 /* Footprint width limit of a tile's FP. */
 #define UIRK_TILE_FP_WMAX(t) ((t).fp_single ? UIRK_FP_WMAX_SINGLE : UIRK_FP_WMAX)
+/* Workgroups of a tile's FP resident per CU (LDS per workgroup: 49 KB with
+ * pairs, 33 KB with single values, of 160 KB): the persistent grid's size, so
+ * that no workgroup waits for a slot and runs its share of items at the end. */
+#define UIRK_TILE_FP_WG_PER_CU(t) ((t).fp_single ? 4 : 3)
 
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
  * FP32 / FP16 / BF16 storage).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
