@@ -143,10 +143,6 @@ typedef struct {
 #define UIRK_TILE_DEFAULT {UIRK_FP_ANGLES, 32, UIRK_BP_TILE, 0}
 /* Footprint width limit of a tile's FP. */
 #define UIRK_TILE_FP_WMAX(t) ((t).fp_single ? UIRK_FP_WMAX_SINGLE : UIRK_FP_WMAX)
-/* Workgroups of a tile's FP resident per CU (LDS per workgroup: 49 KB with
- * pairs, 33 KB with single values, of 160 KB): the persistent grid's size, so
- * that no workgroup waits for a slot and runs its share of items at the end. */
-#define UIRK_TILE_FP_WG_PER_CU(t) ((t).fp_single ? 4 : 3)
 
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
  * FP32 / FP16 / BF16 storage).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
@@ -183,7 +179,10 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
 /* tiles of 64 x tile_rows pixels */
 int uirk_plan_bp(uirk_bp_vec *out, int slice, int n, int tile_rows);
 void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig);
-int uirk_plan_grid(int n_items, int per_cu);
+/* One workgroup per item, each on the XCD whose range holds it: n_xcd times
+ * the largest range (workgroups past their XCD's range exit at once), so
+ * that the hardware's dispatch balances the items' uneven costs. */
+int uirk_plan_grid_xcd(const uirk_xcd_part *part, int n_items);
 int uirk_plan_num_xcd(void);
 /* Splits items [0, n) into n_xcd contiguous ranges of about equal total cost
  * (cost[i] >= 0; NULL = unit cost). */

@@ -14,9 +14,10 @@
 //                 wider than the pairs' (sparse angles on large images),
 //                 k_fp_tab<..., false, ...> (single staged values, fract).
 // Work is described by tables (uirk_fp_item / uirk_bp_item) over any number
-// of slices of any geometry; the kernels are persistent (grid-stride over the
-// table) and every item writes disjoint outputs, so results do not depend on
-// the grid or on which other slices share the launch.
+// of slices of any geometry; the kernels stride over the table (launched with
+// one workgroup per item, so that the hardware's dispatch balances the items'
+// uneven costs) and every item writes disjoint outputs, so results do not
+// depend on the grid or on which other slices share the launch.
 //
 // FP: a workgroup owns 64 detectors x up to 16 angles of one direction family
 //     (one detector and four angles per thread).  It walks the image in

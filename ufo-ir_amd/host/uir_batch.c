@@ -291,7 +291,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
                 cost[i] = uirk_fp_item_cost(&fv[g].v[i]);
             uirk_plan_xcd(cost, (int)fv[g].n, uirk_plan_num_xcd(), &k->part);
             k->n_items = (int)fv[g].n;
-            k->grid = uirk_plan_grid(k->n_items, UIRK_TILE_FP_WG_PER_CU(k->tile));
+            k->grid = uirk_plan_grid_xcd(&k->part, k->n_items);
             if (k->n_items)
                 s = upload(&k->d_items, fv[g].v, fv[g].n * sizeof(uirk_fp_item));
         }
@@ -304,7 +304,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
                 cost[i] = (double)specs[bv[g].v[i].slice].n_angles;
             uirk_plan_xcd(cost, (int)bv[g].n, uirk_plan_num_xcd(), &k->part);
             k->n_items = (int)bv[g].n;
-            k->grid = uirk_plan_grid(k->n_items, 8);
+            k->grid = uirk_plan_grid_xcd(&k->part, k->n_items);
             if (k->n_items)
                 s = upload(&k->d_items, bv[g].v, bv[g].n * sizeof(uirk_bp_item));
         }

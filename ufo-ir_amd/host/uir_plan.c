@@ -253,18 +253,16 @@ void uirk_plan_xcd(const double *cost, int n, int n_xcd, uirk_xcd_part *out)
         out->start[x++] = n;
 }
 
-/* Persistent-grid size: enough workgroups to fill every CU per_cu times. */
-int uirk_plan_grid(int n_items, int per_cu)
+int uirk_plan_grid_xcd(const uirk_xcd_part *part, int n_items)
 {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-            cus = prop.multiProcessorCount;
-        if (cus <= 0)
-            cus = 256;
+    if (n_items <= 0)
+        return 0;
+    if (!part || part->n_xcd <= 1)
+        return n_items;
+    int most = 0;
+    for (int x = 0; x < part->n_xcd; ++x) {
+        const int c = part->start[x + 1] - part->start[x];
+        most = c > most ? c : most;
     }
-    long g = (long)cus * per_cu;
-    return (int)(g < n_items ? g : n_items);
+    return (most > 0 ? most : 1) * part->n_xcd;
 }

@@ -590,7 +590,7 @@ static uir_status tables_build(uir_server *s)
                 for (size_t j = 0; j < s->active[i]->g->n_fp; ++j)
                     s->h_cost[o++] = s->active[i]->g->fp_cost[j];
         uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
-        grp->grid = uirk_plan_grid(grp->n, UIRK_TILE_FP_WG_PER_CU(grp->tile));
+        grp->grid = uirk_plan_grid_xcd(&grp->part, (int)grp->n);
     }
     for (int c = 0; c < SRV_BP_SHAPES; ++c) {
         sgroup *grp = &s->bpg[c];
@@ -600,7 +600,7 @@ static uir_status tables_build(uir_server *s)
                 for (size_t j = 0; j < s->active[i]->g->n_bp; ++j)
                     s->h_cost[o++] = (double)s->active[i]->g->A;
         uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
-        grp->grid = uirk_plan_grid(grp->n, 8);
+        grp->grid = uirk_plan_grid_xcd(&grp->part, (int)grp->n);
     }
     TRY(dgrow((void **)&s->d_fp_sl, &s->dcap_fsl, na, sizeof(uirk_slice)));
     TRY(dgrow((void **)&s->d_bp_sl, &s->dcap_bsl, na, sizeof(uirk_slice)));

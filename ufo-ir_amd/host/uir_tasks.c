@@ -660,7 +660,7 @@ static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float
     sl.r = w_sino;
     sl.ray_w = ray_w;
     return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_fp_items, p->n_fp_items, &p->fp_part, 0, UIR_DTYPE_F32, UIR_DTYPE_F32, NULL,
-                                       uirk_plan_grid(p->n_fp_items, 4), p->d_err, p->res->stream), "FP launch");
+                                       uirk_plan_grid_xcd(&p->fp_part, p->n_fp_items), p->d_err, p->res->stream), "FP launch");
 }
 
 /* backward over all runs (:410-429).  pix_w == NULL: accumulate form,
@@ -685,7 +685,7 @@ static uir_status proj_bp_ex(uir_task *p, uint32_t n, const float *sino, float *
         }
         sl.n_runs = (int32_t)k;
         TRY(uir_check_hip(uirk_bp_table(NULL, &sl, p->d_bp_items, p->n_bp_items, &p->bp_part, 0, UIR_DTYPE_F32, UIR_DTYPE_F32, NULL,
-                                        uirk_plan_grid(p->n_bp_items, 8), p->d_err, p->res->stream), "BP launch"));
+                                        uirk_plan_grid_xcd(&p->bp_part, p->n_bp_items), p->d_err, p->res->stream), "BP launch"));
     }
     return UIR_OK;
 }

@@ -266,7 +266,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, fv.v, fv.n * sizeof(uirk_fp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 1, d_sl, d_items, (int)fv.n, &part, sdt, idt, &tl, uirk_plan_grid((int)fv.n, UIRK_TILE_FP_WG_PER_CU(tl)), d_err,
+            s = time_variant(st, 1, d_sl, d_items, (int)fv.n, &part, sdt, idt, &tl, uirk_plan_grid_xcd(&part, (int)fv.n), d_err,
                              reps, &out->fp_ms[v]);
         if (s == UIR_OK && out->fp_ms[v] < best_fp) {
             best_fp = out->fp_ms[v];
@@ -288,7 +288,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, bv.v, bv.n * sizeof(uirk_bp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 0, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl, uirk_plan_grid((int)bv.n, 8), d_err,
+            s = time_variant(st, 0, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl, uirk_plan_grid_xcd(&part, (int)bv.n), d_err,
                              reps, &out->bp_ms[v]);
         if (s == UIR_OK && out->bp_ms[v] < best_bp) {
             best_bp = out->bp_ms[v];
