@@ -267,14 +267,15 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfl
 // e + (w - kFpPH) d: one FMA per sample, no fract and no second tap.  The
 // origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpPW / 2.
 constexpr int kFpPW = UIRK_FP_WMAX; // widest pair window
+constexpr int kFpPL = 24;           // lines per pair-staged chunk (37 KB of LDS: 4 workgroups per CU)
 constexpr int kFpPs = kFpPW + 1;    // pair line stride (odd)
 constexpr int kFpPH = kFpPW / 2;
 
 template <typename TI>
-__device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpLines * kFpPs], const TI *img, int N, bool hor, int l0,
+__device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], const TI *img, int N, bool hor, int l0,
                                                int nl, int pmin, int W)
 {
-    static_assert(kFpLines == 32 && kFpPW % 64 == 0 && kFpPs > kFpPW, "staging layout");
+    static_assert(kFpPL % 4 == 0 && kFpPL <= 32 && kFpPW % 64 == 0 && kFpPs > kFpPW, "staging layout");
     // Stores are unconditional: positions up to the end of the last 64-wide
     // batch (< kFpPW) and lines up to 31 lie inside the staging area, and
     // entries at positions >= W or lines >= nl are never sampled.  (Guarding
@@ -291,9 +292,9 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpLines * kFpPs], 
         for (int pb = 0; pb < W; pb += 64) {
             const int p = pb + lane;
             const int c = pmin + p;
-            float t0[kFpLines / 4], t1[kFpLines / 4];
+            float t0[kFpPL / 4], t1[kFpPL / 4];
 #pragma unroll
-            for (int i = 0; i < kFpLines / 4; ++i) {
+            for (int i = 0; i < kFpPL / 4; ++i) {
                 const int l = wave + 4 * i;
                 const __amdgpu_buffer_rsrc_t r = buf_rsrc(img + (uint32_t)min(l0 + l, N - 1) * (uint32_t)N,
                                                           l < nl ? (uint32_t)N : 0u);
@@ -302,7 +303,7 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpLines * kFpPs], 
             }
             const float h = (float)(p - kFpPH);
 #pragma unroll
-            for (int i = 0; i < kFpLines / 4; ++i) {
+            for (int i = 0; i < kFpPL / 4; ++i) {
                 const float d = t1[i] - t0[i];
                 lds[(wave + 4 * i) * kFpPs + p] = make_float2(__builtin_fmaf(-h, d, t0[i]), d);
             }
@@ -319,6 +320,8 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpLines * kFpPs], 
         for (int pb = 0; pb < W; pb += 64) {
             if (pb + 16 * wave >= W)
                 break;
+            if (l >= kFpPL)
+                continue; // lanes past the staged lines (one branch around all of a lane's loads)
             const int p0 = pb + 16 * wave + 8 * (lane >> 5);
             const int i0 = (pmin + p0) * N + col;
             float t[9];
@@ -601,11 +604,11 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 //  - A sample costs an add (coordinate), a floor-convert, an address, one
 //    8-byte LDS read and two FMA-class operations.
 template <typename T, typename TI, int APT>
-__global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
+__global__ __launch_bounds__(256, 4) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                   const uirk_fp_item *__restrict__ items, int n_items,
                                                   uirk_xcd_part part, int *__restrict__ err)
 {
-    __shared__ float2 lds[kFpLines * kFpPs];
+    __shared__ float2 lds[kFpPL * kFpPs];
     constexpr int kA = 4 * APT;               // angles per item
     constexpr int kC = (4 * kA + 63) / 64;    // footprint corners per lane
     const int lane = threadIdx.x & 63;
@@ -682,8 +685,8 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
             }
         }
 
-        for (int l0 = item.l_lo; m != 0 && l0 < item.l_hi; l0 += kFpLines) {
-            const int nl = min(kFpLines, item.l_hi - l0);
+        for (int l0 = item.l_lo; m != 0 && l0 < item.l_hi; l0 += kFpPL) {
+            const int nl = min(kFpPL, item.l_hi - l0);
             float lo = 3.0e38f, hi = -3.0e38f;
 #pragma unroll
             for (int e = 0; e < kC; ++e) {
@@ -732,7 +735,7 @@ __global__ __launch_bounds__(256) void k_fp_pairs(const uirk_slice *__restrict__
             // the LDS reads of a step are all in flight before the first is
             // consumed.  The coordinates advance by the same float additions
             // whatever the grouping.
-            constexpr int kQ = APT >= 8 ? 2 : 8;
+            constexpr int kQ = APT >= 8 ? 2 : (APT >= 4 ? 4 : 8);
             for (; l + kQ <= nl; l += kQ) {
                 const float2 *P0 = P + l * kFpPs;
                 float2 v[kQ][APT];
