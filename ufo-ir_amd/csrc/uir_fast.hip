@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256, 4) void k_fp_pairs(const uirk_slice *__restric
 // Backprojection
 // ------------------------------------------------------------------------
 template <typename T, typename TI, bool REF, int NA, int ROWS>
-__global__ __launch_bounds__(256) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
+__global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
 {
