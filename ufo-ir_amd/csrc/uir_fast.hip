@@ -133,23 +133,13 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-// Wave-wide min / max returned as a uniform (scalar) value: a DPP inclusive
-// scan (row shifts, then row broadcasts 15 and 31) leaves the result in lane
-// 63.  Lanes a step does not write keep their value, which min / max ignore.
-template <bool MAX> __device__ __forceinline__ float dpp_step(float v, float t) { return MAX ? fmaxf(v, t) : fminf(v, t); }
-template <bool MAX> __device__ __forceinline__ float wave_reduce_uniform(float v)
+// A double held in lane `ln` (uniform) of every lane's register v.
+__device__ __forceinline__ double readlane_f64(double v, int ln)
 {
-#define UIRK_DPP(ctrl, rm, bm)                                                                                     \
-    v = dpp_step<MAX>(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), ctrl, rm, bm, false)))
-    UIRK_DPP(0x111, 0xf, 0xf); // row_shr:1
-    UIRK_DPP(0x112, 0xf, 0xf); // row_shr:2
-    UIRK_DPP(0x113, 0xf, 0xf); // row_shr:3
-    UIRK_DPP(0x114, 0xf, 0xe); // row_shr:4
-    UIRK_DPP(0x118, 0xf, 0xc); // row_shr:8
-    UIRK_DPP(0x142, 0xa, 0xf); // row_bcast:15
-    UIRK_DPP(0x143, 0xc, 0xf); // row_bcast:31
-#undef UIRK_DPP
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, ln);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), ln);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Workgroup-wide (4 waves) min of lo and max of hi.  Its first barrier also
@@ -598,9 +588,10 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 // coordinates per chunk, the chunk's footprint staged as (e, d) pairs.
 //  - The footprint is the hull of the extreme rays: for one angle a ray's
 //    chunk positions are monotone in the detector index, so the first and
-//    last active detectors bound it.  Each wave evaluates those
-//    4 angles-per-item x 2 detectors x 2 chunk ends itself (no workgroup
-//    reduction, no barrier) with the same operations as the ray lanes.
+//    last active detectors bound it.  Each wave evaluates those angles x 2
+//    detectors x 2 chunk ends itself (no workgroup reduction, no barrier),
+//    with the same operations as the ray lanes, for 64 chunks at a time:
+//    one chunk per lane.
 //  - A sample costs an add (coordinate), a floor-convert, an address, one
 //    8-byte LDS read and two FMA-class operations.
 template <typename T, typename TI, int APT>
@@ -685,22 +676,40 @@ __global__ __launch_bounds__(256, 4) void k_fp_pairs(const uirk_slice *__restric
             }
         }
 
-        for (int l0 = item.l_lo; m != 0 && l0 < item.l_hi; l0 += kFpPL) {
+        // Chunk windows, 64 chunks at a time: lane c evaluates chunk cg + c
+        // at every corner (read from the corner lanes), and each chunk then
+        // reads its lo / hi from that lane (uniform).  The corners' (angle,
+        // detector) pairs sit in lanes 4 k + 2 d (+ 64 per extra corner
+        // register), the same values the per-lane corners hold.
+        float c_lo = 0.0f, c_hi = 0.0f;
+        int ci = 0;
+        for (int l0 = item.l_lo; m != 0 && l0 < item.l_hi; l0 += kFpPL, ++ci) {
             const int nl = min(kFpPL, item.l_hi - l0);
-            float lo = 3.0e38f, hi = -3.0e38f;
+            if ((ci & 63) == 0) {
+                const int cl0 = l0 + lane * kFpPL; // this lane's chunk (beyond l_hi: unused)
+                const double cl0d = (double)cl0;
+                const float nm1 = (float)(min(kFpPL, item.l_hi - cl0) - 1);
+                float lo_c = 3.0e38f, hi_c = -3.0e38f;
 #pragma unroll
-            for (int e = 0; e < kC; ++e) {
-                const float a = (float)(cp[e] + (double)l0 * cs[e]);
-                const float v = cend[e] ? a + (float)(nl - 1) * (float)cs[e] : a;
-                if (con[e]) {
-                    lo = fminf(lo, v);
-                    hi = fmaxf(hi, v);
+                for (int k = 0; k < kA; ++k) {
+                    if (k >= item.na) // uniform
+                        continue;
+#pragma unroll
+                    for (int dsel = 0; dsel < 2; ++dsel) {
+                        const int idx = 4 * k + 2 * dsel;
+                        const double cpe = readlane_f64(cp[idx >> 6], idx & 63);
+                        const double cse = readlane_f64(cs[idx >> 6], idx & 63);
+                        const float av = (float)(cpe + cl0d * cse);
+                        const float bv = av + nm1 * (float)cse;
+                        lo_c = fminf(lo_c, fminf(av, bv));
+                        hi_c = fmaxf(hi_c, fmaxf(av, bv));
+                    }
                 }
+                c_lo = lo_c;
+                c_hi = hi_c;
             }
-            lo = wave_reduce_uniform<false>(lo);
-            hi = wave_reduce_uniform<true>(hi);
-            if (!(lo <= hi))
-                break; // no active ray (the same decision in every wave)
+            const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c_lo), ci & 63));
+            const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c_hi), ci & 63));
             const int pmin = (int)floorf(lo - 0.5f) - 1;
             const int W = (int)floorf(hi - 0.5f) + 3 - pmin;
             // A footprint wider than the staging window is reported and the
