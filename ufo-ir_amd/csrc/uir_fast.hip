@@ -257,7 +257,7 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfl
 // e + (w - kFpPH) d: one FMA per sample, no fract and no second tap.  The
 // origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpPW / 2.
 constexpr int kFpPW = UIRK_FP_WMAX; // widest pair window
-constexpr int kFpPL = 24;           // lines per pair-staged chunk (37 KB of LDS: 4 workgroups per CU)
+constexpr int kFpPL = 20;           // lines per pair-staged chunk (31 KB of LDS: 5 workgroups per CU)
 constexpr int kFpPs = kFpPW + 1;    // pair line stride (odd)
 constexpr int kFpPH = kFpPW / 2;
 
@@ -267,7 +267,7 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], con
 {
     static_assert(kFpPL % 4 == 0 && kFpPL <= 32 && kFpPW % 64 == 0 && kFpPs > kFpPW, "staging layout");
     // Stores are unconditional: positions up to the end of the last 64-wide
-    // batch (< kFpPW) and lines up to 31 lie inside the staging area, and
+    // batch (< kFpPW) and lines below kFpPL lie inside the staging area, and
     // entries at positions >= W or lines >= nl are never sampled.  (Guarding
     // them would let the compiler sink each load into its store's branch and
     // wait for it there.)
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 //  - A sample costs an add (coordinate), a floor-convert, an address, one
 //    8-byte LDS read and two FMA-class operations.
 template <typename T, typename TI, int APT>
-__global__ __launch_bounds__(256, 4) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
+__global__ __launch_bounds__(256, APT == 8 ? 4 : 5) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                   const uirk_fp_item *__restrict__ items, int n_items,
                                                   uirk_xcd_part part, int *__restrict__ err)
 {
