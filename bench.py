@@ -48,6 +48,9 @@ sys.path.insert(0, os.path.join(ROOT, "ufo-ir_amd"))
 METRIC = "slice-iters/s SIRT 1024²×720 ang, batched vs plain @relL2≤1e-3; p99 latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# LDS read bandwidth: 128 bytes per clock per CU (a read-only LDS loop measures
+# 126, with or without lanes sharing addresses) x 256 CUs x 2.4 GHz.
+LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
 
 
 def ellipses(seed, count=20):
@@ -587,12 +590,19 @@ def main():
         roof = {"bound": "mfma", "achieved": round(tfl, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tfl / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_fp_pairs" if kname == "fp" else "k_bp_tab",
-                "note": "compute-bound on the FP32 vector ALU (no MFMA); 4 flop per interpolation sample; "
-                        "hbm frac %.4f at %.1f GB/s algorithmic" % (gbs / HBM_PEAK_GBS, gbs)}
+                "note": "FP32 vector ALU work (no MFMA: no contraction), 4 flop per interpolation sample; "
+                        "hbm frac %.4f at %.1f GB/s algorithmic; the binding resource is LDS reads, "
+                        "one 8-byte (e, d) pair per sample (see lds)" % (gbs / HBM_PEAK_GBS, gbs)}
     else:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_fp_pairs" if kname == "fp" else "k_bp_tab"}
+
+    ksamples = info["fp_samples"] if kname == "fp" else info["bp_samples"]
+    lds_tbs = 8.0 * ksamples / (kms * 1e-3) / 1e12
+    roof["lds"] = {"achieved": round(lds_tbs, 2), "peak": round(LDS_PEAK_TBS, 2), "unit": "TB/s",
+                   "frac": round(lds_tbs / LDS_PEAK_TBS, 4),
+                   "what": "LDS bytes read by the sampling loop, 8 per sample (staging writes not counted)"}
 
     # ---- CPU baseline and FP64 accuracy (the oracle, on this host's cores)
     cpu = None
