@@ -1,7 +1,9 @@
 """FP work-table planner (uirk_plan_fp, host code, no device needed).
 
 The planner cuts each direction run into angle blocks per 64-detector block
-so that every 32-line chunk's footprint fits the staging window.  These tests
+so that every chunk's footprint fits the staging window: the planner bounds
+32-line chunks (the single-value kernel's), and the pair kernel's 20-line
+chunks lie inside them.  These tests
 recompute, in float64 and independently of the planner's own corner
 argument, what the FP kernels do with each item:
   - every (angle, detector block) is planned exactly once, inside one run;
@@ -17,7 +19,7 @@ import pytest
 
 import uir
 
-LINES, DETS = 32, 64
+LINES, DETS = (32, 20), 64  # chunk lines: single-value kernel, pair kernel
 
 
 class Item(C.Structure):
@@ -123,14 +125,15 @@ def test_plan_covers_each_ray_once_and_windows_fit(geom, max_angles, wmax):
         ls = np.nonzero(touch.any(axis=(0, 1)))[0]
         if ls.size:
             assert l_lo <= ls.min() and ls.max() < l_hi, (dirn, a0, na, d0, l_lo, l_hi, ls.min(), ls.max())
-        for l0 in range(l_lo, l_hi, LINES):
-            nl = min(LINES, l_hi - l0)
-            ends = p[:, :, [l0, l0 + nl - 1]]
-            lo, hi = ends.min(), ends.max()
-            if hi < -1.0 or lo > n + 1.0:
-                continue  # the kernel skips chunks outside the image
-            w = np.floor(hi - 0.5) - np.floor(lo - 0.5) + 4
-            assert w <= wmax, (dirn, a0, na, d0, l0, w)
+        for cl in LINES:
+            for l0 in range(l_lo, l_hi, cl):
+                nl = min(cl, l_hi - l0)
+                ends = p[:, :, [l0, l0 + nl - 1]]
+                lo, hi = ends.min(), ends.max()
+                if hi < -1.0 or lo > n + 1.0:
+                    continue  # the kernel skips chunks outside the image
+                w = np.floor(hi - 0.5) - np.floor(lo - 0.5) + 4
+                assert w <= wmax, (dirn, a0, na, d0, cl, l0, w)
     assert (seen == 1).all()
 
 

@@ -19,12 +19,15 @@
 // uneven costs) and every item writes disjoint outputs, so results do not
 // depend on the grid or on which other slices share the launch.
 //
-// FP: a workgroup owns 64 detectors x up to 16 angles of one direction family
-//     (one detector and four angles per thread).  It walks the image in
-//     chunks of 32 lines (rows for the vertical family, columns for the
-//     horizontal one); each chunk's footprint is staged into LDS line-major
-//     (transposed for the horizontal family), so both families read their two
-//     taps as adjacent LDS words (one 8-byte pair in the fast path).
+// FP: a workgroup owns 64 detectors x up to 4 * APT angles of one direction
+//     family (the fast pair path: one angle and APT detectors per thread).
+//     It walks the image in chunks of lines (rows for the vertical family,
+//     columns for the horizontal one: 20 per chunk in the pair path, 32 in
+//     the single-value one); each chunk's footprint is staged into LDS
+//     line-major (transposed for the horizontal family), so both families
+//     read their two taps as adjacent LDS words (one 8-byte pair in the pair
+//     path).  Both projectors are bound by these LDS reads, 8 bytes per
+//     sample.
 // BP: a workgroup owns a 64 x 64 pixel tile (one column and 16 rows per
 //     thread); sinogram rows and their sin/cos are staged 32 angles at a time
 //     over the tile's detector footprint.
