@@ -440,6 +440,9 @@ def main():
     ap.add_argument("--c4-plain-slices", type=int, default=4)
     ap.add_argument("--tile", default=None,
                     help="fp_angles,bp_angles,bp_rows: use this tile shape instead of tuning (no probe launches)")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="measure the LDS read rate and the FP32 FMA issue rate of the device, print them as one "
+                         "JSON line and exit")
     ap.add_argument("--profile-pass", action="store_true",
                     help="for counter collection: build the batch, run 3 iterations with direct launches and one "
                          "512 MiB device-to-device copy (the calibration of the byte counters), then exit")
@@ -468,6 +471,9 @@ def main():
         batch_c3(args, uir)
         return
     res = uir.Resources(0)
+    if args.ceilings:
+        print(json.dumps({"ceilings": uir.probe_ceilings(res)}))
+        return
     n, a, d = args.n, args.angles, args.dets
     step = float(np.float32(np.pi / a))
     relax = 0.25
@@ -603,6 +609,13 @@ def main():
     roof["lds"] = {"achieved": round(lds_tbs, 2), "peak": round(LDS_PEAK_TBS, 2), "unit": "TB/s",
                    "frac": round(lds_tbs / LDS_PEAK_TBS, 4),
                    "what": "LDS bytes read by the sampling loop, 8 per sample (staging writes not counted)"}
+    try:
+        ceil = uir.probe_ceilings(res)
+        roof["lds"]["probe"] = {"tb_s": ceil["lds_tb_s"], "bytes_per_clock_cu": ceil["lds_bytes_per_clock_cu"],
+                                "fp32_fma_tflop_s": ceil["fp32_tflop_s"],
+                                "what": "measured on this device: a read-only LDS loop and an FMA loop"}
+    except uir.UirError as e:
+        roof["lds"]["probe"] = {"error": str(e)}
 
     # ---- CPU baseline and FP64 accuracy (the oracle, on this host's cores)
     cpu = None

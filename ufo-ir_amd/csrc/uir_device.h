@@ -25,6 +25,19 @@ typedef struct {
     int32_t count;
 } uirk_runs;
 
+/* Measured ceilings of the projectors' binding resources on the current
+ * device (uir_probe.hip): LDS read bytes and wave64 FP32 FMAs per clock per
+ * CU, at the device's reported clock.  Returns 0, or 2 (no memory) / 3 (HIP
+ * error). */
+typedef struct {
+    int n_cu;
+    double clock_ghz;
+    double lds_bytes_per_clock_cu;
+    double fma_wave64_per_clock_cu;
+    double lds_ms, fma_ms; /* one launch of each loop */
+} uirk_ceilings;
+int uirk_probe_ceilings(void *stream, uirk_ceilings *c);
+
 /* Reference-compatible Joseph FP (projector-parallel-joseph.cl:36-156) over
  * angles [ang0, ang0 + n_ang).  dir < 0 takes each angle's direction from the
  * LUT (|sin| <= |cos| -> vertical), else forces FP_vert (1) / FP_hor (0).

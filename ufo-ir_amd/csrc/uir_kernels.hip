@@ -6,7 +6,7 @@
 // Built with -ffp-contract=off: every float operation rounds where the
 // reference source rounds, so these kernels reproduce the reference's
 // arithmetic operation for operation (sum order included).  The fast
-// reduced-precision batched kernels live in uir_batch.hip.
+// reduced-precision batched kernels live in uir_fast.hip.
 #include <hip/hip_runtime.h>
 
 #include "uir_device.h"

@@ -91,6 +91,11 @@ class BatchInfo(C.Structure):
                 ("bp_grid", C.c_uint32), ("fp_samples", C.c_double), ("bp_samples", C.c_double)]
 
 
+class Ceilings(C.Structure):
+    _fields_ = [("n_cu", C.c_int), ("clock_ghz", C.c_double), ("lds_bytes_per_clock_cu", C.c_double),
+                ("fma_wave64_per_clock_cu", C.c_double), ("lds_ms", C.c_double), ("fma_ms", C.c_double)]
+
+
 class AsdPocsParams(C.Structure):
     _fields_ = [("beta", C.c_float), ("beta_red", C.c_float), ("alpha", C.c_float), ("alpha_red", C.c_float),
                 ("r_max", C.c_float), ("ng", C.c_uint32), ("df_iterations", C.c_uint32),
@@ -215,6 +220,7 @@ def load():
     L.uir_op_dot_f64.argtypes = [vp, vp, sz, C.POINTER(C.c_double), vp]
     for n in ("uir_op_dx", "uir_op_dxt", "uir_op_dy", "uir_op_dyt", "uir_op_l1_grad"):
         getattr(L, n).argtypes = [vp, vp, u32, u32, vp]
+    L.uirk_probe_ceilings.argtypes = [vp, C.POINTER(Ceilings)]
     _lib = L
     return L
 
@@ -530,6 +536,24 @@ def tune_set(spec, storage, fp_angles, bp_angles, bp_rows):
 
 def tune_clear():
     load().uir_tune_clear()
+
+
+def probe_ceilings(res):
+    """Measured LDS read rate and FP32 FMA issue rate of the device (uir_probe.hip).
+
+    Returns bytes and wave64 FMAs per clock per CU at the device's reported
+    clock, and the chip-wide TB/s and TFLOP/s they imply."""
+    c = Ceilings()
+    rc = load().uirk_probe_ceilings(res.stream, C.byref(c))
+    if rc != 0:
+        raise UirError(rc, "uirk_probe_ceilings failed")
+    hz = c.clock_ghz * 1e9
+    return {"n_cu": c.n_cu, "clock_ghz": c.clock_ghz,
+            "lds_bytes_per_clock_cu": round(c.lds_bytes_per_clock_cu, 2),
+            "lds_tb_s": round(c.lds_bytes_per_clock_cu * c.n_cu * hz / 1e12, 2),
+            "fma_wave64_per_clock_cu": round(c.fma_wave64_per_clock_cu, 3),
+            "fp32_tflop_s": round(c.fma_wave64_per_clock_cu * 64 * 2 * c.n_cu * hz / 1e12, 1),
+            "lds_ms": round(c.lds_ms, 3), "fma_ms": round(c.fma_ms, 3)}
 
 
 class Batch:
