@@ -104,6 +104,15 @@ __device__ __forceinline__ int cvt_flr(float x)
     return r;
 }
 
+// round-to-nearest-even(x) for |x| < 2^22 with one float addition: x + 1.5 2^23
+// lies in [2^23, 2^24), where floats are the integers, so its bits count x's
+// rounded value up from those of 1.5 2^23.  (A float add issues at twice the
+// rate of a float-to-int conversion.)  With x = w - 1/2 this is floor(w),
+// except at integer w, where it may give w - 1: the pair FP reads
+// piecewise-linear (e, d) entries, whose two segments agree at their shared
+// end, so either index gives the same value there.
+__device__ __forceinline__ int rne_index(float x) { return __float_as_int(x + 12582912.0f) - 0x4B400000; }
+
 // x - floor(x); exact for x >= 0 (the callers guarantee it).
 __device__ __forceinline__ float fract_pos(float x) { return __builtin_amdgcn_fractf(x); }
 
@@ -255,9 +264,10 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfl
 
 // Fast FP staging as (e, d) pairs.  For line l and window position p (texel
 // t_p at position pmin + p, zero outside the image):
-//   d = t_{p+1} - t_p,  e = t_p - (p - kFpPH) d,
+//   d = t_{p+1} - t_p,  e = t_p - (p - kFpPH - 1/2) d,
 // so the interpolated value at window coordinate w in [p, p + 1) is
-// e + (w - kFpPH) d: one FMA per sample, no fract and no second tap.  The
+// e + (w - kFpPH - 1/2) d: one FMA per sample, no fract and no second tap,
+// and the entry index is rne_index(w - kFpPH - 1/2), one float add.  The
 // origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpPW / 2.
 constexpr int kFpPW = UIRK_FP_WMAX; // widest pair window
 constexpr int kFpPL = 20;           // lines per pair-staged chunk (31 KB of LDS: 5 workgroups per CU)
@@ -294,7 +304,7 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], con
                 t0[i] = bload(r, img, c);
                 t1[i] = bload(r, img, c + 1);
             }
-            const float h = (float)(p - kFpPH);
+            const float h = (float)(p - kFpPH) - 0.5f; // entries for coordinates w - 1/2
 #pragma unroll
             for (int i = 0; i < kFpPL / 4; ++i) {
                 const float d = t1[i] - t0[i];
@@ -321,7 +331,7 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], con
 #pragma unroll
             for (int k = 0; k < 9; ++k)
                 t[k] = bload(r, img, i0 + k * N);
-            const float h0 = (float)(p0 - kFpPH);
+            const float h0 = (float)(p0 - kFpPH) - 0.5f; // entries for coordinates w - 1/2
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const float d = t[k + 1] - t[k];
@@ -730,9 +740,10 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 5) void k_fp_pairs(const uirk_s
             if (wave * APT >= item.na)
                 continue; // this wave has no angle of the item: it only stages
 
-            // Window coordinate relative to kFpPH; inactive rays sample a
-            // fixed in-window position with zero step and are never written.
-            const float off = 0.5f + (float)pmin + (float)kFpPH;
+            // Window coordinate relative to kFpPH, less 1/2 (rne_index of it is
+            // the sample's entry); inactive rays sample a fixed in-window
+            // position with zero step and are never written.
+            const float off = 1.0f + (float)pmin + (float)kFpPH;
             const float park = lo + 1.0f - off;
             float u[APT], st[APT];
 #pragma unroll
@@ -760,7 +771,7 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 5) void k_fp_pairs(const uirk_s
                         w[q][j] = w[q - 1][j] + st[j];
 #pragma unroll
                     for (int q = 0; q < kQ; ++q)
-                        v[q][j] = P0[q * kFpPs + cvt_flr(w[q][j])];
+                        v[q][j] = P0[q * kFpPs + rne_index(w[q][j])];
                 }
 #pragma unroll
                 for (int j = 0; j < APT; ++j) {
@@ -774,7 +785,7 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 5) void k_fp_pairs(const uirk_s
                 const float2 *P0 = P + l * kFpPs;
 #pragma unroll
                 for (int j = 0; j < APT; ++j) {
-                    const float2 v = P0[cvt_flr(u[j])];
+                    const float2 v = P0[rne_index(u[j])];
                     acc[j] = __builtin_fmaf(u[j], v.y, acc[j] + v.x);
                     u[j] += st[j];
                 }
