@@ -613,7 +613,9 @@ def main():
         ceil = uir.probe_ceilings(res)
         roof["lds"]["probe"] = {"tb_s": ceil["lds_tb_s"], "bytes_per_clock_cu": ceil["lds_bytes_per_clock_cu"],
                                 "fp32_fma_tflop_s": ceil["fp32_tflop_s"],
-                                "what": "measured on this device: a read-only LDS loop and an FMA loop"}
+                                "fma_wave64_per_clock_cu": ceil["fma_wave64_per_clock_cu"],
+                                "cvt_wave64_per_clock_cu": ceil["cvt_wave64_per_clock_cu"],
+                                "what": "measured on this device: a read-only LDS loop, an FMA loop and a float-to-int loop"}
     except uir.UirError as e:
         roof["lds"]["probe"] = {"error": str(e)}
 

@@ -15,4 +15,5 @@ def test_probe_ceilings_are_physical(res):
     assert c["n_cu"] > 0 and c["clock_ghz"] > 0
     assert 0 < c["lds_bytes_per_clock_cu"] <= 256, c
     assert 0 < c["fma_wave64_per_clock_cu"] <= 4, c
+    assert 0 < c["cvt_wave64_per_clock_cu"] <= 4, c
     assert c["lds_ms"] > 0 and c["fma_ms"] > 0

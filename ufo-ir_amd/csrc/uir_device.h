@@ -26,14 +26,15 @@ typedef struct {
 } uirk_runs;
 
 /* Measured ceilings of the projectors' binding resources on the current
- * device (uir_probe.hip): LDS read bytes and wave64 FP32 FMAs per clock per
- * CU, at the device's reported clock.  Returns 0, or 2 (no memory) / 3 (HIP
+ * device (uir_probe.hip): LDS read bytes, wave64 FP32 FMAs and wave64
+ * float-to-int conversions per clock per CU, at the device's reported clock.  Returns 0, or 2 (no memory) / 3 (HIP
  * error). */
 typedef struct {
     int n_cu;
     double clock_ghz;
     double lds_bytes_per_clock_cu;
     double fma_wave64_per_clock_cu;
+    double cvt_wave64_per_clock_cu; /* v_cvt_flr_i32_f32 */
     double lds_ms, fma_ms; /* one launch of each loop */
 } uirk_ceilings;
 int uirk_probe_ceilings(void *stream, uirk_ceilings *c);

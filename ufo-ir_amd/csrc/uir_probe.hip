@@ -2,11 +2,12 @@
 // bound the projectors: LDS reads and FP32 fused multiply-adds.
 //
 // The Joseph FP and BP read one 8-byte (e, d) pair from LDS per sample and
-// issue about five FP32 vector operations per sample, so their roofline is
-// set by the LDS read rate (bytes per clock per CU) and the FP32 issue rate
-// (wave64 FMAs per clock per CU), neither of which is a published figure for
-// this build's instruction mix (no packed FP32).  These loops measure both on
-// the device at hand.
+// issue about five vector operations per sample, so their roofline is set by
+// the LDS read rate (bytes per clock per CU) and the vector issue rates: FP32
+// adds and FMAs (wave64 instructions per clock per CU), and conversions,
+// which issue at a lower rate.  None of these is a published figure for this
+// build's instruction mix (no packed FP32).  These loops measure them on the
+// device at hand.
 #include <hip/hip_runtime.h>
 
 #include "uir_device.h"
@@ -66,6 +67,30 @@ __global__ __launch_bounds__(256) void k_probe_fma(float *__restrict__ out, int 
     out[blockIdx.x * 256 + threadIdx.x] = t;
 }
 
+// Eight independent chains of float-to-int conversions with floor (the
+// projectors' other VALU class: conversions, fract and shift-adds issue at
+// half the rate of float adds and FMAs).  Each operand is the previous
+// result's bits read as a float.
+__global__ __launch_bounds__(256) void k_probe_cvt(float *__restrict__ out, int iters, float seed)
+{
+    int b[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        b[j] = __float_as_int(seed * (float)(threadIdx.x + j));
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < kProbeUnroll; ++k)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(b[j]) : "v"(b[j]));
+    }
+    int t = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        t += b[j];
+    out[blockIdx.x * 256 + threadIdx.x] = (float)t;
+}
+
 } // namespace
 
 int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
@@ -87,7 +112,7 @@ int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
         return 2;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int rc = 0;
-    float ms_lds = 0.0f, ms_fma = 0.0f;
+    float ms_lds = 0.0f, ms_fma = 0.0f, ms_cvt = 0.0f;
     const int lds_iters = 4096, fma_iters = 2048, reps = 5;
     if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
         rc = 3;
@@ -108,6 +133,14 @@ int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
         if (rc == 0 &&
             (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms_fma, ev[0], ev[1]) != hipSuccess))
             rc = 3;
+        hipLaunchKernelGGL(k_probe_cvt, dim3(blocks), dim3(256), 0, st, out, fma_iters, 1.0f);
+        hipEventRecord(ev[0], st);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_probe_cvt, dim3(blocks), dim3(256), 0, st, out, fma_iters, 1.0f);
+        hipEventRecord(ev[1], st);
+        if (rc == 0 &&
+            (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms_cvt, ev[0], ev[1]) != hipSuccess))
+            rc = 3;
         if (rc == 0 && hipGetLastError() != hipSuccess)
             rc = 3;
     }
@@ -119,12 +152,14 @@ int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
         return rc;
     const double clocks_lds = (double)ms_lds / reps * (double)khz; // kHz = clocks per ms
     const double clocks_fma = (double)ms_fma / reps * (double)khz;
+    const double clocks_cvt = (double)ms_cvt / reps * (double)khz;
     const double lds_bytes = (double)blocks * 256 * lds_iters * kProbeUnroll * 8.0;
     const double fma_wave = (double)blocks * 4 * fma_iters * kProbeUnroll * 8.0;
     c->n_cu = cus;
     c->clock_ghz = khz / 1e6;
     c->lds_bytes_per_clock_cu = lds_bytes / cus / clocks_lds;
     c->fma_wave64_per_clock_cu = fma_wave / cus / clocks_fma;
+    c->cvt_wave64_per_clock_cu = fma_wave / cus / clocks_cvt; // same instruction count
     c->lds_ms = ms_lds / reps;
     c->fma_ms = ms_fma / reps;
     return 0;

@@ -93,7 +93,8 @@ class BatchInfo(C.Structure):
 
 class Ceilings(C.Structure):
     _fields_ = [("n_cu", C.c_int), ("clock_ghz", C.c_double), ("lds_bytes_per_clock_cu", C.c_double),
-                ("fma_wave64_per_clock_cu", C.c_double), ("lds_ms", C.c_double), ("fma_ms", C.c_double)]
+                ("fma_wave64_per_clock_cu", C.c_double), ("cvt_wave64_per_clock_cu", C.c_double),
+                ("lds_ms", C.c_double), ("fma_ms", C.c_double)]
 
 
 class AsdPocsParams(C.Structure):
@@ -553,6 +554,7 @@ def probe_ceilings(res):
             "lds_tb_s": round(c.lds_bytes_per_clock_cu * c.n_cu * hz / 1e12, 2),
             "fma_wave64_per_clock_cu": round(c.fma_wave64_per_clock_cu, 3),
             "fp32_tflop_s": round(c.fma_wave64_per_clock_cu * 64 * 2 * c.n_cu * hz / 1e12, 1),
+            "cvt_wave64_per_clock_cu": round(c.cvt_wave64_per_clock_cu, 3),
             "lds_ms": round(c.lds_ms, 3), "fma_ms": round(c.fma_ms, 3)}
 
 
