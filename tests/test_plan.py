@@ -2,7 +2,7 @@
 
 The planner cuts each direction run into angle blocks per 64-detector block
 so that every chunk's footprint fits the staging window: the planner bounds
-32-line chunks (the single-value kernel's), and the pair kernel's 20-line
+32-line chunks (the single-value kernel's), and the pair kernel's 16-line
 chunks lie inside them.  These tests
 recompute, in float64 and independently of the planner's own corner
 argument, what the FP kernels do with each item:
@@ -19,7 +19,7 @@ import pytest
 
 import uir
 
-LINES, DETS = (32, 20), 64  # chunk lines: single-value kernel, pair kernel
+LINES, DETS = (32, 16), 64  # chunk lines: single-value kernel, pair kernel
 
 
 class Item(C.Structure):

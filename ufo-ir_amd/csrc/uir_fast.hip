@@ -22,7 +22,7 @@
 // FP: a workgroup owns 64 detectors x up to 4 * APT angles of one direction
 //     family (the fast pair path: one angle and APT detectors per thread).
 //     It walks the image in chunks of lines (rows for the vertical family,
-//     columns for the horizontal one: 20 per chunk in the pair path, 32 in
+//     columns for the horizontal one: 16 per chunk in the pair path, 32 in
 //     the single-value one); each chunk's footprint is staged into LDS
 //     line-major (transposed for the horizontal family), so both families
 //     read their two taps as adjacent LDS words (one 8-byte pair in the pair
@@ -270,7 +270,7 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfl
 // and the entry index is rne_index(w - kFpPH - 1/2), one float add.  The
 // origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpPW / 2.
 constexpr int kFpPW = UIRK_FP_WMAX; // widest pair window
-constexpr int kFpPL = 20;           // lines per pair-staged chunk (31 KB of LDS: 5 workgroups per CU)
+constexpr int kFpPL = 16;           // lines per pair-staged chunk (25 KB of LDS: 6 workgroups per CU)
 constexpr int kFpPs = kFpPW + 1;    // pair line stride (odd)
 constexpr int kFpPH = kFpPW / 2;
 
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 //  - A sample costs an add (coordinate), a floor-convert, an address, one
 //    8-byte LDS read and two FMA-class operations.
 template <typename T, typename TI, int APT>
-__global__ __launch_bounds__(256, APT == 8 ? 4 : 5) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
+__global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                   const uirk_fp_item *__restrict__ items, int n_items,
                                                   uirk_xcd_part part, int *__restrict__ err)
 {
@@ -754,11 +754,11 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 5) void k_fp_pairs(const uirk_s
             }
             const float2 *P = lds + kFpPH;
             int l = 0;
-            // kQ lines per step (16 or 32 samples per thread), then single lines:
+            // kQ lines per step (8 or 16 samples per thread), then single lines:
             // the LDS reads of a step are all in flight before the first is
             // consumed.  The coordinates advance by the same float additions
             // whatever the grouping.
-            constexpr int kQ = APT >= 8 ? 2 : (APT >= 4 ? 4 : 8);
+            constexpr int kQ = APT >= 4 ? 2 : 8; // APT 4 fits 80 VGPRs (6 waves per SIMD) with 2
             for (; l + kQ <= nl; l += kQ) {
                 const float2 *P0 = P + l * kFpPs;
                 float2 v[kQ][APT];
