@@ -312,28 +312,30 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], con
             }
         }
     } else {
-        // line = column l0 + l (l = lane mod 32), position = row pmin + p.  A
-        // wave instruction covers one row of 32 columns in each half-wave
-        // (coalesced); a thread loads 9 consecutive rows for 8 pairs.  One
-        // resource over the image: rows outside [0, N) read as zero; columns
-        // at or beyond N occur only for lines l >= nl.
-        const int l = lane & 31;
+        // line = column l0 + l (l = lane mod kFpPL), position = row pmin + p.
+        // A wave instruction covers kFpPL consecutive columns of 64 / kFpPL
+        // rows (coalesced per row), every lane busy; a thread loads kPer + 1
+        // consecutive rows for kPer pairs.  One resource over the image: rows
+        // outside [0, N) read as zero; columns at or beyond N occur only for
+        // lines l >= nl.
+        static_assert(64 % kFpPL == 0, "lanes split evenly over the staged lines");
+        constexpr int kSub = 64 / kFpPL;  // row groups per wave instruction
+        constexpr int kPer = 16 / kSub;   // pairs per thread per 16 positions of a wave
+        const int l = lane % kFpPL;
         const int col = l0 + l;
         const __amdgpu_buffer_rsrc_t r = buf_rsrc(img, (uint32_t)N * (uint32_t)N);
         for (int pb = 0; pb < W; pb += 64) {
             if (pb + 16 * wave >= W)
                 break;
-            if (l >= kFpPL)
-                continue; // lanes past the staged lines (one branch around all of a lane's loads)
-            const int p0 = pb + 16 * wave + 8 * (lane >> 5);
+            const int p0 = pb + 16 * wave + kPer * (lane / kFpPL);
             const int i0 = (pmin + p0) * N + col;
-            float t[9];
+            float t[kPer + 1];
 #pragma unroll
-            for (int k = 0; k < 9; ++k)
+            for (int k = 0; k <= kPer; ++k)
                 t[k] = bload(r, img, i0 + k * N);
             const float h0 = (float)(p0 - kFpPH) - 0.5f; // entries for coordinates w - 1/2
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < kPer; ++k) {
                 const float d = t[k + 1] - t[k];
                 lds[l * kFpPs + p0 + k] = make_float2(__builtin_fmaf(-(h0 + (float)k), d, t[k]), d);
             }
