@@ -25,9 +25,13 @@ instead (image edge N in {512, 1024, 2048}, angles U{90..900}, bins
 ceil(N sqrt 2), lambda ~ logU(1e-3, 1e-1), mu = 1, 30 outer x 5 inner sweeps,
 FP16 storage) and prints its own JSON line.
 
-Every batched or served request uses ray_weight_min = 0.1: rays that cross
-less than a tenth of a pixel (image-corner clips) get weight 0; the FP64
-reference uses the same floor.
+Ray-weight floor: --ray-weight-min (default 0, the reference's inv: every
+ray with FP(1) > 0 is weighted 1 / FP(1)).  With a floor, rays that cross
+less than that fraction of a pixel (image-corner clips) get weight 0, and the
+FP64 reference uses the same floor.  The accuracy leg reports the batched
+path against the FP64 SIRT both without the floor and with 0.1, and the line
+carries an explicit gate: rel-L2 <= 1e-3 against the FP64 SIRT of the
+headline's own setting.
 
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
@@ -203,7 +207,7 @@ def batch_c3(args, uir):
         n = sizes[rng.integers(0, len(sizes))]
         a = int(rng.integers(180, 1801))
         specs.append(dict(n=n, n_dets=math.ceil(n * math.sqrt(2)), n_angles=a, step=float(np.float32(np.pi / a)),
-                          relaxation=0.25, ray_weight_min=RAY_WEIGHT_MIN))
+                          relaxation=0.25))
     # sinograms: one random-ellipse phantom per image size, projected at each
     # slice's own angles by the reference-compatible GPU projector
     t0 = time.time()
@@ -228,7 +232,7 @@ def batch_c3(args, uir):
     for i in check:
         sp = specs[i]
         g = O.Geometry(sp["n"], sp["n_dets"], sp["n_angles"], np.float32(sp["step"]))
-        refs[i] = O.sirt64(g, sinos[i].astype(np.float64), args.c3_iters, 0.25, ray_weight_min=RAY_WEIGHT_MIN)
+        refs[i] = O.sirt64(g, sinos[i].astype(np.float64), args.c3_iters, 0.25)
     t_ref = time.time() - t0
     print("c3: FP64 references of %d slices in %.1f s" % (len(check), t_ref), file=sys.stderr, flush=True)
     out = {}
@@ -419,6 +423,8 @@ def main():
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--dets", type=int, default=1448)
     ap.add_argument("--accuracy-iters", type=int, default=10)
+    ap.add_argument("--ray-weight-min", type=float, default=0.0,
+                    help="ray-weight floor of the batched and served SIRT (0: the reference's inv)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 accuracy check")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-qps", type=float, default=100.0)
@@ -502,7 +508,8 @@ def main():
                          *(int(v) for v in args.tile.split(",")))
         else:
             uir.tune_geometry(res, dict(n=n, n_dets=d, n_angles=a, step=step), args.storage, reps=3)
-        specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
+        specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax,
+                      ray_weight_min=args.ray_weight_min)] * args.batch
         bt = uir.Batch(res, specs, args.storage)
         for i in range(args.batch):
             bt.set_sinogram(i, sinos[i % n_distinct])
@@ -547,15 +554,23 @@ def main():
         tune = {"best": {"fp_angles": fa, "bp_angles": ba, "bp_rows": br}}
     else:
         tune = uir.tune_geometry(res, geo, args.storage, reps=3)
-    specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, ray_weight_min=RAY_WEIGHT_MIN)] * args.batch
+    specs = [dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax,
+                  ray_weight_min=args.ray_weight_min)] * args.batch
     bt = uir.Batch(res, specs, args.storage)
     for i in range(args.batch):
         bt.set_sinogram(i, sinos[i % n_distinct])
     info = bt.info()
     fp_ms, bp_ms = bt.time_kernels(5)
-    bt.reset()
-    bt.iterate(args.accuracy_iters, use_graph=False)
-    batched_img = bt.image(0)
+    # accuracy images: slice 0 without the floor (the reference's inv) and with
+    # 0.1, each from a one-slice batch of the same storage and tile shape
+    acc_imgs = {}
+    for rmin in sorted({0.0, 0.1, args.ray_weight_min}):
+        b1 = uir.Batch(res, [dict(specs[0], ray_weight_min=rmin)], args.storage)
+        b1.set_sinogram(0, sinos[0])
+        b1.reset()
+        b1.iterate(args.accuracy_iters, use_graph=False)
+        acc_imgs[rmin] = b1.image(0)
+        b1.close()
     bt.reset()
     bt.iterate(1)                         # capture the one-iteration graph
     for _ in range(args.warmup):
@@ -593,12 +608,12 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch_b%d" % args.batch)
     if tfl / FP32_PEAK_TFLOPS >= gbs / HBM_PEAK_GBS:
-        roof = {"bound": "mfma", "achieved": round(tfl, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tfl / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+        roof = {"bound": "valu", "binding": "lds", "achieved": round(tfl, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tfl / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_fp_pairs" if kname == "fp" else "k_bp_tab",
-                "note": "FP32 vector ALU work (no MFMA: no contraction), 4 flop per interpolation sample; "
-                        "hbm frac %.4f at %.1f GB/s algorithmic; the binding resource is LDS reads, "
-                        "one 8-byte (e, d) pair per sample (see lds)" % (gbs / HBM_PEAK_GBS, gbs)}
+                "note": "FP32 vector ALU work on the FP32 vector peak (no MFMA: no contraction), 4 flop per "
+                        "interpolation sample; hbm frac %.4f at %.1f GB/s algorithmic; the binding resource is "
+                        "LDS reads, one 8-byte (e, d) pair per sample (see lds)" % (gbs / HBM_PEAK_GBS, gbs)}
     else:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -635,17 +650,26 @@ def main():
         cpu = {"value": round(cpu_rate, 4), "unit": "slice-iters/s", "cores": O.num_threads(), "kind": "port",
                "sample": "1 slice at 1024x1024x720x1448, ref-compat FP32 SIRT, (t(3 it) - t(1 it)) / 2"}
         t1 = time.time()
-        ref64 = O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax, ray_weight_min=RAY_WEIGHT_MIN)
+        ref64 = {rmin: O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax, ray_weight_min=rmin)
+                 for rmin in acc_imgs}
         t_ref = time.time() - t1
-        ref64_plain = O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax)
+        st = args.storage
         acc = {"iterations": args.accuracy_iters,
-               "batched_%s_rel_l2_vs_fp64" % args.storage: O.rel_l2(batched_img, ref64),
-               "batched_%s_psnr_db" % args.storage: round(O.psnr(batched_img, ref64), 2),
-               "batched_ray_weight_min": RAY_WEIGHT_MIN,
-               "plain_fp32_rel_l2_vs_fp64": O.rel_l2(plain_img, ref64_plain),
-               "plain_fp32_psnr_db": round(O.psnr(plain_img, ref64_plain), 2),
+               "batched_%s_rel_l2_vs_fp64" % st: O.rel_l2(acc_imgs[args.ray_weight_min], ref64[args.ray_weight_min]),
+               "batched_%s_psnr_db" % st: round(O.psnr(acc_imgs[args.ray_weight_min], ref64[args.ray_weight_min]), 2),
+               "batched_ray_weight_min": args.ray_weight_min,
+               "batched_%s_rel_l2_vs_fp64_unfloored" % st: O.rel_l2(acc_imgs[0.0], ref64[0.0]),
+               "batched_%s_psnr_db_unfloored" % st: round(O.psnr(acc_imgs[0.0], ref64[0.0]), 2),
+               "batched_%s_rel_l2_vs_fp64_floor_0.1" % st: O.rel_l2(acc_imgs[0.1], ref64[0.1]),
+               "unfloored_reference": "FP64 SIRT with the reference's inv (ufo-ir-sirt-task.c:180-185)",
+               "plain_fp32_rel_l2_vs_fp64": O.rel_l2(plain_img, ref64[0.0]),
+               "plain_fp32_psnr_db": round(O.psnr(plain_img, ref64[0.0]), 2),
                "plain_reference": "FP64 SIRT without a ray weight floor (the reference's inv)",
                "fp64_oracle_s": round(t_ref, 1)}
+        gate_err = acc["batched_%s_rel_l2_vs_fp64" % st]
+        acc["gate"] = {"bar": 1e-3, "rel_l2": gate_err, "passed": bool(gate_err <= 1e-3),
+                       "what": "batched result at the headline's ray-weight floor against the FP64 SIRT with "
+                               "the same floor, %d iterations" % args.accuracy_iters}
 
     # ---- serving: Poisson arrivals of SIRT requests of the same shape
     serving = None
@@ -653,7 +677,7 @@ def main():
         del bt
         def make(k):
             return (uir.make_request(n, d, a, step, "sirt", iterations=args.serve_iters, relaxation=relax,
-                                     ray_weight_min=RAY_WEIGHT_MIN), sinos[k % n_distinct])
+                                     ray_weight_min=args.ray_weight_min), sinos[k % n_distinct])
         serving = run_serving(uir, args.storage, make, args.serve_qps, args.serve_requests, 20261204)
         serving["workload"] = "SIRT requests of %dx%d, %d angles, %d bins, %d iterations each, Poisson arrivals" % (
             n, n, a, d, args.serve_iters)

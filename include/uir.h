@@ -398,7 +398,8 @@ uir_status uir_server_submit(uir_server *s, const uir_request *req, const float 
  * UIR_ERR_TIMEOUT leaves the ticket valid; a failed request returns its error. */
 uir_status uir_server_wait(uir_server *s, uint64_t ticket, float *image, double timeout_s, uir_request_stats *stats);
 uir_status uir_server_get_info(uir_server *s, uir_server_info *info);
-/* Device bytes a request holds while admitted (its footprint for admission). */
+/* Device bytes a request holds while admitted (its footprint for admission).
+ * The request is validated as uir_server_submit validates it. */
 uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint64_t *bytes);
 /* Cancels queued requests (their waits return UIR_ERR_CANCELLED), finishes
  * the admitted ones, joins the worker and frees everything.  No other thread

@@ -139,7 +139,7 @@ def test_batch_asdpocs_half_images(res, storage, tol):
         assert err <= tol
 
 
-@pytest.mark.parametrize("shift", [4.5, -6.0])
+@pytest.mark.parametrize("shift", [4.5, -6.0, 3.3])
 def test_batch_asdpocs_shifted_axis(res, shift):
     """The fused SART step and the residual rows with axis_position off the
     detector centre (inactive detectors, shrunken BP window)."""

@@ -147,11 +147,13 @@ def test_batch_rejects_bad_specs(res):
         uir.Batch(res, [_spec(64, 91, 30, relax=1.5)], "f32")
 
 
-@pytest.mark.parametrize("shift", [4.5, -6.0])
+@pytest.mark.parametrize("shift", [4.5, -6.0, 4.3, -6.7])
 def test_batch_shifted_axis_matches_fp64(res, shift):
     """axis_position off the detector centre: the detectors the reference
     skips (projector-parallel-joseph.cl:62-66) carry no weight, and the BP
-    window shrinks (:180-182), on both sides."""
+    window shrinks (:180-182), on both sides.  Shifts of a multiple of 1/2
+    put the clamp edges on bin centres (the BP clamps bins); other shifts put
+    them inside a bin (the BP clamps the coordinate, :203-210)."""
     iters = 10
     n, d, a = GEOMS[0]
     axis = d / 2.0 + shift

@@ -62,7 +62,7 @@ def _tiff(byteorder, w, h, fmt, bits, data_bytes, tiles=None, rows_per_strip=Non
         ent(325, 4, [len(b) for b in blocks])
     else:
         ent(273, 4, offs)
-        ent(278, 4, [rows_per_strip or h])
+        ent(278, 4, [h if rows_per_strip is None else rows_per_strip])
         ent(279, 4, [len(b) for b in blocks])
     ent(277, 3, [1])
     ent(339, 3, [fmt])
@@ -113,6 +113,20 @@ def test_tiff_reads_tiles_and_strips(tmp_path):
     bad.write_bytes(_tiff("II", 35, 20, 3, 32, img.astype("<f4").tobytes(), rows_per_strip=3))
     with pytest.raises(uir.UirError):
         uir.tiff_read(str(bad))
+
+
+def test_tiff_strip_rows_edge_values(tmp_path):
+    img = np.arange(35 * 20, dtype=np.float32).reshape(20, 35)
+    # RowsPerStrip 0 is invalid (TIFF 6.0: at least 1): an error, not a crash
+    z = tmp_path / "z.tif"
+    z.write_bytes(_tiff("II", 35, 20, 3, 32, img.astype("<f4").tobytes(), rows_per_strip=0))
+    with pytest.raises(uir.UirError) as e:
+        uir.tiff_read(str(z))
+    assert "RowsPerStrip" in str(e.value)
+    # a huge RowsPerStrip means one strip (no 32-bit overflow of the row count)
+    h = tmp_path / "h.tif"
+    h.write_bytes(_tiff("II", 35, 20, 3, 32, img.astype("<f4").tobytes(), rows_per_strip=0xFFFFFFF0))
+    assert np.array_equal(uir.tiff_read(str(h)), img)
 
 
 def test_tiff_rejects_garbage(tmp_path):

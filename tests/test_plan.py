@@ -149,3 +149,41 @@ def test_wide_detector_blocks_cut_only_where_needed():
     centre = [mean[d0] for d0 in mean if abs(d0 + 32 - d / 2) < 256]
     edge = [mean[d0] for d0 in mean if abs(d0 + 32 - d / 2) > 600]
     assert min(centre) > 14.0 and min(edge) < min(centre)
+
+
+def _active_bins(d, axis):
+    L = uir.load()
+    L.uir_geom_active_bins.restype = C.c_int
+    L.uir_geom_active_bins.argtypes = [C.c_uint32, C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lo, hi = C.c_int(), C.c_int()
+    ok = L.uir_geom_active_bins(d, axis, C.byref(lo), C.byref(hi))
+    return ok, lo.value, hi.value
+
+
+@pytest.mark.parametrize("d", [363, 1448])
+def test_active_bins_window(d):
+    # projector-parallel-joseph.cl:174-182: a centred axis activates every bin;
+    # a shifted axis drops bins on the side away from it (2 |axis - D/2| of them)
+    assert _active_bins(d, d / 2) == (1, 0, d - 1)
+    ok, lo, hi = _active_bins(d, d / 2 + 10.0)
+    assert ok and lo == 20 and hi == d - 1
+    ok, lo, hi = _active_bins(d, d / 2 - 10.0)
+    assert ok and lo == 0 and hi == d - 1 - 20
+    # the extreme axes that still leave one detector active
+    assert _active_bins(d, 0.5)[0] == 1 and _active_bins(d, d - 0.5)[0] == 1
+    # axes that leave none: the fast kernels' staging would clamp to an empty
+    # range, so the batch and serving paths refuse them
+    for axis in (0.0, 0.25, d - 0.25, float(d), 1.5 * d):
+        ok, lo, hi = _active_bins(d, axis)
+        assert not ok, (axis, lo, hi)
+
+
+@pytest.mark.parametrize("axis", [0.0, 0.25, 363.0, 600.0])
+def test_request_with_no_active_detector_is_refused(axis):
+    req = uir.make_request(256, 363, 180, float(np.pi / 180), axis_position=axis)
+    with pytest.raises(uir.UirError) as e:
+        uir.request_footprint(req)
+    assert "no active detector" in str(e.value)
+    # the default (-1, centred) and an in-range shift are accepted
+    assert uir.request_footprint(uir.make_request(256, 363, 180, float(np.pi / 180))) > 0
+    assert uir.request_footprint(uir.make_request(256, 363, 180, float(np.pi / 180), axis_position=190.0)) > 0

@@ -853,9 +853,17 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
         const float origin_shift = (float)N / 2.0f;
         const float shift = det_shift - rot;
         const float fX = (float)px + 0.5f - origin_shift;
-        // active bins for the fast path's clamp-to-edge staging
-        const int bin_lo = (int)floorf(edge0 - 0.5f);
-        const int bin_hi = (int)floorf(edge1 - 0.5f);
+        // active bins for the fast path's clamp-to-edge staging, kept inside
+        // [0, D - 1]: an axis near 0 or past D leaves the active window empty
+        // or outside the detector, and every staging load must stay in the row
+        const int bin_lo = min(max((int)floorf(edge0 - 0.5f), 0), D - 1);
+        const int bin_hi = min(max((int)floorf(edge1 - 0.5f), 0), D - 1);
+        // Clamping the bins equals the reference's clamp of u (:203-210) when
+        // both edges sit on bin centres (edge - 0.5 an integer: a centred axis,
+        // or one shifted by a multiple of 1/2).  Otherwise an edge falls inside
+        // a bin, and the fast path clamps the coordinate itself (one more
+        // operation per sample; uniform over the slice).
+        const bool clamp_u = !REF && ((edge0 - 0.5f) != floorf(edge0 - 0.5f) || (edge1 - 0.5f) != floorf(edge1 - 0.5f));
         // tile corner coordinates (pixel centres)
         const float cx0 = (float)item.x0 + 0.5f - origin_shift;
         const float cx1 = cx0 + (float)(kBpT - 1);
@@ -891,7 +899,7 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                     const float u01 = cx0 * ct - cy1 * st, u11 = cx1 * ct - cy1 * st;
                     float umn = fminf(fminf(u00, u10), fminf(u01, u11)) + shift;
                     float umx = fmaxf(fmaxf(u00, u10), fmaxf(u01, u11)) + shift;
-                    if (REF) {
+                    if (REF || clamp_u) {
                         umn = fminf(fmaxf(umn, edge0), edge1);
                         umx = fminf(fmaxf(umx, edge0), edge1);
                     }
@@ -921,9 +929,11 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                             const float t = gload(sino, rowo + (uint32_t)bc);
                             v0[e] = bin == bc ? t : 0.0f;
                         } else {
-                            // (t[i], t[i+1]) with clamp-to-edge bins
-                            const int b0 = bin < bin_lo ? bin_lo : (bin > bin_hi ? bin_hi : bin);
-                            const int b1 = bin + 1 < bin_lo ? bin_lo : (bin + 1 > bin_hi ? bin_hi : bin + 1);
+                            // (t[i], t[i+1]) with clamp-to-edge bins; with clamp_u the
+                            // coordinate is clamped instead, and the bins only to the row
+                            const int lo = clamp_u ? 0 : bin_lo, hi = clamp_u ? D - 1 : bin_hi;
+                            const int b0 = bin < lo ? lo : (bin > hi ? hi : bin);
+                            const int b1 = bin + 1 < lo ? lo : (bin + 1 > hi ? hi : bin + 1);
                             v0[e] = gload(sino, rowo + (uint32_t)b0);
                             v1[e] = gload(sino, rowo + (uint32_t)b1);
                         }
@@ -976,19 +986,39 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                         const float2 *P = ldp + k * kBpWb;
                         float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f)) - (float)s_umin[k];
                         const float du = -4.0f * st;
+                        if (!clamp_u) {
 #pragma unroll
-                        for (int g = 0; g < ROWS; g += 8) {
-                            float2 p[8];
-                            float w[8];
+                            for (int g = 0; g < ROWS; g += 8) {
+                                float2 p[8];
+                                float w[8];
 #pragma unroll
-                            for (int h = 0; h < 8; ++h) {
-                                w[h] = __builtin_fmaf((float)h, du, u);
-                                p[h] = P[cvt_flr(w[h])];
+                                for (int h = 0; h < 8; ++h) {
+                                    w[h] = __builtin_fmaf((float)h, du, u);
+                                    p[h] = P[cvt_flr(w[h])];
+                                }
+#pragma unroll
+                                for (int h = 0; h < 8; ++h)
+                                    acc[g + h] = __builtin_fmaf(w[h], p[h].y, acc[g + h] + p[h].x);
+                                u = __builtin_fmaf(8.0f, du, u);
                             }
+                        } else {
+                            // u clamped to [edge0, edge1] (:203-210), in window coordinates
+                            const float wlo = (edge0 - 0.5f) - (float)s_umin[k];
+                            const float whi = (edge1 - 0.5f) - (float)s_umin[k];
 #pragma unroll
-                            for (int h = 0; h < 8; ++h)
-                                acc[g + h] = __builtin_fmaf(w[h], p[h].y, acc[g + h] + p[h].x);
-                            u = __builtin_fmaf(8.0f, du, u);
+                            for (int g = 0; g < ROWS; g += 8) {
+                                float2 p[8];
+                                float w[8];
+#pragma unroll
+                                for (int h = 0; h < 8; ++h) {
+                                    w[h] = __builtin_amdgcn_fmed3f(__builtin_fmaf((float)h, du, u), wlo, whi);
+                                    p[h] = P[cvt_flr(w[h])];
+                                }
+#pragma unroll
+                                for (int h = 0; h < 8; ++h)
+                                    acc[g + h] = __builtin_fmaf(w[h], p[h].y, acc[g + h] + p[h].x);
+                                u = __builtin_fmaf(8.0f, du, u);
+                            }
                         }
                     }
                 }
@@ -1107,25 +1137,28 @@ __global__ __launch_bounds__(256) void k_sart_step(const uirk_sart_slice *__rest
     // BP of bp_angle (projector-parallel-joseph.cl:174-190 geometry, clamp-to-edge bins)
     const bool do_bp = bp_angle >= 0 && bp_angle < S->n_angles;
     float bst = 0.0f, bct = 0.0f, gain = 0.0f;
-    int bin_lo = 0, bin_hi = 0;
+    float w_lo = 0.0f, w_hi = 0.0f;
     const T *row = nullptr;
     if (do_bp) {
         bst = gload(trig, 2 * bp_angle);
         bct = gload(trig, 2 * bp_angle + 1);
         const float half_active = diff < 0 ? (float)D - axis_f : axis_f;
-        bin_lo = (int)floorf(axis_f - half_active + 0.5f - 0.5f);
-        bin_hi = (int)floorf(axis_f + half_active - 0.5f - 0.5f);
+        // the clamp window of u - 0.5 (:180-182, :203-210)
+        w_lo = (axis_f - half_active + 0.5f) - 0.5f;
+        w_hi = (axis_f + half_active - 0.5f) - 0.5f;
         gain = state[blockIdx.y].relax;
         row = uniform((const T *)S->r) + (size_t)bp_angle * D;
     }
     const float shift = (float)D / 2.0f - diff / 2.0f;
-    // The BP taps of one pixel: u = X cos - Y sin + axis - 0.5.
+    // The BP taps of one pixel: u = X cos - Y sin + axis - 0.5, clamped to the
+    // active window; the bins are also kept inside the row whatever the axis.
     auto bp_taps = [&](int px, int py, float &a, float &t0, float &t1) {
         const float fX = (float)px + 0.5f - origin_shift, fY = (float)py + 0.5f - origin_shift;
-        const float u = __builtin_fmaf(-fY, bst, __builtin_fmaf(fX, bct, shift - 0.5f));
+        const float u = __builtin_amdgcn_fmed3f(__builtin_fmaf(-fY, bst, __builtin_fmaf(fX, bct, shift - 0.5f)),
+                                                w_lo, w_hi);
         const int bin = cvt_flr(u);
         a = u - floorf(u);
-        const int b0 = min(max(bin, bin_lo), bin_hi), b1 = min(max(bin + 1, bin_lo), bin_hi);
+        const int b0 = min(max(bin, 0), D - 1), b1 = min(max(bin + 1, 0), D - 1);
         t0 = gload(row, (uint32_t)b0);
         t1 = gload(row, (uint32_t)b1);
     };

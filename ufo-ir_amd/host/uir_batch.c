@@ -109,6 +109,10 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: relaxation outside [0, 1]", i);
         if (!(s->step >= 0.0f) || !(s->axis_position >= -1.0f))
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad step or axis", i);
+        if (!uir_geom_active_bins(s->n_dets, uir_geom_axis(s->axis_position, s->n_dets), NULL, NULL))
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT,
+                                 "slice %u: axis %g leaves no active detector (needs 0.5 <= axis <= n_dets - 0.5)", i,
+                                 (double)s->axis_position);
         if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad ray_weight_min", i);
     }

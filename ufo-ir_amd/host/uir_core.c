@@ -413,6 +413,28 @@ float uir_geom_axis(float axis_property, uint32_t n_dets)
     return axis_property;
 }
 
+/* The active detector window of the back projection (projector-parallel-joseph.cl:
+ * 174-182): u is clamped to [edge0, edge1], so the bins it interpolates lie in
+ * [floor(edge0 - 0.5), floor(edge1 - 0.5)].  edge0 >= 0.5 and edge1 <= D - 0.5
+ * for every axis, so the window lies inside [0, D - 1] when it is not empty;
+ * it is empty for an axis below 0.5 or above D - 0.5 (the rotation axis
+ * outside the first or last detector's centre).  Returns 1 for a non-empty
+ * window, 0 otherwise. */
+int uir_geom_active_bins(uint32_t n_dets, float axis, int *lo, int *hi)
+{
+    const float required_width = axis * 2;
+    const float diff = (float)n_dets - required_width;
+    const float half_active = diff < 0 ? (float)n_dets - axis : axis;
+    const float edge0 = axis - half_active + 0.5f;
+    const float edge1 = axis + half_active - 0.5f;
+    const int l = (int)floorf(edge0 - 0.5f), h = (int)floorf(edge1 - 0.5f);
+    if (lo)
+        *lo = l;
+    if (hi)
+        *hi = h;
+    return edge0 <= edge1 && l <= h && l >= 0 && h <= (int)n_dets - 1;
+}
+
 /* ---- kernel-level ABI ---- */
 
 static uir_workspace g_default_ws[64];

@@ -86,6 +86,7 @@ int uir_fp_single_for(const uir_slice_spec *spec);
 void uir_geom_lut(uint32_t n_angles, float step, float *sin_lut, float *cos_lut);
 uint32_t uir_geom_runs(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uir_subset *out);
 float uir_geom_axis(float axis_property, uint32_t n_dets);
+int uir_geom_active_bins(uint32_t n_dets, float axis, int *lo, int *hi);
 
 #ifdef __cplusplus
 }

@@ -169,10 +169,13 @@ static uint64_t req_bytes(const uir_request *r, size_t ess, size_t esi)
 
 static uint64_t geom_bytes(uint32_t D, uint32_t A) { return align_up((size_t)A * D * 4) + align_up((size_t)A * 8); }
 
+static uir_status check_request(const uir_request *r);
+
 uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint64_t *bytes)
 {
     if (!req || !bytes)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    TRY(check_request(req));
     int sdt = 0, idt = 0;
     if (!uir_storage_split(storage, &sdt, &idt))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
@@ -190,6 +193,10 @@ static uir_status check_request(const uir_request *r)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "image or sinogram of 4 GiB or more");
     if (!(s->step >= 0.0f) || !(s->axis_position >= -1.0f))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad step or axis");
+    if (!uir_geom_active_bins(s->n_dets, uir_geom_axis(s->axis_position, s->n_dets), NULL, NULL))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT,
+                             "axis %g leaves no active detector (needs 0.5 <= axis <= n_dets - 0.5)",
+                             (double)s->axis_position);
     if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad ray_weight_min");
     if (r->iterations == 0)
@@ -674,10 +681,8 @@ static uir_status req_finish(uir_server *s, sreq *q)
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
     TRY(uir_check_hip(hipMemcpyAsync(q->h_image, s->d_tmp, nv * 4, hipMemcpyDeviceToHost, s->stream), "download"));
     TRY(uir_check_hip(hipStreamSynchronize(s->stream), "hipStreamSynchronize"));
-    if (err) {
-        (void)hipMemsetAsync(s->d_err, 0, sizeof(int), s->stream);
+    if (err) /* the server then fails every request of those steps and stops */
         return uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window (flag %d)", err);
-    }
     return UIR_OK;
 }
 
@@ -809,7 +814,13 @@ static void *worker(void *arg)
                     s->active[keep++] = q;
                     continue;
                 }
-                uir_status rs = req_finish(s, q);
+                /* The device flag is shared by every request of the steps since
+                 * the last read: once it is found set, every request retiring in
+                 * this pass fails too (the active ones fail with the server). */
+                uir_status rs = st == UIR_ERR_DEVICE
+                                    ? uir_set_error(UIR_ERR_DEVICE, "projector footprint exceeded the staging window "
+                                                                    "in a step this request took part in")
+                                    : req_finish(s, q);
                 in_flight = 0;
                 char msg[256];
                 snprintf(msg, sizeof msg, "%s", rs == UIR_OK ? "" : uir_last_error());

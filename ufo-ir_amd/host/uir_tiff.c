@@ -273,6 +273,18 @@ uir_status uir_tiff_read(const char *path, uint32_t page, float *dst, size_t cap
         s = uir_set_error(UIR_ERR_UNSUPPORTED, "%s: planar multi-sample TIFF is not supported", path);
         goto out;
     }
+    if (d.w == 0 || d.h == 0) {
+        s = uir_set_error(UIR_ERR_INVALID_ARGUMENT, "%s: empty page (%ux%u)", path, d.w, d.h);
+        goto out;
+    }
+    if ((d.tile_w == 0) != (d.tile_h == 0)) {
+        s = uir_set_error(UIR_ERR_INVALID_ARGUMENT, "%s: TileWidth %u with TileLength %u", path, d.tile_w, d.tile_h);
+        goto out;
+    }
+    if (!d.tile_w && d.rows_per_strip == 0) {
+        s = uir_set_error(UIR_ERR_INVALID_ARGUMENT, "%s: RowsPerStrip is 0", path);
+        goto out;
+    }
     {
         const size_t bps = d.bits / 8, px = bps * d.spp;
         const int tiled = d.tile_w && d.tile_h;
@@ -294,18 +306,19 @@ uir_status uir_tiff_read(const char *path, uint32_t page, float *dst, size_t cap
             for (uint32_t bx = 0; bx < across && s == UIR_OK; ++bx) {
                 const uint32_t k = by * across + bx;
                 const uint32_t boff = arr_get(&t, d.off_pos, d.off_type, k, &err);
-                const uint32_t rows = tiled ? bh : ((by + 1) * bh > d.h ? d.h - by * bh : bh);
+                const uint32_t rows =
+                    tiled ? bh : ((uint64_t)(by + 1) * bh > d.h ? (uint32_t)(d.h - (uint64_t)by * bh) : bh);
                 const size_t need = (size_t)bw * rows * px;
                 if (err || rd_at(&t, boff, blk, need) != 0) {
                     s = uir_set_error(UIR_ERR_INVALID_ARGUMENT, "%s: truncated image data", path);
                     break;
                 }
                 for (uint32_t y = 0; y < rows; ++y) {
-                    const uint32_t gy = by * bh + y;
+                    const uint64_t gy = (uint64_t)by * bh + y;
                     if (gy >= d.h)
                         break;
                     for (uint32_t x = 0; x < bw; ++x) {
-                        const uint32_t gx = bx * bw + x;
+                        const uint64_t gx = (uint64_t)bx * bw + x;
                         if (gx >= d.w)
                             break;
                         dst[(size_t)gy * d.w + gx] = sample(&t, blk + ((size_t)y * bw + x) * px, d.bits, d.fmt);
