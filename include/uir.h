@@ -486,6 +486,17 @@ uir_status uir_op_dot_compat(const float *a, const float *b, uint32_t width, uin
                              float *result, void *stream);
 uir_status uir_op_l2_compat(const float *a, uint32_t width, uint32_t height, float *result, void *stream);
 uir_status uir_op_dot_f64(const float *a, const float *b, size_t n, double *result, void *stream);
+/* uir_op_dot_compat without allocation or synchronisation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under hipGraph
+ * capture, tight loops such as the split-Bregman CGS, harnesses that forbid
+ * hipMalloc).  Same kernels and bit-identical result as uir_op_dot_compat,
+ * written as a float to the caller's device memory d_result, enqueued on
+ * stream and not waited for.  The row partials live in a 256 KiB device array
+ * the library holds per device for the life of the process, so height is at
+ * most 65536 (larger: UIR_ERR_INVALID_ARGUMENT); calls that overlap on one
+ * device race on that array. */
+uir_status uir_op_dot_compat_noalloc(const float *a, const float *b, uint32_t width, uint32_t height,
+                                     float *d_result, void *stream);
 
 /* Dx / Dxt / Dy / Dyt, ufo-ir-gradient-processor.cl:1-51 (periodic). */
 uir_status uir_op_dx(const float *in, float *out, uint32_t width, uint32_t height, void *stream);

@@ -709,3 +709,72 @@ int uirk_l1_grad(const float *in, float *out, uint32_t w, uint32_t h, void *stre
 }
 
 } // extern "C"
+
+// ==========================================================================
+// Allocation-free strided "dot" (uir_op_dot_compat_noalloc)
+// ==========================================================================
+// The same two kernels as uir_op_dot_compat (one float partial per row, then
+// the partials added in row order), with the row partials in this file-scope
+// device array instead of a workspace the library allocates, and the result
+// written to the caller's device memory without a host synchronisation.  Each
+// device holds its own copy of the array for the life of the process.  Every
+// partial the final kernel reads is written by the same call's first kernel,
+// so nothing needs clearing between calls; calls that overlap on one device
+// race on the array.
+constexpr uint32_t kNoAllocMaxRows = 65536; // 256 KiB of FP32 partials
+static __device__ __align__(256) float scratch[kNoAllocMaxRows];
+
+#include "uir_internal.h"
+
+namespace {
+constexpr int kMaxDevices = 64;
+float *g_scratch_dev[kMaxDevices]; // the array's device address, per device
+pthread_mutex_t g_scratch_lock = PTHREAD_MUTEX_INITIALIZER;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name is not a device address).
+uir_status scratch_address(float **out)
+{
+    int dev = -1;
+    uir_status s = uir_check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (s != UIR_OK)
+        return s;
+    if (dev < 0 || dev >= kMaxDevices)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, kMaxDevices);
+    pthread_mutex_lock(&g_scratch_lock);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        s = uir_check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (s == UIR_OK && !p)
+            s = uir_set_error(UIR_ERR_DEVICE, "hipGetSymbolAddress(scratch) gave NULL");
+        if (s == UIR_OK)
+            g_scratch_dev[dev] = (float *)p;
+    }
+    *out = g_scratch_dev[dev];
+    pthread_mutex_unlock(&g_scratch_lock);
+    return s;
+}
+} // namespace
+
+extern "C" uir_status uir_op_dot_compat_noalloc(const float *a, const float *b, uint32_t width, uint32_t height,
+                                                float *d_result, void *stream)
+{
+    if (!a || !b || !d_result)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL pointer");
+    if (height == 0)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "height is 0");
+    if (height > kNoAllocMaxRows)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT,
+                             "height %u exceeds the %u rows the allocation-free dot holds (uir_op_dot_compat "
+                             "takes any height)", height, kNoAllocMaxRows);
+    float *ws = nullptr;
+    uir_status s = scratch_address(&ws);
+    if (s != UIR_OK)
+        return s;
+    const uint32_t part_len = (uint32_t)(((size_t)width * height) / height); // ufo-ir-basic-ops-processor.c:181-183
+    hipLaunchKernelGGL(k_dot_compat_parts, dim3((height + 255) / 256), dim3(256), 0, (hipStream_t)stream, a, b,
+                       height, part_len, ws);
+    hipLaunchKernelGGL(k_dot_compat_final, dim3(1), dim3(64), 0, (hipStream_t)stream, (const float *)ws, height,
+                       d_result);
+    return uir_check_hip(hipGetLastError(), "dot_compat launch");
+}

@@ -217,6 +217,7 @@ def load():
     L.uir_op_normalization.argtypes = [vp, sz, vp]
     L.uir_op_l1_norm.argtypes = [vp, u32, u32, C.POINTER(f32), vp]
     L.uir_op_dot_compat.argtypes = [vp, vp, u32, u32, C.POINTER(f32), vp]
+    L.uir_op_dot_compat_noalloc.argtypes = [vp, vp, u32, u32, vp, vp]
     L.uir_op_l2_compat.argtypes = [vp, u32, u32, C.POINTER(f32), vp]
     L.uir_op_dot_f64.argtypes = [vp, vp, sz, C.POINTER(C.c_double), vp]
     for n in ("uir_op_dx", "uir_op_dxt", "uir_op_dy", "uir_op_dyt", "uir_op_l1_grad"):
