@@ -87,6 +87,8 @@ def lib():
         L.orc_sbtv_cgs.argtypes = [G, f32p, f32p, f32p, C.c_uint32, C.c_float, C.c_float, C.c_int]
         L.orc_sbtv_cgs.restype = C.c_int
         L.orc_sbtv_apply.argtypes = [G, f32p, f32p, C.c_float, C.c_float]
+        L.orc_cgs_dense.argtypes = [f32p, C.c_uint32, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_int]
+        L.orc_cgs_dense.restype = C.c_int
         L.orc_sbtv_shrink.argtypes = [f32p, f32p, C.c_size_t, C.c_float, f32p, f32p]
         L.orc64_fp.argtypes = [G, f64p, f64p]
         L.orc64_bp.argtypes = [G, f64p, f64p]
@@ -299,6 +301,15 @@ def sbtv_apply(geom, x, lam, mu):
 def sbtv_cgs(geom, rhs, x0, max_iter, lam, mu, dot_mode=0):
     x = np.zeros(geom.vol_shape(), np.float32)
     flag = lib().orc_sbtv_cgs(geom.ptr, _f32(rhs), x, _f32(x0), int(max_iter), float(lam), float(mu), int(dot_mode))
+    return x, flag
+
+
+def cgs_dense(M, b, x0, max_iter, w, h, dot_mode=0):
+    """The oracle's CGS (ufo-ir-sbtv-task.c:404-585) on a dense operator M
+    ((w h) x (w h), float rows summed in order); returns (x, flag)."""
+    M, b, x0 = _f32(M), _f32(b).ravel(), _f32(x0).ravel()
+    x = np.zeros_like(b)
+    flag = lib().orc_cgs_dense(M, int(w), int(h), b, x, x0, int(max_iter), int(dot_mode))
     return x, flag
 
 

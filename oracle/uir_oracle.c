@@ -639,14 +639,28 @@ typedef struct {
     float *tempA, *tempAt, *tempD, *tempDxt;
 } sbtv_ctx;
 
-static float sb_dot(const sbtv_ctx *c, const float *a, const float *b)
+/* The CGS of ufo-ir-sbtv-task.c:404-585 on a vector of n = w * h floats
+ * with an operator given as a callback: the SBTV system (processA) or, for
+ * the oracle's own pins, a dense matrix.  dot_mode 0 is the reference's
+ * strided "dot" over a (w, h) buffer (ufo-ir-basic-ops-processor.c:148-196),
+ * 1 a true inner product. */
+typedef void (*cgs_apply_fn)(void *ctx, const float *in, float *out);
+
+typedef struct {
+    size_t n, w, h;
+    int dot_mode;
+    cgs_apply_fn apply;
+    void *ctx;
+} cgs_op;
+
+static float sb_dot(const cgs_op *c, const float *a, const float *b)
 {
     if (c->dot_mode == 0)
-        return orc_dot_compat(a, b, (size_t)c->g->n, (size_t)c->g->n);
-    return dot_plain(a, b, c->nv);
+        return orc_dot_compat(a, b, c->w, c->h);
+    return dot_plain(a, b, c->n);
 }
 
-static float sb_l2(const sbtv_ctx *c, const float *a)
+static float sb_l2(const cgs_op *c, const float *a)
 {
     return (float)sqrt(sb_dot(c, a, a));
 }
@@ -670,10 +684,12 @@ static void sb_process_a(sbtv_ctx *c, const float *in, float *out)
     orc_op_add(out, c->tempAt, out, c->nv);
 }
 
+static void sb_apply(void *ctx, const float *in, float *out) { sb_process_a((sbtv_ctx *)ctx, in, out); }
+
 /* cgs, :404-585 (a port of MATLAB cgs with its stagnation / moresteps logic). */
-static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32_t max_iter)
+static int cgs_core(const cgs_op *c, const float *b, float *x, const float *x0, uint32_t max_iter)
 {
-    const size_t nv = c->nv;
+    const size_t nv = c->n;
     const float tol = 1E-06f;
     const double eps = 2.2204E-16;                          /* :35 */
     float n2b = sb_l2(c, b);
@@ -683,7 +699,7 @@ static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32
     memcpy(xmin, x, nv * sizeof(float));
     float tolb = tol * n2b;
     float *r = (float *)malloc(nv * sizeof(float));
-    sb_process_a(c, x, r);
+    c->apply(c->ctx, x, r);
     orc_op_sub(b, r, r, nv);
     float normr = sb_l2(c, r);
     float normr_act = normr;
@@ -728,7 +744,7 @@ static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32
             orc_op_add_scaled(u, temp_sum, beta, pp, nv);
         }
         memcpy(ph, pp, nv * sizeof(float));
-        sb_process_a(c, ph, vh);
+        c->apply(c->ctx, ph, vh);
         float rtvh = sb_dot(c, rt, vh);
         float alpha = 0;
         if (rtvh == 0) {
@@ -748,12 +764,12 @@ static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32
         else
             stag = 0;
         orc_op_add_scaled(x, uh, alpha, x, nv);
-        sb_process_a(c, uh, qh);
+        c->apply(c->ctx, uh, qh);
         orc_op_add_scaled(r, qh, -alpha, r, nv);
         normr = sb_l2(c, r);
         normr_act = normr;
         if (normr <= tolb || stag >= maxstagsteps || moresteps) {
-            sb_process_a(c, r, tmpa);                        /* :535, applies A to r */
+            c->apply(c->ctx, r, tmpa);                        /* :535, applies A to r */
             orc_op_sub(b, tmpa, r, nv);
             normr_act = sb_l2(c, r);
             if (normr_act <= tolb) {
@@ -779,7 +795,7 @@ static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32
         }
     }
     if (flag != 0) {                                         /* :566-572 */
-        sb_process_a(c, xmin, tmpa);
+        c->apply(c->ctx, xmin, tmpa);
         orc_op_sub(b, tmpa, r, nv);
         if (sb_l2(c, r) <= normr_act)
             memcpy(x, xmin, nv * sizeof(float));
@@ -787,6 +803,39 @@ static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32
     free(xmin); free(r); free(rt); free(u); free(tmpa); free(pp); free(ph);
     free(q); free(vh); free(temp_sum); free(uh); free(qh);
     return flag;
+}
+
+static int sb_cgs(sbtv_ctx *c, const float *b, float *x, const float *x0, uint32_t max_iter)
+{
+    const cgs_op op = {c->nv, (size_t)c->g->n, (size_t)c->g->n, c->dot_mode, sb_apply, c};
+    return cgs_core(&op, b, x, x0, max_iter);
+}
+
+/* A dense n x n operator for the CGS pins: out_i = sum_j M_ij in_j, summed in
+ * float in order of j. */
+typedef struct {
+    const float *M;
+    size_t n;
+} dense_op;
+
+static void dense_apply(void *ctx, const float *in, float *out)
+{
+    const dense_op *d = (const dense_op *)ctx;
+    for (size_t i = 0; i < d->n; ++i) {
+        float acc = 0.0f;
+        for (size_t j = 0; j < d->n; ++j)
+            acc = acc + d->M[i * d->n + j] * in[j];
+        out[i] = acc;
+    }
+}
+
+int orc_cgs_dense(const float *M, uint32_t w, uint32_t h, const float *b, float *x, const float *x0,
+                  uint32_t max_iter, int dot_mode)
+{
+    const size_t n = (size_t)w * h;
+    dense_op d = {M, n};
+    const cgs_op op = {n, w, h, dot_mode, dense_apply, &d};
+    return cgs_core(&op, b, x, x0, max_iter);
 }
 
 static sbtv_ctx sb_make(const orc_geom *g, float lambda, float mu, int dot_mode)

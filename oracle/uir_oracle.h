@@ -120,6 +120,11 @@ void orc_sbtv(const orc_geom *g, float *b, float *u, uint32_t iters, float lambd
 int orc_sbtv_cgs(const orc_geom *g, const float *rhs, float *x, const float *x0, uint32_t max_iter,
                  float lambda, float mu, int dot_mode);
 void orc_sbtv_apply(const orc_geom *g, const float *in, float *out, float lambda, float mu);
+/* The same CGS (ufo-ir-sbtv-task.c:404-585) with a dense (w h) x (w h)
+ * operator M (row-major, each row summed in float in order): for pinning the
+ * CGS control flow on constructed systems.  Returns the flag. */
+int orc_cgs_dense(const float *M, uint32_t w, uint32_t h, const float *b, float *x, const float *x0,
+                  uint32_t max_iter, int dot_mode);
 /* the shrinkage step of update_db on (tx, ty) -> (dx, dy) */
 void orc_sbtv_shrink(const float *tx, const float *ty, size_t n, float lambda, float *dx, float *dy);
 
