@@ -162,7 +162,12 @@ def serve_c5(args, uir):
 
 def sart_c2(args, uir):
     """Plain SART on one 1024 x 1024 slice, 720 angles, 1448 bins, --c2-iters
-    iterations (30), relaxation 0.5 (the reference-compatible SART task, FP32)."""
+    iterations (30), relaxation 0.5 (the reference-compatible SART task, FP32).
+    Beside the time: accuracy against the oracle's FP32 SART (the reference's
+    arithmetic) and its FP64 evaluation, the roofline of the per-angle
+    projection kernels (single-angle launches through the kernel-level ABI,
+    the kernels the sweep runs), and the oracle's SART on the host cores."""
+    import oracle as O
     res = uir.Resources(0)
     n, a, d = 1024, 720, 1448
     step = float(np.float32(np.pi / a))
@@ -173,7 +178,8 @@ def sart_c2(args, uir):
     proj.process(img, s)
     sino = s.numpy().astype(np.float64)
     sino += np.random.default_rng(20261202).normal(0.0, 0.01 * sino.max(), sino.shape)
-    ds = uir.DeviceArray.from_numpy(sino.astype(np.float32))
+    sino32 = sino.astype(np.float32)
+    ds = uir.DeviceArray.from_numpy(sino32)
     out = uir.DeviceArray((n, n))
     pj = uir.Task("parallel-projector", step=step, is_forward=False, image_size=n)
     iters = args.c2_iters
@@ -185,11 +191,53 @@ def sart_c2(args, uir):
     sart.process(ds, out)
     timer.stop(res.stream)
     ms = timer.elapsed_ms()
+    got = out.numpy()
+    print("c2: %d SART iterations in %.1f ms" % (iters, ms), file=sys.stderr, flush=True)
+
+    # per-launch durations of the sweep's single-angle FP and BP (HIP events on
+    # the task's stream around each launch of one sweep)
+    g = O.Geometry(n, d, a, np.float32(step))
+    fp_us, bp_us = sart.time_kernels(ds, out, stride=1)    # after the result was read
+    # algorithmic bytes per launch: FP reads the image and the angle's row and
+    # writes the row; BP reads the row and the image and writes the image
+    fp_bytes = n * n * 4 + 2 * d * 4
+    bp_bytes = 2 * n * n * 4 + d * 4
+    k = ("k_fp_ref_seg", fp_us, fp_bytes) if fp_us >= bp_us else ("k_bp_ref_angle", bp_us, bp_bytes)
+    gbs = k[2] / (k[1] * 1e-6) / 1e9
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": k[0],
+            "launch_us": {"fp": round(fp_us, 2), "bp": round(bp_us, 2)},
+            "bytes_per_launch": {"fp": fp_bytes, "bp": bp_bytes},
+            "what": "one angle per launch (the sweep's structure, ufo-ir-sart-task.c:208-220): the FP streams "
+                    "the image once, the BP reads and writes it; every launch of one sweep bracketed by "
+                    "events (uir_task_time_kernels), so the figures include each launch's start-up"}
+    print("c2: FP %.2f us, BP %.2f us per angle" % (fp_us, bp_us), file=sys.stderr, flush=True)
+
+    # accuracy (full iteration count) and the CPU baseline
+    t0 = time.time()
+    ref32 = O.sart(g, sino32, iters, 0.5)
+    t_ref32 = time.time() - t0
+    print("c2: oracle FP32 SART in %.1f s" % t_ref32, file=sys.stderr, flush=True)
+    t0 = time.time()
+    ref64 = O.sart64(g, sino32.astype(np.float64), iters, 0.5)
+    print("c2: oracle FP64 SART in %.1f s" % (time.time() - t0), file=sys.stderr, flush=True)
+    cpu_it_s = iters / t_ref32
+    acc = {"iterations": iters,
+           "rel_l2_vs_oracle_fp32": O.rel_l2(got, ref32), "bit_exact_vs_oracle_fp32": bool(np.array_equal(got, ref32)),
+           "rel_l2_vs_fp64": O.rel_l2(got, ref64), "psnr_vs_fp64_db": round(O.psnr(got, ref64), 2),
+           "oracle_fp32_rel_l2_vs_fp64": O.rel_l2(ref32, ref64),
+           "note": "1% noise and the reference's ray weights (1 / FP(1), up to ~8e3 on corner-grazing rays) make "
+                   "FP32 and FP64 differ; the GPU path is the reference's FP32 arithmetic"}
     line = {"metric": "plain SART, %d iterations" % iters, "value": round(ms, 2), "unit": "ms", "n_gpus": 1,
             "higher_is_better": False, "dtype": "f32", "data": "synthetic: 20 random ellipses, 1% Gaussian noise",
             "config": {"workload": "SART task, 1024x1024, 720 angles over [0, pi), 1448 bins, relaxation 0.5, "
                                    "ordered single-angle subsets, %d iterations" % iters},
-            "ms_per_iteration": round(ms / iters, 3), "us_per_angle": round(ms / iters / a * 1000, 2)}
+            "ms_per_iteration": round(ms / iters, 3), "us_per_angle": round(ms / iters / a * 1000, 2),
+            "roofline": roof, "accuracy": acc,
+            "cpu_baseline": {"value": round(cpu_it_s, 4), "unit": "iterations/s", "cores": O.num_threads(),
+                             "kind": "port",
+                             "sample": "the oracle's FP32 SART (the reference's arithmetic), same slice, "
+                                       "%d iterations, OpenMP over the host's cores" % iters}}
     print(json.dumps(line), flush=True)
 
 
@@ -466,6 +514,7 @@ def main():
         serve_c5(args, uir)
         return
     if args.c2:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sart_c2(args, uir)
         return
     if args.c4:

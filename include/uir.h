@@ -191,6 +191,14 @@ uir_status uir_task_get_requisition(uir_task *task, const uir_buffer *inputs, ui
  * (ufo-ir-sbtv-task.c:239). */
 uir_status uir_task_process(uir_task *task, const uir_buffer *inputs, uir_buffer *output);
 
+/* Timing of a method's kernels, no counterpart in the reference (for
+ * measurement): for the sart task, one sweep of the launches process() makes
+ * (ufo-ir-sart-task.c:208-220), with every stride-th angle's FP and BP
+ * launches bracketed by events on the task's stream; returns their average
+ * durations in microseconds.  Synchronises; output holds a partial result. */
+uir_status uir_task_time_kernels(uir_task *task, const uir_buffer *inputs, uir_buffer *output, uint32_t stride,
+                                 float *fp_us, float *bp_us);
+
 /* ufo_ir_state_dependent_task_forward / _backward,
  * ufo-ir-state-dependent-task.c:99-137: accumulate into output
  * (out = out + correction_scale * FP(in), out = out + relaxation * BP(in)). */

@@ -1,7 +1,10 @@
 """The JSON graph runner on the GPU: json-sample.json's chain (read ->
 asdpocs with a projector and a SART df-minimizer -> write, one TIFF per
-frame) gives, frame by frame, what the task ABI gives directly; a SIRT chain
-with a multi-page output; and the uir-runjson program."""
+frame) gives, frame by frame, the oracle's ASD-POCS (ufo-ir-asdpocs-task.c:
+411-509 over ufo-ir-sart-task.c:173-228) and what the task ABI gives
+directly; the sample's own properties (10 iterations of a 100-sweep SART
+df-minimizer) on one frame against the oracle; a SIRT chain with a
+multi-page output against the oracle's SIRT; and the uir-runjson program."""
 import json
 import os
 import subprocess
@@ -58,9 +61,37 @@ def test_json_sample_chain_matches_task_api(res, tmp_path):
     proj = uir.Task("parallel-projector", model="joseph", is_forward=False, step=STEP)
     df = uir.Task("sart", relaxation_factor=0.25, num_iterations=3)
     asd = uir.Task("asdpocs", projector=proj, df_minimizer=df, num_iterations=3).setup(res)
+    g = O.Geometry(N_DET, N_DET, N_ANG, np.float32(STEP))
     for i, s in enumerate(sinos):
         got = uir.tiff_read(str(tmp_path / "out" / ("out%03d.tif" % i)))
         assert np.array_equal(got, _direct(res, asd, s))
+        ref = O.asdpocs(g, s, 3, df="sart", df_iters=3, df_relax_initial=0.25)
+        assert O.rel_l2(got, ref) <= 1e-4
+
+
+def test_json_sample_properties_match_oracle(tmp_path):
+    """json-sample.json:17-44 as written (asdpocs num-iterations 10, SART
+    df-minimizer relaxation 0.25 and num-iterations 100, projector step 2
+    degrees) on one 128-bin frame of 90 angles."""
+    print("case json-sample properties, one frame", flush=True)
+    node = {"plugin": "asdpocs", "properties": {
+        "projector": {"plugin": "parallel-projector", "name": "parallel-projector", "package": "ir",
+                      "properties": {"model": "joseph", "is_forward": False, "step": STEP}},
+        "df_minimizer": {"plugin": "sart", "package": "ir", "name": "sart_minimizer",
+                         "properties": {"relaxation_factor": 0.25, "num-iterations": 100}},
+        "num-iterations": 10}}
+    graph = _graph("out/out%03i.tif", node)
+    graph["nodes"][0]["properties"]["path"] = "sino01.tif"
+    sino = _sinos(1)[0]
+    uir.tiff_write(str(tmp_path / "sino01.tif"), sino)
+    rep = uir.json_run(graph, base_dir=str(tmp_path))
+    assert rep["frames"] == 1 and rep["tasks"] == 1
+    got = uir.tiff_read(str(tmp_path / "out" / "out000.tif"))
+    g = O.Geometry(N_DET, N_DET, N_ANG, np.float32(STEP))
+    ref = O.asdpocs(g, sino, 10, df="sart", df_iters=100, df_relax_initial=0.25)
+    err = O.rel_l2(got, ref)
+    print("json-sample ASD-POCS rel-L2 vs oracle %.3e" % err, flush=True)
+    assert err <= 1e-4
 
 
 def test_sirt_chain_multipage_and_cli(res, tmp_path):
@@ -77,8 +108,11 @@ def test_sirt_chain_multipage_and_cli(res, tmp_path):
     assert uir.tiff_info(str(tmp_path / "recon.tif")) == (2, N_DET, N_DET)
     proj = uir.Task("parallel-projector", is_forward=False, step=STEP)
     sirt = uir.Task("sirt", projector=proj, relaxation_factor=0.5, num_iterations=7).setup(res)
+    gm = O.Geometry(N_DET, N_DET, N_ANG, np.float32(STEP))
     for page, i in enumerate((1, 2)):
-        assert np.array_equal(uir.tiff_read(str(tmp_path / "recon.tif"), page), _direct(res, sirt, sinos[i]))
+        got = uir.tiff_read(str(tmp_path / "recon.tif"), page)
+        assert np.array_equal(got, _direct(res, sirt, sinos[i]))
+        assert O.rel_l2(got, O.sirt(gm, sinos[i], 7, 0.5)) <= 1e-4
     # the command-line runner on the same graph, from the graph's directory
     g["nodes"][1]["properties"]["filename"] = "cli-%02d.tif"
     (tmp_path / "g.json").write_text(json.dumps(g))

@@ -59,6 +59,22 @@ int uirk_fp_ref_split(const float *vol, const float *r_sino, float *w_sino, cons
  * pix_w == NULL: accumulate mode (r_vol may equal w_vol).
  * pix_w != NULL: SIRT mode, tmp = sum_runs(relax * S_run) (from 0), then
  * w = tmp * pix_w + r (ufo-ir-sirt-task.c:208-212); posc clamps the result. */
+/* The coordinate at the start of every 64-sample block of every ray of
+ * angles [0, n_angles) ([angle][detector][block], ceil(n / 64) blocks), taken
+ * by the reference's sequential float walk; inactive rays get 0. */
+int uirk_fp_block_starts(const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis, int n_angles,
+                         int dir, float *starts, void *stream);
+/* The same FP as uirk_fp_ref, bit-identical, one wave per ray, from the
+ * block-start table of uirk_fp_block_starts (built for the same geometry,
+ * direction rule and LUTs). */
+int uirk_fp_ref_seg(const float *vol, const float *r_sino, float *w_sino, const float *ray_w,
+                    const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
+                    float corr, int ang0, int n_ang, int dir, const float *starts, void *stream);
+/* BP of one angle in accumulate mode (w = r + relax * BP_ang), bit-identical
+ * to uirk_bp_ref with a one-angle run. */
+int uirk_bp_ref_angle(const float *r_vol, float *w_vol, const float *sino, const float *sin_lut,
+                      const float *cos_lut, int n, int n_dets, float axis, float relax, int ang, void *stream);
+
 int uirk_bp_ref(const float *r_vol, float *w_vol, const float *sino, int n_angles,
                 const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
                 float relax, const uirk_runs *runs, const float *pix_w, int posc, void *stream);

@@ -9,6 +9,8 @@
 // reduced-precision batched kernels live in uir_fast.hip.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "uir_device.h"
 
 namespace {
@@ -197,6 +199,129 @@ __global__ __launch_bounds__(256) void k_fp_ref_split(FpRefArgs p)
 }
 
 // --------------------------------------------------------------------------
+// The same projection with one wave per ray and the block starts from a table.
+// --------------------------------------------------------------------------
+// A ray's coordinate before sample i is the reference's sequential float walk
+// (:82-90): i additions of the step to the start coordinate.  That walk does
+// not depend on the image, so it is taken once per geometry: the table holds,
+// for every ray, the coordinate at the start of each 64-sample block, reached
+// by the same additions (k_fp_block_starts).  A ray's 64-sample blocks then
+// spread over a wave, 16 samples per lane: a lane replays at most 48 additions
+// from its block's start, loads all 32 taps of its samples at once, and the
+// four lanes of a block add their samples into the block sum one after another
+// in sample order (the value passes lane to lane); the block sums are added in
+// block order.  Every sample sees the coordinate the sequential walk sees and
+// every sum is taken in the reference's order: bit-identical to k_fp_ref.
+constexpr int kSegSamples = 16; // samples per lane; 4 lanes per 64-sample block
+
+__global__ __launch_bounds__(256) void k_fp_block_starts(FpRefArgs p, float *__restrict__ starts)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int ang = p.ang0 + blockIdx.y;
+    if (s >= p.n_dets)
+        return;
+    const int N = p.n, n_blocks = (N + kBlockSamples - 1) / kBlockSamples;
+    float pos = 0.0f, step = 0.0f;
+    int dir = 0;
+    float *out = starts + ((size_t)ang * p.n_dets + s) * n_blocks;
+    if (!fp_ray_setup(s, ang, N, p.n_dets, p.axis, p.sin_lut, p.cos_lut, p.dir, pos, step, dir)) {
+        for (int j = 0; j < n_blocks; ++j)
+            out[j] = 0.0f;
+        return;
+    }
+    for (int j = 0; j < n_blocks; ++j) {
+        out[j] = pos;
+#pragma unroll
+        for (int k = 0; k < kBlockSamples; ++k)
+            pos += step;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fp_ref_seg(FpRefArgs p, const float *__restrict__ starts)
+{
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6); // one ray per wave
+    const int ang = p.ang0 + blockIdx.y;
+    if (s >= p.n_dets)
+        return; // uniform over the wave
+    const int N = p.n, n_blocks = (N + kBlockSamples - 1) / kBlockSamples;
+    float pos0 = 0.0f, step = 0.0f;
+    int dir = 0;
+    const size_t o = (size_t)ang * p.n_dets + s;
+    if (!fp_ray_setup(s, ang, N, p.n_dets, p.axis, p.sin_lut, p.cos_lut, p.dir, pos0, step, dir)) {
+        if (lane == 0)
+            fp_inactive(p, o);
+        return;
+    }
+    const float *st = starts + o * n_blocks;
+    const int q = lane & 3; // this lane's quarter of its block
+    float acc = 0.0f;
+    for (int jb0 = 0; jb0 < n_blocks; jb0 += 16) {
+        const int jb = jb0 + (lane >> 2);
+        float pos = jb < n_blocks ? st[jb] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            if (r < q) {
+#pragma unroll
+                for (int k = 0; k < kSegSamples; ++k)
+                    pos += step;
+            }
+        }
+        const int i0 = jb * kBlockSamples + kSegSamples * q;
+        float va[kSegSamples], vb[kSegSamples], w[kSegSamples];
+        int c0[kSegSamples];
+#pragma unroll
+        for (int k = 0; k < kSegSamples; ++k) {
+            const int i = min(i0 + k, N - 1); // lanes past the image load a valid address, unused
+            const float v = pos - 0.5f;
+            const float fv = floorf(v);
+            c0[k] = (int)fv;
+            w[k] = v - fv;
+            const int ca = min(max(c0[k], 0), N - 1), cb = min(max(c0[k] + 1, 0), N - 1);
+            const size_t oa = dir == 0 ? (size_t)ca * N + i : (size_t)i * N + ca;
+            const size_t ob = dir == 0 ? (size_t)cb * N + i : (size_t)i * N + cb;
+            va[k] = p.vol[oa];
+            vb[k] = p.vol[ob];
+            pos += step;
+        }
+#pragma unroll
+        for (int k = 0; k < kSegSamples; ++k)
+            asm volatile("" : "+v"(va[k]), "+v"(vb[k]));
+        float val[kSegSamples];
+#pragma unroll
+        for (int k = 0; k < kSegSamples; ++k) {
+            const float t0 = (c0[k] >= 0 && c0[k] < N) ? va[k] : 0.0f;
+            const float t1 = (c0[k] + 1 >= 0 && c0[k] + 1 < N) ? vb[k] : 0.0f;
+            const float omw = 1.0f - w[k];
+            val[k] = omw * t0 + w[k] * t1;
+        }
+        // the block sum, lane q continuing from lane q - 1 (:84-89 order)
+        float run = 0.0f;
+#pragma unroll
+        for (int stage = 0; stage < 4; ++stage) {
+            const float from = __shfl_up(run, 1, 4);
+            if (q == stage) {
+                float a = stage == 0 ? 0.0f : from;
+#pragma unroll
+                for (int k = 0; k < kSegSamples; ++k)
+                    if (i0 + k < N)
+                        a = a + val[k];
+                run = a;
+            }
+        }
+        // block sums in block order (:89)
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const float bs = __shfl(run, 4 * jj + 3);
+            if (jb0 + jj < n_blocks)
+                acc = acc + bs;
+        }
+    }
+    if (lane == 0)
+        fp_epilogue(p, o, acc);
+}
+
+// --------------------------------------------------------------------------
 // Backprojection, one thread per pixel, per-run accumulation.
 // --------------------------------------------------------------------------
 struct BpRefArgs {
@@ -264,6 +389,74 @@ __global__ __launch_bounds__(256) void k_bp_ref(BpRefArgs p)
             out = out > 0 ? out : 0.0f;
     }
     p.w_vol[o] = out;
+}
+
+// --------------------------------------------------------------------------
+// Backprojection of one angle, four pixels per thread (SART's per-angle BP).
+// --------------------------------------------------------------------------
+// The arithmetic of k_bp_ref with one run of one angle (:196-216) in
+// accumulate mode: w = r + relax * (0 + lerp).  Row-aligned buffers (N a
+// multiple of 4, 16-byte aligned pointers) move four pixels per load and
+// store.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_bp_ref_angle(BpRefArgs p, int ang)
+{
+    const int px0 = (blockIdx.x * 64 + threadIdx.x) * 4;
+    const int py = blockIdx.y * 4 + threadIdx.y;
+    const int N = p.n, D = p.n_dets;
+    if (px0 >= N || py >= N)
+        return;
+    // projector-parallel-joseph.cl:174-190 (detector quantities use n_dets)
+    const float required_width = p.axis * 2;
+    const float diff = (float)D - required_width;
+    const float half_active = diff < 0 ? (float)D - p.axis : p.axis;
+    const float edge0 = p.axis - half_active + 0.5f;
+    const float edge1 = p.axis + half_active - 0.5f;
+    const float rot = diff / 2.0f;
+    const float det_shift = (float)D / 2.0f;
+    const float origin_shift = (float)N / 2.0f;
+    const float shift = det_shift - rot;
+    const float fY = (float)py + 0.5f - origin_shift;
+    const float st = p.sin_lut[ang], ct = p.cos_lut[ang];
+    const float *row = p.sino + (size_t)ang * D;
+    const size_t o = (size_t)py * N + px0;
+    float in[4];
+    if (VEC) {
+        const float4 v = *(const float4 *)(p.r_vol + o);
+        in[0] = v.x, in[1] = v.y, in[2] = v.z, in[3] = v.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            in[k] = px0 + k < N ? p.r_vol[o + k] : 0.0f;
+    }
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float fX = (float)(px0 + k) + 0.5f - origin_shift;
+        float u = fX * ct - fY * st + shift;
+        if (u > edge1)
+            u = edge1;
+        else if (u < edge0)
+            u = edge0;
+        const float w = u - 0.5f;
+        const float fw = floorf(w);
+        const int i0 = (int)fw;
+        const float a = w - fw;
+        const float oma = 1.0f - a;
+        const float t0 = (i0 >= 0 && i0 < D) ? row[i0] : 0.0f;
+        const float t1 = (i0 + 1 >= 0 && i0 + 1 < D) ? row[i0 + 1] : 0.0f;
+        float value = 0.0f;
+        value += oma * t0 + a * t1;
+        out[k] = in[k] + p.relax * value; // :216
+    }
+    if (VEC) {
+        *(float4 *)(p.w_vol + o) = make_float4(out[0], out[1], out[2], out[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (px0 + k < N)
+                p.w_vol[o + k] = out[k];
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -552,6 +745,54 @@ int uirk_fp_ref_split(const float *vol, const float *r_sino, float *w_sino, cons
     FpRefArgs a{vol, r_sino, w_sino, ray_w, sin_lut, cos_lut, n, n_dets, ang0, n_ang, dir, axis, corr};
     dim3 grid((n_dets + kSplitRays - 1) / kSplitRays, n_ang);
     hipLaunchKernelGGL(k_fp_ref_split, grid, dim3(256), 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+int uirk_fp_block_starts(const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis, int n_angles,
+                         int dir, float *starts, void *stream)
+{
+    if (n_angles <= 0 || n_dets <= 0 || n <= 0)
+        return 0;
+    FpRefArgs a{nullptr, nullptr, nullptr, nullptr, sin_lut, cos_lut, n, n_dets, 0, n_angles, dir, axis, 0.0f};
+    dim3 grid((n_dets + 255) / 256, n_angles);
+    hipLaunchKernelGGL(k_fp_block_starts, grid, dim3(256), 0, (hipStream_t)stream, a, starts);
+    return (int)hipGetLastError();
+}
+
+int uirk_fp_ref_seg(const float *vol, const float *r_sino, float *w_sino, const float *ray_w,
+                    const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
+                    float corr, int ang0, int n_ang, int dir, const float *starts, void *stream)
+{
+    if (n_ang <= 0 || n_dets <= 0)
+        return 0;
+    FpRefArgs a{vol, r_sino, w_sino, ray_w, sin_lut, cos_lut, n, n_dets, ang0, n_ang, dir, axis, corr};
+    dim3 grid((n_dets + 3) / 4, n_ang);
+    hipLaunchKernelGGL(k_fp_ref_seg, grid, dim3(256), 0, (hipStream_t)stream, a, starts);
+    return (int)hipGetLastError();
+}
+
+int uirk_bp_ref_angle(const float *r_vol, float *w_vol, const float *sino, const float *sin_lut,
+                      const float *cos_lut, int n, int n_dets, float axis, float relax, int ang, void *stream)
+{
+    if (n <= 0)
+        return 0;
+    BpRefArgs a;
+    memset(&a, 0, sizeof a);
+    a.r_vol = r_vol;
+    a.w_vol = w_vol;
+    a.sino = sino;
+    a.sin_lut = sin_lut;
+    a.cos_lut = cos_lut;
+    a.n = n;
+    a.n_dets = n_dets;
+    a.axis = axis;
+    a.relax = relax;
+    dim3 block(64, 4), grid((n + 255) / 256, (n + 3) / 4);
+    const bool vec = (n & 3) == 0 && (((uintptr_t)r_vol | (uintptr_t)w_vol) & 15) == 0;
+    if (vec)
+        hipLaunchKernelGGL(k_bp_ref_angle<true>, grid, block, 0, (hipStream_t)stream, a, ang);
+    else
+        hipLaunchKernelGGL(k_bp_ref_angle<false>, grid, block, 0, (hipStream_t)stream, a, ang);
     return (int)hipGetLastError();
 }
 

@@ -73,7 +73,7 @@ struct uir_task {
     hipGraph_t sart_graph;
     hipGraphExec_t sart_exec;
     struct {
-        const void *x, *b, *tmp, *ray_w, *sin, *cos;
+        const void *x, *b, *tmp, *ray_w, *sin, *cos, *starts;
         uint32_t n, D, A;
         float axis, relax, corr;
     } sart_key;
@@ -883,55 +883,86 @@ static uir_status sirt_run(uir_task *t, const float *b, float *x, float relax, u
  * ray weighting fused (subset_fp + op_mulRows), then BP of that row into x. */
 /* One SART sweep (ufo-ir-sart-task.c:208-220): per angle in order, FP of that
  * angle with the residual and ray weights fused (w = (b - A_a x) R_a), then
- * its BP x = x + relax A_a^T w.  The FP spreads each ray's 64-sample blocks
- * over 16 threads (bit-identical to the one-thread-per-ray walk): one angle
- * is only n_dets rays. */
+ * its BP x = x + relax A_a^T w.  One angle is only n_dets rays, so the FP
+ * gives each ray a wave, starting its 64-sample blocks from the geometry's
+ * block-start table (starts; without one, 16 threads per ray replay the float
+ * stepping); both are bit-identical to the one-thread-per-ray walk.  The BP
+ * takes four pixels per thread. */
 static uir_status sart_sweep(uir_task *t, mctx *c, const float *b, float *x, float *sino_tmp, const float *ray_w,
-                             float axis)
+                             float axis, const float *starts)
 {
     HIPTRY(hipMemcpyAsync(sino_tmp, b, c->ns * 4, hipMemcpyDeviceToDevice, c->st), "copy");
     for (uint32_t a = 0; a < c->A; ++a) {
-        HIPTRY(uirk_fp_ref_split(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
-                                 c->p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
-        uirk_runs r;
-        memset(&r, 0, sizeof r);
-        r.count = 1;
-        r.offset[0] = a;
-        r.n[0] = 1;
-        HIPTRY(uirk_bp_ref(x, x, sino_tmp, (int)c->A, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
-                           c->p->relaxation, &r, NULL, 0, c->st), "BP launch");
+        if (starts)
+            HIPTRY(uirk_fp_ref_seg(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
+                                   c->p->correction_scale, (int)a, 1, -1, starts, c->st), "FP launch");
+        else
+            HIPTRY(uirk_fp_ref_split(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D,
+                                     axis, c->p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
+        HIPTRY(uirk_bp_ref_angle(x, x, sino_tmp, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
+                                 c->p->relaxation, (int)a, c->st), "BP launch");
     }
     (void)t;
+    return UIR_OK;
+}
+
+/* Bytes of the FP block-start table of a geometry; tables above this bound
+ * are not built (the FP then replays the stepping per thread). */
+#define SART_STARTS_MAX_BYTES ((size_t)4 << 30)
+
+/* SART set-up (ufo-ir-sart-task.c:193-207): ray weights R = inv(FP(1)) per
+ * angle, corr = -1, relax = relaxation, x = 0 (unless warm); with `table`,
+ * also the FP block-start table of the geometry. */
+typedef struct {
+    float *sino_tmp, *ray_w, *starts;
+    float axis;
+} sart_prep;
+
+static uir_status sart_prepare(uir_task *t, float *x, float relax, int warm, int table, mctx *c, sart_prep *sp)
+{
+    float *vol_tmp;
+    TRY(tmp_get(t, 4, c->ns * 4, &sp->sino_tmp));
+    TRY(tmp_get(t, 5, c->nv * 4, &vol_tmp));
+    TRY(tmp_get(t, 6, c->ns * 4, &sp->ray_w));
+    HIPTRY(uirk_set(vol_tmp, c->nv, 1.0f, c->st), "set");          /* :193-200 */
+    HIPTRY(uirk_set(sp->ray_w, c->ns, 0.0f, c->st), "set");
+    c->p->correction_scale = 1.0f;
+    TRY(proj_fp(c->p, c->n, vol_tmp, sp->ray_w, sp->ray_w, NULL));
+    HIPTRY(uirk_inv(sp->ray_w, sp->ray_w, c->ns, c->st), "inv");
+    c->p->correction_scale = -1.0f;                                 /* :205-207 */
+    c->p->relaxation = relax;
+    if (!warm)
+        HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
+    sp->axis = proj_axis(c->p);
+    sp->starts = NULL;
+    /* the rays' block starts: the float stepping taken once per call */
+    const size_t starts_bytes = (size_t)c->A * c->D * ((c->n + 63) / 64) * 4;
+    if (table && starts_bytes <= SART_STARTS_MAX_BYTES) {
+        TRY(tmp_get(t, 25, starts_bytes, &sp->starts));
+        HIPTRY(uirk_fp_block_starts(c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, sp->axis, (int)c->A, -1,
+                                    sp->starts, c->st), "block starts");
+    }
     return UIR_OK;
 }
 
 static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, uint32_t iters, int warm,
                            mctx *c)
 {
-    float *sino_tmp, *vol_tmp, *ray_w;
-    TRY(tmp_get(t, 4, c->ns * 4, &sino_tmp));
-    TRY(tmp_get(t, 5, c->nv * 4, &vol_tmp));
-    TRY(tmp_get(t, 6, c->ns * 4, &ray_w));
-    HIPTRY(uirk_set(vol_tmp, c->nv, 1.0f, c->st), "set");          /* :193-200 */
-    HIPTRY(uirk_set(ray_w, c->ns, 0.0f, c->st), "set");
-    c->p->correction_scale = 1.0f;
-    TRY(proj_fp(c->p, c->n, vol_tmp, ray_w, ray_w, NULL));
-    HIPTRY(uirk_inv(ray_w, ray_w, c->ns, c->st), "inv");
-    c->p->correction_scale = -1.0f;                                 /* :205-207 */
-    c->p->relaxation = relax;
-    if (!warm)
-        HIPTRY(uirk_set(x, c->nv, 0.0f, c->st), "set");
-    const float axis = proj_axis(c->p);
+    sart_prep sp;
+    TRY(sart_prepare(t, x, relax, warm, iters > 0, c, &sp));
     if (iters == 0)
         return UIR_OK;
+    float *sino_tmp = sp.sino_tmp, *ray_w = sp.ray_w, *starts = sp.starts;
+    const float axis = sp.axis;
     /* 2 A launches per sweep: capture one sweep and replay it (:208-220) */
     const int same = t->sart_exec && t->sart_key.x == x && t->sart_key.b == b && t->sart_key.tmp == sino_tmp &&
+                     t->sart_key.starts == starts &&
                      t->sart_key.ray_w == ray_w && t->sart_key.sin == c->p->d_sin && t->sart_key.cos == c->p->d_cos &&
                      t->sart_key.n == c->n && t->sart_key.D == c->D && t->sart_key.A == c->A &&
                      t->sart_key.axis == axis && t->sart_key.relax == c->p->relaxation &&
                      t->sart_key.corr == c->p->correction_scale;
     if (!same && iters == 1) /* a capture pays off from the second replay on */
-        return sart_sweep(t, c, b, x, sino_tmp, ray_w, axis);
+        return sart_sweep(t, c, b, x, sino_tmp, ray_w, axis, starts);
     if (!same) {
         if (t->sart_exec)
             (void)hipGraphExecDestroy(t->sart_exec);
@@ -940,7 +971,7 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
         t->sart_exec = NULL;
         t->sart_graph = NULL;
         HIPTRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        uir_status s = sart_sweep(t, c, b, x, sino_tmp, ray_w, axis);
+        uir_status s = sart_sweep(t, c, b, x, sino_tmp, ray_w, axis, starts);
         hipGraph_t g = NULL;
         const int e = hipStreamEndCapture(c->st, &g);
         if (s != UIR_OK || e != hipSuccess) {
@@ -953,6 +984,7 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
         t->sart_key.x = x;
         t->sart_key.b = b;
         t->sart_key.tmp = sino_tmp;
+        t->sart_key.starts = starts;
         t->sart_key.ray_w = ray_w;
         t->sart_key.sin = c->p->d_sin;
         t->sart_key.cos = c->p->d_cos;
@@ -1347,6 +1379,73 @@ static uir_status projector_process(uir_task *p, const uir_buffer *inputs, uir_b
     if (p->is_forward)
         return uir_projector_forward(p, &inputs[0], out);
     return uir_projector_backward(p, &inputs[0], out);
+}
+
+uir_status uir_task_time_kernels(uir_task *t, const uir_buffer *inputs, uir_buffer *output, uint32_t stride,
+                                 float *fp_us, float *bp_us)
+{
+    if (!t || !inputs || !output || !fp_us || !bp_us)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (t->kind != K_SART)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "kernel timing is for the sart task");
+    if (stride == 0)
+        stride = 1;
+    if (t->res)
+        TRY(uir_check_hip(hipSetDevice(t->res->device), "hipSetDevice"));
+    mctx c;
+    TRY(method_begin(t, inputs, output, &c));
+    const float *b = (const float *)inputs[0].data;
+    float *x = (float *)output->data;
+    sart_prep sp;
+    TRY(sart_prepare(t, x, t->relaxation_factor, 0, 1, &c, &sp));
+    HIPTRY(hipMemcpyAsync(sp.sino_tmp, b, c.ns * 4, hipMemcpyDeviceToDevice, c.st), "copy");
+    hipEvent_t ev[3] = {NULL, NULL, NULL};
+    uir_status s = UIR_OK;
+    for (int i = 0; i < 3 && s == UIR_OK; ++i)
+        s = uir_check_hip(hipEventCreate(&ev[i]), "hipEventCreate");
+    double fp_sum = 0, bp_sum = 0;
+    uint32_t timed = 0;
+    /* one sweep in order, as sart_sweep launches it; every stride-th angle's
+     * FP and BP are each bracketed by events */
+    for (uint32_t a = 0; a < c.A && s == UIR_OK; ++a) {
+        const int tm = a % stride == 0;
+        if (tm)
+            s = uir_check_hip(hipEventRecord(ev[0], c.st), "hipEventRecord");
+        if (s == UIR_OK)
+            s = uir_check_hip(sp.starts ? uirk_fp_ref_seg(x, sp.sino_tmp, sp.sino_tmp, sp.ray_w, c.p->d_sin,
+                                                          c.p->d_cos, (int)c.n, (int)c.D, sp.axis,
+                                                          c.p->correction_scale, (int)a, 1, -1, sp.starts, c.st)
+                                        : uirk_fp_ref_split(x, sp.sino_tmp, sp.sino_tmp, sp.ray_w, c.p->d_sin,
+                                                            c.p->d_cos, (int)c.n, (int)c.D, sp.axis,
+                                                            c.p->correction_scale, (int)a, 1, -1, c.st),
+                              "FP launch");
+        if (s == UIR_OK && tm)
+            s = uir_check_hip(hipEventRecord(ev[1], c.st), "hipEventRecord");
+        if (s == UIR_OK)
+            s = uir_check_hip(uirk_bp_ref_angle(x, x, sp.sino_tmp, c.p->d_sin, c.p->d_cos, (int)c.n, (int)c.D,
+                                                sp.axis, c.p->relaxation, (int)a, c.st), "BP launch");
+        if (s == UIR_OK && tm) {
+            float f = 0, g = 0;
+            s = uir_check_hip(hipEventRecord(ev[2], c.st), "hipEventRecord");
+            if (s == UIR_OK)
+                s = uir_check_hip(hipEventSynchronize(ev[2]), "hipEventSynchronize");
+            if (s == UIR_OK)
+                s = uir_check_hip(hipEventElapsedTime(&f, ev[0], ev[1]), "hipEventElapsedTime");
+            if (s == UIR_OK)
+                s = uir_check_hip(hipEventElapsedTime(&g, ev[1], ev[2]), "hipEventElapsedTime");
+            fp_sum += f;
+            bp_sum += g;
+            ++timed;
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+        if (ev[i])
+            (void)hipEventDestroy(ev[i]);
+    if (s != UIR_OK)
+        return s;
+    *fp_us = timed ? (float)(1e3 * fp_sum / timed) : 0.0f;
+    *bp_us = timed ? (float)(1e3 * bp_sum / timed) : 0.0f;
+    return UIR_OK;
 }
 
 uir_status uir_task_process(uir_task *t, const uir_buffer *inputs, uir_buffer *output)

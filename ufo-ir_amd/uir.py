@@ -162,6 +162,8 @@ def load():
     L.uir_task_setup.argtypes = [vp, vp]
     L.uir_task_get_requisition.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Requisition)]
     L.uir_task_process.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer)]
+    L.uir_task_time_kernels.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer), u32, C.POINTER(f32),
+                                        C.POINTER(f32)]
     L.uir_projector_forward.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer)]
     L.uir_projector_backward.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer)]
     L.uir_parallel_projector_subset_fp.argtypes = [vp, C.POINTER(Buffer), C.POINTER(Buffer), C.POINTER(Subset)]
@@ -414,6 +416,14 @@ class Task:
     def process(self, inp, out):
         bi, bo = inp.buffer(), out.buffer()
         check(load().uir_task_process(self.handle, C.byref(bi), C.byref(bo)))
+
+    def time_kernels(self, inp, out, stride=1):
+        """(FP, BP) average launch durations in us over one timed sweep (sart)."""
+        f, b = C.c_float(), C.c_float()
+        bi, bo = inp.buffer(), out.buffer()
+        check(load().uir_task_time_kernels(self.handle, C.byref(bi), C.byref(bo), int(stride), C.byref(f),
+                                           C.byref(b)))
+        return f.value, b.value
 
     def luts(self):
         s, c, n = C.POINTER(C.c_float)(), C.POINTER(C.c_float)(), C.c_uint32()
