@@ -23,7 +23,7 @@ result on the host) and the served slice-iterations/s are reported beside the
 batched throughput.  --serve-c5 runs the split-Bregman TV serving workload
 instead (image edge N in {512, 1024, 2048}, angles U{90..900}, bins
 ceil(N sqrt 2), lambda ~ logU(1e-3, 1e-1), mu = 1, 30 outer x 5 inner sweeps,
-FP16 storage) and prints its own JSON line.
+FP16 sinograms with FP32 images and math) and prints its own JSON line.
 
 Ray-weight floor: --ray-weight-min (default 0, the reference's inv: every
 ray with FP(1) > 0 is weighted 1 / FP(1)).  With a floor, rays that cross
@@ -149,14 +149,42 @@ def serve_c5(args, uir):
                                ray_weight_min=RAY_WEIGHT_MIN)
         return req, np.ascontiguousarray(pool[n][:a])
 
-    out = run_serving(uir, "f16", make, args.serve_qps_c5, args.serve_requests_c5, 20261206)
+    storage = args.c5_storage
+    out = run_serving(uir, storage, make, args.serve_qps_c5, args.serve_requests_c5, 20261206)
+    print("c5: served %d requests" % args.serve_requests_c5, file=sys.stderr, flush=True)
+
+    # accuracy: requests of the workload's kind (full 30 x 5 sweeps) served
+    # together, against the FP64 oracle of the method
+    import oracle as O
+    checks = [(512, 200, 0.01), (512, 400, 0.05), (1024, 120, 0.002)]
+    reqs = []
+    for (n, a, lam) in checks:
+        d = math.ceil(n * math.sqrt(2))
+        reqs.append((uir.make_request(n, d, a, float(np.float32(np.pi / a)), "sbtv-sirt", iterations=30,
+                                      inner_iterations=5, relaxation=1.0, lam=lam, mu=1.0,
+                                      ray_weight_min=RAY_WEIGHT_MIN), np.ascontiguousarray(pool[n][:a])))
+    with uir.Server(0, storage) as srv:
+        tickets = [srv.submit(r, sn) for r, sn in reqs]
+        imgs = [srv.wait(t, timeout=600)[0] for t in tickets]
+    acc = []
+    for (n, a, lam), (r, sn), img in zip(checks, reqs, imgs):
+        d = math.ceil(n * math.sqrt(2))
+        g = O.Geometry(n, d, a, np.float32(np.pi / a))
+        ref = O.sbsirt64(g, sn, 30, 5, relax=1.0, lam=lam, mu=1.0, ray_weight_min=RAY_WEIGHT_MIN)
+        acc.append({"n": n, "angles": a, "lambda": lam, "rel_l2_vs_fp64": O.rel_l2(img, ref),
+                    "psnr_db": round(O.psnr(img, ref), 2)})
+        print("c5: accuracy N %d, %d angles: %.3e" % (n, a, acc[-1]["rel_l2_vs_fp64"]), file=sys.stderr, flush=True)
     line = {"metric": "split-Bregman TV serving: p50/p99 latency and slice-iters/s", "value": out["slice_iters_per_s"],
-            "unit": "slice-iters/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f16-storage/f32-math",
+            "unit": "slice-iters/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": {"f16": "f16 storage, f32 math", "f16-sino": "f16 sinograms, f32 images, f32 math"}.get(
+                storage, storage),
             "data": "synthetic smooth sinograms (content does not change the work)",
             "config": {"workload": "SBTV_SIRT requests, N in {512, 1024, 2048}, angles U{90..900}, bins ceil(N sqrt 2), "
                                    "lambda logU(1e-3, 1e-1), mu 1, 30 outer x 5 inner sweeps, Poisson arrivals",
-                       "qps": args.serve_qps_c5, "requests": args.serve_requests_c5, "storage": "f16"},
-            "serving": out}
+                       "qps": args.serve_qps_c5, "requests": args.serve_requests_c5, "storage": storage},
+            "serving": out,
+            "accuracy": {"requests": acc, "reference": "FP64 SBTV_SIRT oracle (orc64_sbsirt), same sweeps, "
+                                                       "ray_weight_min %g" % RAY_WEIGHT_MIN}}
     print(json.dumps(line), flush=True)
 
 
@@ -481,6 +509,9 @@ def main():
     ap.add_argument("--serve-c5", action="store_true", help="run the split-Bregman TV serving workload only")
     ap.add_argument("--serve-qps-c5", type=float, default=10.0)
     ap.add_argument("--serve-requests-c5", type=int, default=200)
+    ap.add_argument("--c5-storage", default="f16-sino", choices=["f16", "f16-sino", "f32"],
+                    help="storage of the split-Bregman serving workload (f16-sino: FP16 sinograms, FP32 images; "
+                         "f16 images random-walk to ~6e-3 from FP64 over 30 x 5 sweeps)")
     ap.add_argument("--c2", action="store_true", help="time plain SART on 1024x1024, 720 angles, 30 iterations")
     ap.add_argument("--c2-iters", type=int, default=30)
     ap.add_argument("--c3", action="store_true", help="batched SIRT over 512 slices of uneven size")
@@ -511,6 +542,7 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
     if args.serve_c5:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         serve_c5(args, uir)
         return
     if args.c2:
@@ -699,8 +731,11 @@ def main():
         cpu = {"value": round(cpu_rate, 4), "unit": "slice-iters/s", "cores": O.num_threads(), "kind": "port",
                "sample": "1 slice at 1024x1024x720x1448, ref-compat FP32 SIRT, (t(3 it) - t(1 it)) / 2"}
         t1 = time.time()
-        ref64 = {rmin: O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax, ray_weight_min=rmin)
-                 for rmin in acc_imgs}
+        ref64 = {}
+        for rmin in acc_imgs:
+            ref64[rmin] = O.sirt64(g, sinos[0].astype(np.float64), args.accuracy_iters, relax, ray_weight_min=rmin)
+            print("accuracy: FP64 SIRT, %d iterations, ray_weight_min %g: %.1f s"
+                  % (args.accuracy_iters, rmin, time.time() - t1), file=sys.stderr, flush=True)
         t_ref = time.time() - t1
         st = args.storage
         acc = {"iterations": args.accuracy_iters,

@@ -111,8 +111,10 @@ def test_mixed_batch_largest_slices_match_fp64(res, mixed_refs, storage, tol):
 
 def test_sbtv_sirt_serving_large_requests_match_fp64():
     """SBTV_SIRT requests of the split-Bregman serving workload's sizes, served
-    together in FP16 storage (6 outer x 5 inner sweeps to keep the FP64 oracle
-    short; the serving workload runs 30 x 5)."""
+    together with FP16 sinograms and FP32 images, the workload's storage
+    (6 outer x 5 inner sweeps to keep the FP64 oracle short; the workload runs
+    30 x 5).  With FP16 images the rounding of the image after every sweep
+    random-walks: 9.6e-4 after these 30 sweeps, about 6e-3 after 150."""
     outer, inner, mu = 6, 5, 1.0
     cases = [(512, 725, 400, 0.002), (1024, 1449, 300, 0.02), (2048, 2897, 90, 0.08)]
     reqs, sinos = [], []
@@ -124,7 +126,7 @@ def test_sbtv_sirt_serving_large_requests_match_fp64():
         reqs.append(req)
         sinos.append(b)
     print("case sbtv-sirt serving", [c[:3] for c in cases], flush=True)
-    with uir.Server(0, "f16") as srv:
+    with uir.Server(0, "f16-sino") as srv:
         tickets = [srv.submit(r, s) for r, s in zip(reqs, sinos)]
         out = [srv.wait(t, timeout=120) for t in tickets]
     for k, (n, d, a, lam) in enumerate(cases):

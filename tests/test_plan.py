@@ -187,3 +187,75 @@ def test_request_with_no_active_detector_is_refused(axis):
     # the default (-1, centred) and an in-range shift are accepted
     assert uir.request_footprint(uir.make_request(256, 363, 180, float(np.pi / 180))) > 0
     assert uir.request_footprint(uir.make_request(256, 363, 180, float(np.pi / 180), axis_position=190.0)) > 0
+
+
+def _kernel_windows_and_samples(n, d, a, axis_prop, items, s, c, axis, max_items=None):
+    """Replays, in FP32 where the kernel rounds, what k_fp_pairs computes for
+    each chunk of each item: the staging window from the item's corner rays
+    (its first and last active detector at each angle, at the chunk's first
+    and last line), and every active ray's entry index at every line of the
+    chunk (the coordinate anchored per chunk from double, stepped by float
+    additions, rounded by rne_index).  Returns the worst margins: the smallest
+    entry position and the largest position minus the staged width (both
+    must keep the reads inside [0, W)), and the widest window."""
+    f32 = np.float32
+    kPH, kPL, kPW = 96, 16, 192
+    req = f32(axis) * f32(2)
+    diff = f32(d) - req
+    rot = diff / f32(2)
+    tbase = 0.5 + float(rot)
+    thalf = float(f32(0.5) * f32(d))
+    worst_lo, worst_hi, widest = 10 ** 9, -10 ** 9, 0
+    pick = items if max_items is None else items[::max(1, len(items) // max_items)]
+    for dirn, a0, na, d0, l_lo, l_hi in pick:
+        dets = np.arange(d0, d0 + DETS)
+        on_det = _active(np.minimum(dets, d - 1), d, axis) & (dets < d)
+        if not on_det.any():
+            continue
+        ang = np.arange(a0, a0 + na)
+        st, ct = s[ang], c[ang]
+        hor = dirn == 0
+        det_step = -1.0 / st if hor else 1.0 / ct
+        sdv = ct / st if hor else st / ct
+        # ray anchors of the active rays: p0 = ((tbase + s) - thalf) det_step - N/2 sd + N/2
+        sa = dets[on_det].astype(np.float64)
+        p0 = ((tbase + sa[None, :]) - thalf) * det_step[:, None] - 0.5 * n * sdv[:, None] + 0.5 * n
+        sdf = sdv.astype(f32)
+        s_first, s_last = sa.min(), sa.max()
+        cp = np.stack([((tbase + sx) - thalf) * det_step - 0.5 * n * sdv + 0.5 * n for sx in (s_first, s_last)])
+        for l0 in range(l_lo, l_hi, kPL):
+            nl = min(kPL, l_hi - l0)
+            nm1 = f32(nl - 1)
+            av = (cp + float(l0) * sdv[None, :]).astype(f32)
+            bv = av + nm1 * sdf[None, :]
+            lo, hi = min(av.min(), bv.min()), max(av.max(), bv.max())
+            pmin = int(np.floor(lo - f32(0.5))) - 1
+            W = int(np.floor(hi - f32(0.5))) + 3 - pmin
+            if W > kPW:
+                raise AssertionError("window overflow", dirn, a0, na, d0, l0, W)
+            if pmin + W <= 0 or pmin >= n:
+                continue
+            widest = max(widest, W)
+            off = (f32(1.0) + f32(pmin)) + f32(kPH)
+            u = (p0 + float(l0) * sdv[:, None]).astype(f32) - off
+            for _ in range(nl):
+                idx = (u + f32(12582912.0)).view(np.int32) - 0x4B400000
+                pos = idx + kPH
+                worst_lo = min(worst_lo, int(pos.min()))
+                worst_hi = max(worst_hi, int(pos.max()) - W)
+                u = u + sdf[:, None]
+    return worst_lo, worst_hi, widest
+
+
+@pytest.mark.parametrize("n,d,a,axis_prop,max_items", [
+    (256, 363, 180, -1.0, None),
+    (192, 272, 180, 140.5, None),
+    (1024, 1448, 720, -1.0, 160),
+    (2048, 2897, 900, -1.0, 60),     # items of more than 64 chunks: the 64-chunk window batches
+])
+def test_pair_kernel_reads_stay_in_the_staged_window(n, d, a, axis_prop, max_items):
+    items, _, s, c, axis = _plan(n, d, a, float(np.float32(np.pi / a)), axis_prop, 16, 192)
+    lo, hi, widest = _kernel_windows_and_samples(n, d, a, axis_prop, items, s, c, axis, max_items)
+    assert lo >= 0, lo          # no entry left of the window
+    assert hi <= -1, hi         # no entry at or right of position W
+    assert widest <= 192
