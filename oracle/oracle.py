@@ -27,7 +27,10 @@ class Subset(C.Structure):
 
 class Geom(C.Structure):
     _fields_ = [("n", C.c_int), ("n_dets", C.c_int), ("n_angles", C.c_int), ("axis", C.c_float),
-                ("sin_lut", C.c_void_p), ("cos_lut", C.c_void_p)]
+                ("sin_lut", C.c_void_p), ("cos_lut", C.c_void_p), ("model", C.c_int)]
+
+
+MODELS = {"joseph": 0, "line": 1}
 
 
 class AsdPocsParams(C.Structure):
@@ -89,6 +92,9 @@ def lib():
         L.orc_sbtv_apply.argtypes = [G, f32p, f32p, C.c_float, C.c_float]
         L.orc_cgs_dense.argtypes = [f32p, C.c_uint32, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_int]
         L.orc_cgs_dense.restype = C.c_int
+        L.orc64_line_chord.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc64_line_chord.restype = C.c_double
+        L.orc64_line_dir.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_sbtv_shrink.argtypes = [f32p, f32p, C.c_size_t, C.c_float, f32p, f32p]
         L.orc64_fp.argtypes = [G, f64p, f64p]
         L.orc64_bp.argtypes = [G, f64p, f64p]
@@ -117,15 +123,16 @@ class Geometry:
     axis: projector "axis_position" (-1 selects n_dets / 2).
     """
 
-    def __init__(self, n, n_dets, n_angles, step, axis=-1.0):
+    def __init__(self, n, n_dets, n_angles, step, axis=-1.0, model="joseph"):
         self.n, self.n_dets, self.n_angles = int(n), int(n_dets), int(n_angles)
+        self.model = model
         self.step = float(step)
         self.sin = np.empty(self.n_angles, np.float32)
         self.cos = np.empty(self.n_angles, np.float32)
         lib().orc_lut(self.n_angles, self.step, self.sin, self.cos)
         self.axis = lib().orc_axis_resolve(float(axis), self.n_dets)
         self._g = Geom(self.n, self.n_dets, self.n_angles, self.axis,
-                       self.sin.ctypes.data, self.cos.ctypes.data)
+                       self.sin.ctypes.data, self.cos.ctypes.data, MODELS[model])
         self.runs = self._subsets(lib().orc_runs)
         self.angle_subsets = self._subsets(lib().orc_angle_subsets)
 

@@ -955,10 +955,131 @@ static inline double texel64(const double *img, int w, int h, int i, int j)
     return (i >= 0 && i < w && j >= 0 && j < h) ? img[(size_t)j * w + i] : 0.0;
 }
 
+/* ---- the line model (a second projection model; no reference counterpart) ----
+ *
+ * Ray s at angle theta is the line X cos - Y sin = t_s, t_s = s + 1/2 - axis,
+ * with (cos, sin) the LUT pair scaled to unit length (the float LUT pair is a
+ * unit vector only to ~3e-8) and a component below 2^-20 taken as 0: the
+ * float LUT holds cos(pi/2) as -4.4e-8, the rounding of an exact zero, and a
+ * ramp that narrow would make the weight of a ray along a pixel edge depend on
+ * the last bits of d:
+ * the geometry of the reference's BP (projector-parallel-joseph.cl:174-199,
+ * u = X cos - Y sin + axis).  Its weight on pixel (x, y), centre (X, Y) =
+ * (x + 1/2 - N/2, y + 1/2 - N/2), is the length of that line inside the unit
+ * square: with d = t_s - (X cos - Y sin), M = max(|cos|, |sin|) and
+ * m = min(|cos|, |sin|), a trapezoid in d of height 1/M, flat for
+ * |d| <= (M - m)/2 and zero from |d| >= (M + m)/2.  For m = 0 (theta = 0) the
+ * pixel spans d in [-1/2, 1/2) so that a ray along an edge counts once.
+ * Every ray is active (no inactive-detector rule), and the BP is the exact
+ * transpose of the FP. */
+void orc64_line_dir(float cos_lut, float sin_lut, double *c, double *s)
+{
+    const double r = hypot((double)cos_lut, (double)sin_lut);
+    double uc = cos_lut / r, us = sin_lut / r;
+    const double tiny = 1.0 / 1048576.0;
+    if (fabs(uc) < tiny) {
+        uc = 0.0;
+        us = us < 0.0 ? -1.0 : 1.0;
+    } else if (fabs(us) < tiny) {
+        us = 0.0;
+        uc = uc < 0.0 ? -1.0 : 1.0;
+    }
+    *c = uc;
+    *s = us;
+}
+
+double orc64_line_chord(double d, double c, double s)
+{
+    const double ac = fabs(c), as = fabs(s);
+    const double M = ac > as ? ac : as, m = ac > as ? as : ac;
+    const double h = 1.0 / M;
+    if (m == 0.0)
+        return (d >= -0.5 && d < 0.5) ? h : 0.0;
+    double v = (0.5 * (M + m) - fabs(d)) / m;
+    if (v <= 0.0)
+        return 0.0;
+    if (v > 1.0)
+        v = 1.0;
+    return h * v;
+}
+
+/* FP of the line model, ray by ray: along the ray's major axis, the (at most
+ * three) pixels of each line whose square the ray can cross. */
+static void fp64_line(const orc_geom *g, const double *vol, double *sino)
+{
+    const int N = g->n, D = g->n_dets, A = g->n_angles;
+    const double axis = g->axis, h = 0.5 * N;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int ang = 0; ang < A; ++ang) {
+        for (int s = 0; s < D; ++s) {
+            double c, sn;
+            orc64_line_dir(g->cos_lut[ang], g->sin_lut[ang], &c, &sn);
+            const double t = (double)s + 0.5 - axis;
+            double acc = 0.0;
+            if (fabs(sn) > fabs(c)) { /* columns: Y on the ray = (X cos - t) / sin */
+                for (int x = 0; x < N; ++x) {
+                    const double X = x + 0.5 - h, Yr = (X * c - t) / sn;
+                    const int y0 = (int)floor(Yr + h - 0.5) - 1;
+                    for (int y = y0; y < y0 + 4; ++y) {
+                        if (y < 0 || y >= N)
+                            continue;
+                        const double Y = y + 0.5 - h;
+                        acc += orc64_line_chord(t - (X * c - Y * sn), c, sn) * vol[(size_t)y * N + x];
+                    }
+                }
+            } else { /* rows: X on the ray = (t + Y sin) / cos */
+                for (int y = 0; y < N; ++y) {
+                    const double Y = y + 0.5 - h, Xr = (t + Y * sn) / c;
+                    const int x0 = (int)floor(Xr + h - 0.5) - 1;
+                    for (int x = x0; x < x0 + 4; ++x) {
+                        if (x < 0 || x >= N)
+                            continue;
+                        const double X = x + 0.5 - h;
+                        acc += orc64_line_chord(t - (X * c - Y * sn), c, sn) * vol[(size_t)y * N + x];
+                    }
+                }
+            }
+            sino[(size_t)ang * D + s] = acc;
+        }
+    }
+}
+
+/* BP of the line model, pixel by pixel: the (at most three) bins of each
+ * angle whose ray crosses the pixel; the same weights as fp64_line. */
+static void bp64_line(const orc_geom *g, double *vol, const double *sino)
+{
+    const int N = g->n, D = g->n_dets, A = g->n_angles;
+    const double axis = g->axis, h = 0.5 * N;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int y = 0; y < N; ++y) {
+        for (int x = 0; x < N; ++x) {
+            const double X = x + 0.5 - h, Y = y + 0.5 - h;
+            double acc = 0.0;
+            for (int ang = 0; ang < A; ++ang) {
+                double c, sn;
+                orc64_line_dir(g->cos_lut[ang], g->sin_lut[ang], &c, &sn);
+                const double tc = X * c - Y * sn;
+                const int s0 = (int)floor(tc + axis - 0.5) - 1;
+                for (int s = s0; s < s0 + 4; ++s) {
+                    if (s < 0 || s >= D)
+                        continue;
+                    acc += orc64_line_chord(((double)s + 0.5 - axis) - tc, c, sn) * sino[(size_t)ang * D + s];
+                }
+            }
+            vol[(size_t)y * N + x] = acc;
+        }
+    }
+}
+
 /* Same rays as orc_fp_subset (including the half-pixel offsets of the
- * reference's start coordinate) but v_i = v0 + i * step evaluated in double. */
+ * reference's start coordinate) but v_i = v0 + i * step evaluated in double.
+ * With g->model = 1, the line model above. */
 void orc64_fp(const orc_geom *g, const double *vol, double *sino)
 {
+    if (g->model == 1) {
+        fp64_line(g, vol, sino);
+        return;
+    }
     const int N = g->n, D = g->n_dets, A = g->n_angles;
     const double axis = g->axis;
     const double required_width = 2.0 * axis;
@@ -1004,6 +1125,10 @@ void orc64_fp(const orc_geom *g, const double *vol, double *sino)
  * active detector window, linear interpolation, all in double. */
 void orc64_bp(const orc_geom *g, double *vol, const double *sino)
 {
+    if (g->model == 1) {
+        bp64_line(g, vol, sino);
+        return;
+    }
     const int N = g->n, D = g->n_dets, A = g->n_angles;
     const double axis = g->axis;
     const double diff = (double)D - 2.0 * axis;

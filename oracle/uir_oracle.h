@@ -52,6 +52,9 @@ typedef struct {
     float axis;       /* resolved axis position (>= 0) */
     const float *sin_lut;
     const float *cos_lut;
+    int model;        /* projection model of the FP64 operators: 0 = joseph
+                       * (the reference's), 1 = line (exact ray-pixel
+                       * intersection lengths; no reference counterpart) */
 } orc_geom;
 
 /* ---- geometry (ufo-ir-parallel-projector-task.c) ---- */
@@ -120,6 +123,13 @@ void orc_sbtv(const orc_geom *g, float *b, float *u, uint32_t iters, float lambd
 int orc_sbtv_cgs(const orc_geom *g, const float *rhs, float *x, const float *x0, uint32_t max_iter,
                  float lambda, float mu, int dot_mode);
 void orc_sbtv_apply(const orc_geom *g, const float *in, float *out, float lambda, float mu);
+/* The line model's direction: the LUT pair scaled to unit length, with a
+ * component below 2^-20 taken as exactly 0 (and the other as +-1). */
+void orc64_line_dir(float cos_lut, float sin_lut, double *c, double *s);
+/* The line model's weight: the length of the line X cos - Y sin = t inside
+ * the unit pixel whose centre projects to t - d. */
+double orc64_line_chord(double d, double c, double s);
+
 /* The same CGS (ufo-ir-sbtv-task.c:404-585) with a dense (w h) x (w h)
  * operator M (row-major, each row summed in float in order): for pinning the
  * CGS control flow on constructed systems.  Returns the flag. */
