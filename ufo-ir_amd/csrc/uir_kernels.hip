@@ -255,6 +255,9 @@ __global__ __launch_bounds__(256) void k_fp_ref_seg(FpRefArgs p, const float *__
     }
     const float *st = starts + o * n_blocks;
     const int q = lane & 3; // this lane's quarter of its block
+    // the epilogue's operands, loaded while the samples are taken
+    const float r_in = lane == 0 ? p.r_sino[o] : 0.0f;
+    const float rw_in = (lane == 0 && p.ray_w) ? p.ray_w[o] : 0.0f;
     float acc = 0.0f;
     for (int jb0 = 0; jb0 < n_blocks; jb0 += 16) {
         const int jb = jb0 + (lane >> 2);
@@ -295,11 +298,13 @@ __global__ __launch_bounds__(256) void k_fp_ref_seg(FpRefArgs p, const float *__
             const float omw = 1.0f - w[k];
             val[k] = omw * t0 + w[k] * t1;
         }
-        // the block sum, lane q continuing from lane q - 1 (:84-89 order)
+        // the block sum, lane q continuing from lane q - 1 (:84-89 order);
+        // the hand-over is a DPP row shift by one lane (lanes of a block
+        // share a row of 16)
         float run = 0.0f;
 #pragma unroll
         for (int stage = 0; stage < 4; ++stage) {
-            const float from = __shfl_up(run, 1, 4);
+            const float from = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(run), 0x111, 0xf, 0xf, false));
             if (q == stage) {
                 float a = stage == 0 ? 0.0f : from;
 #pragma unroll
@@ -312,13 +317,17 @@ __global__ __launch_bounds__(256) void k_fp_ref_seg(FpRefArgs p, const float *__
         // block sums in block order (:89)
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj) {
-            const float bs = __shfl(run, 4 * jj + 3);
+            const float bs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(run), 4 * jj + 3));
             if (jb0 + jj < n_blocks)
                 acc = acc + bs;
         }
     }
-    if (lane == 0)
-        fp_epilogue(p, o, acc);
+    if (lane == 0) { // fp_epilogue with the prefetched operands (:92-94)
+        float w = r_in + acc * p.corr;
+        if (p.ray_w)
+            w = w * rw_in;
+        p.w_sino[o] = w;
+    }
 }
 
 // --------------------------------------------------------------------------
