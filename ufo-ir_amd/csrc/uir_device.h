@@ -79,6 +79,41 @@ int uirk_bp_ref(const float *r_vol, float *w_vol, const float *sino, int n_angle
                 const float *sin_lut, const float *cos_lut, int n, int n_dets, float axis,
                 float relax, const uirk_runs *runs, const float *pix_w, int posc, void *stream);
 
+/* ---- The line projection model (uir_line.hip) ----
+ * Ray s of angle a is the line X c - Y s_ = t, t = s + 1/2 - axis, with (c, s_)
+ * the angle's LUT pair scaled to unit length and a component below 2^-20
+ * taken as 0; its weight on a pixel is the length of that line inside the
+ * pixel's unit square (centre (x + 1/2 - n/2, y + 1/2 - n/2)).  With d the
+ * ray's offset from the pixel centre, M = max(|c|, |s_|) and m = min, the
+ * length is (1/M) clamp(((M + m)/2 - |d|) / m, 0, 1); for m = 0 the pixel
+ * takes d in [-1/2, 1/2).  Per sample of a ray (one line of pixels along
+ * its major axis) and per angle of a pixel the model has two taps at
+ * fractional position f in [0, 1) between them, with weights linear in f.
+ * Constants per angle and geometry, computed on the host in double. */
+typedef struct {
+    double fp_base, fp_tk; /* FP: position of detector s at sample i: v = fp_base + s fp_tk + i fp_step */
+    float fp_step;
+    float fp_k, fp_b0, fp_b1; /* FP taps: w0 = clamp(fp_b0 - fp_k f), w1 = clamp(fp_b1 + fp_k f), x 1/M */
+    float hm;                 /* 1/M */
+    float c, s;               /* unit direction */
+    float bp_off;             /* BP: u = x c - Y s_ + bp_off, bins floor(u), floor(u) + 1 */
+    float bp_k, bp_b0, bp_b1; /* BP taps: w0 = clamp(bp_b0 - bp_k f, 0, 1/M), w1 = clamp(bp_b1 + bp_k f, 0, 1/M) */
+    int32_t family;           /* 0: |c| >= |s_|, a sample per image row (taps x, x + 1); 1: per column (y, y + 1) */
+    int32_t aligned;          /* m == 0: one tap, by the half-open rule */
+    int32_t tie0;             /* aligned FP: f == 1/2 belongs to tap 0 */
+} uirk_line_angle;
+
+/* FP of angles [a0, a0 + na), all of one family: w = (r + corr FP) (* ray_w
+ * when ray_w != NULL), r_sino may equal w_sino.  few_rays selects the layout
+ * for launches of few rays (one angle): each ray spread over 16 lanes. */
+int uirk_fp_line(const float *vol, const float *r_sino, float *w_sino, const float *ray_w,
+                 const uirk_line_angle *d_tab, int family, int n, int n_dets, int a0, int na, float corr,
+                 int few_rays, void *stream);
+/* BP of angles [a0, a0 + na): pix_w == NULL: w_vol = r_vol + relax BP;
+ * else w_vol = (relax BP) pix_w + r_vol, clamped at 0 when posc. */
+int uirk_bp_line(const float *r_vol, float *w_vol, const float *sino, const float *pix_w, int posc,
+                 const uirk_line_angle *d_tab, int n, int n_dets, int a0, int na, float relax, void *stream);
+
 /* Elementwise (ufo-ir-basic-ops.cl). */
 int uirk_set(float *out, size_t n, float v, void *stream);
 int uirk_inv(const float *in, float *out, size_t n, void *stream);

@@ -413,6 +413,64 @@ float uir_geom_axis(float axis_property, uint32_t n_dets)
     return axis_property;
 }
 
+/* Constants of the line model per angle (uirk_line_angle) for an n x n image
+ * and rotation axis `axis`: the unit direction from the float LUT pair,
+ * components below 2^-20 taken as 0 (float cos(pi/2) is -4.4e-8, the
+ * rounding of an exact zero); the FP position of detector s at sample i is
+ * v = (t + Y s_) / c + n/2 - 1/2 (family 0, Y the centre of row i) or
+ * (X c - t) / s_ + n/2 - 1/2 (family 1, X the centre of column i) with
+ * t = s + 1/2 - axis, and the two taps at floor(v) and floor(v) + 1 lie at
+ * ray offsets of magnitude M f and M (1 - f); the BP position of pixel (x, y)
+ * is u = X c - Y s_ + axis - 1/2 with offsets f and 1 - f to bins floor(u)
+ * and floor(u) + 1. */
+void uir_line_table(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uint32_t n, float axis,
+                    uirk_line_angle *out)
+{
+    const double h = 0.5 * n, tiny = 1.0 / 1048576.0;
+    for (uint32_t a = 0; a < n_angles; ++a) {
+        uirk_line_angle *L = &out[a];
+        memset(L, 0, sizeof *L);
+        const double r = hypot((double)cos_lut[a], (double)sin_lut[a]);
+        double c = r > 0 ? cos_lut[a] / r : 1.0, s = r > 0 ? sin_lut[a] / r : 0.0;
+        if (fabs(c) < tiny) {
+            c = 0.0;
+            s = s < 0 ? -1.0 : 1.0;
+        } else if (fabs(s) < tiny) {
+            s = 0.0;
+            c = c < 0 ? -1.0 : 1.0;
+        }
+        const double M = fmax(fabs(c), fabs(s)), m = fmin(fabs(c), fabs(s));
+        L->family = fabs(s) > fabs(c);
+        L->aligned = m == 0.0;
+        L->hm = (float)(1.0 / M);
+        L->c = (float)c;
+        L->s = (float)s;
+        double step, tk;
+        if (L->family == 0) {
+            step = s / c;
+            tk = 1.0 / c;
+            L->tie0 = c < 0;
+        } else {
+            step = c / s;
+            tk = -1.0 / s;
+            L->tie0 = s > 0;
+        }
+        L->fp_step = (float)step;
+        L->fp_tk = tk;
+        L->fp_base = (0.5 - h) * step + h - 0.5 + (0.5 - (double)axis) * tk;
+        L->bp_off = (float)((0.5 - h) * c + (double)axis - 0.5);
+        if (!L->aligned) {
+            const double q = m / M;
+            L->fp_k = (float)(1.0 / q);
+            L->fp_b0 = (float)((1.0 + q) / (2.0 * q));
+            L->fp_b1 = (float)((q - 1.0) / (2.0 * q));
+            L->bp_k = (float)(1.0 / (M * m));
+            L->bp_b0 = (float)((M + m) / (2.0 * M * m));
+            L->bp_b1 = (float)(((M + m) / 2.0 - 1.0) / (M * m));
+        }
+    }
+}
+
 /* The active detector window of the back projection (projector-parallel-joseph.cl:
  * 174-182): u is clamped to [edge0, edge1], so the bins it interpolates lie in
  * [floor(edge0 - 0.5), floor(edge1 - 0.5)].  edge0 >= 0.5 and edge1 <= D - 0.5

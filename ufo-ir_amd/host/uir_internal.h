@@ -87,6 +87,8 @@ void uir_geom_lut(uint32_t n_angles, float step, float *sin_lut, float *cos_lut)
 uint32_t uir_geom_runs(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uir_subset *out);
 float uir_geom_axis(float axis_property, uint32_t n_dets);
 int uir_geom_active_bins(uint32_t n_dets, float axis, int *lo, int *hi);
+void uir_line_table(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uint32_t n, float axis,
+                    uirk_line_angle *out);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,14 @@ struct uir_task {
     uirk_xcd_part fp_part, bp_part;
     float *d_trig;
     int *d_err;
+    /* the line model (model "line"): per-angle constants of the current
+     * geometry (host copy and device table) and the angles' family runs */
+    int line;
+    uint32_t lut_gen, line_gen, line_n, line_angles;
+    float line_axis;
+    uirk_line_angle *h_line, *d_line;
+    uir_subset *line_runs;
+    uint32_t n_line_runs;
 
     /* methods (ufo-ir-method-task.c:74-85 and the method tasks) */
     uir_task *projector;
@@ -73,7 +81,7 @@ struct uir_task {
     hipGraph_t sart_graph;
     hipGraphExec_t sart_exec;
     struct {
-        const void *x, *b, *tmp, *ray_w, *sin, *cos, *starts;
+        const void *x, *b, *tmp, *ray_w, *sin, *cos, *starts, *line;
         uint32_t n, D, A;
         float axis, relax, corr;
     } sart_key;
@@ -256,6 +264,9 @@ void uir_task_unref(uir_task *t)
     uir_dfree(t->d_bp_items);
     uir_dfree(t->d_trig);
     uir_dfree(t->d_err);
+    free(t->h_line);
+    uir_dfree(t->d_line);
+    free(t->line_runs);
     free(t->diag);
     uir_task_unref(t->projector);
     uir_task_unref(t->df_minimizer);
@@ -458,10 +469,15 @@ uir_status uir_task_setup(uir_task *t, uir_resources *res)
         t->res = res;
     }
     if (t->kind == K_PROJECTOR) {
-        /* ufo-ir-parallel-projector-task.c:360-388 loads projector-parallel-<model>.cl;
-         * the only model shipped is joseph. */
-        if (strcmp(t->model, "joseph") != 0)
-            return uir_set_error(UIR_ERR_SETUP, "no projection model '%s' (only 'joseph')", t->model);
+        /* ufo-ir-parallel-projector-task.c:360-388 loads projector-parallel-<model>.cl
+         * and fails for a model without kernels; the reference ships joseph,
+         * this library also the line model (uir_line.hip) */
+        if (strcmp(t->model, "joseph") == 0)
+            t->line = 0;
+        else if (strcmp(t->model, "line") == 0)
+            t->line = 1;
+        else
+            return uir_set_error(UIR_ERR_SETUP, "no projection model '%s' (joseph or line)", t->model);
         return UIR_OK;
     }
     if (!t->projector) /* ufo-ir-method-task.c:212-215 */
@@ -506,6 +522,7 @@ static uir_status proj_build_luts(uir_task *p, uint32_t n_angles)
         HIPTRY(hipStreamSynchronize(st), "LUT upload sync");
     }
     p->lut_n = n_angles;
+    p->lut_gen++;
     return UIR_OK;
 }
 
@@ -635,6 +652,68 @@ done:
     return s;
 }
 
+/* The line model's constants for an n x n image in the current geometry,
+ * and the runs of angles of one family. */
+static uir_status line_plan(uir_task *p, uint32_t n)
+{
+    const float axis = proj_axis(p);
+    if ((uint64_t)n * n * 4 >= (1ull << 32) || (uint64_t)p->lut_n * p->n_dets * 4 >= (1ull << 32))
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "image or sinogram of 4 GiB or more");
+    if (p->lut_n > 65535)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "the line model takes at most 65535 angles");
+    if (p->d_line && p->line_gen == p->lut_gen && p->line_n == n && p->line_angles == p->lut_n &&
+        p->line_axis == axis)
+        return UIR_OK;
+    const uint32_t A = p->lut_n ? p->lut_n : 1;
+    uirk_line_angle *h = (uirk_line_angle *)realloc(p->h_line, sizeof *h * A);
+    if (!h)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+    p->h_line = h;
+    uir_subset *runs = (uir_subset *)realloc(p->line_runs, sizeof *runs * A);
+    if (!runs)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+    p->line_runs = runs;
+    uir_line_table(p->h_sin, p->h_cos, p->lut_n, n, axis, h);
+    uint32_t k = 0;
+    for (uint32_t a = 0; a < p->lut_n; ++a) {
+        if (k && runs[k - 1].direction == h[a].family) {
+            runs[k - 1].n++;
+        } else {
+            runs[k].offset = a;
+            runs[k].n = 1;
+            runs[k].direction = h[a].family;
+            ++k;
+        }
+    }
+    p->n_line_runs = k;
+    uir_dfree(p->d_line);
+    p->d_line = NULL;
+    TRY(uir_dmalloc((void **)&p->d_line, sizeof *h * A));
+    HIPTRY(hipMemcpy(p->d_line, h, sizeof *h * p->lut_n, hipMemcpyHostToDevice), "table upload");
+    p->line_gen = p->lut_gen;
+    p->line_n = n;
+    p->line_angles = p->lut_n;
+    p->line_axis = axis;
+    return UIR_OK;
+}
+
+/* Line-model FP of angles [a0, a0 + na): one launch per run of one family. */
+static uir_status line_fp(uir_task *p, uint32_t n, const float *vol, const float *r_sino, float *w_sino,
+                          const float *ray_w, uint32_t a0, uint32_t na, int few_rays, hipStream_t st)
+{
+    TRY(line_plan(p, n));
+    for (uint32_t a = a0; a < a0 + na;) {
+        const int fam = p->h_line[a].family;
+        uint32_t e = a + 1;
+        while (e < a0 + na && p->h_line[e].family == fam)
+            ++e;
+        HIPTRY(uirk_fp_line(vol, r_sino, w_sino, ray_w, p->d_line, fam, (int)n, (int)p->n_dets, (int)a,
+                            (int)(e - a), p->correction_scale, few_rays, st), "FP launch");
+        a = e;
+    }
+    return UIR_OK;
+}
+
 static void proj_slice(const uir_task *p, uint32_t n, uirk_slice *sl)
 {
     memset(sl, 0, sizeof *sl);
@@ -652,6 +731,8 @@ static void proj_slice(const uir_task *p, uint32_t n, uirk_slice *sl)
 static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float *r_sino, float *w_sino,
                           const float *ray_w)
 {
+    if (p->line)
+        return line_fp(p, n, vol, r_sino, w_sino, ray_w, 0, p->lut_n, 0, p->res->stream);
     TRY(proj_plan(p, n));
     uirk_slice sl;
     proj_slice(p, n, &sl);
@@ -667,6 +748,11 @@ static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float
  * vol += relax * BP per run; else the fused SIRT update with POSC option. */
 static uir_status proj_bp_ex(uir_task *p, uint32_t n, const float *sino, float *vol, const float *pix_w, int posc)
 {
+    if (p->line) {
+        TRY(line_plan(p, n));
+        return uir_check_hip(uirk_bp_line(vol, vol, sino, pix_w, posc, p->d_line, (int)n, (int)p->n_dets, 0,
+                                          (int)p->lut_n, p->relaxation, p->res->stream), "BP launch");
+    }
     TRY(proj_plan(p, n));
     if (pix_w && p->n_runs > UIRK_MAX_RUNS)
         return uir_set_error(UIR_ERR_UNSUPPORTED, "more than %d direction runs", UIRK_MAX_RUNS);
@@ -759,6 +845,9 @@ uir_status uir_parallel_projector_subset_fp(uir_task *p, const uir_buffer *volum
     uint32_t n = volume ? volume->dims[0] : 0;
     TRY(check_buf(volume, n, n, "volume"));
     TRY(check_buf(sinogram, p->n_dets, p->lut_n, "sinogram"));
+    if (p->line) /* each angle takes its own family; sub->direction is joseph's */
+        return line_fp(p, n, (const float *)volume->data, (const float *)sinogram->data, (float *)sinogram->data,
+                       NULL, sub->offset, sub->n, sub->n < 8, p->res->stream);
     return uir_check_hip(uirk_fp_ref((const float *)volume->data, (const float *)sinogram->data,
                                      (float *)sinogram->data, NULL, p->d_sin, p->d_cos, (int)n, (int)p->n_dets,
                                      proj_axis(p), p->correction_scale, (int)sub->offset, (int)sub->n,
@@ -774,6 +863,13 @@ uir_status uir_parallel_projector_subset_bp(uir_task *p, uir_buffer *volume, con
     uint32_t n = volume ? volume->dims[0] : 0;
     TRY(check_buf(volume, n, n, "volume"));
     TRY(check_buf(sinogram, p->n_dets, p->lut_n, "sinogram"));
+    if (p->line) {
+        TRY(line_plan(p, n));
+        return uir_check_hip(uirk_bp_line((const float *)volume->data, (float *)volume->data,
+                                          (const float *)sinogram->data, NULL, 0, p->d_line, (int)n, (int)p->n_dets,
+                                          (int)sub->offset, (int)sub->n, p->relaxation, p->res->stream),
+                             "BP launch");
+    }
     uirk_runs r;
     proj_runs_dev(p, sub, 1, &r, 0);
     return uir_check_hip(uirk_bp_ref((const float *)volume->data, (float *)volume->data,
@@ -888,19 +984,37 @@ static uir_status sirt_run(uir_task *t, const float *b, float *x, float relax, u
  * block-start table (starts; without one, 16 threads per ray replay the float
  * stepping); both are bit-identical to the one-thread-per-ray walk.  The BP
  * takes four pixels per thread. */
+static uir_status sart_fp_angle(mctx *c, const float *x, float *sino_tmp, const float *ray_w, float axis,
+                                const float *starts, uint32_t a)
+{
+    uir_task *p = c->p;
+    if (p->line) /* the line model: each ray over 16 lanes */
+        return uir_check_hip(uirk_fp_line(x, sino_tmp, sino_tmp, ray_w, p->d_line, p->h_line[a].family, (int)c->n,
+                                          (int)c->D, (int)a, 1, p->correction_scale, 1, c->st), "FP launch");
+    if (starts)
+        return uir_check_hip(uirk_fp_ref_seg(x, sino_tmp, sino_tmp, ray_w, p->d_sin, p->d_cos, (int)c->n, (int)c->D,
+                                             axis, p->correction_scale, (int)a, 1, -1, starts, c->st), "FP launch");
+    return uir_check_hip(uirk_fp_ref_split(x, sino_tmp, sino_tmp, ray_w, p->d_sin, p->d_cos, (int)c->n, (int)c->D,
+                                           axis, p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
+}
+
+static uir_status sart_bp_angle(mctx *c, float *x, const float *sino_tmp, float axis, uint32_t a)
+{
+    uir_task *p = c->p;
+    if (p->line)
+        return uir_check_hip(uirk_bp_line(x, x, sino_tmp, NULL, 0, p->d_line, (int)c->n, (int)c->D, (int)a, 1,
+                                          p->relaxation, c->st), "BP launch");
+    return uir_check_hip(uirk_bp_ref_angle(x, x, sino_tmp, p->d_sin, p->d_cos, (int)c->n, (int)c->D, axis,
+                                           p->relaxation, (int)a, c->st), "BP launch");
+}
+
 static uir_status sart_sweep(uir_task *t, mctx *c, const float *b, float *x, float *sino_tmp, const float *ray_w,
                              float axis, const float *starts)
 {
     HIPTRY(hipMemcpyAsync(sino_tmp, b, c->ns * 4, hipMemcpyDeviceToDevice, c->st), "copy");
     for (uint32_t a = 0; a < c->A; ++a) {
-        if (starts)
-            HIPTRY(uirk_fp_ref_seg(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
-                                   c->p->correction_scale, (int)a, 1, -1, starts, c->st), "FP launch");
-        else
-            HIPTRY(uirk_fp_ref_split(x, sino_tmp, sino_tmp, ray_w, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D,
-                                     axis, c->p->correction_scale, (int)a, 1, -1, c->st), "FP launch");
-        HIPTRY(uirk_bp_ref_angle(x, x, sino_tmp, c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, axis,
-                                 c->p->relaxation, (int)a, c->st), "BP launch");
+        TRY(sart_fp_angle(c, x, sino_tmp, ray_w, axis, starts, a));
+        TRY(sart_bp_angle(c, x, sino_tmp, axis, a));
     }
     (void)t;
     return UIR_OK;
@@ -937,7 +1051,9 @@ static uir_status sart_prepare(uir_task *t, float *x, float relax, int warm, int
     sp->starts = NULL;
     /* the rays' block starts: the float stepping taken once per call */
     const size_t starts_bytes = (size_t)c->A * c->D * ((c->n + 63) / 64) * 4;
-    if (table && starts_bytes <= SART_STARTS_MAX_BYTES) {
+    if (c->p->line) /* the line model's table, built by the FP above */
+        TRY(line_plan(c->p, c->n));
+    else if (table && starts_bytes <= SART_STARTS_MAX_BYTES) {
         TRY(tmp_get(t, 25, starts_bytes, &sp->starts));
         HIPTRY(uirk_fp_block_starts(c->p->d_sin, c->p->d_cos, (int)c->n, (int)c->D, sp->axis, (int)c->A, -1,
                                     sp->starts, c->st), "block starts");
@@ -956,7 +1072,7 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
     const float axis = sp.axis;
     /* 2 A launches per sweep: capture one sweep and replay it (:208-220) */
     const int same = t->sart_exec && t->sart_key.x == x && t->sart_key.b == b && t->sart_key.tmp == sino_tmp &&
-                     t->sart_key.starts == starts &&
+                     t->sart_key.starts == starts && t->sart_key.line == (c->p->line ? c->p->d_line : NULL) &&
                      t->sart_key.ray_w == ray_w && t->sart_key.sin == c->p->d_sin && t->sart_key.cos == c->p->d_cos &&
                      t->sart_key.n == c->n && t->sart_key.D == c->D && t->sart_key.A == c->A &&
                      t->sart_key.axis == axis && t->sart_key.relax == c->p->relaxation &&
@@ -985,6 +1101,7 @@ static uir_status sart_run(uir_task *t, const float *b, float *x, float relax, u
         t->sart_key.b = b;
         t->sart_key.tmp = sino_tmp;
         t->sart_key.starts = starts;
+        t->sart_key.line = c->p->line ? c->p->d_line : NULL;
         t->sart_key.ray_w = ray_w;
         t->sart_key.sin = c->p->d_sin;
         t->sart_key.cos = c->p->d_cos;
@@ -1412,18 +1529,11 @@ uir_status uir_task_time_kernels(uir_task *t, const uir_buffer *inputs, uir_buff
         if (tm)
             s = uir_check_hip(hipEventRecord(ev[0], c.st), "hipEventRecord");
         if (s == UIR_OK)
-            s = uir_check_hip(sp.starts ? uirk_fp_ref_seg(x, sp.sino_tmp, sp.sino_tmp, sp.ray_w, c.p->d_sin,
-                                                          c.p->d_cos, (int)c.n, (int)c.D, sp.axis,
-                                                          c.p->correction_scale, (int)a, 1, -1, sp.starts, c.st)
-                                        : uirk_fp_ref_split(x, sp.sino_tmp, sp.sino_tmp, sp.ray_w, c.p->d_sin,
-                                                            c.p->d_cos, (int)c.n, (int)c.D, sp.axis,
-                                                            c.p->correction_scale, (int)a, 1, -1, c.st),
-                              "FP launch");
+            s = sart_fp_angle(&c, x, sp.sino_tmp, sp.ray_w, sp.axis, sp.starts, a);
         if (s == UIR_OK && tm)
             s = uir_check_hip(hipEventRecord(ev[1], c.st), "hipEventRecord");
         if (s == UIR_OK)
-            s = uir_check_hip(uirk_bp_ref_angle(x, x, sp.sino_tmp, c.p->d_sin, c.p->d_cos, (int)c.n, (int)c.D,
-                                                sp.axis, c.p->relaxation, (int)a, c.st), "BP launch");
+            s = sart_bp_angle(&c, x, sp.sino_tmp, sp.axis, a);
         if (s == UIR_OK && tm) {
             float f = 0, g = 0;
             s = uir_check_hip(hipEventRecord(ev[2], c.st), "hipEventRecord");
