@@ -33,6 +33,10 @@ path against the FP64 SIRT both without the floor and with 0.1, and the line
 carries an explicit gate: rel-L2 <= 1e-3 against the FP64 SIRT of the
 headline's own setting.
 
+--line measures SIRT with the line projector (the projector's model "line":
+exact ray-pixel chord lengths) on one slice of the same shape and prints its
+own JSON line.
+
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 import argparse
@@ -266,6 +270,147 @@ def sart_c2(args, uir):
                              "kind": "port",
                              "sample": "the oracle's FP32 SART (the reference's arithmetic), same slice, "
                                        "%d iterations, OpenMP over the host's cores" % iters}}
+    print(json.dumps(line), flush=True)
+
+
+def line_samples(n, d, a, step, axis):
+    """(ray, image line) pairs of the line model whose two taps touch the
+    image: the FP's algorithmic samples.  The model's geometry: the float
+    LUT pair scaled to unit length (components below 2^-20 taken as 0); a ray
+    advances one image line per sample along its major axis, at position
+    v = v0 + i step across it, and touches the image while -1 <= v < n."""
+    ang = (np.arange(a, dtype=np.float32) * np.float32(step)).astype(np.float64)
+    sn = np.sin(ang).astype(np.float32).astype(np.float64)
+    cs = np.cos(ang).astype(np.float32).astype(np.float64)
+    r = np.hypot(cs, sn)
+    cs, sn = cs / r, sn / r
+    tiny = 2.0 ** -20
+    cs, sn = (np.where(np.abs(cs) < tiny, 0.0, np.where(np.abs(sn) < tiny, np.sign(cs), cs)),
+              np.where(np.abs(sn) < tiny, 0.0, np.where(np.abs(cs) < tiny, np.sign(sn), sn)))
+    h = n / 2.0
+    t = np.arange(d) + 0.5 - axis
+    total = 0
+    for k in range(a):
+        c, s = cs[k], sn[k]
+        if abs(s) > abs(c):
+            st, v0 = c / s, (0.5 - h) * c / s + h - 0.5 - t / s
+        else:
+            st, v0 = s / c, (0.5 - h) * s / c + h - 0.5 + t / c
+        if st == 0.0:
+            total += int(np.sum((v0 >= -1) & (v0 < n))) * n
+            continue
+        lo, hi = (-1.0 - v0) / st, (n - v0) / st
+        if st > 0:
+            first, end = np.ceil(lo), np.ceil(hi)
+        else:
+            first, end = np.floor(hi) + 1, np.floor(lo) + 1
+        total += int(np.sum(np.maximum(0, np.minimum(n, end) - np.maximum(0, first))))
+    return total
+
+
+def line_sirt(args, uir):
+    """SIRT with the line projector (exact ray-pixel chord lengths, the
+    projector's model "line") on one slice of --n x --n, --angles angles,
+    --dets bins: slice-iterations/s of the SIRT task (FP32), the durations of
+    its FP and BP (one launch each of the table-driven, LDS-staged kernels in
+    their line mode) from HIP events on the task's stream, accuracy after
+    --accuracy-iters iterations
+    against the oracle's FP64 SIRT over the same model, and that oracle on the
+    host cores."""
+    import oracle as O
+    res = uir.Resources(0)
+    n, a, d = args.n, args.angles, args.dets
+    step = float(np.float32(np.pi / a))
+    axis = d / 2.0
+    g = O.Geometry(n, d, a, np.float32(step), model="line")
+    fwd = uir.Task("parallel-projector", model="line", step=step, is_forward=True, angles_num=a, image_size=n,
+                   detectors_num=d).setup(res)
+    img = uir.DeviceArray.from_numpy(phantom(n, 20261203))
+    assert fwd.requisition(img) == (a, d)
+    s = uir.DeviceArray((a, d))
+    fwd.process(img, s)
+    sino = s.numpy().astype(np.float64)
+    sino += np.random.default_rng(20261203).normal(0.0, 0.01 * sino.max(), sino.shape)
+    sino32 = sino.astype(np.float32)
+    ds = uir.DeviceArray.from_numpy(sino32)
+    out = uir.DeviceArray((n, n))
+    pj = uir.Task("parallel-projector", model="line", step=step, is_forward=False, image_size=n)
+    relax = 0.25
+    sirt = uir.Task("sirt", projector=pj, relaxation_factor=relax, num_iterations=1).setup(res)
+    sirt.process(ds, out)
+    timer = uir.Timer()
+    k = max(4, args.steps // 3)
+    times = {}
+    for iters in (1, k + 1):
+        sirt["num_iterations"] = iters
+        res.synchronize()
+        timer.start(res.stream)
+        sirt.process(ds, out)
+        timer.stop(res.stream)
+        times[iters] = timer.elapsed_ms()
+    ms_it = (times[k + 1] - times[1]) / k
+    print("line: SIRT %.3f ms per iteration" % ms_it, file=sys.stderr, flush=True)
+
+    # the projections alone, each repeated `reps` times between two events
+    bwd = uir.Task("parallel-projector", model="line", step=step, is_forward=False, angles_num=a, image_size=n,
+                   detectors_num=d).setup(res)
+    assert bwd.requisition(s) == (n, n)
+    reps = 10
+    fwd.forward(img, s)
+    timer.start(res.stream)
+    for _ in range(reps):
+        fwd.forward(img, s)
+    timer.stop(res.stream)
+    fp_ms = timer.elapsed_ms() / reps
+    bwd.backward(s, out)
+    timer.start(res.stream)
+    for _ in range(reps):
+        bwd.backward(s, out)
+    timer.stop(res.stream)
+    bp_ms = timer.elapsed_ms() / reps
+    fp_samples = line_samples(n, d, a, step, axis)
+    bp_samples = n * n * a
+    print("line: FP %.3f ms (%d samples), BP %.3f ms (%d samples)" % (fp_ms, fp_samples, bp_ms, bp_samples),
+          file=sys.stderr, flush=True)
+    # 8 flop per two-tap sample: two weights (one FMA each) and two FMAs into the sum
+    kname, kms, ks = ("fp", fp_ms, fp_samples) if fp_ms >= bp_ms else ("bp", bp_ms, bp_samples)
+    tfl = 8.0 * ks / (kms * 1e-3) / 1e12
+    nv, ns = n * n, a * d
+    kbytes = (nv * 4 + 2 * ns * 4) if kname == "fp" else (ns * 4 + 2 * nv * 4)
+    gbs = kbytes / (kms * 1e-3) / 1e9
+    roof = {"bound": "valu", "achieved": round(tfl, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tfl / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "k_fp_pairs (line mode)" if kname == "fp" else "k_bp_tab (line mode)",
+            "ms": {"fp": round(fp_ms, 4), "bp": round(bp_ms, 4)},
+            "samples": {"fp": fp_samples, "bp": bp_samples},
+            "note": "FP32 vector work, 8 flop per two-tap sample (two chord weights, two FMAs into the sum); "
+                    "the taps are read from LDS-staged image lines (FP) or sinogram rows (BP); hbm frac %.4f "
+                    "at %.1f GB/s algorithmic (%d bytes: image and sinogram streamed once)"
+                    % (gbs / HBM_PEAK_GBS, gbs, kbytes)}
+    sirt["num_iterations"] = args.accuracy_iters
+    sirt.process(ds, out)
+    got = out.numpy()
+    acc = cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        ref = O.sirt64(g, sino32.astype(np.float64), args.accuracy_iters, relax)
+        t_ref = time.time() - t0
+        print("line: oracle FP64 SIRT, %d iterations in %.1f s" % (args.accuracy_iters, t_ref), file=sys.stderr,
+              flush=True)
+        acc = {"iterations": args.accuracy_iters, "rel_l2_vs_fp64": O.rel_l2(got, ref),
+               "psnr_vs_fp64_db": round(O.psnr(got, ref), 2),
+               "reference": "the oracle's FP64 SIRT over the line model (orc64_sirt), same data"}
+        cpu = {"value": round(args.accuracy_iters / t_ref, 4), "unit": "slice-iterations/s",
+               "cores": O.num_threads(), "kind": "port",
+               "sample": "the oracle's FP64 SIRT over the line model, the same slice, %d iterations (the ray and "
+                         "pixel weights, two more projections, included), OpenMP over the host's cores"
+                         % args.accuracy_iters}
+    line = {"metric": "SIRT slice-iterations/s with the line projector", "value": round(1000.0 / ms_it, 2),
+            "unit": "slice-iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic: 20 random ellipses, 1% Gaussian noise",
+            "config": {"workload": "SIRT task, line projector, %dx%d, %d angles over [0, pi), %d bins, relaxation "
+                                   "%g, one slice per process()" % (n, n, a, d, relax)},
+            "ms_per_iteration": round(ms_it, 4), "roofline": roof, "accuracy": acc, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 
@@ -514,6 +659,8 @@ def main():
                          "f16 images random-walk to ~6e-3 from FP64 over 30 x 5 sweeps)")
     ap.add_argument("--c2", action="store_true", help="time plain SART on 1024x1024, 720 angles, 30 iterations")
     ap.add_argument("--c2-iters", type=int, default=30)
+    ap.add_argument("--line", action="store_true", help="SIRT with the line projector (exact chord lengths) on "
+                    "one --n x --n slice")
     ap.add_argument("--c3", action="store_true", help="batched SIRT over 512 slices of uneven size")
     ap.add_argument("--c3-slices", type=int, default=512)
     ap.add_argument("--c3-iters", type=int, default=100)
@@ -552,6 +699,10 @@ def main():
     if args.c4:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         asdpocs_c4(args, uir)
+        return
+    if args.line:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        line_sirt(args, uir)
         return
     if args.c3:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -135,6 +135,12 @@ typedef struct uir_task uir_task;
  * model (string, "joseph"), angles_num (uint, 0 = from sinogram)
  * (ufo-ir-projector-task.c:68-93, ufo-ir-state-dependent-task.c:75-80,
  *  ufo-ir-parallel-projector-task.c:123-137).
+ * model selects the projection model at setup (:360-388; the reference ships
+ * "joseph"): "joseph", or "line", exact ray-pixel intersection lengths with
+ * every ray active and the BP the transpose of the FP (no reference
+ * counterpart); any other name fails setup with UIR_ERR_SETUP.  The method
+ * tasks, subsets and the kernel timing take the projector's model; the
+ * batch and server paths are joseph only.
  * Extension property image_size (uint, 0 = n_dets): the N x N image edge. */
 uir_task *uir_ir_parallel_projector_task_new(void);
 /* ufo_ir_sirt_task_new, ufo-ir-sirt-task.c:142-146.  Properties: projector

@@ -160,6 +160,10 @@ typedef struct {
     const float *tv;
     float tv_scale;
     int32_t pad_;
+    /* line model (mode 2) only: 8 floats per angle, (fp_k, fp_b, 1/M, 0,
+     * bp_k, bp_b0, bp_b1, 1/M) (uir_line_fast_consts); trig then holds the
+     * model's unit directions */
+    const float *line;
 } uirk_slice;
 
 /* FP work item: angles [a0, a0 + na) of one direction family (0 = horizontal,
@@ -210,7 +214,9 @@ typedef struct {
 #define UIRK_TILE_FP_WMAX(t) ((t).fp_single ? UIRK_FP_WMAX_SINGLE : UIRK_FP_WMAX)
 
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
- * FP32 / FP16 / BF16 storage).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
+ * FP32 / FP16 / BF16 storage), 2 = the fast path with the line model (weights
+ * from each slice's `line` table; the tile's bp_rows (64 or 32) only, the rest
+ * default; BP: pix_w, when given, multiplies the update).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
  * sinogram buffers (b, r); img_dtype: of the image buffers (x, x_out), either
  * dtype or FP32 (the reference-compatible mode takes FP32 only).  d_slices: device
  * descriptor array indexed by item.slice, or NULL to use *one (host memory,
@@ -241,6 +247,10 @@ typedef struct {
  * success, -1 on allocation failure. */
 int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
                  const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax);
+/* The same for the line model: trig pairs are its unit directions and ray
+ * positions lie half a line step further along than joseph's. */
+int uirk_plan_fp_line(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_unit,
+                      const float *cos_unit, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax);
 /* tiles of 64 x tile_rows pixels */
 int uirk_plan_bp(uirk_bp_vec *out, int slice, int n, int tile_rows);
 void uirk_plan_trig(const float *sin_lut, const float *cos_lut, uint32_t n, float *trig);

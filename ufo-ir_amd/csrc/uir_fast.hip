@@ -113,6 +113,15 @@ __device__ __forceinline__ int cvt_flr(float x)
 // end, so either index gives the same value there.
 __device__ __forceinline__ int rne_index(float x) { return __float_as_int(x + 12582912.0f) - 0x4B400000; }
 
+// The line model's remapped fraction at coordinate x of the pair FP, whose
+// entry is rne_index(x): f - 1/2 = x - rne(x) (exact), and
+// phi = clamp((f - 1/2) k + b, 0, 1).
+__device__ __forceinline__ float line_phi(float x, float k, float b)
+{
+    const float fh = x - ((x + 12582912.0f) - 12582912.0f);
+    return __builtin_amdgcn_fmed3f(__builtin_fmaf(fh, k, b), 0.0f, 1.0f);
+}
+
 // x - floor(x); exact for x >= 0 (the callers guarantee it).
 __device__ __forceinline__ float fract_pos(float x) { return __builtin_amdgcn_fractf(x); }
 
@@ -269,12 +278,13 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, const __hip_bfl
 // e + (w - kFpPH - 1/2) d: one FMA per sample, no fract and no second tap,
 // and the entry index is rne_index(w - kFpPH - 1/2), one float add.  The
 // origin kFpPH is the window's middle, which keeps |w - kFpPH| <= kFpPW / 2.
+// PLAIN (the line model) stages e = t_p: its samples remap the fraction.
 constexpr int kFpPW = UIRK_FP_WMAX; // widest pair window
 constexpr int kFpPL = 16;           // lines per pair-staged chunk (25 KB of LDS: 6 workgroups per CU)
 constexpr int kFpPs = kFpPW + 1;    // pair line stride (odd)
 constexpr int kFpPH = kFpPW / 2;
 
-template <typename TI>
+template <typename TI, bool PLAIN>
 __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], const TI *img, int N, bool hor, int l0,
                                                int nl, int pmin, int W)
 {
@@ -308,7 +318,7 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], con
 #pragma unroll
             for (int i = 0; i < kFpPL / 4; ++i) {
                 const float d = t1[i] - t0[i];
-                lds[(wave + 4 * i) * kFpPs + p] = make_float2(__builtin_fmaf(-h, d, t0[i]), d);
+                lds[(wave + 4 * i) * kFpPs + p] = make_float2(PLAIN ? t0[i] : __builtin_fmaf(-h, d, t0[i]), d);
             }
         }
     } else {
@@ -337,7 +347,7 @@ __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], con
 #pragma unroll
             for (int k = 0; k < kPer; ++k) {
                 const float d = t[k + 1] - t[k];
-                lds[l * kFpPs + p0 + k] = make_float2(__builtin_fmaf(-(h0 + (float)k), d, t[k]), d);
+                lds[l * kFpPs + p0 + k] = make_float2(PLAIN ? t[k] : __builtin_fmaf(-(h0 + (float)k), d, t[k]), d);
             }
         }
     }
@@ -609,7 +619,15 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 //    one chunk per lane.
 //  - A sample costs an add (coordinate), a floor-convert, an address, one
 //    8-byte LDS read and two FMA-class operations.
-template <typename T, typename TI, int APT>
+//  - LINE: the line model (exact chord lengths).  Along a ray the chords of
+//    one line of pixels sum to 1/M, so a sample is the lerp e + phi(f) d of
+//    the plain pair (t_p, t_{p+1} - t_p) with the fraction f remapped by
+//    phi(f) = clamp((f - 1/2) fp_k + fp_b, 0, 1) (a ramp of width m / M about
+//    1/2, or a step with the tie rule for axis-parallel rays), times 1/M.
+//    Positions lie half a line step further along than joseph's (the
+//    model's exact geometry), and every ray is active.  Four more
+//    operations per sample.
+template <typename T, typename TI, int APT, bool LINE>
 __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                   const uirk_fp_item *__restrict__ items, int n_items,
                                                   uirk_xcd_part part, int *__restrict__ err)
@@ -640,7 +658,8 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
         const float diff = (float)D - required_width;
         const float rot = diff / 2.0f;
         const bool in_range = s < D;
-        const bool det_on = in_range && !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width));
+        const bool det_on =
+            in_range && (LINE || !((diff < 0 && (float)s < fabsf(diff)) || (diff > 0 && (float)s > required_width)));
         const double tbase = 0.5 + (double)rot;
         const double thalf = (double)(0.5f * (float)D);
 
@@ -655,20 +674,28 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
         float stp[APT], acc[APT];
         double p0d[APT], sd[APT];
         bool on[APT], ray_in[APT];
+        float lk = 0.0f, lb = 0.0f, lhm = 1.0f; // LINE: the angle's phi and 1/M
         {
             // fast_ray, with the angle's quantities computed once
             const int ang = item.a0 + (ang_on ? k_ang : 0);
             const float st = gload(trig, 2 * ang), ct = gload(trig, 2 * ang + 1);
             const double det_step = hor ? -1.0 / (double)st : 1.0 / (double)ct;
             const double sdv = hor ? (double)ct / (double)st : (double)st / (double)ct;
+            if constexpr (LINE) {
+                const float *lt = uniform(S->line);
+                lk = gload(lt, 8 * ang);
+                lb = gload(lt, 8 * ang + 1);
+                lhm = gload(lt, 8 * ang + 2);
+            }
 #pragma unroll
             for (int j = 0; j < APT; ++j) {
                 const int sj = item.d0 + lane % kDL + kDL * j;
                 ray_in[j] = sj < D;
-                const bool dj_on =
-                    ray_in[j] && !((diff < 0 && (float)sj < fabsf(diff)) || (diff > 0 && (float)sj > required_width));
+                const bool dj_on = ray_in[j] && (LINE || !((diff < 0 && (float)sj < fabsf(diff)) ||
+                                                           (diff > 0 && (float)sj > required_width)));
                 on[j] = dj_on && ang_on;
-                p0d[j] = ((tbase + (double)sj) - thalf) * det_step - 0.5 * (double)N * sdv + 0.5 * (double)N;
+                p0d[j] = ((tbase + (double)sj) - thalf) * det_step - 0.5 * (double)N * sdv + 0.5 * (double)N +
+                         (LINE ? 0.5 * sdv : 0.0);
                 sd[j] = sdv;
                 stp[j] = (float)sdv;
                 acc[j] = 0.0f;
@@ -688,6 +715,8 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
                 cend[e] = idx & 1;
                 const int sx = (idx & 2) ? s_last : s_first;
                 fast_ray(trig, item.a0 + (con[e] ? k : 0), hor, (tbase + (double)sx) - thalf, N, cp[e], cs[e]);
+                if (LINE)
+                    cp[e] += 0.5 * cs[e];
             }
         }
 
@@ -737,7 +766,7 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
             if (outside)
                 continue;
             __syncthreads(); // the previous chunk's reads are done
-            fp_stage_pairs(lds, img, N, hor, l0, nl, pmin, W);
+            fp_stage_pairs<TI, LINE>(lds, img, N, hor, l0, nl, pmin, W);
             __syncthreads();
             if (wave * APT >= item.na)
                 continue; // this wave has no angle of the item: it only stages
@@ -779,7 +808,8 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
                 for (int j = 0; j < APT; ++j) {
 #pragma unroll
                     for (int q = 0; q < kQ; ++q)
-                        acc[j] = __builtin_fmaf(w[q][j], v[q][j].y, acc[j] + v[q][j].x);
+                        acc[j] = __builtin_fmaf(LINE ? line_phi(w[q][j], lk, lb) : w[q][j], v[q][j].y,
+                                                acc[j] + v[q][j].x);
                     u[j] = w[kQ - 1][j] + st[j];
                 }
             }
@@ -788,7 +818,7 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
 #pragma unroll
                 for (int j = 0; j < APT; ++j) {
                     const float2 v = P0[rne_index(u[j])];
-                    acc[j] = __builtin_fmaf(u[j], v.y, acc[j] + v.x);
+                    acc[j] = __builtin_fmaf(LINE ? line_phi(u[j], lk, lb) : u[j], v.y, acc[j] + v.x);
                     u[j] += st[j];
                 }
             }
@@ -806,7 +836,7 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
             // :92-94; an inactive detector keeps the value as read
             float w = gload(bsrc, o);
             if (on[j])
-                w = w + acc[j] * corr;
+                w = w + (LINE ? acc[j] * lhm : acc[j]) * corr;
             if (ray_w)
                 w = w * gload(ray_w, o);
             gstore(rdst, o, w);
@@ -817,16 +847,24 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
-template <typename T, typename TI, bool REF, int NA, int ROWS>
-__global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
+// LINE (fast path only): the line model.  Bins outside the detector read 0
+// (every ray is active, no clamp), bins are staged as single values (a
+// sample reads two adjacent words in one instruction), and a sample weighs
+// them by the chord lengths of the pixel on the two rays,
+// w0 = clamp(bp_b0 - bp_k (f - 1/2), 0, 1/M), w1 = clamp(bp_b1 + bp_k (f - 1/2),
+// 0, 1/M), f the fraction of the detector coordinate; pix_w, when given,
+// multiplies the update (the model's pixel weights are not uniform).
+template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE>
+__global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                 const uirk_bp_item *__restrict__ items, int n_items,
                                                 uirk_xcd_part part, int *__restrict__ err)
 {
-    // reference-compatible path: plain staged bins; fast path: (e, d) pairs
-    __shared__ float lds[REF ? NA * kBpWb : 1];
-    __shared__ float2 ldp[REF ? 1 : NA * kBpWb];
+    // reference-compatible path and line model: plain staged bins; fast path: (e, d) pairs
+    __shared__ float lds[(REF || LINE) ? NA * kBpWb : 1];
+    __shared__ float2 ldp[(REF || LINE) ? 1 : NA * kBpWb];
     __shared__ int s_umin[NA];
     __shared__ float s_trig[2 * NA];
+    __shared__ float4 s_line[LINE ? NA : 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const ItemRange range = item_range(part, n_items);
@@ -863,7 +901,8 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
         // or one shifted by a multiple of 1/2).  Otherwise an edge falls inside
         // a bin, and the fast path clamps the coordinate itself (one more
         // operation per sample; uniform over the slice).
-        const bool clamp_u = !REF && ((edge0 - 0.5f) != floorf(edge0 - 0.5f) || (edge1 - 0.5f) != floorf(edge1 - 0.5f));
+        const bool clamp_u =
+            !REF && !LINE && ((edge0 - 0.5f) != floorf(edge0 - 0.5f) || (edge1 - 0.5f) != floorf(edge1 - 0.5f));
         // tile corner coordinates (pixel centres)
         const float cx0 = (float)item.x0 + 0.5f - origin_shift;
         const float cx1 = cx0 + (float)(kBpT - 1);
@@ -909,6 +948,11 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                     s_umin[threadIdx.x] = umin;
                     s_trig[2 * threadIdx.x] = st;
                     s_trig[2 * threadIdx.x + 1] = ct;
+                    if constexpr (LINE) {
+                        const float *lt = uniform(S->line);
+                        s_line[threadIdx.x] = make_float4(gload(lt, 8 * ang + 4), gload(lt, 8 * ang + 5),
+                                                          gload(lt, 8 * ang + 6), gload(lt, 8 * ang + 7));
+                    }
                 }
                 __syncthreads();
                 {
@@ -924,7 +968,7 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                         const int i = lane + 64 * (e & 1);
                         const uint32_t rowo = (uint32_t)(abase + k) * (uint32_t)D;
                         const int bin = s_umin[k] + i;
-                        if constexpr (REF) {
+                        if constexpr (REF || LINE) {
                             const int bc = min(max(bin, 0), D - 1);
                             const float t = gload(sino, rowo + (uint32_t)bc);
                             v0[e] = bin == bc ? t : 0.0f;
@@ -943,7 +987,7 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                         const int k = wave + 4 * (e >> 1);
                         const int i = lane + 64 * (e & 1);
                         if (k < na && i < kBpWb) {
-                            if constexpr (REF) {
+                            if constexpr (REF || LINE) {
                                 lds[k * kBpWb + i] = v0[e];
                             } else {
                                 // (e, d) with d = t[i+1] - t[i], e = t[i] - i d: the value at
@@ -986,7 +1030,29 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                         const float2 *P = ldp + k * kBpWb;
                         float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f)) - (float)s_umin[k];
                         const float du = -4.0f * st;
-                        if (!clamp_u) {
+                        if constexpr (LINE) {
+                            const float4 L = s_line[k]; // (bp_k, bp_b0, bp_b1, 1/M)
+                            const float *B = lds + k * kBpWb;
+#pragma unroll
+                            for (int g = 0; g < ROWS; g += 8) {
+                                float2 p[8];
+                                float fh[8];
+#pragma unroll
+                                for (int h = 0; h < 8; ++h) {
+                                    const float w = __builtin_fmaf((float)h, du, u); // >= 1: inside the window
+                                    const int i0 = cvt_flr(w);
+                                    p[h] = make_float2(B[i0], B[i0 + 1]);
+                                    fh[h] = fract_pos(w) - 0.5f;
+                                }
+#pragma unroll
+                                for (int h = 0; h < 8; ++h) {
+                                    const float w0 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], -L.x, L.y), 0.0f, L.w);
+                                    const float w1 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], L.x, L.z), 0.0f, L.w);
+                                    acc[g + h] = __builtin_fmaf(w1, p[h].y, __builtin_fmaf(w0, p[h].x, acc[g + h]));
+                                }
+                                u = __builtin_fmaf(8.0f, du, u);
+                            }
+                        } else if (!clamp_u) {
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
                                 float2 p[8];
@@ -1053,6 +1119,8 @@ __global__ __launch_bounds__(256, REF ? 1 : 6) void k_bp_tab(const uirk_slice *_
                         out = out * gload(pix_w, o);
                         out = out + gload(xin, o);
                     }
+                } else if (LINE && pix_w) {
+                    out = __builtin_fmaf(acc[q] * gain, gload(pix_w, o), gload(xin, o));
                 } else {
                     out = __builtin_fmaf(acc[q], gain, gload(xin, o));
                     if (tv) // split-Bregman TV term, evaluated at the same x
@@ -1455,27 +1523,31 @@ int launch_fp(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *i
     return (int)hipGetLastError();
 }
 
-template <typename T, typename TI, int APT>
+template <typename T, typename TI, int APT, bool LINE = false>
 int launch_fp_pairs(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items, int n,
                     const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fp_pairs<T, TI, APT>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_fp_pairs<T, TI, APT, LINE>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
     return (int)hipGetLastError();
 }
 
-template <typename T, typename TI, bool REF, int NA, int ROWS>
+template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE = false>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part, err);
+    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS, LINE>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part,
+                       err);
     return (int)hipGetLastError();
 }
 
-// fast-path FP over the tile variants: 16, 8 or 32 angles per item (4, 2 or 8 per thread)
+// fast-path FP over the tile variants: 16, 8 or 32 angles per item (4, 2 or 8 per thread);
+// the line model takes the default tile
 template <typename T, typename TI>
-int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_fp_item *items,
-                   int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+int launch_fp_fast(const uirk_tile &tile, bool line, const uirk_slice *sl, const uirk_slice &one,
+                   const uirk_fp_item *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
+    if (line)
+        return launch_fp_pairs<T, TI, 4, true>(sl, one, items, n, part, grid, err, st);
     if (tile.fp_single) {
         if (tile.fp_angles == 8)
             return launch_fp<T, TI, false, 2>(sl, one, items, n, part, grid, err, st);
@@ -1492,9 +1564,12 @@ int launch_fp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice
 
 // fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
 template <typename T, typename TI>
-int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items,
-                   int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+int launch_bp_fast(const uirk_tile &tile, bool line, const uirk_slice *sl, const uirk_slice &one,
+                   const uirk_bp_item *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
+    if (line) // 32 staged angles; tiles of 64 or 32 rows
+        return tile.bp_rows == 32 ? launch_bp<T, TI, false, 32, 8, true>(sl, one, items, n, part, grid, err, st)
+                                  : launch_bp<T, TI, false, 32, 16, true>(sl, one, items, n, part, grid, err, st);
     const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
     if (na16 && r8)
         return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, err, st);
@@ -1508,12 +1583,13 @@ int launch_bp_fast(const uirk_tile &tile, const uirk_slice *sl, const uirk_slice
 // (sinogram, image) storage pairs: both FP32, both FP16, both BF16, or
 // FP16 / BF16 sinograms with an FP32 image
 template <bool FP>
-int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, const uirk_slice *sl, const uirk_slice &one,
-                const void *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
+int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, bool line, const uirk_slice *sl,
+                const uirk_slice &one, const void *items, int n, const uirk_xcd_part &part, int grid, int *err,
+                hipStream_t st)
 {
-#define UIRK_LAUNCH(TS, TIM)                                                                                     \
-    return FP ? launch_fp_fast<TS, TIM>(tile, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st)     \
-              : launch_bp_fast<TS, TIM>(tile, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)
+#define UIRK_LAUNCH(TS, TIM)                                                                                        \
+    return FP ? launch_fp_fast<TS, TIM>(tile, line, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st) \
+              : launch_bp_fast<TS, TIM>(tile, line, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)
     if (img_dtype == UIR_DTYPE_F32) {
         switch (dtype) {
         case UIR_DTYPE_F16: UIRK_LAUNCH(__half, float);
@@ -1568,7 +1644,10 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_fp<float, float, true, 4>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    return launch_fast<true>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    if (mode != 1 && mode != 2)
+        return (int)hipErrorInvalidValue;
+    return launch_fast<true>(dtype, img_dtype, tl, mode == 2, d_slices, single, d_items, n_items, pt, grid, d_err,
+                             st);
 }
 
 int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
@@ -1660,7 +1739,10 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    return launch_fast<false>(dtype, img_dtype, tl, d_slices, single, d_items, n_items, pt, grid, d_err, st);
+    if (mode != 1 && mode != 2)
+        return (int)hipErrorInvalidValue;
+    return launch_fast<false>(dtype, img_dtype, tl, mode == 2, d_slices, single, d_items, n_items, pt, grid, d_err,
+                              st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

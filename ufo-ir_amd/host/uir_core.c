@@ -422,9 +422,18 @@ float uir_geom_axis(float axis_property, uint32_t n_dets)
  * t = s + 1/2 - axis, and the two taps at floor(v) and floor(v) + 1 lie at
  * ray offsets of magnitude M f and M (1 - f); the BP position of pixel (x, y)
  * is u = X c - Y s_ + axis - 1/2 with offsets f and 1 - f to bins floor(u)
- * and floor(u) + 1. */
+ * and floor(u) + 1.
+ * trig_unit (optional): (sin, cos) of the unit direction per angle, the
+ * trig table of the fast path's line mode; fast_k (optional): its 8
+ * constants per angle in terms of f - 1/2: the FP's phi = clamp(fp_k (f -
+ * 1/2) + fp_b, 0, 1) (fp_k = M / m, fp_b = 1/2; axis-parallel: a step, fp_k
+ * = 2^60, fp_b = 0 or 1 by the tie rule), 1/M, 0, and the BP's weights
+ * clamp(bp_b0 - bp_k (f - 1/2), 0, 1/M), clamp(bp_b1 + bp_k (f - 1/2), 0,
+ * 1/M) (bp_k = 1 / (M m), bp_b0 = bp_b1 = (M + m - 1) / (2 M m);
+ * axis-parallel: bp_k = 2^60, bp_b0 = 1, bp_b1 = 0: f = 1/2 goes to the
+ * lower bin), 1/M. */
 void uir_line_table(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uint32_t n, float axis,
-                    uirk_line_angle *out)
+                    uirk_line_angle *out, float *trig_unit, float *fast_k)
 {
     const double h = 0.5 * n, tiny = 1.0 / 1048576.0;
     for (uint32_t a = 0; a < n_angles; ++a) {
@@ -467,6 +476,28 @@ void uir_line_table(const float *sin_lut, const float *cos_lut, uint32_t n_angle
             L->bp_k = (float)(1.0 / (M * m));
             L->bp_b0 = (float)((M + m) / (2.0 * M * m));
             L->bp_b1 = (float)(((M + m) / 2.0 - 1.0) / (M * m));
+        }
+        if (trig_unit) {
+            trig_unit[2 * a] = (float)s;
+            trig_unit[2 * a + 1] = (float)c;
+        }
+        if (fast_k) {
+            float *k = fast_k + 8 * (size_t)a;
+            const float step = 1152921504606846976.0f; /* 2^60 */
+            if (L->aligned) {
+                k[0] = step;
+                k[1] = L->tie0 ? 0.0f : 1.0f;
+                k[4] = step;
+                k[5] = 1.0f;
+                k[6] = 0.0f;
+            } else {
+                k[0] = (float)(M / m);
+                k[1] = 0.5f;
+                k[4] = (float)(1.0 / (M * m));
+                k[5] = k[6] = (float)((M + m - 1.0) / (2.0 * M * m));
+            }
+            k[2] = k[7] = L->hm;
+            k[3] = 0.0f;
         }
     }
 }

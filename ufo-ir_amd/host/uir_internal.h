@@ -88,7 +88,7 @@ uint32_t uir_geom_runs(const float *sin_lut, const float *cos_lut, uint32_t n_an
 float uir_geom_axis(float axis_property, uint32_t n_dets);
 int uir_geom_active_bins(uint32_t n_dets, float axis, int *lo, int *hi);
 void uir_line_table(const float *sin_lut, const float *cos_lut, uint32_t n_angles, uint32_t n, float axis,
-                    uirk_line_angle *out);
+                    uirk_line_angle *out, float *trig_unit, float *fast_k);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,9 @@
  * positions are affine in the detector index and the line index, so the
  * footprint width over a chunk is a convex function of the chunk's position
  * and attains its maximum at the first or the last chunk; both are evaluated
- * exactly here (with a margin for float stepping).
+ * exactly here (with a margin for float stepping).  The line model's ray
+ * positions lie half a line step further along than joseph's (`shift`, in
+ * line steps).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -47,7 +49,7 @@ static int bp_push(uirk_bp_vec *v, uirk_bp_item it)
 /* Footprint width (in staged positions, as the kernel computes it) of the rays
  * of angles [a0, a0 + k) and detectors [d0, d0 + 64) over lines [l0, l0 + C). */
 static int window_width(int n, int n_dets, double axis, const float *s, const float *c, int dir, int a0, int k,
-                        int d0, int l0)
+                        int d0, int l0, double shift)
 {
     const int C = UIRK_FP_LINES;
     const int d1 = d0 + UIRK_FP_DETS - 1 < n_dets - 1 ? d0 + UIRK_FP_DETS - 1 : n_dets - 1;
@@ -67,7 +69,7 @@ static int window_width(int n, int n_dets, double axis, const float *s, const fl
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j) {
                 const double t = (double)ds[i] + 0.5 - axis;
-                const double p = t * det_step - 0.5 * n * step + 0.5 * n + (double)ls[j] * step;
+                const double p = t * det_step - 0.5 * n * step + 0.5 * n + ((double)ls[j] + shift) * step;
                 if (p < lo) lo = p;
                 if (p > hi) hi = p;
             }
@@ -81,11 +83,11 @@ static int window_width(int n, int n_dets, double axis, const float *s, const fl
  * the staging window in every chunk: the width is convex in the chunk's
  * first line, so the first and the last chunk bound it. */
 static int block_fits(int n, int n_dets, float axis, const float *s, const float *c, int dir, int a0, int k, int d0,
-                      int wmax)
+                      int wmax, double shift)
 {
     const int llast = ((n - 1) / UIRK_FP_LINES) * UIRK_FP_LINES;
-    return window_width(n, n_dets, axis, s, c, dir, a0, k, d0, 0) <= wmax &&
-           window_width(n, n_dets, axis, s, c, dir, a0, k, d0, llast) <= wmax;
+    return window_width(n, n_dets, axis, s, c, dir, a0, k, d0, 0, shift) <= wmax &&
+           window_width(n, n_dets, axis, s, c, dir, a0, k, d0, llast, shift) <= wmax;
 }
 
 /* Lines [*lo, *hi) outside which no ray of the block (angles [a0, a0 + k),
@@ -95,7 +97,7 @@ static int block_fits(int n, int n_dets, float axis, const float *s, const float
  * the detector block is bounded by its two extreme rays.  A margin of 2
  * positions covers the float rounding of stepped coordinates. */
 static void line_range(int n, int n_dets, double axis, const float *s, const float *c, int dir, int a0, int k,
-                       int d0, int *lo, int *hi)
+                       int d0, int *lo, int *hi, double shift)
 {
     const int d1 = d0 + UIRK_FP_DETS - 1 < n_dets - 1 ? d0 + UIRK_FP_DETS - 1 : n_dets - 1;
     const double plo = -0.5 - 2.0, phi = n + 0.5 + 2.0;
@@ -111,7 +113,7 @@ static void line_range(int n, int n_dets, double axis, const float *s, const flo
         }
         const int ds[2] = {d0, d1};
         for (int i = 0; i < 2; ++i) {
-            const double p0 = ((double)ds[i] + 0.5 - axis) * det_step - 0.5 * n * step + 0.5 * n;
+            const double p0 = ((double)ds[i] + 0.5 - axis) * det_step - 0.5 * n * step + 0.5 * n + shift * step;
             double l_a, l_b;
             if (fabs(step) < 1e-12) {
                 if (p0 < plo || p0 > phi)
@@ -149,8 +151,9 @@ static void line_range(int n, int n_dets, double axis, const float *s, const flo
     *hi = (int)b;
 }
 
-int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
-                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax)
+static int plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
+                   const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax,
+                   double shift)
 {
     if (max_angles < 1 || max_angles > UIRK_FP_ANGLES_MAX)
         max_angles = UIRK_FP_ANGLES;
@@ -167,7 +170,7 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
                  * no active angle skips the sampling. */
                 const int per_wave = max_angles >= 4 ? max_angles / 4 : 1;
                 int k = off + cnt - a < max_angles ? off + cnt - a : max_angles;
-                while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, wmax))
+                while (k > 1 && !block_fits(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, wmax, shift))
                     k = k > per_wave ? ((k - 1) / per_wave) * per_wave : k - 1;
                 uirk_fp_item it;
                 memset(&it, 0, sizeof it);
@@ -176,7 +179,7 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
                 it.a0 = a;
                 it.na = k;
                 it.d0 = d0;
-                line_range(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, &it.l_lo, &it.l_hi);
+                line_range(n, n_dets, axis, sin_lut, cos_lut, dir, a, k, d0, &it.l_lo, &it.l_hi, shift);
                 if (fp_push(out, it))
                     return -1;
                 a += k;
@@ -184,6 +187,18 @@ int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, con
         }
     }
     return 0;
+}
+
+int uirk_plan_fp(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_lut,
+                 const float *cos_lut, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax)
+{
+    return plan_fp(out, slice, n, n_dets, axis, sin_lut, cos_lut, subs, n_subs, max_angles, wmax, 0.0);
+}
+
+int uirk_plan_fp_line(uirk_fp_vec *out, int slice, int n, int n_dets, float axis, const float *sin_unit,
+                      const float *cos_unit, const uir_subset *subs, uint32_t n_subs, int max_angles, int wmax)
+{
+    return plan_fp(out, slice, n, n_dets, axis, sin_unit, cos_unit, subs, n_subs, max_angles, wmax, 0.5);
 }
 
 int uirk_plan_bp(uirk_bp_vec *out, int slice, int n, int tile_rows)

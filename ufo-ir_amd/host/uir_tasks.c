@@ -59,6 +59,13 @@ struct uir_task {
     uirk_line_angle *h_line, *d_line;
     uir_subset *line_runs;
     uint32_t n_line_runs;
+    /* the line model on the table-driven fast path (mode 2): unit-direction
+     * trig pairs, per-angle constants, work tables and their partitions */
+    float *d_ltrig, *d_lk;
+    uirk_fp_item *d_lfp_items;
+    uirk_bp_item *d_lbp_items;
+    int n_lfp_items, n_lbp_items;
+    uirk_xcd_part lfp_part, lbp_part;
 
     /* methods (ufo-ir-method-task.c:74-85 and the method tasks) */
     uir_task *projector;
@@ -267,6 +274,10 @@ void uir_task_unref(uir_task *t)
     free(t->h_line);
     uir_dfree(t->d_line);
     free(t->line_runs);
+    uir_dfree(t->d_ltrig);
+    uir_dfree(t->d_lk);
+    uir_dfree(t->d_lfp_items);
+    uir_dfree(t->d_lbp_items);
     free(t->diag);
     uir_task_unref(t->projector);
     uir_task_unref(t->df_minimizer);
@@ -652,13 +663,23 @@ done:
     return s;
 }
 
-/* The line model's constants for an n x n image in the current geometry,
- * and the runs of angles of one family. */
+/* The line model's constants for an n x n image in the current geometry:
+ * the per-angle table of the single-angle kernels (uir_line.hip), the runs
+ * of angles of one family, and the fast path's trig pairs, constants and
+ * work tables (mode 2 of the table kernels, for projections over all
+ * angles). */
+/* BP tiles of the task path's line mode: 64 x 32 pixels, twice as many
+ * workgroups as 64 x 64 tiles (one slice per launch leaves few). */
+#define LINE_BP_ROWS 32
+static const uirk_tile g_line_tile = {UIRK_FP_ANGLES, 32, LINE_BP_ROWS, 0};
+
 static uir_status line_plan(uir_task *p, uint32_t n)
 {
     const float axis = proj_axis(p);
     if ((uint64_t)n * n * 4 >= (1ull << 32) || (uint64_t)p->lut_n * p->n_dets * 4 >= (1ull << 32))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "image or sinogram of 4 GiB or more");
+    if (n > 16384)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "the line model takes images of at most 16384 x 16384");
     if (p->lut_n > 65535)
         return uir_set_error(UIR_ERR_UNSUPPORTED, "the line model takes at most 65535 angles");
     if (p->d_line && p->line_gen == p->lut_gen && p->line_n == n && p->line_angles == p->lut_n &&
@@ -673,35 +694,138 @@ static uir_status line_plan(uir_task *p, uint32_t n)
     if (!runs)
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
     p->line_runs = runs;
-    uir_line_table(p->h_sin, p->h_cos, p->lut_n, n, axis, h);
+    float *trig = (float *)malloc(sizeof(float) * 2 * A), *lk = (float *)malloc(sizeof(float) * 8 * A);
+    float *su = (float *)malloc(sizeof(float) * A), *cu = (float *)malloc(sizeof(float) * A);
+    uirk_fp_vec fv = {0};
+    uirk_bp_vec bv = {0};
+    double *cost = NULL;
+    uir_status st = UIR_OK;
+    if (!trig || !lk || !su || !cu) {
+        st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+        goto done;
+    }
+    uir_line_table(p->h_sin, p->h_cos, p->lut_n, n, axis, h, trig, lk);
+    /* runs of one family; as work-table subsets their direction is joseph's
+     * (1 = a sample per row, the line model's family 0) */
     uint32_t k = 0;
     for (uint32_t a = 0; a < p->lut_n; ++a) {
-        if (k && runs[k - 1].direction == h[a].family) {
+        if (k && runs[k - 1].direction == !h[a].family) {
             runs[k - 1].n++;
         } else {
             runs[k].offset = a;
             runs[k].n = 1;
-            runs[k].direction = h[a].family;
+            runs[k].direction = !h[a].family;
             ++k;
         }
     }
     p->n_line_runs = k;
+    for (uint32_t a = 0; a < p->lut_n; ++a) {
+        su[a] = trig[2 * a];
+        cu[a] = trig[2 * a + 1];
+    }
+    if (uirk_plan_fp_line(&fv, 0, (int)n, (int)p->n_dets, axis, su, cu, runs, k, UIRK_FP_ANGLES, UIRK_FP_WMAX) ||
+        uirk_plan_bp(&bv, 0, (int)n, LINE_BP_ROWS)) {
+        st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+        goto done;
+    }
     uir_dfree(p->d_line);
+    uir_dfree(p->d_ltrig);
+    uir_dfree(p->d_lk);
+    uir_dfree(p->d_lfp_items);
+    uir_dfree(p->d_lbp_items);
     p->d_line = NULL;
-    TRY(uir_dmalloc((void **)&p->d_line, sizeof *h * A));
-    HIPTRY(hipMemcpy(p->d_line, h, sizeof *h * p->lut_n, hipMemcpyHostToDevice), "table upload");
+    p->d_ltrig = p->d_lk = NULL;
+    p->d_lfp_items = NULL;
+    p->d_lbp_items = NULL;
+    if ((st = uir_dmalloc((void **)&p->d_line, sizeof *h * A)) != UIR_OK ||
+        (st = uir_dmalloc((void **)&p->d_ltrig, sizeof(float) * 2 * A)) != UIR_OK ||
+        (st = uir_dmalloc((void **)&p->d_lk, sizeof(float) * 8 * A)) != UIR_OK ||
+        (st = uir_dmalloc((void **)&p->d_lfp_items, (fv.n ? fv.n : 1) * sizeof(uirk_fp_item))) != UIR_OK ||
+        (st = uir_dmalloc((void **)&p->d_lbp_items, (bv.n ? bv.n : 1) * sizeof(uirk_bp_item))) != UIR_OK)
+        goto done;
+    if (!p->d_err) {
+        if ((st = uir_dmalloc((void **)&p->d_err, sizeof(int))) != UIR_OK ||
+            (st = uir_check_hip(hipMemset(p->d_err, 0, sizeof(int)), "memset")) != UIR_OK)
+            goto done;
+    }
+    if ((st = uir_check_hip(hipMemcpy(p->d_line, h, sizeof *h * p->lut_n, hipMemcpyHostToDevice), "table upload")) !=
+            UIR_OK ||
+        (st = uir_check_hip(hipMemcpy(p->d_ltrig, trig, sizeof(float) * 2 * p->lut_n, hipMemcpyHostToDevice),
+                            "table upload")) != UIR_OK ||
+        (st = uir_check_hip(hipMemcpy(p->d_lk, lk, sizeof(float) * 8 * p->lut_n, hipMemcpyHostToDevice),
+                            "table upload")) != UIR_OK ||
+        (st = uir_check_hip(hipMemcpy(p->d_lfp_items, fv.v, fv.n * sizeof(uirk_fp_item), hipMemcpyHostToDevice),
+                            "table upload")) != UIR_OK ||
+        (st = uir_check_hip(hipMemcpy(p->d_lbp_items, bv.v, bv.n * sizeof(uirk_bp_item), hipMemcpyHostToDevice),
+                            "table upload")) != UIR_OK)
+        goto done;
+    p->n_lfp_items = (int)fv.n;
+    p->n_lbp_items = (int)bv.n;
+    cost = (double *)malloc(sizeof(double) * (fv.n ? fv.n : 1));
+    if (!cost) {
+        st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+        goto done;
+    }
+    for (size_t i = 0; i < fv.n; ++i)
+        cost[i] = uirk_fp_item_cost(&fv.v[i]);
+    uirk_plan_xcd(cost, (int)fv.n, uirk_plan_num_xcd(), &p->lfp_part);
+    uirk_plan_xcd(NULL, (int)bv.n, uirk_plan_num_xcd(), &p->lbp_part);
     p->line_gen = p->lut_gen;
     p->line_n = n;
     p->line_angles = p->lut_n;
     p->line_axis = axis;
-    return UIR_OK;
+done:
+    free(trig);
+    free(lk);
+    free(su);
+    free(cu);
+    free(fv.v);
+    free(bv.v);
+    free(cost);
+    if (st != UIR_OK) {
+        uir_dfree(p->d_line);
+        p->d_line = NULL; /* rebuilt on the next call */
+    }
+    return st;
 }
 
-/* Line-model FP of angles [a0, a0 + na): one launch per run of one family. */
+/* A slice descriptor of the fast path's line mode for an n x n image. */
+static void line_slice(const uir_task *p, uint32_t n, uirk_slice *sl)
+{
+    memset(sl, 0, sizeof *sl);
+    sl->trig = p->d_ltrig;
+    sl->line = p->d_lk;
+    sl->n = (int32_t)n;
+    sl->n_dets = (int32_t)p->n_dets;
+    sl->n_angles = (int32_t)p->lut_n;
+    sl->axis = proj_axis(p);
+    sl->corr = p->correction_scale;
+    sl->relax = p->relaxation;
+    sl->pix_scale = 1.0f;
+    sl->n_runs = 1;
+    sl->run_off[0] = 0;
+    sl->run_n[0] = p->lut_n;
+}
+
+/* Line-model FP of angles [a0, a0 + na): over all angles one launch of the
+ * fast path's table kernel, else one launch of the single-angle kernels per
+ * run of one family. */
 static uir_status line_fp(uir_task *p, uint32_t n, const float *vol, const float *r_sino, float *w_sino,
                           const float *ray_w, uint32_t a0, uint32_t na, int few_rays, hipStream_t st)
 {
     TRY(line_plan(p, n));
+    if (a0 == 0 && na == p->lut_n && !few_rays) {
+        uirk_slice sl;
+        line_slice(p, n, &sl);
+        sl.x = vol;
+        sl.b = r_sino;
+        sl.r = w_sino;
+        sl.ray_w = ray_w;
+        return uir_check_hip(uirk_fp_table(NULL, &sl, p->d_lfp_items, p->n_lfp_items, &p->lfp_part, 2, UIR_DTYPE_F32,
+                                           UIR_DTYPE_F32, NULL, uirk_plan_grid_xcd(&p->lfp_part, p->n_lfp_items),
+                                           p->d_err, st),
+                             "FP launch");
+    }
     for (uint32_t a = a0; a < a0 + na;) {
         const int fam = p->h_line[a].family;
         uint32_t e = a + 1;
@@ -748,10 +872,19 @@ static uir_status proj_fp(uir_task *p, uint32_t n, const float *vol, const float
  * vol += relax * BP per run; else the fused SIRT update with POSC option. */
 static uir_status proj_bp_ex(uir_task *p, uint32_t n, const float *sino, float *vol, const float *pix_w, int posc)
 {
-    if (p->line) {
+    if (p->line) { /* the fast path's table kernel in line mode */
         TRY(line_plan(p, n));
-        return uir_check_hip(uirk_bp_line(vol, vol, sino, pix_w, posc, p->d_line, (int)n, (int)p->n_dets, 0,
-                                          (int)p->lut_n, p->relaxation, p->res->stream), "BP launch");
+        uirk_slice sl;
+        line_slice(p, n, &sl);
+        sl.b = sino;
+        sl.x = vol;
+        sl.x_out = vol;
+        sl.pix_w = pix_w;
+        sl.posc = posc;
+        return uir_check_hip(uirk_bp_table(NULL, &sl, p->d_lbp_items, p->n_lbp_items, &p->lbp_part, 2, UIR_DTYPE_F32,
+                                           UIR_DTYPE_F32, &g_line_tile, uirk_plan_grid_xcd(&p->lbp_part, p->n_lbp_items),
+                                           p->d_err, p->res->stream),
+                             "BP launch");
     }
     TRY(proj_plan(p, n));
     if (pix_w && p->n_runs > UIRK_MAX_RUNS)
