@@ -310,13 +310,13 @@ def line_samples(n, d, a, step, axis):
 
 def line_sirt(args, uir):
     """SIRT with the line projector (exact ray-pixel chord lengths, the
-    projector's model "line") on one slice of --n x --n, --angles angles,
-    --dets bins: slice-iterations/s of the SIRT task (FP32), the durations of
-    its FP and BP (one launch each of the table-driven, LDS-staged kernels in
-    their line mode) from HIP events on the task's stream, accuracy after
-    --accuracy-iters iterations
-    against the oracle's FP64 SIRT over the same model, and that oracle on the
-    host cores."""
+    projector's model "line") at --n x --n, --angles angles, --dets bins:
+    slice-iterations/s of a --batch-slice batch in --storage (`value`) and of
+    the SIRT task, one FP32 slice per process() (`plain_fp32`); the durations
+    of the FP and BP launches (the table-driven, LDS-staged kernels in their
+    line mode) from HIP events on the stream they run on; accuracy of both
+    paths after --accuracy-iters iterations against the oracle's FP64 SIRT
+    over the same model, and that oracle on the host cores."""
     import oracle as O
     res = uir.Resources(0)
     n, a, d = args.n, args.angles, args.dets
@@ -370,26 +370,57 @@ def line_sirt(args, uir):
     bp_ms = timer.elapsed_ms() / reps
     fp_samples = line_samples(n, d, a, step, axis)
     bp_samples = n * n * a
-    print("line: FP %.3f ms (%d samples), BP %.3f ms (%d samples)" % (fp_ms, fp_samples, bp_ms, bp_samples),
+    print("line: task FP %.3f ms (%d samples), BP %.3f ms (%d samples)" % (fp_ms, fp_samples, bp_ms, bp_samples),
           file=sys.stderr, flush=True)
+    sirt["num_iterations"] = args.accuracy_iters
+    sirt.process(ds, out)
+    plain_img = out.numpy()
+
+    # ---- batched: --batch slices of the same shape in --storage, one FP and
+    # one BP launch per iteration over every slice, replayed from a hipGraph
+    spec = dict(n=n, n_dets=d, n_angles=a, step=step, relaxation=relax, model="line")
+    bt = uir.Batch(res, [spec] * args.batch, args.storage)
+    for i in range(args.batch):
+        bt.set_sinogram(i, sino32)
+    bfp_ms, bbp_ms = bt.time_kernels(5)
+    b1 = uir.Batch(res, [spec], args.storage)
+    b1.set_sinogram(0, sino32)
+    b1.reset()
+    b1.iterate(args.accuracy_iters, use_graph=False)
+    batch_img = b1.image(0)
+    b1.close()
+    bt.reset()
+    bt.iterate(1)
+    for _ in range(args.warmup):
+        bt.iterate(1)
+    res.synchronize()
+    timer.start(res.stream)
+    for _ in range(args.steps):
+        bt.iterate(1)
+    timer.stop(res.stream)
+    total_ms = timer.elapsed_ms()
+    ms_per_step = total_ms / args.steps
+    value = args.batch * args.steps / (total_ms / 1000.0)
+    print("line: batched %s, %d slices: %.3f ms per iteration (FP %.3f, BP %.3f ms per launch)"
+          % (args.storage, args.batch, ms_per_step, bfp_ms, bbp_ms), file=sys.stderr, flush=True)
+
     # 8 flop per two-tap sample: two weights (one FMA each) and two FMAs into the sum
-    kname, kms, ks = ("fp", fp_ms, fp_samples) if fp_ms >= bp_ms else ("bp", bp_ms, bp_samples)
-    tfl = 8.0 * ks / (kms * 1e-3) / 1e12
+    es = 2 if args.storage in ("f16", "bf16", "f16-sino", "bf16-sino") else 4
+    ei = 2 if args.storage in ("f16", "bf16") else 4
     nv, ns = n * n, a * d
-    kbytes = (nv * 4 + 2 * ns * 4) if kname == "fp" else (ns * 4 + 2 * nv * 4)
+    kname, kms, ks = ("fp", bfp_ms, fp_samples) if bfp_ms >= bbp_ms else ("bp", bbp_ms, bp_samples)
+    tfl = 8.0 * ks * args.batch / (kms * 1e-3) / 1e12
+    kbytes = args.batch * ((nv * ei + 2 * ns * es + ns * 4) if kname == "fp" else (ns * es + 2 * nv * ei + nv * 4))
     gbs = kbytes / (kms * 1e-3) / 1e9
     roof = {"bound": "valu", "achieved": round(tfl, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tfl / FP32_PEAK_TFLOPS, 4), "traffic": None,
             "kernel": "k_fp_pairs (line mode)" if kname == "fp" else "k_bp_tab (line mode)",
-            "ms": {"fp": round(fp_ms, 4), "bp": round(bp_ms, 4)},
-            "samples": {"fp": fp_samples, "bp": bp_samples},
-            "note": "FP32 vector work, 8 flop per two-tap sample (two chord weights, two FMAs into the sum); "
-                    "the taps are read from LDS-staged image lines (FP) or sinogram rows (BP); hbm frac %.4f "
-                    "at %.1f GB/s algorithmic (%d bytes: image and sinogram streamed once)"
-                    % (gbs / HBM_PEAK_GBS, gbs, kbytes)}
-    sirt["num_iterations"] = args.accuracy_iters
-    sirt.process(ds, out)
-    got = out.numpy()
+            "ms": {"fp": round(bfp_ms, 4), "bp": round(bbp_ms, 4)},
+            "samples_per_slice": {"fp": fp_samples, "bp": bp_samples},
+            "note": "one launch over the %d-slice batch; FP32 vector work, 8 flop per two-tap sample (two chord "
+                    "weights, two FMAs into the sum); the taps are read from LDS-staged image lines (FP) or "
+                    "sinogram rows (BP); hbm frac %.4f at %.1f GB/s algorithmic (%d bytes: images, sinograms "
+                    "and weights streamed once)" % (args.batch, gbs / HBM_PEAK_GBS, gbs, kbytes)}
     acc = cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -397,20 +428,30 @@ def line_sirt(args, uir):
         t_ref = time.time() - t0
         print("line: oracle FP64 SIRT, %d iterations in %.1f s" % (args.accuracy_iters, t_ref), file=sys.stderr,
               flush=True)
-        acc = {"iterations": args.accuracy_iters, "rel_l2_vs_fp64": O.rel_l2(got, ref),
-               "psnr_vs_fp64_db": round(O.psnr(got, ref), 2),
+        acc = {"iterations": args.accuracy_iters,
+               "batched_%s_rel_l2_vs_fp64" % args.storage: O.rel_l2(batch_img, ref),
+               "batched_%s_psnr_db" % args.storage: round(O.psnr(batch_img, ref), 2),
+               "plain_fp32_rel_l2_vs_fp64": O.rel_l2(plain_img, ref),
+               "gate": {"rel_l2_max": 1e-3, "pass": bool(O.rel_l2(batch_img, ref) <= 1e-3)},
                "reference": "the oracle's FP64 SIRT over the line model (orc64_sirt), same data"}
         cpu = {"value": round(args.accuracy_iters / t_ref, 4), "unit": "slice-iterations/s",
                "cores": O.num_threads(), "kind": "port",
                "sample": "the oracle's FP64 SIRT over the line model, the same slice, %d iterations (the ray and "
                          "pixel weights, two more projections, included), OpenMP over the host's cores"
                          % args.accuracy_iters}
-    line = {"metric": "SIRT slice-iterations/s with the line projector", "value": round(1000.0 / ms_it, 2),
-            "unit": "slice-iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
-            "data": "synthetic: 20 random ellipses, 1% Gaussian noise",
-            "config": {"workload": "SIRT task, line projector, %dx%d, %d angles over [0, pi), %d bins, relaxation "
-                                   "%g, one slice per process()" % (n, n, a, d, relax)},
-            "ms_per_iteration": round(ms_it, 4), "roofline": roof, "accuracy": acc, "cpu_baseline": cpu}
+    line = {"metric": "SIRT slice-iterations/s with the line projector, batched vs plain", "value": round(value, 2),
+            "unit": "slice-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 math, %s storage" % args.storage,
+            "data": "synthetic: 20 random ellipses, line FP, 1% Gaussian noise",
+            "config": {"workload": "batched SIRT with the line projector, %d slices of %dx%d, %d angles over [0, pi), "
+                                   "%d bins, relaxation %g" % (args.batch, n, n, a, d, relax),
+                       "batch": args.batch, "storage": args.storage},
+            "plain_fp32": {"value": round(1000.0 / ms_it, 2), "unit": "slice-iterations/s",
+                           "ms_per_iteration": round(ms_it, 4),
+                           "kernels_ms": {"fp": round(fp_ms, 4), "bp": round(bp_ms, 4)},
+                           "path": "SIRT task with a line projector, one slice per process(), FP32"},
+            "roofline": roof, "accuracy": acc, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

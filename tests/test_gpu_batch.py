@@ -191,3 +191,50 @@ def test_batch_sparse_and_dense_angles_in_one_batch(res, storage, tol):
         err = O.rel_l2(bt.image(i), refs[i])
         print(storage, geoms[i], "rel-L2 vs FP64", err)
         assert err <= tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16-sino", 1e-3)])
+def test_batch_line_and_joseph_slices_match_fp64(res, storage, tol):
+    """Slices of both projection models in one batch (the line model's FP and
+    BP run as separate launches of the table kernels' line mode, with their own
+    ray weights and per-pixel weights): each matches the FP64 SIRT of its own
+    model; a line slice with the ray-weight floor and an odd image edge (rays
+    along pixel edges at 90 degrees) included.  The joseph slice takes the
+    floor: on noisy data its corner-grazing rays (weights up to ~1e4) leave
+    FP32 and FP64 apart without it (test_batch_noisy_with_ray_weight_floor...)."""
+    iters = 20
+    cases = [(128, 182, 90, "line", 0.0), (96, 136, 120, "joseph", 0.1), (97, 137, 60, "line", 0.1),
+             (160, 227, 75, "line", 0.0)]
+    specs = [dict(_spec(n, d, a), model=m, ray_weight_min=rmin) for (n, d, a, m, rmin) in cases]
+    print("case batch line + joseph", storage, cases, flush=True)
+    bt = uir.Batch(res, specs, storage)
+    refs = []
+    for i, (n, d, a, m, rmin) in enumerate(cases):
+        g = O.Geometry(n, d, a, np.float32(np.pi / a), model=m)
+        b = O.fp64(g, ellipse_phantom(n, seed=700 + i).astype(np.float64))
+        b = b + np.random.default_rng(700 + i).normal(0.0, 0.01 * b.max(), b.shape)
+        b32 = b.astype(np.float32)
+        bt.set_sinogram(i, b32)
+        refs.append(O.sirt64(g, b32.astype(np.float64), iters, 0.5, ray_weight_min=rmin))
+    bt.reset()
+    bt.iterate(iters)
+    for i, c in enumerate(cases):
+        err = O.rel_l2(bt.image(i), refs[i])
+        print(storage, c, "rel-L2 vs FP64", err, flush=True)
+        assert err <= tol
+    # a line slice alone in a batch gives the same image as in the mix
+    solo = uir.Batch(res, [specs[0]], storage)
+    g0 = O.Geometry(128, 182, 90, np.float32(np.pi / 90), model="line")
+    b0 = O.fp64(g0, ellipse_phantom(128, seed=700).astype(np.float64))
+    b0 = (b0 + np.random.default_rng(700).normal(0.0, 0.01 * b0.max(), b0.shape)).astype(np.float32)
+    solo.set_sinogram(0, b0)
+    solo.reset()
+    solo.iterate(iters)
+    assert np.array_equal(solo.image(0), bt.image(0))
+
+
+def test_batch_asdpocs_refuses_line_slices(res):
+    bt = uir.Batch(res, [dict(_spec(64, 91, 30), model="line")], "f32")
+    with pytest.raises(uir.UirError) as e:
+        bt.asdpocs(1)
+    assert e.value.status == 4

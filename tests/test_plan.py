@@ -189,6 +189,21 @@ def test_request_with_no_active_detector_is_refused(axis):
     assert uir.request_footprint(uir.make_request(256, 363, 180, float(np.pi / 180), axis_position=190.0)) > 0
 
 
+def test_line_model_requests_are_refused_by_the_server():
+    """The server's kernels and weights are joseph's: a request of the line
+    model is refused as unsupported, an unknown model as invalid."""
+    req = uir.make_request(256, 363, 180, float(np.pi / 180), model="line")
+    with pytest.raises(uir.UirError) as e:
+        uir.request_footprint(req)
+    assert e.value.status == 4 and "joseph" in str(e.value)
+    req.slice.model = 7
+    with pytest.raises(uir.UirError) as e:
+        uir.request_footprint(req)
+    assert e.value.status == 1
+    assert uir.slice_spec(dict(n=8, n_dets=8, n_angles=4, step=0.1, model="line")).model == 1
+    assert uir.slice_spec(dict(n=8, n_dets=8, n_angles=4, step=0.1)).model == 0
+
+
 def _kernel_windows_and_samples(n, d, a, axis_prop, items, s, c, axis, max_items=None):
     """Replays, in FP32 where the kernel rounds, what k_fp_pairs computes for
     each chunk of each item: the staging window from the item's corner rays

@@ -262,6 +262,9 @@ uir_status uir_batch_asdpocs(uir_batch *b, uint32_t iterations, const uir_asdpoc
     if (!(p->beta_red >= 0.0f) || !(p->alpha >= 0.0f) || !(p->alpha_red >= 0.0f) || !(p->r_max >= 0.0f) ||
         !isfinite(p->beta) || p->ng > 1000 || p->df_iterations == 0)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad ASD-POCS parameters");
+    for (uint32_t g = 0; g < b->n_geoms; ++g)
+        if (b->geoms[g].model != 0) /* its fused SART step is joseph's */
+            return uir_set_error(UIR_ERR_UNSUPPORTED, "batched ASD-POCS takes the joseph model only");
     TRY(uir_check_hip(hipSetDevice(b->res->device), "hipSetDevice"));
     if (!b->asd)
         TRY(asd_create(b, &b->asd));

@@ -5,7 +5,9 @@
  * measured sinogram b (A*D) and the weighted residual r (A*D) in the storage
  * dtype, and, per distinct geometry, the FP32 ray weights R = 1/FP(1) (A*D)
  * and the (sin, cos) table.  The pixel weight of the fast projector pair is
- * the scalar 1/A (its backprojection of ones is exactly A).
+ * the scalar 1/A (its backprojection of ones is exactly A).  A geometry of the
+ * line model adds its per-angle constants and an FP32 image of pixel weights
+ * C = 1/BP(1); its slices run in the table kernels' line mode.
  *
  * One iteration = one FP launch over every slice's FP work items
  * (r = (b - A x) * R) and one BP launch over every slice's tiles
@@ -34,7 +36,8 @@ static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 static int same_geom(const uir_slice_spec *s, const geom_t *g)
 {
     return s->n == g->n && s->n_dets == g->D && s->n_angles == g->A && s->step == g->step &&
-           uir_geom_axis(s->axis_position, s->n_dets) == g->axis && s->ray_weight_min == g->r_min;
+           uir_geom_axis(s->axis_position, s->n_dets) == g->axis && s->ray_weight_min == g->r_min &&
+           s->model == g->model;
 }
 
 void uir_batch_free(uir_batch *b)
@@ -51,6 +54,8 @@ void uir_batch_free(uir_batch *b)
         free(b->geoms[i].h_sin);
         free(b->geoms[i].h_cos);
         free(b->geoms[i].runs);
+        free(b->geoms[i].h_lk);
+        free(b->geoms[i].h_line);
     }
     free(b->geoms);
     free(b->geom_of);
@@ -65,6 +70,8 @@ void uir_batch_free(uir_batch *b)
         uir_dfree(b->fpg[g].d_items);
     for (int g = 0; g < UIR_BP_SHAPES; ++g)
         uir_dfree(b->bpg[g].d_items);
+    uir_dfree(b->fpl.d_items);
+    uir_dfree(b->bpl.d_items);
     uir_dfree(b->d_err);
     uir_asd_free(b->asd);
     uir_resources_free(b->res);
@@ -86,6 +93,43 @@ uir_status uir_ray_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, flo
     TRY(uir_check_hip(uirk_ray_weights64(d_trig, (int)n, (int)D, (int)A, axis, r_min, d_ray_w, stream),
                       "ray weights"));
     return uir_check_hip(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+/* The line model's weights of one geometry from its single-angle kernels:
+ * R = 1/FP(1) (0 where FP(1) = 0 or below r_min) and C = 1/BP(1) (0 where
+ * BP(1) = 0), with the geometry's tables already on the device.
+ * Synchronises the stream. */
+static uir_status line_weights(hipStream_t st, const geom_t *G)
+{
+    const size_t nv = (size_t)G->n * G->n, ns = (size_t)G->A * G->D;
+    float *ones_v = NULL, *ones_s = NULL;
+    uir_status s = uir_dmalloc((void **)&ones_v, nv * 4);
+    if (s == UIR_OK)
+        s = uir_dmalloc((void **)&ones_s, ns * 4);
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_set(ones_v, nv, 1.0f, st), "set");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_set(ones_s, ns, 1.0f, st), "set");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_set(G->d_ray_w, ns, 0.0f, st), "set");
+    for (uint32_t r = 0; r < G->n_runs && s == UIR_OK; ++r)
+        s = uir_check_hip(uirk_fp_line(ones_v, G->d_ray_w, G->d_ray_w, NULL, G->d_line, !G->runs[r].direction,
+                                       (int)G->n, (int)G->D, (int)G->runs[r].offset, (int)G->runs[r].n, 1.0f, 0, st),
+                          "FP launch");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_inv_min(G->d_ray_w, G->d_ray_w, ns, G->r_min, st), "inv");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_set(G->d_pix_w, nv, 0.0f, st), "set");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_bp_line(G->d_pix_w, G->d_pix_w, ones_s, NULL, 0, G->d_line, (int)G->n, (int)G->D, 0,
+                                       (int)G->A, 1.0f, st), "BP launch");
+    if (s == UIR_OK)
+        s = uir_check_hip(uirk_inv(G->d_pix_w, G->d_pix_w, nv, st), "inv");
+    if (s == UIR_OK)
+        s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uir_dfree(ones_v);
+    uir_dfree(ones_s);
+    return s;
 }
 
 uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_spec *specs, uint32_t n_slices,
@@ -115,6 +159,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
                                  (double)s->axis_position);
         if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
             return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: bad ray_weight_min", i);
+        if (s->model != 0 && s->model != 1)
+            return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "slice %u: unknown model %d", i, s->model);
     }
     TRY(uir_check_hip(hipSetDevice(res->device), "hipSetDevice"));
     uir_batch *b = (uir_batch *)calloc(1, sizeof *b);
@@ -135,8 +181,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     b->r = (void **)calloc(n_slices, sizeof(void *));
     uir_status s = UIR_OK;
     uirk_slice *fp_sl = NULL, *bp_sl = NULL;
-    uirk_fp_vec fv[UIR_FP_SHAPES] = {{0}};
-    uirk_bp_vec bv[UIR_BP_SHAPES] = {{0}};
+    uirk_fp_vec fv[UIR_FP_SHAPES] = {{0}}, fvl = {0};
+    uirk_bp_vec bv[UIR_BP_SHAPES] = {{0}}, bvl = {0};
     if (!b->specs || !b->geom_of || !b->geoms || !b->x || !b->b || !b->r) {
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
         goto fail;
@@ -156,7 +202,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             G->step = specs[i].step;
             G->axis = uir_geom_axis(specs[i].axis_position, specs[i].n_dets);
             G->r_min = specs[i].ray_weight_min;
-            G->tile = uir_tile_for(&specs[i], storage);
+            G->model = specs[i].model;
+            G->tile = G->model ? (uirk_tile)UIRK_TILE_DEFAULT : uir_tile_for(&specs[i], storage);
             G->h_sin = (float *)malloc(sizeof(float) * G->A);
             G->h_cos = (float *)malloc(sizeof(float) * G->A);
             G->runs = (uir_subset *)malloc(sizeof(uir_subset) * G->A);
@@ -165,7 +212,38 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
                 goto fail;
             }
             uir_geom_lut(G->A, G->step, G->h_sin, G->h_cos);
-            G->n_runs = uir_geom_runs(G->h_sin, G->h_cos, G->A, G->runs);
+            if (G->model) {
+                /* the unit directions replace the LUT; runs of one family */
+                G->h_lk = (float *)malloc(sizeof(float) * 8 * G->A);
+                G->h_line = (uirk_line_angle *)malloc(sizeof(uirk_line_angle) * G->A);
+                float *tu = (float *)malloc(sizeof(float) * 2 * G->A);
+                if (!G->h_lk || !G->h_line || !tu) {
+                    free(tu);
+                    s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+                    goto fail;
+                }
+                uir_line_table(G->h_sin, G->h_cos, G->A, G->n, G->axis, G->h_line, tu, G->h_lk);
+                for (uint32_t a = 0; a < G->A; ++a) {
+                    G->h_sin[a] = tu[2 * a];
+                    G->h_cos[a] = tu[2 * a + 1];
+                }
+                free(tu);
+                uint32_t k = 0;
+                for (uint32_t a = 0; a < G->A; ++a) {
+                    const int dir = !G->h_line[a].family;
+                    if (k && G->runs[k - 1].direction == dir) {
+                        G->runs[k - 1].n++;
+                    } else {
+                        G->runs[k].offset = a;
+                        G->runs[k].n = 1;
+                        G->runs[k].direction = dir;
+                        ++k;
+                    }
+                }
+                G->n_runs = k;
+            } else {
+                G->n_runs = uir_geom_runs(G->h_sin, G->h_cos, G->A, G->runs);
+            }
         }
         b->geom_of[i] = g;
     }
@@ -176,8 +254,13 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         const size_t nv = (size_t)specs[i].n * specs[i].n, ns = (size_t)specs[i].n_angles * specs[i].n_dets;
         total += align_up(nv * b->esi) + 2 * align_up(ns * b->ess);
     }
-    for (uint32_t g = 0; g < b->n_geoms; ++g)
-        total += align_up((size_t)b->geoms[g].A * b->geoms[g].D * 4) + align_up((size_t)b->geoms[g].A * 8);
+    for (uint32_t g = 0; g < b->n_geoms; ++g) {
+        const geom_t *G = &b->geoms[g];
+        total += align_up((size_t)G->A * G->D * 4) + align_up((size_t)G->A * 8);
+        if (G->model)
+            total += align_up((size_t)G->A * 32) + align_up((size_t)G->n * G->n * 4) +
+                     align_up((size_t)G->A * sizeof(uirk_line_angle));
+    }
     {
         size_t free_b = 0;
         if ((s = uir_mem_get_info(&free_b, NULL)) != UIR_OK)
@@ -207,6 +290,14 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             p += align_up((size_t)G->A * G->D * 4);
             G->d_trig = (float *)p;
             p += align_up((size_t)G->A * 8);
+            if (G->model) {
+                G->d_lk = (float *)p;
+                p += align_up((size_t)G->A * 32);
+                G->d_pix_w = (float *)p;
+                p += align_up((size_t)G->n * G->n * 4);
+                G->d_line = (uirk_line_angle *)p;
+                p += align_up((size_t)G->A * sizeof(uirk_line_angle));
+            }
         }
     }
     if ((s = uir_check_hip(hipMemset(b->mem, 0, total), "hipMemset")) != UIR_OK)
@@ -224,9 +315,19 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         uirk_plan_trig(G->h_sin, G->h_cos, G->A, trig);
         s = uir_check_hip(hipMemcpy(G->d_trig, trig, sizeof(float) * 2 * G->A, hipMemcpyHostToDevice), "hipMemcpy");
         free(trig);
-        if (s != UIR_OK ||
-            (s = uir_ray_weights(b->res->stream, G->n, G->D, G->A, G->axis, G->d_trig, G->r_min, G->d_ray_w)) != UIR_OK)
+        if (s != UIR_OK)
             goto fail;
+        if (G->model) {
+            if ((s = uir_check_hip(hipMemcpy(G->d_lk, G->h_lk, sizeof(float) * 8 * G->A, hipMemcpyHostToDevice),
+                                   "hipMemcpy")) != UIR_OK ||
+                (s = uir_check_hip(hipMemcpy(G->d_line, G->h_line, sizeof(uirk_line_angle) * G->A,
+                                             hipMemcpyHostToDevice), "hipMemcpy")) != UIR_OK ||
+                (s = line_weights(b->res->stream, G)) != UIR_OK)
+                goto fail;
+        } else if ((s = uir_ray_weights(b->res->stream, G->n, G->D, G->A, G->axis, G->d_trig, G->r_min,
+                                        G->d_ray_w)) != UIR_OK) {
+            goto fail;
+        }
     }
 
     /* descriptors and work tables */
@@ -261,9 +362,20 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         k->n_runs = 1;
         k->run_off[0] = 0;
         k->run_n[0] = G->A;
-        if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
-                         G->runs, G->n_runs, G->tile.fp_angles, UIRK_TILE_FP_WMAX(G->tile)) ||
-            uirk_plan_bp(&bv[bp_shape(&G->tile)], (int)i, (int)G->n, G->tile.bp_rows)) {
+        if (G->model) { /* line: its constants, and per-pixel weights in place of 1/A */
+            f->line = G->d_lk;
+            k->line = G->d_lk;
+            k->pix_w = G->d_pix_w;
+            k->pix_scale = 1.0f;
+            if (uirk_plan_fp_line(&fvl, (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos, G->runs,
+                                  G->n_runs, UIRK_FP_ANGLES, UIRK_FP_WMAX) ||
+                uirk_plan_bp(&bvl, (int)i, (int)G->n, UIRK_BP_TILE)) {
+                s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+                goto fail;
+            }
+        } else if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
+                                G->runs, G->n_runs, G->tile.fp_angles, UIRK_TILE_FP_WMAX(G->tile)) ||
+                   uirk_plan_bp(&bv[bp_shape(&G->tile)], (int)i, (int)G->n, G->tile.bp_rows)) {
             s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
             goto fail;
         }
@@ -276,7 +388,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     /* Tables stay slice-major; each XCD takes a contiguous, cost-balanced range,
      * so its L2 holds the images and sinograms of a few slices only. */
     {
-        size_t nmax = 1;
+        size_t nmax = fvl.n > bvl.n ? fvl.n : bvl.n;
+        nmax = nmax > 1 ? nmax : 1;
         for (int g = 0; g < UIR_FP_SHAPES; ++g)
             nmax = fv[g].n > nmax ? fv[g].n : nmax;
         for (int g = 0; g < UIR_BP_SHAPES; ++g)
@@ -312,6 +425,28 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             if (k->n_items)
                 s = upload(&k->d_items, bv[g].v, bv[g].n * sizeof(uirk_bp_item));
         }
+        if (s == UIR_OK) { /* the line model's groups */
+            kgroup *k = &b->fpl;
+            k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
+            for (size_t i = 0; i < fvl.n; ++i)
+                cost[i] = uirk_fp_item_cost(&fvl.v[i]);
+            uirk_plan_xcd(cost, (int)fvl.n, uirk_plan_num_xcd(), &k->part);
+            k->n_items = (int)fvl.n;
+            k->grid = uirk_plan_grid_xcd(&k->part, k->n_items);
+            if (k->n_items)
+                s = upload(&k->d_items, fvl.v, fvl.n * sizeof(uirk_fp_item));
+        }
+        if (s == UIR_OK) {
+            kgroup *k = &b->bpl;
+            k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
+            for (size_t i = 0; i < bvl.n; ++i)
+                cost[i] = (double)specs[bvl.v[i].slice].n_angles;
+            uirk_plan_xcd(cost, (int)bvl.n, uirk_plan_num_xcd(), &k->part);
+            k->n_items = (int)bvl.n;
+            k->grid = uirk_plan_grid_xcd(&k->part, k->n_items);
+            if (k->n_items)
+                s = upload(&k->d_items, bvl.v, bvl.n * sizeof(uirk_bp_item));
+        }
         free(cost);
         if (s != UIR_OK)
             goto fail;
@@ -325,6 +460,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         free(fv[g].v);
     for (int g = 0; g < UIR_BP_SHAPES; ++g)
         free(bv[g].v);
+    free(fvl.v);
+    free(bvl.v);
     *out = b;
     return UIR_OK;
 fail:
@@ -334,6 +471,8 @@ fail:
         free(fv[g].v);
     for (int g = 0; g < UIR_BP_SHAPES; ++g)
         free(bv[g].v);
+    free(fvl.v);
+    free(bvl.v);
     uir_batch_free(b);
     return s;
 }
@@ -377,6 +516,11 @@ static uir_status launch_fp_groups(uir_batch *b)
                                             b->sdt, b->idt, &k->tile, k->grid, b->d_err, b->res->stream),
                               "FP launch"));
     }
+    if (b->fpl.n_items)
+        TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, (const uirk_fp_item *)b->fpl.d_items, b->fpl.n_items,
+                                        &b->fpl.part, 2, b->sdt, b->idt, &b->fpl.tile, b->fpl.grid, b->d_err,
+                                        b->res->stream),
+                          "FP launch"));
     return UIR_OK;
 }
 
@@ -389,6 +533,11 @@ static uir_status launch_bp_groups(uir_batch *b)
                                             b->sdt, b->idt, &k->tile, k->grid, b->d_err, b->res->stream),
                               "BP launch"));
     }
+    if (b->bpl.n_items)
+        TRY(uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, (const uirk_bp_item *)b->bpl.d_items, b->bpl.n_items,
+                                        &b->bpl.part, 2, b->sdt, b->idt, &b->bpl.tile, b->bpl.grid, b->d_err,
+                                        b->res->stream),
+                          "BP launch"));
     return UIR_OK;
 }
 
@@ -479,6 +628,10 @@ uir_status uir_batch_get_info(const uir_batch *b, uir_batch_info *info)
         nb += (uint64_t)b->bpg[g].n_items;
         info->bp_grid += (uint32_t)b->bpg[g].grid * (b->bpg[g].n_items > 0);
     }
+    nf += (uint64_t)b->fpl.n_items;
+    nb += (uint64_t)b->bpl.n_items;
+    info->fp_grid += (uint32_t)b->fpl.grid * (b->fpl.n_items > 0);
+    info->bp_grid += (uint32_t)b->bpl.grid * (b->bpl.n_items > 0);
     info->device_bytes = b->mem_bytes + nf * sizeof(uirk_fp_item) + nb * sizeof(uirk_bp_item) +
                          2ull * b->n_slices * sizeof(uirk_slice);
     info->n_slices = b->n_slices;

@@ -12,10 +12,18 @@ typedef struct {
     float step, axis;        /* resolved axis */
     float r_min;             /* ray weight floor */
     uirk_tile tile;          /* tile shape (autotuned per geometry, or the default) */
-    float *h_sin, *h_cos;
+    float *h_sin, *h_cos;    /* line model: its unit directions */
     float *d_trig, *d_ray_w; /* carved from the batch allocation */
-    uir_subset *runs;
+    uir_subset *runs;        /* line model: runs of one family (direction = joseph's) */
     uint32_t n_runs;
+    int32_t model;           /* 0 joseph, 1 line */
+    /* line model: per-angle constants of the table kernels (8 floats) and of
+     * the single-angle kernels, and the pixel weights 1 / BP(1) (N x N);
+     * device copies carved from the batch allocation */
+    float *h_lk;
+    uirk_line_angle *h_line;
+    float *d_lk, *d_pix_w;
+    uirk_line_angle *d_line;
 } geom_t;
 
 #define UIR_FP_SHAPES 6 /* FP angles 16, 8, 32, with pair staging, then with single values */
@@ -45,6 +53,8 @@ struct uir_batch {
     /* one launch per tile-shape group: FP by angles per item (2 shapes), BP by
      * staged angles x tile rows (4 shapes); each group's table is slice-major */
     kgroup fpg[UIR_FP_SHAPES], bpg[UIR_BP_SHAPES];
+    /* the line model's slices: one FP and one BP group (default tile, mode 2) */
+    kgroup fpl, bpl;
     int *d_err;
     double fp_samples, bp_samples;
     hipGraph_t graph;

@@ -199,6 +199,9 @@ static uir_status check_request(const uir_request *r)
                              (double)s->axis_position);
     if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad ray_weight_min");
+    if (s->model != 0)
+        return uir_set_error(s->model == 1 ? UIR_ERR_UNSUPPORTED : UIR_ERR_INVALID_ARGUMENT,
+                             s->model == 1 ? "the server takes the joseph model only" : "unknown model %d", s->model);
     if (r->iterations == 0)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
     if (r->method == UIR_METHOD_SIRT) {

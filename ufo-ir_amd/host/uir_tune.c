@@ -328,6 +328,8 @@ uir_status uir_tune_geometry(uir_resources *res, const uir_slice_spec *spec, uir
 {
     if (!res)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL resources");
+    if (spec && spec->model != 0) /* the line model runs the default tile */
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "tile tuning is for the joseph model");
     uir_status s = uir_check_hip(hipSetDevice(res->device), "hipSetDevice");
     if (s != UIR_OK)
         return s;

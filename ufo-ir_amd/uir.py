@@ -49,7 +49,7 @@ class GeometryDims(C.Structure):
 class SliceSpec(C.Structure):
     _fields_ = [("n", C.c_uint32), ("n_dets", C.c_uint32), ("n_angles", C.c_uint32), ("step", C.c_float),
                 ("axis_position", C.c_float), ("relaxation", C.c_float), ("positive_constraint", C.c_int32),
-                ("ray_weight_min", C.c_float)]
+                ("ray_weight_min", C.c_float), ("model", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -509,12 +509,17 @@ DTYPES = {"f32": DTYPE_F32, "f16": DTYPE_F16, "bf16": DTYPE_BF16,
           "f16-sino": 3, "bf16-sino": 4}   # sinograms FP16 / BF16, images FP32
 
 
+MODELS = {"joseph": 0, "line": 1}
+
+
 def slice_spec(s):
     """A uir_slice_spec from a dict (n, n_dets, n_angles, step, axis_position,
-    relaxation, positive_constraint, ray_weight_min)."""
+    relaxation, positive_constraint, ray_weight_min, model: "joseph" or
+    "line")."""
     return SliceSpec(int(s["n"]), int(s["n_dets"]), int(s["n_angles"]), float(s["step"]),
                      float(s.get("axis_position", -1.0)), float(s.get("relaxation", 0.25)),
-                     int(bool(s.get("positive_constraint", False))), float(s.get("ray_weight_min", 0.0)))
+                     int(bool(s.get("positive_constraint", False))), float(s.get("ray_weight_min", 0.0)),
+                     MODELS[s.get("model", "joseph")])
 
 
 def _tile_dict(t):
@@ -654,10 +659,13 @@ STATUS_TIMEOUT = 9
 
 
 def make_request(n, n_dets, n_angles, step, method="sirt", iterations=10, inner_iterations=1, relaxation=0.25,
-                 lam=0.1, mu=1.0, axis_position=-1.0, positive_constraint=False, ray_weight_min=0.0):
-    """A uir_request (method "sirt" or "sbtv-sirt")."""
+                 lam=0.1, mu=1.0, axis_position=-1.0, positive_constraint=False, ray_weight_min=0.0,
+                 model="joseph"):
+    """A uir_request (method "sirt" or "sbtv-sirt"; the server takes the
+    joseph model only)."""
     return Request(SliceSpec(int(n), int(n_dets), int(n_angles), float(step), float(axis_position),
-                             float(relaxation), int(bool(positive_constraint)), float(ray_weight_min)),
+                             float(relaxation), int(bool(positive_constraint)), float(ray_weight_min),
+                             MODELS[model]),
                    METHODS[method], int(iterations), int(inner_iterations), float(lam), float(mu))
 
 
