@@ -193,7 +193,7 @@ def test_batch_sparse_and_dense_angles_in_one_batch(res, storage, tol):
         assert err <= tol
 
 
-@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16-sino", 1e-3)])
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16-sino", 1e-3), ("f16", 1e-3), ("bf16-sino", 3e-3)])
 def test_batch_line_and_joseph_slices_match_fp64(res, storage, tol):
     """Slices of both projection models in one batch (the line model's FP and
     BP run as separate launches of the table kernels' line mode, with their own

@@ -4,7 +4,9 @@ frame) gives, frame by frame, the oracle's ASD-POCS (ufo-ir-asdpocs-task.c:
 411-509 over ufo-ir-sart-task.c:173-228) and what the task ABI gives
 directly; the sample's own properties (10 iterations of a 100-sweep SART
 df-minimizer) on one frame against the oracle; a SIRT chain with a
-multi-page output against the oracle's SIRT; and the uir-runjson program."""
+multi-page output against the oracle's SIRT; the projector's model property
+through the graph ("line" against the oracle's FP64 line model, an unknown
+model refused at setup); and the uir-runjson program."""
 import json
 import os
 import subprocess
@@ -123,3 +125,29 @@ def test_sirt_chain_multipage_and_cli(res, tmp_path):
     for page in range(2):
         assert np.array_equal(uir.tiff_read(str(tmp_path / ("cli-%02d.tif" % page))),
                               uir.tiff_read(str(tmp_path / "recon.tif"), page))
+
+
+def test_graph_projector_model_line(tmp_path):
+    """model "line" in the graph's projector node (the property of
+    ufo-ir-parallel-projector-task.c:123-128) selects the line model: the SIRT
+    chain equals the oracle's FP64 SIRT over that model; an unknown model
+    fails the run at setup, as the reference fails to load its kernels."""
+    print("case graph line model", flush=True)
+    gl = O.Geometry(N_DET, N_DET, N_ANG, np.float32(STEP), model="line")
+    sino = O.fp64(gl, ellipse_phantom(N_DET, seed=510).astype(np.float64)).astype(np.float32)
+    uir.tiff_write(str(tmp_path / "sino.tif"), sino)
+    node = {"plugin": "sirt", "properties": {
+        "projector": {"plugin": "parallel-projector", "package": "ir",
+                      "properties": {"model": "line", "is_forward": False, "step": STEP}},
+        "relaxation_factor": 0.5, "num-iterations": 20}}
+    rep = uir.json_run(_graph("out/line%03i.tif", node), base_dir=str(tmp_path))
+    assert rep["frames"] == 1
+    got = uir.tiff_read(str(tmp_path / "out" / "line000.tif"))
+    ref = O.sirt64(gl, sino.astype(np.float64), 20, 0.5)
+    err = O.rel_l2(got, ref)
+    print("graph line SIRT rel-L2 vs FP64 %.3e" % err, flush=True)
+    assert err <= 1e-4
+    node["properties"]["projector"]["properties"]["model"] = "cone"
+    with pytest.raises(uir.UirError) as e:
+        uir.json_run(_graph("out/cone%03i.tif", node), base_dir=str(tmp_path))
+    assert "cone" in str(e.value)

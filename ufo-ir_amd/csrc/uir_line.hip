@@ -5,6 +5,10 @@
 // joseph).  The model and the per-angle constants are described with
 // uirk_line_angle in uir_device.h; the BP is the transpose of the FP.
 //
+// These kernels serve launches over one angle or a few (SART's sweep,
+// projector subsets) and the line model's weights.  Projections over all
+// angles run the table-driven kernels of uir_fast.hip in their line mode.
+//
 // FP: one ray per R lanes (R = 64: a thread per ray; R = 4: a ray over 16
 // lanes, interleaved along it), so that a wave's loads coalesce: rays of
 // family 0 advance one image row per sample and neighbouring rays read
