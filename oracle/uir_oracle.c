@@ -1173,7 +1173,10 @@ static void sirt64_core(const orc_geom *g, const double *b, double *x, uint32_t 
     orc64_fp(g, ones_v, R);
     for (size_t i = 0; i < ns; ++i) R[i] = (R[i] != 0 && R[i] >= r_min) ? 1.0 / R[i] : 0.0;
     orc64_bp(g, C, ones_s);
-    for (size_t i = 0; i < nv; ++i) C[i] = C[i] != 0 ? 1.0 / C[i] : 0.0;
+    /* the line model takes no pixel covered by less than ORC_LINE_PIX_MIN of
+     * chord length (a corner touch) */
+    const double cmin = g->model == 1 ? ORC_LINE_PIX_MIN : 0.0;
+    for (size_t i = 0; i < nv; ++i) C[i] = (C[i] != 0 && C[i] >= cmin) ? 1.0 / C[i] : 0.0;
     if (!warm)
         for (size_t i = 0; i < nv; ++i) x[i] = 0.0;
     for (uint32_t it = 0; it < iters; ++it) {

@@ -123,6 +123,11 @@ void orc_sbtv(const orc_geom *g, float *b, float *u, uint32_t iters, float lambd
 int orc_sbtv_cgs(const orc_geom *g, const float *rhs, float *x, const float *x0, uint32_t max_iter,
                  float lambda, float mu, int dot_mode);
 void orc_sbtv_apply(const orc_geom *g, const float *in, float *out, float lambda, float mu);
+/* SIRT pixel weights of the line model: C = 1 / BP(1) where BP(1) >=
+ * ORC_LINE_PIX_MIN, else 0.  A pixel that rays only touch at a corner has
+ * BP(1) = 0 exactly and ~1e-7 in FP32 arithmetic; the floor keeps FP32 from
+ * giving it C ~ 1e7 and a full-size update. */
+#define ORC_LINE_PIX_MIN (1.0 / 1024.0)
 /* The line model's direction: the LUT pair scaled to unit length, with a
  * component below 2^-20 taken as exactly 0 (and the other as +-1). */
 void orc64_line_dir(float cos_lut, float sin_lut, double *c, double *s);

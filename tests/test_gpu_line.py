@@ -203,3 +203,42 @@ def test_line_asdpocs_textbook_matches_fp64(res):
     err = O.rel_l2(got, ref)
     print("rel-L2 %.3e" % err, flush=True)
     assert err <= METHOD_TOL
+
+
+EDGES = [  # (image edge, detectors, angles, axis): degenerate sizes and axes
+    (1, 1, 1, -1.0),
+    (2, 3, 1, -1.0),
+    (5, 2, 7, -1.0),
+    (8, 1, 4, -1.0),
+    (3, 9, 180, -1.0),
+    (16, 23, 12, 0.0),      # axis at the first detector's edge
+    (16, 23, 12, 40.0),     # axis beyond the detector: most rays miss the image
+    (65, 93, 33, 46.25),
+]
+
+
+@pytest.mark.parametrize("n,d,a,axis", EDGES)
+def test_line_edge_shapes_match_fp64(res, n, d, a, axis):
+    """Tiny images, one detector, one angle, axes at or beyond the detector's
+    ends (the task path accepts any axis, as the reference does): FP, BP and
+    10 SIRT iterations against the FP64 line model."""
+    print("case line edge", n, d, a, axis, flush=True)
+    g = _geom(n, d, a, axis)
+    rng = np.random.default_rng(n * 1000 + d)
+    x = rng.random((n, n))
+    y = rng.random((a, d))
+    fp = _fp(res, g, x)
+    bp = _bp(res, g, y)
+    fref = O.fp64(g, x.astype(np.float32).astype(np.float64))
+    bref = O.bp64(g, y.astype(np.float32).astype(np.float64))
+    assert np.allclose(fp, fref, rtol=1e-5, atol=1e-5 * max(1.0, np.abs(fref).max())), np.abs(fp - fref).max()
+    assert np.allclose(bp, bref, rtol=1e-5, atol=1e-5 * max(1.0, np.abs(bref).max())), np.abs(bp - bref).max()
+    b = fref.astype(np.float32)
+    m = _method(res, g, "sirt", num_iterations=10, relaxation_factor=0.5)
+    got = _run(m, b, n)
+    ref = O.sirt64(g, b.astype(np.float64), 10, 0.5)
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-5 * max(1.0, np.abs(ref).max())), np.abs(got - ref).max()
+    m = _method(res, g, "sart", num_iterations=2, relaxation_factor=0.5)
+    got = _run(m, b, n)
+    ref = O.sart64(g, b.astype(np.float64), 2, 0.5)
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-5 * max(1.0, np.abs(ref).max())), np.abs(got - ref).max()

@@ -9,7 +9,10 @@ inside the pixel's unit square.
   - the FP equals a brute-force sum over every pixel of clipped lengths;
   - FP(1) is the length of each ray inside the whole image square;
   - at theta = 0 with the bins on the columns, the FP is the column sums;
-  - the BP is the exact transpose of the FP (<A x, y> = <x, A^T y>)."""
+  - the BP is the exact transpose of the FP (<A x, y> = <x, A^T y>);
+  - SIRT over the model equals the dense iteration x += relax C A^T R (b - A x)
+    with C = 1 / BP(1) only where BP(1) >= 2^-10: pixels that rays merely
+    graze keep their value."""
 import numpy as np
 import pytest
 
@@ -142,3 +145,31 @@ def test_bp_is_the_exact_transpose(n, d, a, axis):
     gj = O.Geometry(n, d, a, np.float32(np.pi / a), axis)
     lj, rj = float(np.sum(O.fp64(gj, x) * y)), float(np.sum(x * O.bp64(gj, y)))
     assert abs(lj - rj) > 1e-6 * (abs(lj) + abs(rj))
+
+
+def test_sirt_pixel_weight_floor():
+    """One detector whose ray passes 1e-5 beside the pixel corners at 45
+    degrees: the pixels it grazes have BP(1) in (0, 2^-10) and are never
+    updated; the others follow the dense iteration."""
+    n, d, a = 4, 1, 4
+    g = O.Geometry(n, d, a, np.float32(np.pi / a), 0.5 + 1e-5, model="line")
+    nv, ns = n * n, a * d
+    A = np.zeros((ns, nv))
+    for j in range(nv):
+        e = np.zeros(nv)
+        e[j] = 1.0
+        A[:, j] = O.fp64(g, e.reshape(n, n)).ravel()
+    colsum = A.sum(axis=0)
+    grazed = (colsum > 0) & (colsum < 2.0 ** -10)
+    assert grazed.any() and (colsum >= 2.0 ** -10).any()
+    assert np.allclose(O.bp64(g, np.ones((a, d))).ravel(), colsum, rtol=1e-12, atol=1e-15)
+    rowsum = A.sum(axis=1)
+    R = np.where(rowsum != 0, 1.0 / np.where(rowsum != 0, rowsum, 1.0), 0.0)
+    C = np.where(colsum >= 2.0 ** -10, 1.0 / np.where(colsum > 0, colsum, 1.0), 0.0)
+    b = np.random.default_rng(5).random(ns) + 1.0
+    x = np.zeros(nv)
+    for _ in range(6):
+        x = x + 0.5 * C * (A.T @ (R * (b - A @ x)))
+    got = O.sirt64(g, b.reshape(a, d), 6, 0.5).ravel()
+    assert np.allclose(got, x, rtol=1e-12, atol=1e-12)
+    assert np.all(got[grazed] == 0.0) and np.any(got[colsum >= 2.0 ** -10] != 0.0)

@@ -103,6 +103,12 @@ typedef struct {
     int32_t tie0;             /* aligned FP: f == 1/2 belongs to tap 0 */
 } uirk_line_angle;
 
+/* SIRT pixel weights of the line model: C = 1 / BP(1) where BP(1) >= this,
+ * else 0 (a thousandth of a pixel's chord).  A pixel that rays only touch at
+ * a corner has BP(1) = 0 exactly and about 1e-7 in FP32; without the floor it
+ * would get C ~ 1e7 and the full residual of those rays. */
+#define UIRK_LINE_PIX_MIN (1.0f / 1024.0f)
+
 /* FP of angles [a0, a0 + na), all of one family: w = (r + corr FP) (* ray_w
  * when ray_w != NULL), r_sino may equal w_sino.  few_rays selects the layout
  * for launches of few rays (one angle): each ray spread over 16 lanes. */

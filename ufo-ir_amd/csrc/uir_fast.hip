@@ -626,7 +626,10 @@ __global__ __launch_bounds__(256) void k_fp_tab(const uirk_slice *__restrict__ s
 //    1/2, or a step with the tie rule for axis-parallel rays), times 1/M.
 //    Positions lie half a line step further along than joseph's (the
 //    model's exact geometry), and every ray is active.  Four more
-//    operations per sample.
+//    operations per sample.  The remap multiplies position errors by M / m,
+//    so a chunk's start coordinate is rounded once from double, relative to
+//    the window, and each line's coordinate is one FMA from it (joseph steps
+//    by float additions).
 template <typename T, typename TI, int APT, bool LINE>
 __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_slice *__restrict__ slices, uirk_slice one,
                                                   const uirk_fp_item *__restrict__ items, int n_items,
@@ -779,8 +782,12 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
             float u[APT], st[APT];
 #pragma unroll
             for (int j = 0; j < APT; ++j) {
-                const float pos = (float)(p0d[j] + (double)l0 * sd[j]);
-                u[j] = on[j] ? pos - off : park;
+                if (LINE) {
+                    u[j] = on[j] ? (float)(p0d[j] + (double)l0 * sd[j] - (double)off) : park;
+                } else {
+                    const float pos = (float)(p0d[j] + (double)l0 * sd[j]);
+                    u[j] = on[j] ? pos - off : park;
+                }
                 st[j] = on[j] ? stp[j] : 0.0f;
             }
             const float2 *P = lds + kFpPH;
@@ -796,10 +803,10 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
                 float w[kQ][APT];
 #pragma unroll
                 for (int j = 0; j < APT; ++j) {
-                    w[0][j] = u[j];
+                    w[0][j] = LINE ? __builtin_fmaf((float)l, st[j], u[j]) : u[j];
 #pragma unroll
                     for (int q = 1; q < kQ; ++q)
-                        w[q][j] = w[q - 1][j] + st[j];
+                        w[q][j] = LINE ? __builtin_fmaf((float)(l + q), st[j], u[j]) : w[q - 1][j] + st[j];
 #pragma unroll
                     for (int q = 0; q < kQ; ++q)
                         v[q][j] = P0[q * kFpPs + rne_index(w[q][j])];
@@ -810,16 +817,19 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
                     for (int q = 0; q < kQ; ++q)
                         acc[j] = __builtin_fmaf(LINE ? line_phi(w[q][j], lk, lb) : w[q][j], v[q][j].y,
                                                 acc[j] + v[q][j].x);
-                    u[j] = w[kQ - 1][j] + st[j];
+                    if (!LINE) // LINE: u stays the chunk's start
+                        u[j] = w[kQ - 1][j] + st[j];
                 }
             }
             for (; l < nl; ++l) {
                 const float2 *P0 = P + l * kFpPs;
 #pragma unroll
                 for (int j = 0; j < APT; ++j) {
-                    const float2 v = P0[rne_index(u[j])];
-                    acc[j] = __builtin_fmaf(LINE ? line_phi(u[j], lk, lb) : u[j], v.y, acc[j] + v.x);
-                    u[j] += st[j];
+                    const float uj = LINE ? __builtin_fmaf((float)l, st[j], u[j]) : u[j];
+                    const float2 v = P0[rne_index(uj)];
+                    acc[j] = __builtin_fmaf(LINE ? line_phi(uj, lk, lb) : uj, v.y, acc[j] + v.x);
+                    if (!LINE)
+                        u[j] += st[j];
                 }
             }
         }
@@ -865,6 +875,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
     __shared__ int s_umin[NA];
     __shared__ float s_trig[2 * NA];
     __shared__ float4 s_line[LINE ? NA : 1];
+    __shared__ float s_base[LINE ? NA : 1]; // LINE: window coordinate at the tile corner
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const ItemRange range = item_range(part, n_items);
@@ -952,6 +963,10 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                         const float *lt = uniform(S->line);
                         s_line[threadIdx.x] = make_float4(gload(lt, 8 * ang + 4), gload(lt, 8 * ang + 5),
                                                           gload(lt, 8 * ang + 6), gload(lt, 8 * ang + 7));
+                        // u - 1/2 - umin at the tile corner, rounded once: the
+                        // weights multiply coordinate errors by 1 / (M m)
+                        s_base[threadIdx.x] = (float)((double)cx0 * (double)ct - (double)cy0 * (double)st +
+                                                      (double)shift - 0.5 - (double)umin);
                     }
                 }
                 __syncthreads();
@@ -1033,6 +1048,8 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                         if constexpr (LINE) {
                             const float4 L = s_line[k]; // (bp_k, bp_b0, bp_b1, 1/M)
                             const float *B = lds + k * kBpWb;
+                            // this thread's column and first row, relative to the tile corner
+                            u = __builtin_fmaf((float)lane, ct, __builtin_fmaf(-(float)wave, st, s_base[k]));
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
                                 float2 p[8];

@@ -97,7 +97,7 @@ uir_status uir_ray_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, flo
 
 /* The line model's weights of one geometry from its single-angle kernels:
  * R = 1/FP(1) (0 where FP(1) = 0 or below r_min) and C = 1/BP(1) (0 where
- * BP(1) = 0), with the geometry's tables already on the device.
+ * BP(1) < UIRK_LINE_PIX_MIN), with the geometry's tables already on the device.
  * Synchronises the stream. */
 static uir_status line_weights(hipStream_t st, const geom_t *G)
 {
@@ -124,7 +124,7 @@ static uir_status line_weights(hipStream_t st, const geom_t *G)
         s = uir_check_hip(uirk_bp_line(G->d_pix_w, G->d_pix_w, ones_s, NULL, 0, G->d_line, (int)G->n, (int)G->D, 0,
                                        (int)G->A, 1.0f, st), "BP launch");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_inv(G->d_pix_w, G->d_pix_w, nv, st), "inv");
+        s = uir_check_hip(uirk_inv_min(G->d_pix_w, G->d_pix_w, nv, UIRK_LINE_PIX_MIN, st), "inv");
     if (s == UIR_OK)
         s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     uir_dfree(ones_v);

@@ -1084,7 +1084,10 @@ static uir_status sirt_weights(uir_task *t, mctx *c, float **ray_w, float **pix_
     HIPTRY(uirk_set(sino_tmp, c->ns, 1.0f, c->st), "set");
     HIPTRY(uirk_set(*pix_w, c->nv, 0.0f, c->st), "set");
     TRY(proj_bp(c->p, c->n, sino_tmp, *pix_w));
-    HIPTRY(uirk_inv(*pix_w, *pix_w, c->nv, c->st), "inv");
+    if (c->p->line) /* no pixel covered by less than UIRK_LINE_PIX_MIN of chord */
+        HIPTRY(uirk_inv_min(*pix_w, *pix_w, c->nv, UIRK_LINE_PIX_MIN, c->st), "inv");
+    else
+        HIPTRY(uirk_inv(*pix_w, *pix_w, c->nv, c->st), "inv");
     return UIR_OK;
 }
 
