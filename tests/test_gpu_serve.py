@@ -69,6 +69,46 @@ def test_serve_sirt_equals_batch(res, storage):
         assert got[i][1]["steps"] == iters
 
 
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16-sino", 1e-3)])
+def test_serve_line_sirt_equals_batch_and_fp64(res, storage, tol):
+    """SIRT requests of the line model served beside joseph requests (one of
+    them of the same geometry, so the geometry cache must tell the models
+    apart): each equals the batch engine's slice bit for bit and the oracle's
+    FP64 SIRT of its own model.  A split-Bregman request of the line model is
+    refused at submission."""
+    iters = 8
+    cases = [(128, 182, 90, "line", 0.1), (128, 182, 90, "joseph", 0.1), (97, 137, 60, "line", 0.1)]
+    print("case serve line + joseph", storage, cases, flush=True)
+    sinos, refs = [], []
+    for i, (n, d, a, m, rmin) in enumerate(cases):
+        g = O.Geometry(n, d, a, np.float32(np.pi / a), model=m)
+        b = O.fp64(g, ellipse_phantom(n, seed=730 + i).astype(np.float64))
+        b = (b + np.random.default_rng(730 + i).normal(0.0, 0.01 * b.max(), b.shape)).astype(np.float32)
+        sinos.append(b)
+        refs.append(O.sirt64(g, b.astype(np.float64), iters, 0.5, ray_weight_min=rmin))
+    reqs = [uir.make_request(n, d, a, _step(a), "sirt", iterations=iters, relaxation=0.5, ray_weight_min=rmin,
+                             model=m) for (n, d, a, m, rmin) in cases]
+    with uir.Server(0, storage) as srv:
+        got = _run(srv, reqs, sinos)
+        with pytest.raises(uir.UirError) as e:
+            srv.submit(uir.make_request(97, 137, 60, _step(60), "sbtv-sirt", iterations=2, model="line"),
+                       sinos[2])
+        assert e.value.status == 4
+    specs = [dict(n=n, n_dets=d, n_angles=a, step=_step(a), relaxation=0.5, ray_weight_min=rmin, model=m)
+             for (n, d, a, m, rmin) in cases]
+    bt = uir.Batch(res, specs, storage)
+    for i, b in enumerate(sinos):
+        bt.set_sinogram(i, b)
+    bt.reset()
+    bt.iterate(iters, use_graph=False)
+    for i, c in enumerate(cases):
+        err = O.rel_l2(got[i][0], refs[i])
+        print(storage, c, "rel-L2 vs FP64 %.3e" % err, flush=True)
+        assert np.array_equal(got[i][0], bt.image(i)), c
+        assert got[i][1]["steps"] == iters
+        assert err <= tol
+
+
 def test_serve_result_same_alone_and_in_mix():
     n, d, a = GEOMS[0]
     _, b = _data(n, d, a, 21, noise=0.01)

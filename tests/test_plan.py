@@ -189,12 +189,22 @@ def test_request_with_no_active_detector_is_refused(axis):
     assert uir.request_footprint(uir.make_request(256, 363, 180, float(np.pi / 180), axis_position=190.0)) > 0
 
 
-def test_line_model_requests_are_refused_by_the_server():
-    """The server's kernels and weights are joseph's: a request of the line
-    model is refused as unsupported, an unknown model as invalid."""
-    req = uir.make_request(256, 363, 180, float(np.pi / 180), model="line")
+def test_server_line_model_requests():
+    """The server takes SIRT requests of the line model, whose geometry holds
+    the model's angle table and N x N pixel weights on top of joseph's ray
+    weights; split-Bregman requests of the line model are refused as
+    unsupported (their step bound is joseph's), an unknown model as invalid."""
+    n, d, a = 256, 363, 180
+    req = uir.make_request(n, d, a, float(np.pi / 180), model="line")
+    joseph = uir.request_footprint(uir.make_request(n, d, a, float(np.pi / 180)))
+    align = lambda b: (b + 255) // 256 * 256
+    # 8 floats per angle, the pixel weights, and 72 bytes per angle
+    # (uirk_line_angle: 2 doubles, 11 floats, 3 int32)
+    extra = align(a * 32) + align(n * n * 4) + align(a * 72)
+    assert uir.request_footprint(req) == joseph + extra
+    sb = uir.make_request(n, d, a, float(np.pi / 180), method="sbtv-sirt", model="line")
     with pytest.raises(uir.UirError) as e:
-        uir.request_footprint(req)
+        uir.request_footprint(sb)
     assert e.value.status == 4 and "joseph" in str(e.value)
     req.slice.model = 7
     with pytest.raises(uir.UirError) as e:

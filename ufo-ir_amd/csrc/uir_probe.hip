@@ -119,25 +119,32 @@ int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
     } else {
         // warm-up launch of each, then reps timed launches
         hipLaunchKernelGGL(k_probe_lds, dim3(blocks), dim3(256), 0, st, out, lds_iters, 1.0f);
-        hipEventRecord(ev[0], st);
+        if (hipEventRecord(ev[0], st) != hipSuccess)
+            rc = 3;
         for (int r = 0; r < reps; ++r)
             hipLaunchKernelGGL(k_probe_lds, dim3(blocks), dim3(256), 0, st, out, lds_iters, 1.0f);
-        hipEventRecord(ev[1], st);
-        if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms_lds, ev[0], ev[1]) != hipSuccess)
+        if (hipEventRecord(ev[1], st) != hipSuccess)
+            rc = 3;
+        if (rc == 0 &&
+            (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms_lds, ev[0], ev[1]) != hipSuccess))
             rc = 3;
         hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(256), 0, st, out, fma_iters, 1.0f);
-        hipEventRecord(ev[0], st);
+        if (hipEventRecord(ev[0], st) != hipSuccess)
+            rc = 3;
         for (int r = 0; r < reps; ++r)
             hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(256), 0, st, out, fma_iters, 1.0f);
-        hipEventRecord(ev[1], st);
+        if (hipEventRecord(ev[1], st) != hipSuccess)
+            rc = 3;
         if (rc == 0 &&
             (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms_fma, ev[0], ev[1]) != hipSuccess))
             rc = 3;
         hipLaunchKernelGGL(k_probe_cvt, dim3(blocks), dim3(256), 0, st, out, fma_iters, 1.0f);
-        hipEventRecord(ev[0], st);
+        if (hipEventRecord(ev[0], st) != hipSuccess)
+            rc = 3;
         for (int r = 0; r < reps; ++r)
             hipLaunchKernelGGL(k_probe_cvt, dim3(blocks), dim3(256), 0, st, out, fma_iters, 1.0f);
-        hipEventRecord(ev[1], st);
+        if (hipEventRecord(ev[1], st) != hipSuccess)
+            rc = 3;
         if (rc == 0 &&
             (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms_cvt, ev[0], ev[1]) != hipSuccess))
             rc = 3;
@@ -146,8 +153,8 @@ int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
     }
     for (int i = 0; i < 2; ++i)
         if (ev[i])
-            hipEventDestroy(ev[i]);
-    hipFree(out);
+            (void)hipEventDestroy(ev[i]);
+    (void)hipFree(out);
     if (rc)
         return rc;
     const double clocks_lds = (double)ms_lds / reps * (double)khz; // kHz = clocks per ms

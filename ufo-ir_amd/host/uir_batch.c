@@ -96,12 +96,15 @@ uir_status uir_ray_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, flo
 }
 
 /* The line model's weights of one geometry from its single-angle kernels:
- * R = 1/FP(1) (0 where FP(1) = 0 or below r_min) and C = 1/BP(1) (0 where
- * BP(1) < UIRK_LINE_PIX_MIN), with the geometry's tables already on the device.
- * Synchronises the stream. */
-static uir_status line_weights(hipStream_t st, const geom_t *G)
+ * R = 1/FP(1) (0 where FP(1) = 0 or below r_min) into d_ray_w (A x D) and
+ * C = 1/BP(1) (0 where BP(1) < UIRK_LINE_PIX_MIN) into d_pix_w (n x n), with
+ * the geometry's angle table d_line already on the device and runs its
+ * family runs.  Synchronises the stream. */
+uir_status uir_line_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, float r_min, const uir_subset *runs,
+                            uint32_t n_runs, const uirk_line_angle *d_line, float *d_ray_w, float *d_pix_w)
 {
-    const size_t nv = (size_t)G->n * G->n, ns = (size_t)G->A * G->D;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nv = (size_t)n * n, ns = (size_t)A * D;
     float *ones_v = NULL, *ones_s = NULL;
     uir_status s = uir_dmalloc((void **)&ones_v, nv * 4);
     if (s == UIR_OK)
@@ -111,25 +114,57 @@ static uir_status line_weights(hipStream_t st, const geom_t *G)
     if (s == UIR_OK)
         s = uir_check_hip(uirk_set(ones_s, ns, 1.0f, st), "set");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_set(G->d_ray_w, ns, 0.0f, st), "set");
-    for (uint32_t r = 0; r < G->n_runs && s == UIR_OK; ++r)
-        s = uir_check_hip(uirk_fp_line(ones_v, G->d_ray_w, G->d_ray_w, NULL, G->d_line, !G->runs[r].direction,
-                                       (int)G->n, (int)G->D, (int)G->runs[r].offset, (int)G->runs[r].n, 1.0f, 0, st),
+        s = uir_check_hip(uirk_set(d_ray_w, ns, 0.0f, st), "set");
+    for (uint32_t r = 0; r < n_runs && s == UIR_OK; ++r)
+        s = uir_check_hip(uirk_fp_line(ones_v, d_ray_w, d_ray_w, NULL, d_line, !runs[r].direction, (int)n, (int)D,
+                                       (int)runs[r].offset, (int)runs[r].n, 1.0f, 0, st),
                           "FP launch");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_inv_min(G->d_ray_w, G->d_ray_w, ns, G->r_min, st), "inv");
+        s = uir_check_hip(uirk_inv_min(d_ray_w, d_ray_w, ns, r_min, st), "inv");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_set(G->d_pix_w, nv, 0.0f, st), "set");
+        s = uir_check_hip(uirk_set(d_pix_w, nv, 0.0f, st), "set");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_bp_line(G->d_pix_w, G->d_pix_w, ones_s, NULL, 0, G->d_line, (int)G->n, (int)G->D, 0,
-                                       (int)G->A, 1.0f, st), "BP launch");
+        s = uir_check_hip(uirk_bp_line(d_pix_w, d_pix_w, ones_s, NULL, 0, d_line, (int)n, (int)D, 0, (int)A, 1.0f, st),
+                          "BP launch");
     if (s == UIR_OK)
-        s = uir_check_hip(uirk_inv_min(G->d_pix_w, G->d_pix_w, nv, UIRK_LINE_PIX_MIN, st), "inv");
+        s = uir_check_hip(uirk_inv_min(d_pix_w, d_pix_w, nv, UIRK_LINE_PIX_MIN, st), "inv");
     if (s == UIR_OK)
         s = uir_check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     uir_dfree(ones_v);
     uir_dfree(ones_s);
     return s;
+}
+
+/* The line model's host tables of one geometry: its angle table h_line and
+ * fast-kernel constants h_lk (8 floats per angle); the unit directions replace
+ * the (sin, cos) LUT in place, and runs / n_runs become the runs of one
+ * direction family each. */
+uir_status uir_line_geom(uint32_t A, uint32_t n, float axis, float *sin_lut, float *cos_lut, uirk_line_angle *h_line,
+                         float *h_lk, uir_subset *runs, uint32_t *n_runs)
+{
+    float *tu = (float *)malloc(sizeof(float) * 2 * A);
+    if (!tu)
+        return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
+    uir_line_table(sin_lut, cos_lut, A, n, axis, h_line, tu, h_lk);
+    for (uint32_t a = 0; a < A; ++a) {
+        sin_lut[a] = tu[2 * a];
+        cos_lut[a] = tu[2 * a + 1];
+    }
+    free(tu);
+    uint32_t k = 0;
+    for (uint32_t a = 0; a < A; ++a) {
+        const int dir = !h_line[a].family;
+        if (k && runs[k - 1].direction == dir) {
+            runs[k - 1].n++;
+        } else {
+            runs[k].offset = a;
+            runs[k].n = 1;
+            runs[k].direction = dir;
+            ++k;
+        }
+    }
+    *n_runs = k;
+    return UIR_OK;
 }
 
 uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_spec *specs, uint32_t n_slices,
@@ -213,34 +248,15 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             }
             uir_geom_lut(G->A, G->step, G->h_sin, G->h_cos);
             if (G->model) {
-                /* the unit directions replace the LUT; runs of one family */
                 G->h_lk = (float *)malloc(sizeof(float) * 8 * G->A);
                 G->h_line = (uirk_line_angle *)malloc(sizeof(uirk_line_angle) * G->A);
-                float *tu = (float *)malloc(sizeof(float) * 2 * G->A);
-                if (!G->h_lk || !G->h_line || !tu) {
-                    free(tu);
+                if (!G->h_lk || !G->h_line) {
                     s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
                     goto fail;
                 }
-                uir_line_table(G->h_sin, G->h_cos, G->A, G->n, G->axis, G->h_line, tu, G->h_lk);
-                for (uint32_t a = 0; a < G->A; ++a) {
-                    G->h_sin[a] = tu[2 * a];
-                    G->h_cos[a] = tu[2 * a + 1];
-                }
-                free(tu);
-                uint32_t k = 0;
-                for (uint32_t a = 0; a < G->A; ++a) {
-                    const int dir = !G->h_line[a].family;
-                    if (k && G->runs[k - 1].direction == dir) {
-                        G->runs[k - 1].n++;
-                    } else {
-                        G->runs[k].offset = a;
-                        G->runs[k].n = 1;
-                        G->runs[k].direction = dir;
-                        ++k;
-                    }
-                }
-                G->n_runs = k;
+                if ((s = uir_line_geom(G->A, G->n, G->axis, G->h_sin, G->h_cos, G->h_line, G->h_lk, G->runs,
+                                       &G->n_runs)) != UIR_OK)
+                    goto fail;
             } else {
                 G->n_runs = uir_geom_runs(G->h_sin, G->h_cos, G->A, G->runs);
             }
@@ -322,7 +338,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
                                    "hipMemcpy")) != UIR_OK ||
                 (s = uir_check_hip(hipMemcpy(G->d_line, G->h_line, sizeof(uirk_line_angle) * G->A,
                                              hipMemcpyHostToDevice), "hipMemcpy")) != UIR_OK ||
-                (s = line_weights(b->res->stream, G)) != UIR_OK)
+                (s = uir_line_weights(b->res->stream, G->n, G->D, G->A, G->r_min, G->runs, G->n_runs, G->d_line,
+                                      G->d_ray_w, G->d_pix_w)) != UIR_OK)
                 goto fail;
         } else if ((s = uir_ray_weights(b->res->stream, G->n, G->D, G->A, G->axis, G->d_trig, G->r_min,
                                         G->d_ray_w)) != UIR_OK) {

@@ -57,6 +57,14 @@ void uir_dfree(void *ptr);
  * FP(1) = 0 or FP(1) < r_min; synchronises the stream (uir_batch.c). */
 uir_status uir_ray_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, float axis, const float *d_trig,
                            float r_min, float *d_ray_w);
+/* The line model: a geometry's host tables (the unit directions replace the
+ * LUT; runs of one direction family each), and its weights R = 1/FP(1) and
+ * C = 1/BP(1) from the angle table on the device; synchronises the stream
+ * (uir_batch.c). */
+uir_status uir_line_geom(uint32_t A, uint32_t n, float axis, float *sin_lut, float *cos_lut, uirk_line_angle *h_line,
+                         float *h_lk, uir_subset *runs, uint32_t *n_runs);
+uir_status uir_line_weights(void *stream, uint32_t n, uint32_t D, uint32_t A, float r_min, const uir_subset *runs,
+                            uint32_t n_runs, const uirk_line_angle *d_line, float *d_ray_w, float *d_pix_w);
 
 /* A storage mode split into the sinogram and image element types; returns 0
  * for an unknown mode. */

@@ -19,7 +19,10 @@
  *   5. retire requests whose iterations are done: copy the image to the
  *      host, release its memory, wake its waiter.
  * At most two steps are in flight, so arrivals are seen within about two step
- * times.  Every FP and BP work item writes only its own slice, so a request's
+ * times.  SIRT requests of the line projector model (uir_slice_spec.model = 1)
+ * carry its per-angle constants and pixel weights C = 1/BP(1) in their
+ * geometry and share the steps in work tables of their own (the table
+ * kernels' line mode).  Every FP and BP work item writes only its own slice, so a request's
  * result does not depend on what shares its steps.
  *
  * SBTV_SIRT is split-Bregman TV (the outer loop of ufo-ir-sbtv-task.c:
@@ -30,7 +33,8 @@
  *   P = 1 / (mu A_n + 8 lambda),
  * A_n = the angle count (= A^T R A 1 for the fast projector pair) and 8 the
  * row-sum bound of D^T D, so P^-1 majorises the subproblem's Hessian and any
- * relaxation w in (0, 1] converges.  u, d and b start at 0.
+ * relaxation w in (0, 1] converges.  u, d and b start at 0.  That bound
+ * holds for Joseph's pair only, so SBTV_SIRT takes the joseph model.
  */
 #define _GNU_SOURCE
 #include <errno.h>
@@ -66,11 +70,22 @@ typedef struct sgeom {
     double last_used;
     uirk_tile tile;        /* tuned or default tile shape */
     int fp_shape, bp_shape; /* group indices of that shape */
+    int32_t model;          /* 0 joseph, 1 line */
+    /* line model: per-angle constants of the table kernels (8 floats) and of
+     * the single-angle kernels, and the pixel weights 1 / BP(1) (n x n),
+     * carved from mem */
+    float *d_lk, *d_pix_w;
+    uirk_line_angle *d_line;
     struct sgeom *next;
 } sgeom;
 
-#define SRV_FP_SHAPES 6
-#define SRV_BP_SHAPES 4
+/* work-table groups: FP angles 16, 8, 32 with pair staging, then with single
+ * values, then the line model; BP 32 or 16 angles x 64 or 32 rows, then the
+ * line model */
+#define SRV_FP_SHAPES 7
+#define SRV_BP_SHAPES 5
+#define SRV_FP_LINE 6
+#define SRV_BP_LINE 4
 
 typedef struct {
     uirk_tile tile;
@@ -167,7 +182,15 @@ static uint64_t req_bytes(const uir_request *r, size_t ess, size_t esi)
     return b + REQ_SLACK;
 }
 
-static uint64_t geom_bytes(uint32_t D, uint32_t A) { return align_up((size_t)A * D * 4) + align_up((size_t)A * 8); }
+static uint64_t geom_bytes(const uir_slice_spec *sp)
+{
+    const uint64_t A = sp->n_angles;
+    uint64_t b = align_up((size_t)A * sp->n_dets * 4) + align_up((size_t)A * 8);
+    if (sp->model == 1)
+        b += align_up((size_t)A * 32) + align_up((size_t)sp->n * sp->n * 4) +
+             align_up((size_t)A * sizeof(uirk_line_angle));
+    return b;
+}
 
 static uir_status check_request(const uir_request *r);
 
@@ -180,7 +203,7 @@ uir_status uir_request_footprint(const uir_request *req, uir_dtype storage, uint
     if (!uir_storage_split(storage, &sdt, &idt))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown storage dtype");
     *bytes = req_bytes(req, uir_dtype_size(sdt), uir_dtype_size(idt)) +
-             geom_bytes(req->slice.n_dets, req->slice.n_angles);
+             geom_bytes(&req->slice);
     return UIR_OK;
 }
 
@@ -199,9 +222,8 @@ static uir_status check_request(const uir_request *r)
                              (double)s->axis_position);
     if (!(s->ray_weight_min >= 0.0f) || !isfinite(s->ray_weight_min))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad ray_weight_min");
-    if (s->model != 0)
-        return uir_set_error(s->model == 1 ? UIR_ERR_UNSUPPORTED : UIR_ERR_INVALID_ARGUMENT,
-                             s->model == 1 ? "the server takes the joseph model only" : "unknown model %d", s->model);
+    if (s->model != 0 && s->model != 1)
+        return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown model %d", s->model);
     if (r->iterations == 0)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
     if (r->method == UIR_METHOD_SIRT) {
@@ -211,6 +233,8 @@ static uir_status check_request(const uir_request *r)
     }
     if (r->method != UIR_METHOD_SBTV_SIRT)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown method %u", r->method);
+    if (s->model != 0)
+        return uir_set_error(UIR_ERR_UNSUPPORTED, "SBTV_SIRT takes the joseph model only");
     if (!(s->relaxation > 0.0f && s->relaxation <= 1.0f))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "relaxation outside (0, 1]");
     if (r->inner_iterations == 0)
@@ -250,7 +274,7 @@ static sgeom *geom_find(uir_server *s, const uir_slice_spec *sp)
     const float axis = uir_geom_axis(sp->axis_position, sp->n_dets);
     for (sgeom *g = s->geoms; g; g = g->next)
         if (g->n == sp->n && g->D == sp->n_dets && g->A == sp->n_angles && g->step == sp->step && g->axis == axis &&
-            g->r_min == sp->ray_weight_min)
+            g->r_min == sp->ray_weight_min && g->model == sp->model)
             return g;
     return NULL;
 }
@@ -267,20 +291,29 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
     g->step = sp->step;
     g->axis = uir_geom_axis(sp->axis_position, sp->n_dets);
     g->r_min = sp->ray_weight_min;
-    g->bytes = geom_bytes(g->D, g->A);
+    g->model = sp->model;
+    g->bytes = geom_bytes(sp);
     g->h_sin = (float *)malloc(sizeof(float) * g->A);
     g->h_cos = (float *)malloc(sizeof(float) * g->A);
     g->runs = (uir_subset *)malloc(sizeof(uir_subset) * g->A);
     float *trig = (float *)malloc(sizeof(float) * 2 * g->A);
+    float *h_lk = g->model ? (float *)malloc(sizeof(float) * 8 * g->A) : NULL;
+    uirk_line_angle *h_line = g->model ? (uirk_line_angle *)malloc(sizeof(uirk_line_angle) * g->A) : NULL;
     uir_status st = UIR_OK;
-    if (!g->h_sin || !g->h_cos || !g->runs || !trig) {
+    if (!g->h_sin || !g->h_cos || !g->runs || !trig || (g->model && (!h_lk || !h_line))) {
         st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
         goto fail;
     }
     uir_geom_lut(g->A, g->step, g->h_sin, g->h_cos);
-    g->n_runs = uir_geom_runs(g->h_sin, g->h_cos, g->A, g->runs);
+    if (g->model) { /* the unit directions replace the LUT; runs of one family */
+        if ((st = uir_line_geom(g->A, g->n, g->axis, g->h_sin, g->h_cos, h_line, h_lk, g->runs, &g->n_runs)) !=
+            UIR_OK)
+            goto fail;
+    } else {
+        g->n_runs = uir_geom_runs(g->h_sin, g->h_cos, g->A, g->runs);
+    }
     uirk_plan_trig(g->h_sin, g->h_cos, g->A, trig);
-    if (s->autotune) {
+    if (s->autotune && !g->model) {
         uir_tile t;
         if (uir_tune_get(sp, s->dtype, &t) == UIR_ERR_NOT_FOUND) {
             uir_tune_result tr;
@@ -288,15 +321,24 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
                 goto fail;
         }
     }
-    g->tile = uir_tile_for(sp, s->dtype);
-    g->fp_shape = (g->tile.fp_angles == 8 ? 1 : (g->tile.fp_angles == 32 ? 2 : 0)) + 3 * (g->tile.fp_single != 0);
-    g->bp_shape = (g->tile.bp_angles == 16) * 2 + (g->tile.bp_rows == 32);
+    if (g->model) { /* the line model's groups take the default tile */
+        g->tile = (uirk_tile)UIRK_TILE_DEFAULT;
+        g->fp_shape = SRV_FP_LINE;
+        g->bp_shape = SRV_BP_LINE;
+    } else {
+        g->tile = uir_tile_for(sp, s->dtype);
+        g->fp_shape = (g->tile.fp_angles == 8 ? 1 : (g->tile.fp_angles == 32 ? 2 : 0)) + 3 * (g->tile.fp_single != 0);
+        g->bp_shape = (g->tile.bp_angles == 16) * 2 + (g->tile.bp_rows == 32);
+    }
     {
         uirk_fp_vec fv = {0};
         uirk_bp_vec bv = {0};
-        if (uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs, g->n_runs,
-                         g->tile.fp_angles, UIRK_TILE_FP_WMAX(g->tile)) ||
-            uirk_plan_bp(&bv, 0, (int)g->n, g->tile.bp_rows)) {
+        const int fail_fp = g->model ? uirk_plan_fp_line(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos,
+                                                         g->runs, g->n_runs, g->tile.fp_angles,
+                                                         UIRK_TILE_FP_WMAX(g->tile))
+                                     : uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs,
+                                                    g->n_runs, g->tile.fp_angles, UIRK_TILE_FP_WMAX(g->tile));
+        if (fail_fp || uirk_plan_bp(&bv, 0, (int)g->n, g->tile.bp_rows)) {
             free(fv.v);
             free(bv.v);
             st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
@@ -321,13 +363,35 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
     if ((st = uir_check_hip(hipMemcpyAsync(g->d_trig, trig, sizeof(float) * 2 * g->A, hipMemcpyHostToDevice,
                                            s->stream), "hipMemcpyAsync")) != UIR_OK)
         goto fail;
-    if ((st = uir_ray_weights(s->stream, g->n, g->D, g->A, g->axis, g->d_trig, g->r_min, g->d_ray_w)) != UIR_OK)
+    if (g->model) {
+        char *p = (char *)g->d_trig + align_up((size_t)g->A * 8);
+        g->d_lk = (float *)p;
+        p += align_up((size_t)g->A * 32);
+        g->d_pix_w = (float *)p;
+        p += align_up((size_t)g->n * g->n * 4);
+        g->d_line = (uirk_line_angle *)p;
+        if ((st = uir_check_hip(hipMemcpyAsync(g->d_lk, h_lk, sizeof(float) * 8 * g->A, hipMemcpyHostToDevice,
+                                               s->stream), "hipMemcpyAsync")) != UIR_OK ||
+            (st = uir_check_hip(hipMemcpyAsync(g->d_line, h_line, sizeof(uirk_line_angle) * g->A,
+                                               hipMemcpyHostToDevice, s->stream), "hipMemcpyAsync")) != UIR_OK ||
+            (st = uir_line_weights(s->stream, g->n, g->D, g->A, g->r_min, g->runs, g->n_runs, g->d_line, g->d_ray_w,
+                                   g->d_pix_w)) != UIR_OK)
+            goto fail;
+    } else if ((st = uir_ray_weights(s->stream, g->n, g->D, g->A, g->axis, g->d_trig, g->r_min, g->d_ray_w)) !=
+               UIR_OK) {
         goto fail;
+    }
     free(trig);
+    free(h_lk);
+    free(h_line);
     *out = g;
     return UIR_OK;
 fail:
+    /* the stream may still read the host tables: wait before freeing them */
+    (void)hipStreamSynchronize(s->stream);
     free(trig);
+    free(h_lk);
+    free(h_line);
     geom_free(g);
     return st;
 }
@@ -367,7 +431,7 @@ static void geom_trim_idle(uir_server *s)
 /* Bytes admitting q needs now (its buffers, plus its geometry unless resident). */
 static uint64_t admit_bytes(uir_server *s, const sreq *q)
 {
-    return q->bytes + (geom_find(s, &q->rq.slice) ? 0 : geom_bytes(q->rq.slice.n_dets, q->rq.slice.n_angles));
+    return q->bytes + (geom_find(s, &q->rq.slice) ? 0 : geom_bytes(&q->rq.slice));
 }
 
 /* Under the lock: pops the requests that fit, in arrival order. */
@@ -574,6 +638,11 @@ static uir_status tables_build(uir_server *s)
             k->pix_scale = q->rq.mu * P;
             k->tv = q->tv;
             k->tv_scale = q->rq.slice.relaxation * q->rq.lambda * P;
+        } else if (g->model) { /* line: its constants, and per-pixel weights in place of 1/A */
+            f->line = g->d_lk;
+            k->line = g->d_lk;
+            k->pix_w = g->d_pix_w;
+            k->pix_scale = 1.0f;
         } else {
             k->pix_scale = 1.0f / (float)g->A;
         }
@@ -647,12 +716,12 @@ static uir_status step_launch(uir_server *s)
     for (int c = 0; c < SRV_FP_SHAPES; ++c)
         if (s->fpg[c].n)
             TRY(uir_check_hip(uirk_fp_table(s->d_fp_sl, NULL, (const uirk_fp_item *)s->fpg[c].d_items, s->fpg[c].n,
-                                            &s->fpg[c].part, 1, s->sdt, s->idt, &s->fpg[c].tile, s->fpg[c].grid, s->d_err,
+                                            &s->fpg[c].part, c == SRV_FP_LINE ? 2 : 1, s->sdt, s->idt, &s->fpg[c].tile, s->fpg[c].grid, s->d_err,
                                             s->stream), "FP launch"));
     for (int c = 0; c < SRV_BP_SHAPES; ++c)
         if (s->bpg[c].n)
             TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, (const uirk_bp_item *)s->bpg[c].d_items, s->bpg[c].n,
-                                            &s->bpg[c].part, 1, s->sdt, s->idt, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
+                                            &s->bpg[c].part, c == SRV_BP_LINE ? 2 : 1, s->sdt, s->idt, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
                                             s->stream), "BP launch"));
     for (uint32_t i = 0; i < s->n_active; ++i) {
         sreq *q = s->active[i];
@@ -913,11 +982,15 @@ uir_status uir_server_new(int device, const uir_server_config *cfg, uir_server *
     s->autotune = cfg->autotune;
     for (int c = 0; c < SRV_FP_SHAPES; ++c) {
         s->fpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
+        if (c == SRV_FP_LINE)
+            continue;
         s->fpg[c].tile.fp_angles = c % 3 == 1 ? 8 : (c % 3 == 2 ? 32 : UIRK_FP_ANGLES);
         s->fpg[c].tile.fp_single = c / 3;
     }
     for (int c = 0; c < SRV_BP_SHAPES; ++c) {
         s->bpg[c].tile = (uirk_tile)UIRK_TILE_DEFAULT;
+        if (c == SRV_BP_LINE)
+            continue;
         s->bpg[c].tile.bp_angles = c >= 2 ? 16 : 32;
         s->bpg[c].tile.bp_rows = (c & 1) ? 32 : UIRK_BP_TILE;
     }
@@ -953,7 +1026,7 @@ uir_status uir_server_submit(uir_server *s, const uir_request *req, const float 
     if (!s || !req || !sinogram || !ticket)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "NULL argument");
     TRY(check_request(req));
-    const uint64_t need = req_bytes(req, s->ess, s->esi) + geom_bytes(req->slice.n_dets, req->slice.n_angles);
+    const uint64_t need = req_bytes(req, s->ess, s->esi) + geom_bytes(&req->slice);
     if (need > s->budget)
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "request needs %llu device bytes, the budget is %llu",
                              (unsigned long long)need, (unsigned long long)s->budget);
