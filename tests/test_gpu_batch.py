@@ -47,6 +47,46 @@ def test_batch_matches_fp64(res, storage, tol):
         assert err <= tol
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16-sino", "f16", "bf16-sino"])
+def test_batch_slice_pairs_equal_single_slices(res, storage):
+    """Consecutive slices of one geometry share their work items (slice pairs:
+    one coordinate, address and 16-byte LDS read per sample serve both slices).
+    Each slice's image must equal, bit for bit, that slice reconstructed alone
+    (a batch of one: no pair), including a slice left unpaired between pairs,
+    an off-centre axis (the BP's clamped coordinate), sparse angles (the
+    single-value FP, whose items do not pair, with paired BP items), distinct
+    relaxations and POSC within a pair."""
+    iters = 7
+    cases = [  # (n, d, a, axis_position, relax, posc)
+        (128, 182, 90, -1.0, 0.5, False),
+        (128, 182, 90, -1.0, 0.3, True),
+        (128, 182, 90, -1.0, 0.5, False),
+        (97, 137, 60, 70.3, 0.5, False),
+        (97, 137, 60, 70.3, 0.7, False),
+        (256, 363, 24, -1.0, 0.5, False),
+        (256, 363, 24, -1.0, 0.5, True),
+    ]
+    print("case slice pairs", storage, cases, flush=True)
+    specs = [dict(n=n, n_dets=d, n_angles=a, step=float(np.float32(np.pi / a)), axis_position=ax, relaxation=rl,
+                  positive_constraint=pc) for (n, d, a, ax, rl, pc) in cases]
+    sinos = []
+    for i, (n, d, a, ax, rl, pc) in enumerate(cases):
+        g = O.Geometry(n, d, a, np.float32(np.pi / a), ax)
+        b = O.fp64(g, ellipse_phantom(n, seed=900 + i).astype(np.float64))
+        sinos.append((b + np.random.default_rng(900 + i).normal(0.0, 0.01 * b.max(), b.shape)).astype(np.float32))
+    bt = uir.Batch(res, specs, storage)
+    for i, b in enumerate(sinos):
+        bt.set_sinogram(i, b)
+    bt.reset()
+    bt.iterate(iters)
+    for i, c in enumerate(cases):
+        solo = uir.Batch(res, [specs[i]], storage)
+        solo.set_sinogram(0, sinos[i])
+        solo.reset()
+        solo.iterate(iters)
+        assert np.array_equal(bt.image(i), solo.image(0)), (i, c)
+
+
 @pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16", 1e-3)])
 def test_batch_noisy_with_ray_weight_floor_matches_fp64(res, storage, tol):
     """1% noise: with ray_weight_min = 0.1 in both the batch and the oracle the

@@ -180,9 +180,11 @@ typedef struct {
     int32_t slice, dir, a0, na, d0, l_lo, l_hi, pad;
 } uirk_fp_item;
 
-/* BP work item: the 64 x 64 pixel tile at (x0, y0) of one slice. */
+/* BP work item: the 64 x 64 pixel tile at (x0, y0) of one slice.  pair (mode
+ * 3 only): the tile of slice + 1 too, which has the same geometry (the batch
+ * sets it, not the planner). */
 typedef struct {
-    int32_t slice, x0, y0, pad;
+    int32_t slice, x0, y0, pair;
 } uirk_bp_item;
 
 /* XCD-aware partition of a work table: workgroup g runs on XCD g % n_xcd
@@ -220,7 +222,10 @@ typedef struct {
 #define UIRK_TILE_FP_WMAX(t) ((t).fp_single ? UIRK_FP_WMAX_SINGLE : UIRK_FP_WMAX)
 
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
- * FP32 / FP16 / BF16 storage), 2 = the fast path with the line model (weights
+ * FP32 / FP16 / BF16 storage), 3 = (BP only) the fast path over slice pairs
+ * (items with pair set cover slices slice and slice + 1 of one geometry, with
+ * the same results as mode 1; d_slices required), 2 = the fast path with the
+ * line model (weights
  * from each slice's `line` table; the tile's bp_rows (64 or 32) only, the rest
  * default; BP: pix_w, when given, multiplies the update).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
  * sinogram buffers (b, r); img_dtype: of the image buffers (x, x_out), either
@@ -269,6 +274,9 @@ int uirk_plan_num_xcd(void);
  * (cost[i] >= 0; NULL = unit cost). */
 void uirk_plan_xcd(const double *cost, int n, int n_xcd, uirk_xcd_part *out);
 double uirk_fp_item_cost(const uirk_fp_item *it);
+/* Relative cost of a slice-pair BP item against a single-slice one (shared
+ * coordinates and addresses). */
+#define UIRK_PAIR_COST 1.6
 /* FP32 <-> storage conversions */
 int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream);
 

@@ -284,6 +284,24 @@ constexpr int kFpPL = 16;           // lines per pair-staged chunk (25 KB of LDS
 constexpr int kFpPs = kFpPW + 1;    // pair line stride (odd)
 constexpr int kFpPH = kFpPW / 2;
 
+// A staged (e, d) pair, or (e, d, e', d') for a slice pair (the BP's mode 3).
+template <bool PAIR> struct fp_entry {
+    using type = float2;
+};
+template <> struct fp_entry<true> {
+    using type = float4;
+};
+template <bool PAIR> __device__ __forceinline__ typename fp_entry<PAIR>::type make_entry(float e, float d, float e2,
+                                                                                         float d2);
+template <> __device__ __forceinline__ float2 make_entry<false>(float e, float d, float, float)
+{
+    return make_float2(e, d);
+}
+template <> __device__ __forceinline__ float4 make_entry<true>(float e, float d, float e2, float d2)
+{
+    return make_float4(e, d, e2, d2);
+}
+
 template <typename TI, bool PLAIN>
 __device__ __forceinline__ void fp_stage_pairs(float2 (&lds)[kFpPL * kFpPs], const TI *img, int N, bool hor, int l0,
                                                int nl, int pmin, int W)
@@ -864,14 +882,20 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
 // w0 = clamp(bp_b0 - bp_k (f - 1/2), 0, 1/M), w1 = clamp(bp_b1 + bp_k (f - 1/2),
 // 0, 1/M), f the fraction of the detector coordinate; pix_w, when given,
 // multiplies the update (the model's pixel weights are not uniform).
-template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE>
-__global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const uirk_slice *__restrict__ slices, uirk_slice one,
-                                                const uirk_bp_item *__restrict__ items, int n_items,
-                                                uirk_xcd_part part, int *__restrict__ err)
+// PAIR (fast path only): an item with item.pair set also backprojects slice
+// item.slice + 1 of the same geometry; both slices' (e, d) pairs share a
+// 16-byte entry, so a pixel-angle's coordinate, address and one LDS read
+// serve both (the same operations per slice as alone).
+template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE, bool PAIR>
+__global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 4) : 6))) void k_bp_tab(
+    const uirk_slice *__restrict__ slices, uirk_slice one, const uirk_bp_item *__restrict__ items, int n_items,
+    uirk_xcd_part part, int *__restrict__ err)
 {
+    static_assert(!PAIR || (!REF && !LINE), "slice pairs: the joseph fast path only");
+    using E = typename fp_entry<PAIR>::type;
     // reference-compatible path and line model: plain staged bins; fast path: (e, d) pairs
     __shared__ float lds[(REF || LINE) ? NA * kBpWb : 1];
-    __shared__ float2 ldp[(REF || LINE) ? 1 : NA * kBpWb];
+    __shared__ E ldp[(REF || LINE) ? 1 : NA * kBpWb];
     __shared__ int s_umin[NA];
     __shared__ float s_trig[2 * NA];
     __shared__ float4 s_line[LINE ? NA : 1];
@@ -884,7 +908,10 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
         const uirk_bp_item item = items[it];
         const uirk_slice *S = slices ? &slices[item.slice] : &one;
         const int N = S->n, D = S->n_dets;
+        const bool paired = PAIR && item.pair != 0 && slices;
+        const uirk_slice *S2 = paired ? &slices[item.slice + 1] : S;
         const T *sino = uniform((const T *)S->b);
+        const T *sino2 = PAIR ? uniform((const T *)S2->b) : sino;
         const float *trig = uniform(S->trig);
         const float relax = S->relax;
         const int n_runs = S->n_runs;
@@ -921,11 +948,13 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
         const float cy1 = cy0 + (float)(4 * ROWS - 1);
         const float fY0 = (float)(item.y0 + wave) + 0.5f - origin_shift;
 
-        float acc[ROWS], val[ROWS];
+        float acc[ROWS], val[ROWS], acc2[PAIR ? ROWS : 1];
 #pragma unroll
         for (int k = 0; k < ROWS; ++k) {
             acc[k] = 0.0f;
             val[k] = 0.0f;
+            if constexpr (PAIR)
+                acc2[k] = 0.0f;
         }
         if (REF && !S->pix_w) {
             const TI *xin = uniform((const TI *)S->x);
@@ -975,7 +1004,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                     // all loads first, then the LDS writes
                     constexpr int kPer = (NA / 4) * 2;
                     static_assert(kBpWb <= 128, "two bins per lane");
-                    float v0[kPer], v1[kPer];
+                    float v0[kPer], v1[kPer], y0[PAIR ? kPer : 1], y1[PAIR ? kPer : 1];
 #pragma unroll
                     for (int e = 0; e < kPer; ++e) {
                         // clamped, always valid indices: no branches around the loads
@@ -995,6 +1024,10 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                             const int b1 = bin + 1 < lo ? lo : (bin + 1 > hi ? hi : bin + 1);
                             v0[e] = gload(sino, rowo + (uint32_t)b0);
                             v1[e] = gload(sino, rowo + (uint32_t)b1);
+                            if constexpr (PAIR) {
+                                y0[e] = gload(sino2, rowo + (uint32_t)b0);
+                                y1[e] = gload(sino2, rowo + (uint32_t)b1);
+                            }
                         }
                     }
 #pragma unroll
@@ -1008,7 +1041,10 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                                 // (e, d) with d = t[i+1] - t[i], e = t[i] - i d: the value at
                                 // window coordinate w in [i, i+1) is e + w d
                                 const float d = v1[e] - v0[e];
-                                ldp[k * kBpWb + i] = make_float2(__builtin_fmaf(-(float)i, d, v0[e]), d);
+                                const float d2 = PAIR ? y1[e] - y0[e] : 0.0f;
+                                ldp[k * kBpWb + i] =
+                                    make_entry<PAIR>(__builtin_fmaf(-(float)i, d, v0[e]), d,
+                                                     PAIR ? __builtin_fmaf(-(float)i, d2, y0[e]) : 0.0f, d2);
                             }
                         }
                     }
@@ -1042,7 +1078,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                     } else {
                         // coordinates relative to the staged window: the address of a
                         // sample is then P + 8 floor(uu), one shift-add
-                        const float2 *P = ldp + k * kBpWb;
+                        const E *P = ldp + k * kBpWb;
                         float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f)) - (float)s_umin[k];
                         const float du = -4.0f * st;
                         if constexpr (LINE) {
@@ -1072,7 +1108,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                         } else if (!clamp_u) {
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
-                                float2 p[8];
+                                E p[8];
                                 float w[8];
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
@@ -1080,8 +1116,11 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                                     p[h] = P[cvt_flr(w[h])];
                                 }
 #pragma unroll
-                                for (int h = 0; h < 8; ++h)
+                                for (int h = 0; h < 8; ++h) {
                                     acc[g + h] = __builtin_fmaf(w[h], p[h].y, acc[g + h] + p[h].x);
+                                    if constexpr (PAIR)
+                                        acc2[g + h] = __builtin_fmaf(w[h], p[h].w, acc2[g + h] + p[h].z);
+                                }
                                 u = __builtin_fmaf(8.0f, du, u);
                             }
                         } else {
@@ -1090,7 +1129,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                             const float whi = (edge1 - 0.5f) - (float)s_umin[k];
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
-                                float2 p[8];
+                                E p[8];
                                 float w[8];
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
@@ -1098,8 +1137,11 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                                     p[h] = P[cvt_flr(w[h])];
                                 }
 #pragma unroll
-                                for (int h = 0; h < 8; ++h)
+                                for (int h = 0; h < 8; ++h) {
                                     acc[g + h] = __builtin_fmaf(w[h], p[h].y, acc[g + h] + p[h].x);
+                                    if constexpr (PAIR)
+                                        acc2[g + h] = __builtin_fmaf(w[h], p[h].w, acc2[g + h] + p[h].z);
+                                }
                                 u = __builtin_fmaf(8.0f, du, u);
                             }
                         }
@@ -1115,14 +1157,14 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
             }
         }
 
-        if (px < N) {
-            const TI *xin = uniform((const TI *)S->x);
-            TI *xout = uniform((TI *)S->x_out);
-            const float *pix_w = uniform(S->pix_w);
-            const int posc = S->posc;
-            const float gain = S->relax * S->pix_scale;
-            const float *tv = REF ? nullptr : uniform(S->tv);
-            const float tv_scale = S->tv_scale;
+        auto epilogue = [&](const uirk_slice *Sx, const float(&a)[ROWS]) {
+            const TI *xin = uniform((const TI *)Sx->x);
+            TI *xout = uniform((TI *)Sx->x_out);
+            const float *pix_w = uniform(Sx->pix_w);
+            const int posc = Sx->posc;
+            const float gain = Sx->relax * Sx->pix_scale;
+            const float *tv = REF ? nullptr : uniform(Sx->tv);
+            const float tv_scale = Sx->tv_scale;
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
                 const int py = item.y0 + wave + 4 * q;
@@ -1131,21 +1173,28 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : 6)) void k_bp_tab(const 
                 const uint32_t o = (uint32_t)py * (uint32_t)N + (uint32_t)px;
                 float out;
                 if (REF) {
-                    out = acc[q];
+                    out = a[q];
                     if (pix_w) { // SIRT form (ufo-ir-sirt-task.c:208-212)
                         out = out * gload(pix_w, o);
                         out = out + gload(xin, o);
                     }
                 } else if (LINE && pix_w) {
-                    out = __builtin_fmaf(acc[q] * gain, gload(pix_w, o), gload(xin, o));
+                    out = __builtin_fmaf(a[q] * gain, gload(pix_w, o), gload(xin, o));
                 } else {
-                    out = __builtin_fmaf(acc[q], gain, gload(xin, o));
+                    out = __builtin_fmaf(a[q], gain, gload(xin, o));
                     if (tv) // split-Bregman TV term, evaluated at the same x
                         out = __builtin_fmaf(tv_scale, gload(tv, o), out);
                 }
                 if (posc)
                     out = out > 0 ? out : 0.0f;
                 gstore(xout, o, out);
+            }
+        };
+        if (px < N) {
+            epilogue(S, acc);
+            if constexpr (PAIR) {
+                if (paired)
+                    epilogue(S2, acc2);
             }
         }
     }
@@ -1548,19 +1597,19 @@ int launch_fp_pairs(const uirk_slice *sl, const uirk_slice &one, const uirk_fp_i
     return (int)hipGetLastError();
 }
 
-template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE = false>
+template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE = false, bool PAIR = false>
 int launch_bp(const uirk_slice *sl, const uirk_slice &one, const uirk_bp_item *items, int n,
               const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS, LINE>), dim3(grid), dim3(256), 0, st, sl, one, items, n, part,
-                       err);
+    hipLaunchKernelGGL((k_bp_tab<T, TI, REF, NA, ROWS, LINE, PAIR>), dim3(grid), dim3(256), 0, st, sl, one, items, n,
+                       part, err);
     return (int)hipGetLastError();
 }
 
 // fast-path FP over the tile variants: 16, 8 or 32 angles per item (4, 2 or 8 per thread);
 // the line model takes the default tile
 template <typename T, typename TI>
-int launch_fp_fast(const uirk_tile &tile, bool line, const uirk_slice *sl, const uirk_slice &one,
+int launch_fp_fast(const uirk_tile &tile, bool line, bool, const uirk_slice *sl, const uirk_slice &one,
                    const uirk_fp_item *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
     if (line)
@@ -1581,13 +1630,22 @@ int launch_fp_fast(const uirk_tile &tile, bool line, const uirk_slice *sl, const
 
 // fast-path BP over the tile variants: 32 or 16 staged angles, tiles of 64 or 32 rows
 template <typename T, typename TI>
-int launch_bp_fast(const uirk_tile &tile, bool line, const uirk_slice *sl, const uirk_slice &one,
+int launch_bp_fast(const uirk_tile &tile, bool line, bool pair, const uirk_slice *sl, const uirk_slice &one,
                    const uirk_bp_item *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
     if (line) // 32 staged angles; tiles of 64 or 32 rows
         return tile.bp_rows == 32 ? launch_bp<T, TI, false, 32, 8, true>(sl, one, items, n, part, grid, err, st)
                                   : launch_bp<T, TI, false, 32, 16, true>(sl, one, items, n, part, grid, err, st);
     const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
+    if (pair) {
+        if (na16 && r8)
+            return launch_bp<T, TI, false, 16, 8, false, true>(sl, one, items, n, part, grid, err, st);
+        if (na16)
+            return launch_bp<T, TI, false, 16, 16, false, true>(sl, one, items, n, part, grid, err, st);
+        if (r8)
+            return launch_bp<T, TI, false, 32, 8, false, true>(sl, one, items, n, part, grid, err, st);
+        return launch_bp<T, TI, false, 32, 16, false, true>(sl, one, items, n, part, grid, err, st);
+    }
     if (na16 && r8)
         return launch_bp<T, TI, false, 16, 8>(sl, one, items, n, part, grid, err, st);
     if (na16)
@@ -1600,13 +1658,15 @@ int launch_bp_fast(const uirk_tile &tile, bool line, const uirk_slice *sl, const
 // (sinogram, image) storage pairs: both FP32, both FP16, both BF16, or
 // FP16 / BF16 sinograms with an FP32 image
 template <bool FP>
-int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, bool line, const uirk_slice *sl,
+int launch_fast(int dtype, int img_dtype, const uirk_tile &tile, bool line, bool pair, const uirk_slice *sl,
                 const uirk_slice &one, const void *items, int n, const uirk_xcd_part &part, int grid, int *err,
                 hipStream_t st)
 {
-#define UIRK_LAUNCH(TS, TIM)                                                                                        \
-    return FP ? launch_fp_fast<TS, TIM>(tile, line, sl, one, (const uirk_fp_item *)items, n, part, grid, err, st) \
-              : launch_bp_fast<TS, TIM>(tile, line, sl, one, (const uirk_bp_item *)items, n, part, grid, err, st)
+#define UIRK_LAUNCH(TS, TIM)                                                                                         \
+    return FP ? launch_fp_fast<TS, TIM>(tile, line, pair, sl, one, (const uirk_fp_item *)items, n, part, grid, err, \
+                                        st)                                                                          \
+              : launch_bp_fast<TS, TIM>(tile, line, pair, sl, one, (const uirk_bp_item *)items, n, part, grid, err, \
+                                        st)
     if (img_dtype == UIR_DTYPE_F32) {
         switch (dtype) {
         case UIR_DTYPE_F16: UIRK_LAUNCH(__half, float);
@@ -1663,8 +1723,8 @@ int uirk_fp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
     }
     if (mode != 1 && mode != 2)
         return (int)hipErrorInvalidValue;
-    return launch_fast<true>(dtype, img_dtype, tl, mode == 2, d_slices, single, d_items, n_items, pt, grid, d_err,
-                             st);
+    return launch_fast<true>(dtype, img_dtype, tl, mode == 2, false, d_slices, single, d_items, n_items, pt, grid,
+                             d_err, st);
 }
 
 int uirk_ray_weights64(const float *d_trig, int n, int n_dets, int n_angles, float axis, float r_min, float *d_ray_w,
@@ -1756,10 +1816,10 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    if (mode != 1 && mode != 2)
+    if ((mode != 1 && mode != 2 && mode != 3) || (mode == 3 && !d_slices))
         return (int)hipErrorInvalidValue;
-    return launch_fast<false>(dtype, img_dtype, tl, mode == 2, d_slices, single, d_items, n_items, pt, grid, d_err,
-                              st);
+    return launch_fast<false>(dtype, img_dtype, tl, mode == 2, mode == 3, d_slices, single, d_items, n_items, pt, grid,
+                              d_err, st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

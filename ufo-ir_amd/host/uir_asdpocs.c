@@ -236,7 +236,7 @@ static uir_status launch_asd_iteration(uir_batch *b, const uir_asdpocs_params *p
     for (int g = 0; g < UIR_FP_SHAPES; ++g) {
         const kgroup *kg = &b->fpg[g];
         if (kg->n_items)
-            HIP(uirk_fp_table(a->d_dd_sl, NULL, (const uirk_fp_item *)kg->d_items, kg->n_items, &kg->part, 1, b->sdt,
+            HIP(uirk_fp_table(a->d_dd_sl, NULL, (const uirk_fp_item *)kg->d_items, kg->n_items, &kg->part, kg->mode, b->sdt,
                               b->idt, &kg->tile, kg->grid, b->d_err, st), "residual FP launch");
     }
     HIP(uirk_asd_sino_norm(b->sdt, a->d_sl, S, nblk, tb, a->d_partials, st), "residual norm");

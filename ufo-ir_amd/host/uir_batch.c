@@ -216,6 +216,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     b->r = (void **)calloc(n_slices, sizeof(void *));
     uir_status s = UIR_OK;
     uirk_slice *fp_sl = NULL, *bp_sl = NULL;
+    char *lead = NULL; /* slice i leads a pair with slice i + 1 */
     uirk_fp_vec fv[UIR_FP_SHAPES] = {{0}}, fvl = {0};
     uirk_bp_vec bv[UIR_BP_SHAPES] = {{0}}, bvl = {0};
     if (!b->specs || !b->geom_of || !b->geoms || !b->x || !b->b || !b->r) {
@@ -350,10 +351,17 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
     /* descriptors and work tables */
     fp_sl = (uirk_slice *)calloc(n_slices, sizeof *fp_sl);
     bp_sl = (uirk_slice *)calloc(n_slices, sizeof *bp_sl);
-    if (!fp_sl || !bp_sl) {
+    lead = (char *)calloc(n_slices, 1);
+    if (!fp_sl || !bp_sl || !lead) {
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
         goto fail;
     }
+    /* consecutive joseph slices of one geometry pair up in the BP (mode 3) */
+    for (uint32_t i = 0; i + 1 < n_slices; ++i)
+        if (b->geom_of[i + 1] == b->geom_of[i] && !b->geoms[b->geom_of[i]].model) {
+            lead[i] = 1;
+            ++i;
+        }
     for (uint32_t i = 0; i < n_slices; ++i) {
         const geom_t *G = &b->geoms[b->geom_of[i]];
         uirk_slice *f = &fp_sl[i], *k = &bp_sl[i];
@@ -390,11 +398,20 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
                 goto fail;
             }
-        } else if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
-                                G->runs, G->n_runs, G->tile.fp_angles, UIRK_TILE_FP_WMAX(G->tile)) ||
-                   uirk_plan_bp(&bv[bp_shape(&G->tile)], (int)i, (int)G->n, G->tile.bp_rows)) {
-            s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
-            goto fail;
+        } else {
+            /* BP slice pairs: slice i + 1 of the same geometry joins slice i's
+             * BP items (pair set) */
+            const int second = i > 0 && lead[i - 1];
+            uirk_bp_vec *bvg = &bv[bp_shape(&G->tile)];
+            const size_t b0 = bvg->n;
+            if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
+                             G->runs, G->n_runs, G->tile.fp_angles, UIRK_TILE_FP_WMAX(G->tile)) ||
+                (!second && uirk_plan_bp(bvg, (int)i, (int)G->n, G->tile.bp_rows))) {
+                s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+                goto fail;
+            }
+            for (size_t j = b0; j < bvg->n && lead[i]; ++j)
+                bvg->v[j].pair = 1;
         }
         double m = 0;
         for (uint32_t a = 0; a < G->A; ++a)
@@ -421,6 +438,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
             k->tile.fp_angles = k_fp_shape_angles[g % 3];
             k->tile.fp_single = g / 3;
+            k->mode = 1;
             for (size_t i = 0; i < fv[g].n; ++i)
                 cost[i] = uirk_fp_item_cost(&fv[g].v[i]);
             uirk_plan_xcd(cost, (int)fv[g].n, uirk_plan_num_xcd(), &k->part);
@@ -434,8 +452,11 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
             k->tile.bp_angles = g >= 2 ? 16 : 32;
             k->tile.bp_rows = (g & 1) ? 32 : UIRK_BP_TILE;
-            for (size_t i = 0; i < bv[g].n; ++i)
-                cost[i] = (double)specs[bv[g].v[i].slice].n_angles;
+            k->mode = 1;
+            for (size_t i = 0; i < bv[g].n; ++i) {
+                cost[i] = (double)specs[bv[g].v[i].slice].n_angles * (bv[g].v[i].pair ? UIRK_PAIR_COST : 1.0);
+                k->mode = bv[g].v[i].pair ? 3 : k->mode;
+            }
             uirk_plan_xcd(cost, (int)bv[g].n, uirk_plan_num_xcd(), &k->part);
             k->n_items = (int)bv[g].n;
             k->grid = uirk_plan_grid_xcd(&k->part, k->n_items);
@@ -473,6 +494,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         goto fail;
     free(fp_sl);
     free(bp_sl);
+    free(lead);
     for (int g = 0; g < UIR_FP_SHAPES; ++g)
         free(fv[g].v);
     for (int g = 0; g < UIR_BP_SHAPES; ++g)
@@ -484,6 +506,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
 fail:
     free(fp_sl);
     free(bp_sl);
+    free(lead);
     for (int g = 0; g < UIR_FP_SHAPES; ++g)
         free(fv[g].v);
     for (int g = 0; g < UIR_BP_SHAPES; ++g)
@@ -529,7 +552,7 @@ static uir_status launch_fp_groups(uir_batch *b)
     for (int g = 0; g < UIR_FP_SHAPES; ++g) {
         const kgroup *k = &b->fpg[g];
         if (k->n_items)
-            TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, (const uirk_fp_item *)k->d_items, k->n_items, &k->part, 1,
+            TRY(uir_check_hip(uirk_fp_table(b->d_fp_sl, NULL, (const uirk_fp_item *)k->d_items, k->n_items, &k->part, k->mode,
                                             b->sdt, b->idt, &k->tile, k->grid, b->d_err, b->res->stream),
                               "FP launch"));
     }
@@ -546,7 +569,7 @@ static uir_status launch_bp_groups(uir_batch *b)
     for (int g = 0; g < UIR_BP_SHAPES; ++g) {
         const kgroup *k = &b->bpg[g];
         if (k->n_items)
-            TRY(uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, (const uirk_bp_item *)k->d_items, k->n_items, &k->part, 1,
+            TRY(uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, (const uirk_bp_item *)k->d_items, k->n_items, &k->part, k->mode,
                                             b->sdt, b->idt, &k->tile, k->grid, b->d_err, b->res->stream),
                               "BP launch"));
     }

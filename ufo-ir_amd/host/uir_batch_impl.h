@@ -33,6 +33,7 @@ typedef struct {
     uirk_tile tile;
     void *d_items;
     int n_items, grid;
+    int mode; /* launch mode: 1, 3 (the table holds slice pairs) or 2 (line) */
     uirk_xcd_part part;
 } kgroup;
 

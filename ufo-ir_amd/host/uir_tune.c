@@ -123,9 +123,9 @@ uirk_tile uir_tile_for(const uir_slice_spec *spec, uir_dtype storage)
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 /* Average ms of `reps` launches of one variant (after one untimed launch). */
-static uir_status time_variant(hipStream_t st, int fp, const uirk_slice *d_sl, const void *d_items, int n_items,
-                               const uirk_xcd_part *part, int sdt, int idt, const uirk_tile *tile, int grid,
-                               int *d_err, uint32_t reps, float *ms)
+static uir_status time_variant(hipStream_t st, int fp, int mode, const uirk_slice *d_sl, const void *d_items,
+                               int n_items, const uirk_xcd_part *part, int sdt, int idt, const uirk_tile *tile,
+                               int grid, int *d_err, uint32_t reps, float *ms)
 {
     hipEvent_t e0, e1;
     uir_status s = uir_check_hip(hipEventCreate(&e0), "hipEventCreate");
@@ -140,10 +140,10 @@ static uir_status time_variant(hipStream_t st, int fp, const uirk_slice *d_sl, c
             s = uir_check_hip(hipEventRecord(e0, st), "hipEventRecord");
         if (s != UIR_OK)
             break;
-        const int e = fp ? uirk_fp_table(d_sl, NULL, (const uirk_fp_item *)d_items, n_items, part, 1, sdt, idt, tile,
-                                         grid, d_err, st)
-                         : uirk_bp_table(d_sl, NULL, (const uirk_bp_item *)d_items, n_items, part, 1, sdt, idt, tile,
-                                         grid, d_err, st);
+        const int e = fp ? uirk_fp_table(d_sl, NULL, (const uirk_fp_item *)d_items, n_items, part, mode, sdt, idt,
+                                         tile, grid, d_err, st)
+                         : uirk_bp_table(d_sl, NULL, (const uirk_bp_item *)d_items, n_items, part, mode, sdt, idt,
+                                         tile, grid, d_err, st);
         s = uir_check_hip(e, fp ? "FP launch" : "BP launch");
     }
     if (s == UIR_OK)
@@ -266,8 +266,8 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, fv.v, fv.n * sizeof(uirk_fp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 1, d_sl, d_items, (int)fv.n, &part, sdt, idt, &tl, uirk_plan_grid_xcd(&part, (int)fv.n), d_err,
-                             reps, &out->fp_ms[v]);
+            s = time_variant(st, 1, 1, d_sl, d_items, (int)fv.n, &part, sdt, idt, &tl,
+                             uirk_plan_grid_xcd(&part, (int)fv.n), d_err, reps, &out->fp_ms[v]);
         if (s == UIR_OK && out->fp_ms[v] < best_fp) {
             best_fp = out->fp_ms[v];
             out->best.fp_angles = tl.fp_angles;
@@ -277,9 +277,14 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
     for (int v = 0; v < 4 && s == UIR_OK; ++v) {
         const uirk_tile tl = {UIRK_FP_ANGLES, k_bp_variants[v].bp_angles, k_bp_variants[v].bp_rows, 0};
         uirk_bp_vec bv = {0};
-        for (uint32_t i = 0; i < B && s == UIR_OK; ++i)
+        /* slice pairs, as a batch of this geometry runs them */
+        for (uint32_t i = 0; i < B && s == UIR_OK; i += 2) {
+            const size_t b0 = bv.n;
             if (uirk_plan_bp(&bv, (int)i, (int)n, tl.bp_rows))
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+            for (size_t k = b0; k < bv.n && i + 1 < B; ++k)
+                bv.v[k].pair = 1;
+        }
         uirk_xcd_part part;
         uirk_plan_xcd(NULL, (int)bv.n, n_xcd, &part);
         uir_dfree(d_items);
@@ -288,8 +293,8 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, bv.v, bv.n * sizeof(uirk_bp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 0, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl, uirk_plan_grid_xcd(&part, (int)bv.n), d_err,
-                             reps, &out->bp_ms[v]);
+            s = time_variant(st, 0, B > 1 ? 3 : 1, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl,
+                             uirk_plan_grid_xcd(&part, (int)bv.n), d_err, reps, &out->bp_ms[v]);
         if (s == UIR_OK && out->bp_ms[v] < best_bp) {
             best_bp = out->bp_ms[v];
             out->best.bp_angles = tl.bp_angles;
