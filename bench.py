@@ -56,9 +56,12 @@ sys.path.insert(0, os.path.join(ROOT, "ufo-ir_amd"))
 METRIC = "slice-iters/s SIRT 1024²×720 ang, batched vs plain @relL2≤1e-3; p99 latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# LDS read bandwidth: 128 bytes per clock per CU (a read-only LDS loop measures
-# 126, with or without lanes sharing addresses) x 256 CUs x 2.4 GHz.
-LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
+# LDS read rate of the projectors' 8-byte reads, one instruction each:
+# 139.5 bytes per clock per CU measured (uir_probe.hip's read loop) x 256 CUs x
+# 2.4 GHz.  (16-byte reads reach 216 only when the 16 lanes of a pass read 16
+# consecutive entries; at the projectors' spaced addresses they take twice
+# the clocks of an 8-byte read.)
+LDS_PEAK_TBS = 139.5 * 256 * 2.4e9 / 1e12
 
 
 def ellipses(seed, count=20):

@@ -48,14 +48,16 @@ def test_batch_matches_fp64(res, storage, tol):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16-sino", "f16", "bf16-sino"])
-def test_batch_slice_pairs_equal_single_slices(res, storage):
-    """Consecutive slices of one geometry share their work items (slice pairs:
-    one coordinate, address and 16-byte LDS read per sample serve both slices).
-    Each slice's image must equal, bit for bit, that slice reconstructed alone
-    (a batch of one: no pair), including a slice left unpaired between pairs,
-    an off-centre axis (the BP's clamped coordinate), sparse angles (the
-    single-value FP, whose items do not pair, with paired BP items), distinct
-    relaxations and POSC within a pair."""
+def test_batch_slice_pairs_equal_single_slices(res, storage, monkeypatch):
+    """Consecutive slices of one geometry share their BP work items (slice
+    pairs: one coordinate, address and 16-byte LDS read per pixel-angle serve
+    both slices).  Each slice's image must equal, bit for bit, that slice
+    reconstructed alone (a batch of one: no pair), including a slice left
+    unpaired between pairs, an off-centre axis (the BP's clamped coordinate),
+    sparse angles (the single-value FP), distinct relaxations and POSC within
+    a pair.  These batches are too small for the default pairing threshold
+    (4 workgroups per CU after pairing), so the test lowers it to 0."""
+    monkeypatch.setenv("UIR_PAIR_MIN_ITEMS", "0")
     iters = 7
     cases = [  # (n, d, a, axis_position, relax, posc)
         (128, 182, 90, -1.0, 0.5, False),
@@ -85,6 +87,38 @@ def test_batch_slice_pairs_equal_single_slices(res, storage):
         solo.reset()
         solo.iterate(iters)
         assert np.array_equal(bt.image(i), solo.image(0)), (i, c)
+
+
+def test_batch_pairs_slices_by_default_only_at_scale(res, monkeypatch):
+    """With the default threshold, BP slice pairs are used only while the
+    paired table keeps 4 workgroups per CU: ten 1024 x 1024 slices of one
+    geometry pair up (half the BP items), two do not.  The paired results
+    equal the single-slice ones bit for bit."""
+    monkeypatch.delenv("UIR_PAIR_MIN_ITEMS", raising=False)
+    n, d, a, iters = 1024, 1448, 90, 2
+    spec = dict(n=n, n_dets=d, n_angles=a, step=float(np.float32(np.pi / a)), relaxation=0.5)
+    print("case default pairing", n, d, a, flush=True)
+    rng = np.random.default_rng(5)
+    sinos = [rng.uniform(0.0, 50.0, (a, d)).astype(np.float32) for _ in range(10)]
+    n_cu = uir.probe_ceilings(res)["n_cu"]
+    solo = uir.Batch(res, [spec], "f16-sino")
+    per = solo.info()["bp_items"]
+    big = uir.Batch(res, [spec] * 10, "f16-sino")
+    assert 5 * per >= 4 * n_cu
+    assert big.info()["bp_items"] == 5 * per
+    if per < 4 * n_cu:  # one pair would leave fewer than 4 items per CU
+        small = uir.Batch(res, [spec] * 2, "f16-sino")
+        assert small.info()["bp_items"] == 2 * per
+        small.close()
+    for i, b in enumerate(sinos):
+        big.set_sinogram(i, b)
+    big.reset()
+    big.iterate(iters)
+    for i, b in enumerate(sinos):
+        solo.set_sinogram(0, b)
+        solo.reset()
+        solo.iterate(iters)
+        assert np.array_equal(big.image(i), solo.image(0)), i
 
 
 @pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16", 1e-3)])

@@ -52,20 +52,25 @@ def _run(server, reqs, sinos):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_serve_sirt_equals_batch(res, storage):
-    """Served SIRT equals the batch engine's SIRT bit for bit."""
+def test_serve_sirt_equals_batch(res, storage, monkeypatch):
+    """Served SIRT equals the batch engine's SIRT bit for bit.  The last
+    request repeats the first one's geometry: with the pairing threshold at 0
+    the server runs the two as a BP slice pair whenever both are admitted,
+    the batch (where they are not adjacent) as single slices."""
+    monkeypatch.setenv("UIR_PAIR_MIN_ITEMS", "0")
     iters = 6
-    sinos = [_data(n, d, a, 10 + i)[1].astype(np.float32) for i, (n, d, a) in enumerate(GEOMS)]
+    geoms = GEOMS + [GEOMS[0]]
+    sinos = [_data(n, d, a, 10 + i)[1].astype(np.float32) for i, (n, d, a) in enumerate(geoms)]
     with uir.Server(0, storage) as srv:
-        got = _run(srv, [_sirt(n, d, a, iters) for (n, d, a) in GEOMS], sinos)
-    specs = [dict(n=n, n_dets=d, n_angles=a, step=_step(a), relaxation=0.5) for (n, d, a) in GEOMS]
+        got = _run(srv, [_sirt(n, d, a, iters) for (n, d, a) in geoms], sinos)
+    specs = [dict(n=n, n_dets=d, n_angles=a, step=_step(a), relaxation=0.5) for (n, d, a) in geoms]
     bt = uir.Batch(res, specs, storage)
     for i, s in enumerate(sinos):
         bt.set_sinogram(i, s)
     bt.reset()
     bt.iterate(iters, use_graph=False)
-    for i in range(len(GEOMS)):
-        assert np.array_equal(got[i][0], bt.image(i)), GEOMS[i]
+    for i in range(len(geoms)):
+        assert np.array_equal(got[i][0], bt.image(i)), geoms[i]
         assert got[i][1]["steps"] == iters
 
 

@@ -277,6 +277,12 @@ double uirk_fp_item_cost(const uirk_fp_item *it);
 /* Relative cost of a slice-pair BP item against a single-slice one (shared
  * coordinates and addresses). */
 #define UIRK_PAIR_COST 1.6
+/* BP slice pairs halve the workgroups of a launch and hold fewer waves per
+ * SIMD, so they are used only while the paired table keeps at least this
+ * many items: 4 workgroups per CU of the current device, or the value of the
+ * environment variable UIR_PAIR_MIN_ITEMS when set (0: pair whenever two
+ * slices of one geometry are adjacent). */
+int uirk_plan_pair_min_items(void);
 /* FP32 <-> storage conversions */
 int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream);
 

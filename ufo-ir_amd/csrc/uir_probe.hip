@@ -20,7 +20,10 @@ constexpr int kProbeUnroll = 16;
 // Each thread reads 16 pairs per iteration at addresses 128 entries apart
 // from a per-iteration base below kProbeLds / 2, so every index is below
 // kProbeLds / 2 + 15 * 128 < kProbeLds.  Lanes read consecutive entries.
-__global__ __launch_bounds__(256) void k_probe_lds(float *__restrict__ out, int iters, float seed)
+// Every read is one 8-byte instruction, as in the projectors, whose addresses
+// are data-dependent: m (0 at run time) keeps the compiler from merging two
+// reads into one two-address instruction, which moves fewer bytes per clock.
+__global__ __launch_bounds__(256) void k_probe_lds(float *__restrict__ out, int iters, float seed, int m)
 {
     __shared__ float2 s[kProbeLds];
     for (int i = threadIdx.x; i < kProbeLds; i += 256)
@@ -32,7 +35,7 @@ __global__ __launch_bounds__(256) void k_probe_lds(float *__restrict__ out, int 
         const int b0 = (base + it * 64) & (kProbeLds / 2 - 1);
 #pragma unroll
         for (int k = 0; k < kProbeUnroll; ++k) {
-            const float2 v = s[b0 + k * 128];
+            const float2 v = s[((b0 + k * 128) ^ m) & (kProbeLds - 1)];
             if (k & 1) {
                 a0 += v.x;
                 a1 += v.y;
@@ -118,11 +121,11 @@ int uirk_probe_ceilings(void *stream, uirk_ceilings *c)
         rc = 3;
     } else {
         // warm-up launch of each, then reps timed launches
-        hipLaunchKernelGGL(k_probe_lds, dim3(blocks), dim3(256), 0, st, out, lds_iters, 1.0f);
+        hipLaunchKernelGGL(k_probe_lds, dim3(blocks), dim3(256), 0, st, out, lds_iters, 1.0f, 0);
         if (hipEventRecord(ev[0], st) != hipSuccess)
             rc = 3;
         for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_probe_lds, dim3(blocks), dim3(256), 0, st, out, lds_iters, 1.0f);
+            hipLaunchKernelGGL(k_probe_lds, dim3(blocks), dim3(256), 0, st, out, lds_iters, 1.0f, 0);
         if (hipEventRecord(ev[1], st) != hipSuccess)
             rc = 3;
         if (rc == 0 &&

@@ -356,12 +356,25 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
         goto fail;
     }
-    /* consecutive joseph slices of one geometry pair up in the BP (mode 3) */
-    for (uint32_t i = 0; i + 1 < n_slices; ++i)
-        if (b->geom_of[i + 1] == b->geom_of[i] && !b->geoms[b->geom_of[i]].model) {
-            lead[i] = 1;
-            ++i;
+    /* consecutive joseph slices of one geometry pair up in the BP (mode 3),
+     * if the paired joseph table still fills the device */
+    {
+        size_t paired_items = 0;
+        for (uint32_t i = 0; i < n_slices; ++i) {
+            const geom_t *G = &b->geoms[b->geom_of[i]];
+            const size_t tiles = (size_t)((G->n + UIRK_BP_TILE - 1) / UIRK_BP_TILE) *
+                                 ((G->n + G->tile.bp_rows - 1) / G->tile.bp_rows);
+            if (G->model)
+                continue;
+            paired_items += tiles;
+            if (i + 1 < n_slices && b->geom_of[i + 1] == b->geom_of[i]) {
+                lead[i] = 1;
+                ++i;
+            }
         }
+        if (paired_items < (size_t)uirk_plan_pair_min_items())
+            memset(lead, 0, n_slices);
+    }
     for (uint32_t i = 0; i < n_slices; ++i) {
         const geom_t *G = &b->geoms[b->geom_of[i]];
         uirk_slice *f = &fp_sl[i], *k = &bp_sl[i];

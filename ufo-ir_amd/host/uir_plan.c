@@ -240,6 +240,23 @@ int uirk_plan_num_xcd(void)
     return nx;
 }
 
+int uirk_plan_pair_min_items(void)
+{
+    const char *env = getenv("UIR_PAIR_MIN_ITEMS"); /* override: 0 pairs always */
+    if (env && *env)
+        return atoi(env);
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            n = 4 * cus;
+        else
+            n = 1024;
+    }
+    return n;
+}
+
 double uirk_fp_item_cost(const uirk_fp_item *it)
 {
     return (double)it->na * (double)(it->l_hi - it->l_lo) + 16.0; /* + epilogue */

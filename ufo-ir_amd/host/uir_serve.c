@@ -92,6 +92,7 @@ typedef struct {
     void *h_items, *d_items;
     size_t hcap, dcap;
     int n, grid;
+    int mode; /* BP: 1, 3 (slice pairs) or 2 (line) */
     uirk_xcd_part part;
 } sgroup;
 
@@ -148,7 +149,8 @@ struct uir_server {
      * tile shape (FP: 6 shapes, BP: 4), each launched when non-empty */
     uirk_slice *h_fp_sl, *h_bp_sl;
     double *h_cost;
-    size_t hcap_fsl, hcap_bsl, hcap_cost;
+    uint32_t *ord; /* descriptor slot -> active index */
+    size_t hcap_fsl, hcap_bsl, hcap_cost, hcap_ord;
     uirk_slice *d_fp_sl, *d_bp_sl;
     size_t dcap_fsl, dcap_bsl;
     sgroup fpg[SRV_FP_SHAPES], bpg[SRV_BP_SHAPES];
@@ -580,7 +582,10 @@ static uir_status dgrow(void **p, size_t *cap, size_t need, size_t es)
 }
 
 /* Rebuilds descriptors and work tables of the admitted set.  The stream is
- * idle (the caller synchronised it), so the device tables can be rewritten. */
+ * idle (the caller synchronised it), so the device tables can be rewritten.
+ * Descriptors are laid out geometry by geometry (slot j holds request
+ * ord[j]), so that consecutive joseph requests of one geometry share BP items
+ * as slice pairs (mode 3). */
 static uir_status tables_build(uir_server *s)
 {
     const uint32_t na = s->n_active;
@@ -596,8 +601,23 @@ static uir_status tables_build(uir_server *s)
         ncost = nb[c] > ncost ? nb[c] : ncost;
     if (grow((void **)&s->h_fp_sl, &s->hcap_fsl, na, sizeof(uirk_slice)) ||
         grow((void **)&s->h_bp_sl, &s->hcap_bsl, na, sizeof(uirk_slice)) ||
-        grow((void **)&s->h_cost, &s->hcap_cost, ncost, sizeof(double)))
+        grow((void **)&s->h_cost, &s->hcap_cost, ncost, sizeof(double)) ||
+        grow((void **)&s->ord, &s->hcap_ord, na, sizeof(uint32_t)))
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
+    {
+        /* ord: active requests grouped by geometry, in order of first arrival */
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < na; ++i) {
+            int seen = 0;
+            for (uint32_t j = 0; j < i && !seen; ++j)
+                seen = s->active[j]->g == s->active[i]->g;
+            if (seen)
+                continue;
+            for (uint32_t j = i; j < na; ++j)
+                if (s->active[j]->g == s->active[i]->g)
+                    s->ord[k++] = j;
+        }
+    }
     for (int c = 0; c < SRV_FP_SHAPES; ++c)
         if (grow(&s->fpg[c].h_items, &s->fpg[c].hcap, nf[c], sizeof(uirk_fp_item)))
             return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
@@ -608,8 +628,20 @@ static uir_status tables_build(uir_server *s)
         s->fpg[c].n = 0;
     for (int c = 0; c < SRV_BP_SHAPES; ++c)
         s->bpg[c].n = 0;
+    /* BP pairs only while the paired joseph table fills the device */
+    size_t paired = 0;
     for (uint32_t i = 0; i < na; ++i) {
-        const sreq *q = s->active[i];
+        const sgeom *g = s->active[s->ord[i]]->g;
+        if (g->model)
+            continue;
+        paired += g->n_bp;
+        if (i + 1 < na && s->active[s->ord[i + 1]]->g == g)
+            ++i;
+    }
+    const int pair_ok = paired >= (size_t)uirk_plan_pair_min_items();
+    int lead = 0; /* slot i - 1 led a BP pair */
+    for (uint32_t i = 0; i < na; ++i) {
+        const sreq *q = s->active[s->ord[i]];
         const sgeom *g = q->g;
         uirk_slice *f = &s->h_fp_sl[i], *k = &s->h_bp_sl[i];
         memset(f, 0, sizeof *f);
@@ -654,30 +686,34 @@ static uir_status tables_build(uir_server *s)
             fi[fg->n].slice = (int32_t)i;
             ++fg->n;
         }
-        for (size_t j = 0; j < g->n_bp; ++j) {
+        /* a joseph slot followed by one of its geometry leads a BP pair;
+         * the second slot of a pair has no BP items of its own */
+        const int second = lead;
+        lead = pair_ok && !second && !g->model && i + 1 < na && s->active[s->ord[i + 1]]->g == g;
+        for (size_t j = 0; j < g->n_bp && !second; ++j) {
             bi[bg->n] = g->bp[j];
             bi[bg->n].slice = (int32_t)i;
+            bi[bg->n].pair = lead;
             ++bg->n;
         }
     }
     /* per group: cost-balanced XCD ranges (the items stay slice-major) */
     for (int c = 0; c < SRV_FP_SHAPES; ++c) {
         sgroup *grp = &s->fpg[c];
-        size_t o = 0;
-        for (uint32_t i = 0; i < na; ++i)
-            if (s->active[i]->g->fp_shape == c)
-                for (size_t j = 0; j < s->active[i]->g->n_fp; ++j)
-                    s->h_cost[o++] = s->active[i]->g->fp_cost[j];
+        const uirk_fp_item *it = (const uirk_fp_item *)grp->h_items;
+        for (int j = 0; j < grp->n; ++j)
+            s->h_cost[j] = uirk_fp_item_cost(&it[j]);
         uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
         grp->grid = uirk_plan_grid_xcd(&grp->part, (int)grp->n);
     }
     for (int c = 0; c < SRV_BP_SHAPES; ++c) {
         sgroup *grp = &s->bpg[c];
-        size_t o = 0;
-        for (uint32_t i = 0; i < na; ++i)
-            if (s->active[i]->g->bp_shape == c)
-                for (size_t j = 0; j < s->active[i]->g->n_bp; ++j)
-                    s->h_cost[o++] = (double)s->active[i]->g->A;
+        const uirk_bp_item *it = (const uirk_bp_item *)grp->h_items;
+        grp->mode = c == SRV_BP_LINE ? 2 : 1;
+        for (int j = 0; j < grp->n; ++j) {
+            s->h_cost[j] = (double)s->h_bp_sl[it[j].slice].n_angles * (it[j].pair ? UIRK_PAIR_COST : 1.0);
+            grp->mode = it[j].pair ? 3 : grp->mode;
+        }
         uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
         grp->grid = uirk_plan_grid_xcd(&grp->part, (int)grp->n);
     }
@@ -721,7 +757,7 @@ static uir_status step_launch(uir_server *s)
     for (int c = 0; c < SRV_BP_SHAPES; ++c)
         if (s->bpg[c].n)
             TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, (const uirk_bp_item *)s->bpg[c].d_items, s->bpg[c].n,
-                                            &s->bpg[c].part, c == SRV_BP_LINE ? 2 : 1, s->sdt, s->idt, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
+                                            &s->bpg[c].part, s->bpg[c].mode, s->sdt, s->idt, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
                                             s->stream), "BP launch"));
     for (uint32_t i = 0; i < s->n_active; ++i) {
         sreq *q = s->active[i];
@@ -1172,6 +1208,7 @@ void uir_server_free(uir_server *s)
     free(s->h_fp_sl);
     free(s->h_bp_sl);
     free(s->h_cost);
+    free(s->ord);
     for (int c = 0; c < SRV_FP_SHAPES; ++c) {
         free(s->fpg[c].h_items);
         uir_dfree(s->fpg[c].d_items);

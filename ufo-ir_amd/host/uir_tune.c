@@ -277,12 +277,15 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
     for (int v = 0; v < 4 && s == UIR_OK; ++v) {
         const uirk_tile tl = {UIRK_FP_ANGLES, k_bp_variants[v].bp_angles, k_bp_variants[v].bp_rows, 0};
         uirk_bp_vec bv = {0};
-        /* slice pairs, as a batch of this geometry runs them */
-        for (uint32_t i = 0; i < B && s == UIR_OK; i += 2) {
+        /* slice pairs where a batch would use them: the paired table of the
+         * probe fills the device */
+        const size_t tiles = (size_t)((n + UIRK_BP_TILE - 1) / UIRK_BP_TILE) * ((n + tl.bp_rows - 1) / tl.bp_rows);
+        const int pairs = B > 1 && (B + 1) / 2 * tiles >= (size_t)uirk_plan_pair_min_items();
+        for (uint32_t i = 0; i < B && s == UIR_OK; i += pairs ? 2 : 1) {
             const size_t b0 = bv.n;
             if (uirk_plan_bp(&bv, (int)i, (int)n, tl.bp_rows))
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
-            for (size_t k = b0; k < bv.n && i + 1 < B; ++k)
+            for (size_t k = b0; k < bv.n && pairs && i + 1 < B; ++k)
                 bv.v[k].pair = 1;
         }
         uirk_xcd_part part;
@@ -293,7 +296,7 @@ uir_status uir_tune_run(void *stream, const uir_slice_spec *spec, uir_dtype stor
             s = uir_check_hip(hipMemcpyAsync(d_items, bv.v, bv.n * sizeof(uirk_bp_item), hipMemcpyHostToDevice, st),
                               "hipMemcpyAsync");
         if (s == UIR_OK)
-            s = time_variant(st, 0, B > 1 ? 3 : 1, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl,
+            s = time_variant(st, 0, pairs ? 3 : 1, d_sl, d_items, (int)bv.n, &part, sdt, idt, &tl,
                              uirk_plan_grid_xcd(&part, (int)bv.n), d_err, reps, &out->bp_ms[v]);
         if (s == UIR_OK && out->bp_ms[v] < best_bp) {
             best_bp = out->bp_ms[v];
