@@ -55,24 +55,30 @@ def test_batch_slice_pairs_equal_single_slices(res, storage, monkeypatch):
     reconstructed alone (a batch of one: no pair), including a slice left
     unpaired between pairs, an off-centre axis (the BP's clamped coordinate),
     sparse angles (the single-value FP), distinct relaxations and POSC within
-    a pair.  These batches are too small for the default pairing threshold
-    (4 workgroups per CU after pairing), so the test lowers it to 0."""
+    a pair, and pairs of the line model (shared chord weights).  These
+    batches are too small for the default pairing threshold (4 workgroups
+    per CU after pairing), so the test lowers it to 0."""
     monkeypatch.setenv("UIR_PAIR_MIN_ITEMS", "0")
     iters = 7
-    cases = [  # (n, d, a, axis_position, relax, posc)
-        (128, 182, 90, -1.0, 0.5, False),
-        (128, 182, 90, -1.0, 0.3, True),
-        (128, 182, 90, -1.0, 0.5, False),
-        (97, 137, 60, 70.3, 0.5, False),
-        (97, 137, 60, 70.3, 0.7, False),
-        (256, 363, 24, -1.0, 0.5, False),
-        (256, 363, 24, -1.0, 0.5, True),
+    cases = [  # (n, d, a, axis_position, relax, posc, model)
+        (128, 182, 90, -1.0, 0.5, False, "joseph"),
+        (128, 182, 90, -1.0, 0.3, True, "joseph"),
+        (128, 182, 90, -1.0, 0.5, False, "joseph"),
+        (97, 137, 60, 70.3, 0.5, False, "joseph"),
+        (97, 137, 60, 70.3, 0.7, False, "joseph"),
+        (256, 363, 24, -1.0, 0.5, False, "joseph"),
+        (256, 363, 24, -1.0, 0.5, True, "joseph"),
+        (128, 182, 90, -1.0, 0.5, False, "line"),
+        (128, 182, 90, -1.0, 0.4, True, "line"),
+        (97, 137, 60, 40.0, 0.5, False, "line"),
+        (97, 137, 60, 40.0, 0.5, False, "line"),
     ]
     print("case slice pairs", storage, cases, flush=True)
     specs = [dict(n=n, n_dets=d, n_angles=a, step=float(np.float32(np.pi / a)), axis_position=ax, relaxation=rl,
-                  positive_constraint=pc) for (n, d, a, ax, rl, pc) in cases]
+                  positive_constraint=pc, model=m, ray_weight_min=0.1 if m == "line" else 0.0)
+             for (n, d, a, ax, rl, pc, m) in cases]
     sinos = []
-    for i, (n, d, a, ax, rl, pc) in enumerate(cases):
+    for i, (n, d, a, ax, rl, pc, m) in enumerate(cases):
         g = O.Geometry(n, d, a, np.float32(np.pi / a), ax)
         b = O.fp64(g, ellipse_phantom(n, seed=900 + i).astype(np.float64))
         sinos.append((b + np.random.default_rng(900 + i).normal(0.0, 0.01 * b.max(), b.shape)).astype(np.float32))

@@ -224,8 +224,8 @@ typedef struct {
 /* mode: 0 = reference-compatible FP32 arithmetic, 1 = fast path (FP32 math on
  * FP32 / FP16 / BF16 storage), 3 = (BP only) the fast path over slice pairs
  * (items with pair set cover slices slice and slice + 1 of one geometry, with
- * the same results as mode 1; d_slices required), 2 = the fast path with the
- * line model (weights
+ * the same results as mode 1; d_slices required), 4 = (BP only) mode 2 over
+ * slice pairs, 2 = the fast path with the line model (weights
  * from each slice's `line` table; the tile's bp_rows (64 or 32) only, the rest
  * default; BP: pix_w, when given, multiplies the update).  dtype: uir_dtype (FP32 / FP16 / BF16) of the
  * sinogram buffers (b, r); img_dtype: of the image buffers (x, x_out), either

@@ -885,16 +885,19 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
 // PAIR (fast path only): an item with item.pair set also backprojects slice
 // item.slice + 1 of the same geometry; both slices' (e, d) pairs share a
 // 16-byte entry, so a pixel-angle's coordinate, address and one LDS read
-// serve both (the same operations per slice as alone).
+// serve both (the same operations per slice as alone).  With LINE the staged
+// bin holds both slices' values (t, t'), and the chord weights are shared.
 template <typename T, typename TI, bool REF, int NA, int ROWS, bool LINE, bool PAIR>
 __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 4) : 6))) void k_bp_tab(
     const uirk_slice *__restrict__ slices, uirk_slice one, const uirk_bp_item *__restrict__ items, int n_items,
     uirk_xcd_part part, int *__restrict__ err)
 {
-    static_assert(!PAIR || (!REF && !LINE), "slice pairs: the joseph fast path only");
+    static_assert(!PAIR || !REF, "slice pairs: the fast path only");
     using E = typename fp_entry<PAIR>::type;
-    // reference-compatible path and line model: plain staged bins; fast path: (e, d) pairs
-    __shared__ float lds[(REF || LINE) ? NA * kBpWb : 1];
+    // reference-compatible path and line model: plain staged bins (line pairs:
+    // both slices' values); fast path: (e, d) pairs
+    __shared__ float lds[(REF || (LINE && !PAIR)) ? NA * kBpWb : 1];
+    __shared__ float2 ldl[(LINE && PAIR) ? NA * kBpWb : 1];
     __shared__ E ldp[(REF || LINE) ? 1 : NA * kBpWb];
     __shared__ int s_umin[NA];
     __shared__ float s_trig[2 * NA];
@@ -1016,6 +1019,10 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                             const int bc = min(max(bin, 0), D - 1);
                             const float t = gload(sino, rowo + (uint32_t)bc);
                             v0[e] = bin == bc ? t : 0.0f;
+                            if constexpr (PAIR) {
+                                const float t2 = gload(sino2, rowo + (uint32_t)bc);
+                                y0[e] = bin == bc ? t2 : 0.0f;
+                            }
                         } else {
                             // (t[i], t[i+1]) with clamp-to-edge bins; with clamp_u the
                             // coordinate is clamped instead, and the bins only to the row
@@ -1035,7 +1042,9 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                         const int k = wave + 4 * (e >> 1);
                         const int i = lane + 64 * (e & 1);
                         if (k < na && i < kBpWb) {
-                            if constexpr (REF || LINE) {
+                            if constexpr (LINE && PAIR) {
+                                ldl[k * kBpWb + i] = make_float2(v0[e], y0[e]);
+                            } else if constexpr (REF || LINE) {
                                 lds[k * kBpWb + i] = v0[e];
                             } else {
                                 // (e, d) with d = t[i+1] - t[i], e = t[i] - i d: the value at
@@ -1083,27 +1092,55 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                         const float du = -4.0f * st;
                         if constexpr (LINE) {
                             const float4 L = s_line[k]; // (bp_k, bp_b0, bp_b1, 1/M)
-                            const float *B = lds + k * kBpWb;
                             // this thread's column and first row, relative to the tile corner
                             u = __builtin_fmaf((float)lane, ct, __builtin_fmaf(-(float)wave, st, s_base[k]));
+                            if constexpr (PAIR) {
+                                const float2 *B2 = ldl + k * kBpWb;
 #pragma unroll
-                            for (int g = 0; g < ROWS; g += 8) {
-                                float2 p[8];
-                                float fh[8];
+                                for (int g = 0; g < ROWS; g += 8) {
+                                    float2 p[8], q[8];
+                                    float fh[8];
 #pragma unroll
-                                for (int h = 0; h < 8; ++h) {
-                                    const float w = __builtin_fmaf((float)h, du, u); // >= 1: inside the window
-                                    const int i0 = cvt_flr(w);
-                                    p[h] = make_float2(B[i0], B[i0 + 1]);
-                                    fh[h] = fract_pos(w) - 0.5f;
+                                    for (int h = 0; h < 8; ++h) {
+                                        const float w = __builtin_fmaf((float)h, du, u); // >= 1: inside the window
+                                        const int i0 = cvt_flr(w);
+                                        p[h] = B2[i0];
+                                        q[h] = B2[i0 + 1];
+                                        fh[h] = fract_pos(w) - 0.5f;
+                                    }
+#pragma unroll
+                                    for (int h = 0; h < 8; ++h) {
+                                        const float w0 =
+                                            __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], -L.x, L.y), 0.0f, L.w);
+                                        const float w1 =
+                                            __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], L.x, L.z), 0.0f, L.w);
+                                        acc[g + h] = __builtin_fmaf(w1, q[h].x, __builtin_fmaf(w0, p[h].x, acc[g + h]));
+                                        acc2[g + h] =
+                                            __builtin_fmaf(w1, q[h].y, __builtin_fmaf(w0, p[h].y, acc2[g + h]));
+                                    }
+                                    u = __builtin_fmaf(8.0f, du, u);
                                 }
+                            } else {
+                                const float *B = lds + k * kBpWb;
 #pragma unroll
-                                for (int h = 0; h < 8; ++h) {
-                                    const float w0 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], -L.x, L.y), 0.0f, L.w);
-                                    const float w1 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], L.x, L.z), 0.0f, L.w);
-                                    acc[g + h] = __builtin_fmaf(w1, p[h].y, __builtin_fmaf(w0, p[h].x, acc[g + h]));
+                                for (int g = 0; g < ROWS; g += 8) {
+                                    float2 p[8];
+                                    float fh[8];
+#pragma unroll
+                                    for (int h = 0; h < 8; ++h) {
+                                        const float w = __builtin_fmaf((float)h, du, u); // >= 1: inside the window
+                                        const int i0 = cvt_flr(w);
+                                        p[h] = make_float2(B[i0], B[i0 + 1]);
+                                        fh[h] = fract_pos(w) - 0.5f;
+                                    }
+#pragma unroll
+                                    for (int h = 0; h < 8; ++h) {
+                                        const float w0 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], -L.x, L.y), 0.0f, L.w);
+                                        const float w1 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], L.x, L.z), 0.0f, L.w);
+                                        acc[g + h] = __builtin_fmaf(w1, p[h].y, __builtin_fmaf(w0, p[h].x, acc[g + h]));
+                                    }
+                                    u = __builtin_fmaf(8.0f, du, u);
                                 }
-                                u = __builtin_fmaf(8.0f, du, u);
                             }
                         } else if (!clamp_u) {
 #pragma unroll
@@ -1633,7 +1670,10 @@ template <typename T, typename TI>
 int launch_bp_fast(const uirk_tile &tile, bool line, bool pair, const uirk_slice *sl, const uirk_slice &one,
                    const uirk_bp_item *items, int n, const uirk_xcd_part &part, int grid, int *err, hipStream_t st)
 {
-    if (line) // 32 staged angles; tiles of 64 or 32 rows
+    if (line && pair) // 32 staged angles; tiles of 64 or 32 rows
+        return tile.bp_rows == 32 ? launch_bp<T, TI, false, 32, 8, true, true>(sl, one, items, n, part, grid, err, st)
+                                  : launch_bp<T, TI, false, 32, 16, true, true>(sl, one, items, n, part, grid, err, st);
+    if (line)
         return tile.bp_rows == 32 ? launch_bp<T, TI, false, 32, 8, true>(sl, one, items, n, part, grid, err, st)
                                   : launch_bp<T, TI, false, 32, 16, true>(sl, one, items, n, part, grid, err, st);
     const bool na16 = tile.bp_angles == 16, r8 = tile.bp_rows == 32;
@@ -1816,10 +1856,10 @@ int uirk_bp_table(const uirk_slice *d_slices, const uirk_slice *one, const uirk_
             return (int)hipErrorInvalidValue;
         return launch_bp<float, float, true, 32, 16>(d_slices, single, d_items, n_items, pt, grid, d_err, st);
     }
-    if ((mode != 1 && mode != 2 && mode != 3) || (mode == 3 && !d_slices))
+    if (mode < 1 || mode > 4 || (mode >= 3 && !d_slices))
         return (int)hipErrorInvalidValue;
-    return launch_fast<false>(dtype, img_dtype, tl, mode == 2, mode == 3, d_slices, single, d_items, n_items, pt, grid,
-                              d_err, st);
+    return launch_fast<false>(dtype, img_dtype, tl, mode == 2 || mode == 4, mode >= 3, d_slices, single, d_items,
+                              n_items, pt, grid, d_err, st);
 }
 
 int uirk_sb_tv(const void *u, int dtype, const float *ex, const float *ey, float *tv, uint32_t n, void *stream)

@@ -356,15 +356,15 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
         goto fail;
     }
-    /* consecutive joseph slices of one geometry pair up in the BP (mode 3),
-     * if the paired joseph table still fills the device */
-    {
+    /* consecutive slices of one geometry pair up in the BP (modes 3 and 4),
+     * per model if its paired table still fills the device */
+    for (int32_t model = 0; model < 2; ++model) {
         size_t paired_items = 0;
         for (uint32_t i = 0; i < n_slices; ++i) {
             const geom_t *G = &b->geoms[b->geom_of[i]];
             const size_t tiles = (size_t)((G->n + UIRK_BP_TILE - 1) / UIRK_BP_TILE) *
                                  ((G->n + G->tile.bp_rows - 1) / G->tile.bp_rows);
-            if (G->model)
+            if (G->model != model)
                 continue;
             paired_items += tiles;
             if (i + 1 < n_slices && b->geom_of[i + 1] == b->geom_of[i]) {
@@ -373,7 +373,8 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             }
         }
         if (paired_items < (size_t)uirk_plan_pair_min_items())
-            memset(lead, 0, n_slices);
+            for (uint32_t i = 0; i < n_slices; ++i)
+                lead[i] = b->geoms[b->geom_of[i]].model == model ? 0 : lead[i];
     }
     for (uint32_t i = 0; i < n_slices; ++i) {
         const geom_t *G = &b->geoms[b->geom_of[i]];
@@ -405,12 +406,16 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             k->line = G->d_lk;
             k->pix_w = G->d_pix_w;
             k->pix_scale = 1.0f;
+            const int second = i > 0 && lead[i - 1];
+            const size_t b0 = bvl.n;
             if (uirk_plan_fp_line(&fvl, (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos, G->runs,
                                   G->n_runs, UIRK_FP_ANGLES, UIRK_FP_WMAX) ||
-                uirk_plan_bp(&bvl, (int)i, (int)G->n, UIRK_BP_TILE)) {
+                (!second && uirk_plan_bp(&bvl, (int)i, (int)G->n, UIRK_BP_TILE))) {
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
                 goto fail;
             }
+            for (size_t j = b0; j < bvl.n && lead[i]; ++j)
+                bvl.v[j].pair = 1;
         } else {
             /* BP slice pairs: slice i + 1 of the same geometry joins slice i's
              * BP items (pair set) */
@@ -490,8 +495,11 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         if (s == UIR_OK) {
             kgroup *k = &b->bpl;
             k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
-            for (size_t i = 0; i < bvl.n; ++i)
-                cost[i] = (double)specs[bvl.v[i].slice].n_angles;
+            k->mode = 2;
+            for (size_t i = 0; i < bvl.n; ++i) {
+                cost[i] = (double)specs[bvl.v[i].slice].n_angles * (bvl.v[i].pair ? UIRK_PAIR_COST : 1.0);
+                k->mode = bvl.v[i].pair ? 4 : k->mode;
+            }
             uirk_plan_xcd(cost, (int)bvl.n, uirk_plan_num_xcd(), &k->part);
             k->n_items = (int)bvl.n;
             k->grid = uirk_plan_grid_xcd(&k->part, k->n_items);
@@ -588,7 +596,7 @@ static uir_status launch_bp_groups(uir_batch *b)
     }
     if (b->bpl.n_items)
         TRY(uir_check_hip(uirk_bp_table(b->d_bp_sl, NULL, (const uirk_bp_item *)b->bpl.d_items, b->bpl.n_items,
-                                        &b->bpl.part, 2, b->sdt, b->idt, &b->bpl.tile, b->bpl.grid, b->d_err,
+                                        &b->bpl.part, b->bpl.mode, b->sdt, b->idt, &b->bpl.tile, b->bpl.grid, b->d_err,
                                         b->res->stream),
                           "BP launch"));
     return UIR_OK;

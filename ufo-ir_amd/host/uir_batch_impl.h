@@ -33,7 +33,7 @@ typedef struct {
     uirk_tile tile;
     void *d_items;
     int n_items, grid;
-    int mode; /* launch mode: 1, 3 (the table holds slice pairs) or 2 (line) */
+    int mode; /* launch mode: 1, 3 (the table holds slice pairs), 2 (line) or 4 (line pairs) */
     uirk_xcd_part part;
 } kgroup;
 

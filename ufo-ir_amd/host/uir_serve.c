@@ -92,7 +92,7 @@ typedef struct {
     void *h_items, *d_items;
     size_t hcap, dcap;
     int n, grid;
-    int mode; /* BP: 1, 3 (slice pairs) or 2 (line) */
+    int mode; /* BP: 1, 3 (slice pairs), 2 (line) or 4 (line pairs) */
     uirk_xcd_part part;
 } sgroup;
 
@@ -584,8 +584,8 @@ static uir_status dgrow(void **p, size_t *cap, size_t need, size_t es)
 /* Rebuilds descriptors and work tables of the admitted set.  The stream is
  * idle (the caller synchronised it), so the device tables can be rewritten.
  * Descriptors are laid out geometry by geometry (slot j holds request
- * ord[j]), so that consecutive joseph requests of one geometry share BP items
- * as slice pairs (mode 3). */
+ * ord[j]), so that consecutive requests of one geometry share BP items as
+ * slice pairs (modes 3 and 4). */
 static uir_status tables_build(uir_server *s)
 {
     const uint32_t na = s->n_active;
@@ -628,17 +628,16 @@ static uir_status tables_build(uir_server *s)
         s->fpg[c].n = 0;
     for (int c = 0; c < SRV_BP_SHAPES; ++c)
         s->bpg[c].n = 0;
-    /* BP pairs only while the paired joseph table fills the device */
-    size_t paired = 0;
+    /* BP pairs only while the paired table of the model fills the device */
+    size_t paired[2] = {0, 0};
     for (uint32_t i = 0; i < na; ++i) {
         const sgeom *g = s->active[s->ord[i]]->g;
-        if (g->model)
-            continue;
-        paired += g->n_bp;
+        paired[g->model != 0] += g->n_bp;
         if (i + 1 < na && s->active[s->ord[i + 1]]->g == g)
             ++i;
     }
-    const int pair_ok = paired >= (size_t)uirk_plan_pair_min_items();
+    const int pair_ok[2] = {paired[0] >= (size_t)uirk_plan_pair_min_items(),
+                            paired[1] >= (size_t)uirk_plan_pair_min_items()};
     int lead = 0; /* slot i - 1 led a BP pair */
     for (uint32_t i = 0; i < na; ++i) {
         const sreq *q = s->active[s->ord[i]];
@@ -689,7 +688,7 @@ static uir_status tables_build(uir_server *s)
         /* a joseph slot followed by one of its geometry leads a BP pair;
          * the second slot of a pair has no BP items of its own */
         const int second = lead;
-        lead = pair_ok && !second && !g->model && i + 1 < na && s->active[s->ord[i + 1]]->g == g;
+        lead = pair_ok[g->model != 0] && !second && i + 1 < na && s->active[s->ord[i + 1]]->g == g;
         for (size_t j = 0; j < g->n_bp && !second; ++j) {
             bi[bg->n] = g->bp[j];
             bi[bg->n].slice = (int32_t)i;
@@ -712,7 +711,7 @@ static uir_status tables_build(uir_server *s)
         grp->mode = c == SRV_BP_LINE ? 2 : 1;
         for (int j = 0; j < grp->n; ++j) {
             s->h_cost[j] = (double)s->h_bp_sl[it[j].slice].n_angles * (it[j].pair ? UIRK_PAIR_COST : 1.0);
-            grp->mode = it[j].pair ? 3 : grp->mode;
+            grp->mode = it[j].pair ? (c == SRV_BP_LINE ? 4 : 3) : grp->mode;
         }
         uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
         grp->grid = uirk_plan_grid_xcd(&grp->part, (int)grp->n);
