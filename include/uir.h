@@ -140,8 +140,8 @@ typedef struct uir_task uir_task;
  * every ray active and the BP the transpose of the FP (no reference
  * counterpart); any other name fails setup with UIR_ERR_SETUP.  The method
  * tasks, subsets and the kernel timing take the projector's model, the
- * batched and served SIRT take it per slice (uir_slice_spec.model); batched
- * ASD-POCS and the server's SBTV_SIRT are joseph only.
+ * batched SIRT and the server take it per slice (uir_slice_spec.model);
+ * batched ASD-POCS is joseph only.
  * Extension property image_size (uint, 0 = n_dets): the N x N image edge. */
 uir_task *uir_ir_parallel_projector_task_new(void);
 /* ufo_ir_sirt_task_new, ufo-ir-sirt-task.c:142-146.  Properties: projector
@@ -256,9 +256,8 @@ typedef struct {
     float    ray_weight_min;
     /* Projection model (the projector's "model"): 0 = joseph, 1 = line (exact
      * ray-pixel intersection lengths; its pixel weights 1 / BP(1) are an image
-     * per geometry).  The batched and served SIRT take both, side by side;
-     * batched ASD-POCS and the server's SBTV_SIRT take joseph only
-     * (UIR_ERR_UNSUPPORTED). */
+     * per geometry).  The batched SIRT and the server take both, side by
+     * side; batched ASD-POCS takes joseph only (UIR_ERR_UNSUPPORTED). */
     int32_t  model;
 } uir_slice_spec;
 

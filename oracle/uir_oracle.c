@@ -1387,15 +1387,25 @@ void orc64_sbsirt(const orc_geom *g, const double *f, double *u, uint32_t outer,
     double *bp = (double *)calloc(nv, sizeof(double));
     double *tv = (double *)calloc(nv, sizeof(double));
     double *st = (double *)calloc(4 * nv, sizeof(double)); /* bx, by, ex, ey */
-    if (!R || !ones_v || !r || !bp || !tv || !st) {
-        free(R); free(ones_v); free(r); free(bp); free(tv); free(st);
+    double *P = (double *)calloc(nv, sizeof(double));
+    if (!R || !ones_v || !r || !bp || !tv || !st || !P) {
+        free(R); free(ones_v); free(r); free(bp); free(tv); free(st); free(P);
         return;
     }
     double *bx = st, *by = st + nv, *ex = st + 2 * nv, *ey = st + 3 * nv;
     for (size_t i = 0; i < nv; ++i) ones_v[i] = 1.0;
     orc64_fp(g, ones_v, R);
     for (size_t i = 0; i < ns; ++i) R[i] = (R[i] != 0 && R[i] >= r_min) ? 1.0 / R[i] : 0.0;
-    const double P = 1.0 / (mu * (double)g->n_angles + 8.0 * lambda);
+    if (g->model == 1) { /* per pixel: BP(1) of the model */
+        for (size_t i = 0; i < ns; ++i) r[i] = 1.0;
+        orc64_bp(g, bp, r);
+        for (size_t i = 0; i < nv; ++i) {
+            const double b1 = bp[i] >= ORC_LINE_PIX_MIN ? bp[i] : 0.0;
+            P[i] = (b1 > 0 || lambda > 0) ? 1.0 / (mu * b1 + 8.0 * lambda) : 0.0;
+        }
+    } else {
+        for (size_t i = 0; i < nv; ++i) P[i] = 1.0 / (mu * (double)g->n_angles + 8.0 * lambda);
+    }
     for (size_t i = 0; i < nv; ++i) u[i] = 0.0;
     for (uint32_t k = 0; k < outer; ++k) {
         for (uint32_t j = 0; j < inner; ++j) {
@@ -1404,12 +1414,12 @@ void orc64_sbsirt(const orc_geom *g, const double *f, double *u, uint32_t outer,
             orc64_bp(g, bp, r);
             orc64_sb_tv(u, ex, ey, tv, n);
             for (size_t i = 0; i < nv; ++i) {
-                u[i] += relax * P * (mu * bp[i] + lambda * tv[i]);
+                u[i] += relax * P[i] * (mu * bp[i] + lambda * tv[i]);
                 if (posc && u[i] < 0) u[i] = 0;
             }
         }
         if (k + 1 < outer)
             orc64_sb_shrink(u, bx, by, ex, ey, n, lambda);
     }
-    free(R); free(ones_v); free(r); free(bp); free(tv); free(st);
+    free(R); free(ones_v); free(r); free(bp); free(tv); free(st); free(P);
 }

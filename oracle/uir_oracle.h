@@ -157,7 +157,11 @@ void orc64_asdpocs(const orc_geom *g, const double *b, double *out, uint32_t ite
 
 /* split-Bregman TV with SIRT-preconditioned sweeps for the u-subproblem (FP64):
  * outer Bregman iterations x inner sweeps, relaxation w, TV weight lambda,
- * data weight mu; orc64_sb_tv / orc64_sb_shrink are its two TV steps */
+ * data weight mu; orc64_sb_tv / orc64_sb_shrink are its two TV steps.  The
+ * preconditioner is P = 1 / (mu A + 8 lambda) for joseph (A angles) and, per
+ * pixel, P = 1 / (mu BP(1) + 8 lambda) for the line model, with BP(1) taken
+ * as 0 below ORC_LINE_PIX_MIN (then P = 1 / (8 lambda), or 0 for lambda = 0,
+ * as the line SIRT's pixel weight). */
 void orc64_sb_tv(const double *u, const double *ex, const double *ey, double *tv, int n);
 void orc64_sb_shrink(const double *u, double *bx, double *by, double *ex, double *ey, int n, double lambda);
 void orc64_sbsirt(const orc_geom *g, const double *f, double *u, uint32_t outer, uint32_t inner, double relax,

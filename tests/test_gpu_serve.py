@@ -79,8 +79,7 @@ def test_serve_line_sirt_equals_batch_and_fp64(res, storage, tol):
     """SIRT requests of the line model served beside joseph requests (one of
     them of the same geometry, so the geometry cache must tell the models
     apart): each equals the batch engine's slice bit for bit and the oracle's
-    FP64 SIRT of its own model.  A split-Bregman request of the line model is
-    refused at submission."""
+    FP64 SIRT of its own model."""
     iters = 8
     cases = [(128, 182, 90, "line", 0.1), (128, 182, 90, "joseph", 0.1), (97, 137, 60, "line", 0.1)]
     print("case serve line + joseph", storage, cases, flush=True)
@@ -95,10 +94,6 @@ def test_serve_line_sirt_equals_batch_and_fp64(res, storage, tol):
                              model=m) for (n, d, a, m, rmin) in cases]
     with uir.Server(0, storage) as srv:
         got = _run(srv, reqs, sinos)
-        with pytest.raises(uir.UirError) as e:
-            srv.submit(uir.make_request(97, 137, 60, _step(60), "sbtv-sirt", iterations=2, model="line"),
-                       sinos[2])
-        assert e.value.status == 4
     specs = [dict(n=n, n_dets=d, n_angles=a, step=_step(a), relaxation=0.5, ray_weight_min=rmin, model=m)
              for (n, d, a, m, rmin) in cases]
     bt = uir.Batch(res, specs, storage)
@@ -166,6 +161,35 @@ def test_serve_sbtv_sirt_matches_fp64(storage, noise, rmin, tol):
     for i in range(len(reqs)):
         err = O.rel_l2(out[i][0], refs[i])
         print(storage, "noise", noise, GEOMS[i], "SBTV_SIRT rel-L2 vs FP64", err)
+        assert err <= tol
+        assert out[i][1]["steps"] == outer * inner
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 2e-5), ("f16-sino", 1e-3)])
+def test_serve_sbtv_sirt_line_model_matches_fp64(storage, tol):
+    """Split-Bregman requests of the line model (its preconditioner is an
+    image, P = 1 / (mu BP(1) + 8 lambda) per pixel, formed in the BP's
+    epilogue from the pixel weights) beside a joseph one of the same shape:
+    each matches the oracle's FP64 method of its own model."""
+    outer, inner, mu, relax = 5, 4, 1.0, 1.0
+    cases = [(128, 182, 90, "line", 0.05), (97, 137, 60, "line", 0.2), (128, 182, 90, "joseph", 0.05)]
+    print("case serve sbtv line", storage, cases, flush=True)
+    reqs, sinos, refs = [], [], []
+    for i, (n, d, a, m, lam) in enumerate(cases):
+        g = O.Geometry(n, d, a, np.float32(np.pi / a), model=m)
+        b = O.fp64(g, ellipse_phantom(n, seed=770 + i).astype(np.float64))
+        b = (b + np.random.default_rng(770 + i).normal(0.0, 0.01 * b.max(), b.shape)).astype(np.float32)
+        req = uir.make_request(n, d, a, _step(a), "sbtv-sirt", iterations=outer, inner_iterations=inner,
+                               relaxation=relax, lam=lam, mu=mu, ray_weight_min=0.1, model=m)
+        reqs.append(req)
+        sinos.append(b)
+        refs.append(O.sbsirt64(g, b.astype(np.float64), outer, inner, relax=relax, lam=lam, mu=mu,
+                               ray_weight_min=0.1))
+    with uir.Server(0, storage) as srv:
+        out = _run(srv, reqs, sinos)
+    for i, c in enumerate(cases):
+        err = O.rel_l2(out[i][0], refs[i])
+        print(storage, c, "SBTV_SIRT rel-L2 vs FP64 %.3e" % err, flush=True)
         assert err <= tol
         assert out[i][1]["steps"] == outer * inner
 

@@ -501,10 +501,12 @@ def test_cgs_matches_scipy_with_a_true_dot():
 
 # ------------------- split-Bregman TV with SIRT sweeps (this build's method) --
 
-def test_sbsirt_without_tv_is_sirt():
+@pytest.mark.parametrize("model", ["joseph", "line"])
+def test_sbsirt_without_tv_is_sirt(model):
     """lambda = 0 removes the TV term: the sweeps are SIRT with relaxation w and
-    pixel weight 1/A (catches a wrong preconditioner or a missing R)."""
-    g = geom(24, 34, 30)
+    pixel weight 1/A (joseph) or 1/BP(1) with the line SIRT's floor (catches
+    a wrong preconditioner or a missing R)."""
+    g = O.Geometry(24, 34, 30, np.float32(PI / 30), -1.0, model=model)
     f = O.fp64(g, gaussian_blobs(24, [(-3.0, 2.0, 4.0, 1.0), (5.0, -4.0, 2.5, 0.6)]))
     got = O.sbsirt64(g, f, 4, 3, relax=0.8, lam=0.0, mu=2.0)
     want = O.sirt64(g, f, 12, relax=0.8)
@@ -551,13 +553,17 @@ def test_sb_shrink_matches_reference_update_db():
     assert np.allclose(ex, (gx - nbx) - nbx, atol=1e-12)
 
 
-def test_sbsirt_as_matrix_iteration():
+@pytest.mark.parametrize("model", ["joseph", "line"])
+def test_sbsirt_as_matrix_iteration(model):
     """The whole loop from dense FP64 system matrices and periodic difference
     matrices: inner sweeps u += w P (mu B R (f - A u) + lambda D^T (e - D u)),
     then the shrink (catches a shrink after the last outer iteration, a
-    swapped mu / lambda, or a sweep count off by one)."""
+    swapped mu / lambda, or a sweep count off by one).  P = 1 / (mu A + 8
+    lambda) for joseph; for the line model, per pixel, 1 / (mu BP(1) + 8
+    lambda) with BP(1) = the row sums of B, taken as 0 below 2^-10 (catches a
+    scalar P or the joseph angle count in its place)."""
     n, d, a = 8, 12, 10
-    g = geom(n, d, a)
+    g = O.Geometry(n, d, a, np.float32(PI / a), -1.0, model=model)
     A = np.stack([O.fp64(g, np.eye(n * n)[k].reshape(n, n)).ravel() for k in range(n * n)], axis=1)
     B = np.stack([O.bp64(g, np.eye(a * d)[k].reshape(a, d)).ravel() for k in range(a * d)], axis=1)
     I = np.eye(n)
@@ -568,7 +574,12 @@ def test_sbsirt_as_matrix_iteration():
     f = A @ x_true
     R = 1 / np.where(A.sum(1) != 0, A.sum(1), np.inf)
     w, lam, mu, outer, inner = 0.9, 0.3, 1.5, 3, 2
-    P = 1 / (mu * a + 8 * lam)
+    if model == "line":
+        b1 = B.sum(1)
+        P = 1 / (mu * np.where(b1 >= 2.0 ** -10, b1, 0.0) + 8 * lam)
+        assert np.ptp(P) > 0.01 * P.max()  # not a constant in disguise
+    else:
+        P = 1 / (mu * a + 8 * lam)
     u = np.zeros(n * n)
     bx = np.zeros(n * n)
     by = np.zeros(n * n)

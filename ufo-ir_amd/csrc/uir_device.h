@@ -168,7 +168,10 @@ typedef struct {
     int32_t pad_;
     /* line model (mode 2) only: 8 floats per angle, (fp_k, fp_b, 1/M, 0,
      * bp_k, bp_b0, bp_b1, 1/M) (uir_line_fast_consts); trig then holds the
-     * model's unit directions */
+     * model's unit directions.  With tv set, the BP is a split-Bregman sweep
+     * x_out = x + relax P (mu sum + lambda tv), P = pix_w / (mu + 8 lambda
+     * pix_w) per pixel (1 / (8 lambda) where pix_w = 0), with mu in pix_scale
+     * and lambda in tv_scale. */
     const float *line;
 } uirk_slice;
 

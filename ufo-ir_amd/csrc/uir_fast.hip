@@ -1215,6 +1215,13 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                         out = out * gload(pix_w, o);
                         out = out + gload(xin, o);
                     }
+                } else if (LINE && tv) {
+                    // split-Bregman sweep of the line model: P = 1 / (mu BP(1) + 8 lambda)
+                    // per pixel, from C = 1 / BP(1) (0: BP(1) below the floor)
+                    const float C = gload(pix_w, o), mu = Sx->pix_scale, lam = Sx->tv_scale;
+                    const float P = C > 0.0f ? C / __builtin_fmaf(8.0f * lam, C, mu) : 1.0f / (8.0f * lam);
+                    out = __builtin_fmaf(Sx->relax * P, __builtin_fmaf(mu, a[q], lam * gload(tv, o)),
+                                         gload(xin, o));
                 } else if (LINE && pix_w) {
                     out = __builtin_fmaf(a[q] * gain, gload(pix_w, o), gload(xin, o));
                 } else {

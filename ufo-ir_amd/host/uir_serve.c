@@ -33,8 +33,9 @@
  *   P = 1 / (mu A_n + 8 lambda),
  * A_n = the angle count (= A^T R A 1 for the fast projector pair) and 8 the
  * row-sum bound of D^T D, so P^-1 majorises the subproblem's Hessian and any
- * relaxation w in (0, 1] converges.  u, d and b start at 0.  That bound
- * holds for Joseph's pair only, so SBTV_SIRT takes the joseph model.
+ * relaxation w in (0, 1] converges.  u, d and b start at 0.  For the line
+ * model A_n is the pixel's BP(1) (= A^T R A 1 up to the rays R drops), so P
+ * is an image: the BP kernel forms it per pixel from the weights 1 / BP(1).
  */
 #define _GNU_SOURCE
 #include <errno.h>
@@ -235,8 +236,6 @@ static uir_status check_request(const uir_request *r)
     }
     if (r->method != UIR_METHOD_SBTV_SIRT)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "unknown method %u", r->method);
-    if (s->model != 0)
-        return uir_set_error(UIR_ERR_UNSUPPORTED, "SBTV_SIRT takes the joseph model only");
     if (!(s->relaxation > 0.0f && s->relaxation <= 1.0f))
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "relaxation outside (0, 1]");
     if (r->inner_iterations == 0)
@@ -664,16 +663,21 @@ static uir_status tables_build(uir_server *s)
         k->ray_w = NULL;
         k->posc = q->rq.slice.positive_constraint ? 1 : 0;
         k->relax = q->rq.slice.relaxation;
-        if (is_sb(&q->rq)) {
-            const float P = 1.0f / (q->rq.mu * (float)g->A + 8.0f * q->rq.lambda);
-            k->pix_scale = q->rq.mu * P;
-            k->tv = q->tv;
-            k->tv_scale = q->rq.slice.relaxation * q->rq.lambda * P;
-        } else if (g->model) { /* line: its constants, and per-pixel weights in place of 1/A */
+        if (g->model) { /* line: its constants, and per-pixel weights in place of 1/A */
             f->line = g->d_lk;
             k->line = g->d_lk;
             k->pix_w = g->d_pix_w;
             k->pix_scale = 1.0f;
+            if (is_sb(&q->rq)) { /* the kernel forms P per pixel from pix_w */
+                k->pix_scale = q->rq.mu;
+                k->tv = q->tv;
+                k->tv_scale = q->rq.lambda;
+            }
+        } else if (is_sb(&q->rq)) {
+            const float P = 1.0f / (q->rq.mu * (float)g->A + 8.0f * q->rq.lambda);
+            k->pix_scale = q->rq.mu * P;
+            k->tv = q->tv;
+            k->tv_scale = q->rq.slice.relaxation * q->rq.lambda * P;
         } else {
             k->pix_scale = 1.0f / (float)g->A;
         }

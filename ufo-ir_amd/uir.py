@@ -661,8 +661,8 @@ STATUS_TIMEOUT = 9
 def make_request(n, n_dets, n_angles, step, method="sirt", iterations=10, inner_iterations=1, relaxation=0.25,
                  lam=0.1, mu=1.0, axis_position=-1.0, positive_constraint=False, ray_weight_min=0.0,
                  model="joseph"):
-    """A uir_request (method "sirt" or "sbtv-sirt"; "sbtv-sirt" takes the
-    joseph model only)."""
+    """A uir_request (method "sirt" or "sbtv-sirt"; model "joseph" or
+    "line")."""
     return Request(SliceSpec(int(n), int(n_dets), int(n_angles), float(step), float(axis_position),
                              float(relaxation), int(bool(positive_constraint)), float(ray_weight_min),
                              MODELS[model]),
