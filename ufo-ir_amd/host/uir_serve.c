@@ -19,11 +19,12 @@
  *   5. retire requests whose iterations are done: copy the image to the
  *      host, release its memory, wake its waiter.
  * At most two steps are in flight, so arrivals are seen within about two step
- * times.  SIRT requests of the line projector model (uir_slice_spec.model = 1)
+ * times.  Requests of the line projector model (uir_slice_spec.model = 1)
  * carry its per-angle constants and pixel weights C = 1/BP(1) in their
  * geometry and share the steps in work tables of their own (the table
- * kernels' line mode).  Every FP and BP work item writes only its own slice, so a request's
- * result does not depend on what shares its steps.
+ * kernels' line mode).  Every FP and BP work item writes only its own slice
+ * (two adjacent slices for a BP slice pair, with the arithmetic of one), so
+ * a request's result does not depend on what shares its steps.
  *
  * SBTV_SIRT is split-Bregman TV (the outer loop of ufo-ir-sbtv-task.c:
  * b = mu A^T f + lambda D^T (d - b), u-subproblem, shrink, Bregman update)
@@ -147,7 +148,8 @@ struct uir_server {
     sgeom *geoms;
 
     /* tables (worker thread only): slice descriptors, and one work table per
-     * tile shape (FP: 6 shapes, BP: 4), each launched when non-empty */
+     * tile shape (FP: 6 shapes and the line model's, BP: 4 and the line
+     * model's), each launched when non-empty */
     uirk_slice *h_fp_sl, *h_bp_sl;
     double *h_cost;
     uint32_t *ord; /* descriptor slot -> active index */
@@ -689,8 +691,9 @@ static uir_status tables_build(uir_server *s)
             fi[fg->n].slice = (int32_t)i;
             ++fg->n;
         }
-        /* a joseph slot followed by one of its geometry leads a BP pair;
-         * the second slot of a pair has no BP items of its own */
+        /* a slot followed by one of its geometry leads a BP pair (when the
+         * model's paired table fills the device); the second slot of a pair
+         * has no BP items of its own */
         const int second = lead;
         lead = pair_ok[g->model != 0] && !second && i + 1 < na && s->active[s->ord[i + 1]]->g == g;
         for (size_t j = 0; j < g->n_bp && !second; ++j) {
@@ -755,13 +758,15 @@ static uir_status step_launch(uir_server *s)
     for (int c = 0; c < SRV_FP_SHAPES; ++c)
         if (s->fpg[c].n)
             TRY(uir_check_hip(uirk_fp_table(s->d_fp_sl, NULL, (const uirk_fp_item *)s->fpg[c].d_items, s->fpg[c].n,
-                                            &s->fpg[c].part, c == SRV_FP_LINE ? 2 : 1, s->sdt, s->idt, &s->fpg[c].tile, s->fpg[c].grid, s->d_err,
-                                            s->stream), "FP launch"));
+                                            &s->fpg[c].part, c == SRV_FP_LINE ? 2 : 1, s->sdt, s->idt,
+                                            &s->fpg[c].tile, s->fpg[c].grid, s->d_err, s->stream),
+                              "FP launch"));
     for (int c = 0; c < SRV_BP_SHAPES; ++c)
         if (s->bpg[c].n)
             TRY(uir_check_hip(uirk_bp_table(s->d_bp_sl, NULL, (const uirk_bp_item *)s->bpg[c].d_items, s->bpg[c].n,
-                                            &s->bpg[c].part, s->bpg[c].mode, s->sdt, s->idt, &s->bpg[c].tile, s->bpg[c].grid, s->d_err,
-                                            s->stream), "BP launch"));
+                                            &s->bpg[c].part, s->bpg[c].mode, s->sdt, s->idt, &s->bpg[c].tile,
+                                            s->bpg[c].grid, s->d_err, s->stream),
+                              "BP launch"));
     for (uint32_t i = 0; i < s->n_active; ++i) {
         sreq *q = s->active[i];
         q->done_steps++;
