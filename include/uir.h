@@ -141,7 +141,7 @@ typedef struct uir_task uir_task;
  * counterpart); any other name fails setup with UIR_ERR_SETUP.  The method
  * tasks, subsets and the kernel timing take the projector's model, the
  * batched SIRT and the server take it per slice (uir_slice_spec.model);
- * batched ASD-POCS is joseph only.
+ * batched ASD-POCS takes batches of one model.
  * Extension property image_size (uint, 0 = n_dets): the N x N image edge. */
 uir_task *uir_ir_parallel_projector_task_new(void);
 /* ufo_ir_sirt_task_new, ufo-ir-sirt-task.c:142-146.  Properties: projector
@@ -257,7 +257,8 @@ typedef struct {
     /* Projection model (the projector's "model"): 0 = joseph, 1 = line (exact
      * ray-pixel intersection lengths; its pixel weights 1 / BP(1) are an image
      * per geometry).  The batched SIRT and the server take both, side by
-     * side; batched ASD-POCS takes joseph only (UIR_ERR_UNSUPPORTED). */
+     * side; batched ASD-POCS takes either, one model per batch
+     * (UIR_ERR_UNSUPPORTED for a mix). */
     int32_t  model;
 } uir_slice_spec;
 
@@ -314,7 +315,8 @@ void       uir_asdpocs_params_default(uir_asdpocs_params *p);
 /* `iterations` ASD-POCS iterations of every slice from x = 0; the result
  * replaces the slice images (uir_batch_get_image).  Asynchronous on the batch
  * stream.  The first call allocates the method's own buffers (a second image
- * and an FP32 gradient per slice). */
+ * and an FP32 gradient per slice).  The batch's slices must share one
+ * projection model (UIR_ERR_UNSUPPORTED otherwise). */
 uir_status uir_batch_asdpocs(uir_batch *batch, uint32_t iterations, const uir_asdpocs_params *p, int use_graph);
 /* Rows (dd, dp, dg, dtgv) of slice `slice` from the last uir_batch_asdpocs,
  * one per iteration (as uir_asdpocs_get_diagnostics); synchronises. */

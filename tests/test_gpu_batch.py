@@ -313,8 +313,10 @@ def test_batch_line_and_joseph_slices_match_fp64(res, storage, tol):
     assert np.array_equal(solo.image(0), bt.image(0))
 
 
-def test_batch_asdpocs_refuses_line_slices(res):
-    bt = uir.Batch(res, [dict(_spec(64, 91, 30), model="line")], "f32")
+def test_batch_asdpocs_refuses_mixed_models(res):
+    """Batched ASD-POCS runs one data step for the whole batch: slices of the
+    two projection models together are refused."""
+    bt = uir.Batch(res, [dict(_spec(64, 91, 30), model="line"), _spec(64, 91, 30)], "f32")
     with pytest.raises(uir.UirError) as e:
         bt.asdpocs(1)
     assert e.value.status == 4

@@ -357,10 +357,17 @@ typedef struct {
     float *part;
     int32_t n, n_dets, n_angles, pad;
     float axis, corr;
+    const uirk_line_angle *line; /* the line model (uirk_sart_line) only */
 } uirk_sart_slice;
 int uirk_sart_step(const uirk_sart_slice *d_slices, const uirk_asd_state *d_state, int n_slices, int max_n,
                    int dtype, int img_dtype, int bp_angle, int fp_angle, int *d_err, void *stream);
 int uirk_sart_rows(const uirk_sart_slice *d_slices, int n_slices, int max_d, int dtype, int angle, void *stream);
+/* The same SART angle for slices of the line model (uir_line.hip), in two
+ * launches: r[angle] = (b + corr FP) * ray_w, then x_out = x_in + relax
+ * A^T r[angle] pixel by pixel (x_out may equal x_in); descriptors need line
+ * (the geometry's angle table) and trig (its unit directions). */
+int uirk_sart_line(const uirk_sart_slice *d_slices, const uirk_asd_state *d_state, int n_slices, int max_n, int max_d,
+                   int dtype, int img_dtype, int angle, void *stream);
 
 /* Finite differences (ufo-ir-gradient-processor.cl) and l1_grad. */
 int uirk_diff(int kind, const float *in, float *out, uint32_t w, uint32_t h, void *stream);

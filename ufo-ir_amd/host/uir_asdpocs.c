@@ -14,6 +14,9 @@
  *   dd          r = b - A x over all angles (the batch's FP tables), |r|;
  *   TV          ng times: g = l1_grad(x), |g|, x -= dtgv / |g| g;
  *   dg          |x - x_prev|, then the dtgv / beta updates.
+ * A batch of the line model runs each SART angle as the model's FP rows and
+ * BP update over every slice (uirk_sart_line: two launches, the image updated
+ * in place) and its dd with the line model's FP table.
  * Norms are FP64 reductions on the device (L1 for the reference, L2 for the
  * textbook variant) and every scalar the method needs lives in a per-slice
  * state in device memory, so an iteration has no host synchronisation and is
@@ -35,6 +38,7 @@ struct uir_asd {
     uirk_sart_slice *d_sart[2]; /* fused SART step: x -> g, g -> x (g is free during the data step) */
     uirk_slice *d_dd_sl;       /* residual FP r = b - A x for dd */
     uint32_t n_max, d_max, a_max;
+    int line;                  /* every slice has the line model */
     uirk_asd_state *d_state;
     double *d_partials;
     int nblk;
@@ -94,6 +98,7 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
     if (!a)
         return uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
     const uint32_t S = b->n_slices;
+    a->line = b->geoms[b->geom_of[0]].model != 0;
     uir_status s = UIR_OK;
     uirk_asd_slice *sl = (uirk_asd_slice *)calloc(S, sizeof *sl);
     uirk_sart_slice *sa = (uirk_sart_slice *)calloc(2 * (size_t)S, sizeof *sa);
@@ -160,6 +165,10 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
             f->n_angles = (int32_t)G->A;
             f->axis = G->axis;
             f->corr = -1.0f;
+            if (a->line) { /* in place, with the model's angle table */
+                f->x_out = b->x[i];
+                f->line = G->d_line;
+            }
             sa[S + i] = *f;
             sa[S + i].x_in = sl[i].g;
             sa[S + i].x_out = b->x[i];
@@ -174,6 +183,7 @@ static uir_status asd_create(uir_batch *b, struct uir_asd **out)
             dd[i].n_angles = (int32_t)G->A;
             dd[i].axis = G->axis;
             dd[i].corr = -1.0f;
+            dd[i].line = a->line ? G->d_lk : NULL;
         }
     }
     if ((s = upload((void **)&a->d_sl, sl, S * sizeof *sl)) != UIR_OK ||
@@ -218,7 +228,11 @@ static uir_status launch_asd_iteration(uir_batch *b, const uir_asdpocs_params *p
      * angle j, from buffer j % 2 to (j + 1) % 2 (x, g); the last launch only
      * backprojects.  An odd launch count leaves the image in g. */
     const uint32_t A = a->a_max;
-    for (uint32_t pass = 0; pass < p->df_iterations; ++pass) {
+    for (uint32_t pass = 0; pass < p->df_iterations && a->line; ++pass)
+        for (uint32_t j = 0; j < A; ++j)
+            HIP(uirk_sart_line(a->d_sart[0], a->d_state, S, (int)a->n_max, (int)a->d_max, b->sdt, b->idt, (int)j, st),
+                "SART line launch");
+    for (uint32_t pass = 0; pass < p->df_iterations && !a->line; ++pass) {
         for (uint32_t j = 0; j <= A; ++j) {
             const int fp = j < A ? (int)j : -1;
             HIP(uirk_sart_step(a->d_sart[j % 2], a->d_state, S, (int)a->n_max, b->sdt, b->idt, (int)j - 1, fp, b->d_err,
@@ -236,9 +250,12 @@ static uir_status launch_asd_iteration(uir_batch *b, const uir_asdpocs_params *p
     for (int g = 0; g < UIR_FP_SHAPES; ++g) {
         const kgroup *kg = &b->fpg[g];
         if (kg->n_items)
-            HIP(uirk_fp_table(a->d_dd_sl, NULL, (const uirk_fp_item *)kg->d_items, kg->n_items, &kg->part, kg->mode, b->sdt,
-                              b->idt, &kg->tile, kg->grid, b->d_err, st), "residual FP launch");
+            HIP(uirk_fp_table(a->d_dd_sl, NULL, (const uirk_fp_item *)kg->d_items, kg->n_items, &kg->part, kg->mode,
+                              b->sdt, b->idt, &kg->tile, kg->grid, b->d_err, st), "residual FP launch");
     }
+    if (b->fpl.n_items) /* the line model's table (mode 2) */
+        HIP(uirk_fp_table(a->d_dd_sl, NULL, (const uirk_fp_item *)b->fpl.d_items, b->fpl.n_items, &b->fpl.part, 2,
+                          b->sdt, b->idt, &b->fpl.tile, b->fpl.grid, b->d_err, st), "residual FP launch");
     HIP(uirk_asd_sino_norm(b->sdt, a->d_sl, S, nblk, tb, a->d_partials, st), "residual norm");
     FINISH(UIRK_ASD_DD);
     const uint32_t ng = tb ? p->ng : 20; /* tvstd :573 takes 20 steps; ng is unused there */
@@ -262,9 +279,9 @@ uir_status uir_batch_asdpocs(uir_batch *b, uint32_t iterations, const uir_asdpoc
     if (!(p->beta_red >= 0.0f) || !(p->alpha >= 0.0f) || !(p->alpha_red >= 0.0f) || !(p->r_max >= 0.0f) ||
         !isfinite(p->beta) || p->ng > 1000 || p->df_iterations == 0)
         return uir_set_error(UIR_ERR_INVALID_ARGUMENT, "bad ASD-POCS parameters");
-    for (uint32_t g = 0; g < b->n_geoms; ++g)
-        if (b->geoms[g].model != 0) /* its fused SART step is joseph's */
-            return uir_set_error(UIR_ERR_UNSUPPORTED, "batched ASD-POCS takes the joseph model only");
+    for (uint32_t g = 1; g < b->n_geoms; ++g)
+        if (b->geoms[g].model != b->geoms[0].model) /* one data step for the whole batch */
+            return uir_set_error(UIR_ERR_UNSUPPORTED, "batched ASD-POCS takes slices of one projection model");
     TRY(uir_check_hip(hipSetDevice(b->res->device), "hipSetDevice"));
     if (!b->asd)
         TRY(asd_create(b, &b->asd));
