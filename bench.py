@@ -978,7 +978,11 @@ def main():
         "config": {"workload": "batched SIRT, %d slices of %dx%d, %d angles over [0, pi), %d bins, relaxation %.2f"
                                % (args.batch, n, n, a, d, relax),
                    "batch": args.batch, "image": n, "angles": a, "bins": d, "storage": args.storage,
-                   "tile": tune["best"]},
+                   "tile": tune["best"],
+                   # consecutive slices of the geometry share BP items (slice pairs) when the
+                   # paired table still fills the device
+                   "bp_slice_pairs": bool(info["bp_items"] * 2 <= args.batch * ((n + 63) // 64)
+                                          * ((n + tune["best"]["bp_rows"] - 1) // tune["best"]["bp_rows"]))},
         "plain_fp32": {"value": round(plain_rate, 2), "unit": "slice-iters/s", "ms_per_iteration": round(plain_ms, 4),
                        "path": "SIRT task, one slice per process(), reference-compatible FP32"},
         "speedup_vs_plain": round(value / plain_rate, 3),
