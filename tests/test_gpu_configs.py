@@ -2,7 +2,8 @@
 
   - the headline batch: 1024 x 1024 slices, 720 angles, 1448 bins, 1 % noise,
     FP16 sinograms (f16-sino), 10 SIRT iterations, with and without the ray
-    weight floor, against the FP64 SIRT (ufo-ir-sirt-task.c:176-215);
+    weight floor, as a BP slice pair and as single slices, against the FP64
+    SIRT (ufo-ir-sirt-task.c:176-215); the same shape with the line model;
   - plain SART on 1024 x 1024 / 720 / 1448 with 1 % noise, 2 sweeps,
     relaxation 0.5, against the reference-compatible FP32 oracle
     (ufo-ir-sart-task.c:193-220);
@@ -38,27 +39,63 @@ def _spec(n, d, a, relax, rmin=0.0):
 
 
 @pytest.mark.parametrize("rmin", [0.0, 0.1])
-def test_headline_batch_f16_sino_matches_fp64(res, rmin):
+def test_headline_batch_f16_sino_matches_fp64(res, rmin, monkeypatch):
     """Two slices of the metric's shape, FP16 sinograms, 10 iterations; the
     FP64 reference applies the same ray-weight floor (0 is the reference's
-    inv, ufo-ir-basic-ops.cl:18-36)."""
+    inv, ufo-ir-basic-ops.cl:18-36).  The two slices run once as a BP slice
+    pair (the headline's path at 32 slices) and once as single slices, with
+    bit-identical images."""
     n, d, a, iters, relax = 1024, 1448, 720, 10, 0.25
     print("case headline f16-sino", n, d, a, "ray_weight_min", rmin, flush=True)
     g = _geom(n, d, a)
     sinos = [_noisy(g, 20261203 + k) for k in range(2)]
-    bt = uir.Batch(res, [_spec(n, d, a, relax, rmin)] * 2, "f16-sino")
-    for k, s in enumerate(sinos):
-        bt.set_sinogram(k, s.astype(np.float32))
-    bt.reset()
-    bt.iterate(iters)
+    images, items = {}, {}
+    for pairs in ("0", "1000000000"):
+        monkeypatch.setenv("UIR_PAIR_MIN_ITEMS", pairs)
+        bt = uir.Batch(res, [_spec(n, d, a, relax, rmin)] * 2, "f16-sino")
+        items[pairs] = bt.info()["bp_items"]
+        for k, s in enumerate(sinos):
+            bt.set_sinogram(k, s.astype(np.float32))
+        bt.reset()
+        bt.iterate(iters)
+        images[pairs] = [bt.image(k) for k in range(2)]
+        bt.close()
+    assert 2 * items["0"] == items["1000000000"]  # paired, then single
+    for k in range(2):
+        assert np.array_equal(images["0"][k], images["1000000000"][k]), k
+    bt_images = images["0"]
     for k, s in enumerate(sinos):
         # the reference starts from the FP32 data handed to the batch (the
         # device's FP16 rounding of it counts against the batched path)
         ref = O.sirt64(g, s.astype(np.float32).astype(np.float64), iters, relax, ray_weight_min=rmin)
-        got = bt.image(k)
+        got = bt_images[k]
         err = O.rel_l2(got, ref)
         print("slice", k, "rel-L2 vs FP64 %.3e, PSNR %.1f dB" % (err, O.psnr(got, ref)), flush=True)
         assert err <= 1e-3
+
+
+def test_line_headline_shape_matches_fp64(res, monkeypatch):
+    """The line model at the metric's shape: two slices with the same data run
+    as a BP slice pair (mode 4), FP16 sinograms, 10 iterations, against the
+    oracle's FP64 SIRT over the model; both slices of the pair give the same
+    image."""
+    n, d, a, iters, relax = 1024, 1448, 720, 10, 0.25
+    print("case line headline f16-sino", n, d, a, flush=True)
+    monkeypatch.setenv("UIR_PAIR_MIN_ITEMS", "0")
+    g = O.Geometry(n, d, a, np.float32(np.pi / a), model="line")
+    sino = _noisy(g, 20261205).astype(np.float32)
+    bt = uir.Batch(res, [dict(_spec(n, d, a, relax), model="line")] * 2, "f16-sino")
+    assert bt.info()["bp_items"] == 256
+    for k in range(2):
+        bt.set_sinogram(k, sino)
+    bt.reset()
+    bt.iterate(iters)
+    got = [bt.image(k) for k in range(2)]
+    assert np.array_equal(got[0], got[1])
+    ref = O.sirt64(g, sino.astype(np.float64), iters, relax)
+    err = O.rel_l2(got[0], ref)
+    print("line rel-L2 vs FP64 %.3e" % err, flush=True)
+    assert err <= 1e-3
 
 
 def test_sart_task_full_shape_matches_oracle(res):
