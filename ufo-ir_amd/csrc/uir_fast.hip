@@ -113,6 +113,21 @@ __device__ __forceinline__ int cvt_flr(float x)
 // end, so either index gives the same value there.
 __device__ __forceinline__ int rne_index(float x) { return __float_as_int(x + 12582912.0f) - 0x4B400000; }
 
+// Entry rne_index(x) of an LDS table through a base biased by the bits of
+// 1.5 2^23 (lds_rne_base, once per table): the sample's address is then one
+// shift-add of x + 1.5 2^23's bits.  Unsigned 32-bit arithmetic wraps, so
+// the biased base may lie outside LDS; the sum is the entry's address.
+template <typename E> using lds_ptr = const __attribute__((address_space(3))) E *;
+template <typename E> __device__ __forceinline__ uint32_t lds_rne_base(const E *table)
+{
+    return (uint32_t)(uintptr_t)(lds_ptr<E>)table - 0x4B400000u * (uint32_t)sizeof(E);
+}
+template <typename E> __device__ __forceinline__ E lds_rne_read(uint32_t base, float x)
+{
+    const uint32_t a = (uint32_t)__float_as_int(x + 12582912.0f) * (uint32_t)sizeof(E) + base;
+    return *(const E *)(lds_ptr<E>)(uintptr_t)a;
+}
+
 // The line model's remapped fraction at coordinate x of the pair FP, whose
 // entry is rne_index(x): f - 1/2 = x - rne(x) (exact), and
 // phi = clamp((f - 1/2) k + b, 0, 1).
@@ -1047,13 +1062,15 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                             } else if constexpr (REF || LINE) {
                                 lds[k * kBpWb + i] = v0[e];
                             } else {
-                                // (e, d) with d = t[i+1] - t[i], e = t[i] - i d: the value at
-                                // window coordinate w in [i, i+1) is e + w d
+                                // (e, d) with d = t[i+1] - t[i], e = t[i] - (i - 1/2) d: the
+                                // value at window coordinate w in [i, i+1] is e + (w - 1/2) d,
+                                // and rne_index(w - 1/2) is i or (at w = i or i + 1 exactly)
+                                // a neighbour whose entry gives the same value
                                 const float d = v1[e] - v0[e];
                                 const float d2 = PAIR ? y1[e] - y0[e] : 0.0f;
-                                ldp[k * kBpWb + i] =
-                                    make_entry<PAIR>(__builtin_fmaf(-(float)i, d, v0[e]), d,
-                                                     PAIR ? __builtin_fmaf(-(float)i, d2, y0[e]) : 0.0f, d2);
+                                const float ih = (float)i - 0.5f;
+                                ldp[k * kBpWb + i] = make_entry<PAIR>(__builtin_fmaf(-ih, d, v0[e]), d,
+                                                                      PAIR ? __builtin_fmaf(-ih, d2, y0[e]) : 0.0f, d2);
                             }
                         }
                     }
@@ -1085,10 +1102,12 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                             }
                         }
                     } else {
-                        // coordinates relative to the staged window: the address of a
-                        // sample is then P + 8 floor(uu), one shift-add
-                        const E *P = ldp + k * kBpWb;
-                        float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 0.5f)) - (float)s_umin[k];
+                        // coordinates relative to the staged window, less 1/2: a
+                        // sample's entry is rne_index of it, one float add (floor-converts
+                        // and shifts issue at half the rate of float adds), and its address
+                        // one shift-add
+                        const uint32_t P = lds_rne_base(ldp + k * kBpWb);
+                        float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 1.0f)) - (float)s_umin[k];
                         const float du = -4.0f * st;
                         if constexpr (LINE) {
                             const float4 L = s_line[k]; // (bp_k, bp_b0, bp_b1, 1/M)
@@ -1150,7 +1169,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
                                     w[h] = __builtin_fmaf((float)h, du, u);
-                                    p[h] = P[cvt_flr(w[h])];
+                                    p[h] = lds_rne_read<E>(P, w[h]);
                                 }
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
@@ -1161,9 +1180,9 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                 u = __builtin_fmaf(8.0f, du, u);
                             }
                         } else {
-                            // u clamped to [edge0, edge1] (:203-210), in window coordinates
-                            const float wlo = (edge0 - 0.5f) - (float)s_umin[k];
-                            const float whi = (edge1 - 0.5f) - (float)s_umin[k];
+                            // u clamped to [edge0, edge1] (:203-210), in window coordinates less 1/2
+                            const float wlo = (edge0 - 1.0f) - (float)s_umin[k];
+                            const float whi = (edge1 - 1.0f) - (float)s_umin[k];
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
                                 E p[8];
@@ -1171,7 +1190,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
                                     w[h] = __builtin_amdgcn_fmed3f(__builtin_fmaf((float)h, du, u), wlo, whi);
-                                    p[h] = P[cvt_flr(w[h])];
+                                    p[h] = lds_rne_read<E>(P, w[h]);
                                 }
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
