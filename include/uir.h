@@ -331,8 +331,12 @@ uir_status uir_batch_asdpocs_diagnostics(uir_batch *batch, uint32_t slice, float
 /* each on a probe of the geometry and keeps the fastest per geometry and    */
 /* storage dtype; batches and the server then plan each geometry's work with */
 /* its tuned shape (the default 16 / 32 / 64 when untuned).  Variants apply  */
-/* the same arithmetic; only the anchors of the FP32 ray and pixel          */
-/* coordinates move with the tile, so results agree to FP32 rounding.       */
+/* the same arithmetic.  The BP's shape does not change any bit: a pixel's   */
+/* window and coordinates are those of its 64 x 64 tile, and a batch or      */
+/* server step whose BP table would not fill the device (4 work-groups per   */
+/* CU) takes tiles of 32 rows whatever the tuned rows.  The FP's ray         */
+/* coordinates are anchored per work item, so FP variants agree to FP32      */
+/* rounding.                                                                  */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     int32_t fp_angles;  /* 16, 8 or 32 */

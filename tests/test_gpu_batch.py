@@ -96,9 +96,11 @@ def test_batch_slice_pairs_equal_single_slices(res, storage, monkeypatch):
 
 
 def test_batch_pairs_slices_by_default_only_at_scale(res, monkeypatch):
-    """With the default threshold, BP slice pairs are used only while the
-    paired table keeps 4 workgroups per CU: ten 1024 x 1024 slices of one
-    geometry pair up (half the BP items), two do not.  The paired results
+    """With the default threshold (4 workgroups per CU), a BP table that
+    would hold no more items than that takes tiles of 32 rows instead of 64,
+    and BP slice pairs are used only while the paired table keeps at least
+    that many: ten 1024 x 1024 slices of one geometry pair up on 64-row tiles
+    (half the BP items), one or two take 32-row tiles unpaired.  The results
     equal the single-slice ones bit for bit."""
     monkeypatch.delenv("UIR_PAIR_MIN_ITEMS", raising=False)
     n, d, a, iters = 1024, 1448, 90, 2
@@ -106,16 +108,22 @@ def test_batch_pairs_slices_by_default_only_at_scale(res, monkeypatch):
     print("case default pairing", n, d, a, flush=True)
     rng = np.random.default_rng(5)
     sinos = [rng.uniform(0.0, 50.0, (a, d)).astype(np.float32) for _ in range(10)]
-    n_cu = uir.probe_ceilings(res)["n_cu"]
+    fill = 4 * uir.probe_ceilings(res)["n_cu"]
+    tiles = (n // 64) ** 2  # 64 x 64 tiles of one slice (the untuned BP tile)
+
+    def expected(k):  # BP items of k slices: the two rules
+        per = 2 * tiles if k * tiles <= fill else tiles
+        paired = (k // 2 + k % 2) * per
+        return paired if paired >= fill else k * per
+
     solo = uir.Batch(res, [spec], "f16-sino")
-    per = solo.info()["bp_items"]
     big = uir.Batch(res, [spec] * 10, "f16-sino")
-    assert 5 * per >= 4 * n_cu
-    assert big.info()["bp_items"] == 5 * per
-    if per < 4 * n_cu:  # one pair would leave fewer than 4 items per CU
-        small = uir.Batch(res, [spec] * 2, "f16-sino")
-        assert small.info()["bp_items"] == 2 * per
-        small.close()
+    small = uir.Batch(res, [spec] * 2, "f16-sino")
+    assert 10 * tiles > fill and 5 * tiles >= fill and 2 * tiles <= fill  # the cases meant
+    assert solo.info()["bp_items"] == expected(1) == 2 * tiles
+    assert big.info()["bp_items"] == expected(10) == 5 * tiles
+    assert small.info()["bp_items"] == expected(2) == 4 * tiles
+    small.close()
     for i, b in enumerate(sinos):
         big.set_sinogram(i, b)
     big.reset()
@@ -216,7 +224,9 @@ def test_positive_constraint_and_info(res):
     bt.iterate(10)
     assert bt.image(0).min() >= 0.0
     info = bt.info()
-    assert info["n_slices"] == 1 and info["fp_items"] > 0 and info["bp_items"] == 4
+    # 128 x 128: 2 x 2 tiles of 64 x 64, too few to fill the device, so 2 x 4
+    # tiles of 32 rows
+    assert info["n_slices"] == 1 and info["fp_items"] > 0 and info["bp_items"] == 8
     assert info["bp_samples"] == a * n * n
 
 

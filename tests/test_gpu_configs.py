@@ -43,8 +43,9 @@ def test_headline_batch_f16_sino_matches_fp64(res, rmin, monkeypatch):
     """Two slices of the metric's shape, FP16 sinograms, 10 iterations; the
     FP64 reference applies the same ray-weight floor (0 is the reference's
     inv, ufo-ir-basic-ops.cl:18-36).  The two slices run once as a BP slice
-    pair (the headline's path at 32 slices) and once as single slices, with
-    bit-identical images."""
+    pair on 64-row tiles (the headline's path at 32 slices) and once as
+    single slices on 32-row tiles (what the default thresholds give two
+    slices), with bit-identical images."""
     n, d, a, iters, relax = 1024, 1448, 720, 10, 0.25
     print("case headline f16-sino", n, d, a, "ray_weight_min", rmin, flush=True)
     g = _geom(n, d, a)
@@ -60,7 +61,7 @@ def test_headline_batch_f16_sino_matches_fp64(res, rmin, monkeypatch):
         bt.iterate(iters)
         images[pairs] = [bt.image(k) for k in range(2)]
         bt.close()
-    assert 2 * items["0"] == items["1000000000"]  # paired, then single
+    assert 4 * items["0"] == items["1000000000"]  # paired 64-row tiles, then single 32-row ones
     for k in range(2):
         assert np.array_equal(images["0"][k], images["1000000000"][k]), k
     bt_images = images["0"]

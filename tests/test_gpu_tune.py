@@ -97,3 +97,41 @@ def test_server_autotune_uses_the_tuned_shape(res):
     batched, = _run(res, [spec], [b], "f16", 5)      # the batch plans with the same tuned shape
     uir.tune_clear()
     assert np.array_equal(served, batched)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16-sino", "bf16"])
+def test_bp_shape_does_not_change_results(res, monkeypatch, storage):
+    """The BP's shape changes the launch, not the arithmetic: a pixel's BP
+    window and sample coordinates are those of its 64 x 64 super-tile and its
+    angle sum runs in angle order.  With the FP's shape fixed, every BP shape
+    (32 or 16 staged angles, tiles of 64 or 32 rows), with slice pairs
+    (UIR_PAIR_MIN_ITEMS=0: the tuned rows) and without (a huge threshold,
+    which also selects 32-row tiles for a table that small), gives the same
+    bits: joseph with a centred and an off-centre axis (the clamped
+    coordinate), and the line model."""
+    cases = [  # (n, d, a, axis_position, model), two slices each
+        (160, 227, 75, -1.0, "joseph"),
+        (97, 137, 60, 70.3, "joseph"),
+        (128, 182, 90, -1.0, "line"),
+    ]
+    specs, sinos = [], []
+    for k, (n, d, a, ax, m) in enumerate(cases):
+        g = O.Geometry(n, d, a, np.float32(np.pi / a), ax)
+        for j in range(2):
+            specs.append(dict(_spec(n, d, a), axis_position=ax, model=m, positive_constraint=j == 1,
+                              ray_weight_min=0.1 if m == "line" else 0.0))
+            sinos.append(O.fp64(g, ellipse_phantom(n, seed=40 + 2 * k + j).astype(np.float64)))
+    print("case bp shapes", storage, cases, flush=True)
+    got = {}
+    for pairs in ("0", "1000000000"):
+        monkeypatch.setenv("UIR_PAIR_MIN_ITEMS", pairs)
+        for ba, br in [(32, 64), (32, 32), (16, 64), (16, 32)]:
+            uir.tune_clear()
+            for sp in specs[:4]:
+                uir.tune_set(sp, storage, 16, ba, br)
+            got[(pairs, ba, br)] = _run(res, specs, sinos, storage, 4)
+    uir.tune_clear()
+    ref = got[("0", 32, 64)]
+    for key, imgs in got.items():
+        for i in range(len(specs)):
+            assert np.array_equal(imgs[i], ref[i]), (key, i)

@@ -286,6 +286,12 @@ double uirk_fp_item_cost(const uirk_fp_item *it);
  * environment variable UIR_PAIR_MIN_ITEMS when set (0: pair whenever two
  * slices of one geometry are adjacent). */
 int uirk_plan_pair_min_items(void);
+/* 1 if a BP table of this many items (at the tuned tile rows, before
+ * pairing) holds no more than uirk_plan_pair_min_items(): too few workgroups
+ * to fill the device, so its tiles take 32 rows instead of 64.  A pixel's BP
+ * result does not depend on its tile's rows (its window and sample
+ * coordinates are those of its 64 x 64 super-tile). */
+int uirk_plan_bp_rows32(size_t items);
 /* FP32 <-> storage conversions */
 int uirk_convert(const void *src, int src_dtype, void *dst, int dst_dtype, size_t n, void *stream);
 

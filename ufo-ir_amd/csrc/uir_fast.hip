@@ -113,6 +113,16 @@ __device__ __forceinline__ int cvt_flr(float x)
 // end, so either index gives the same value there.
 __device__ __forceinline__ int rne_index(float x) { return __float_as_int(x + 12582912.0f) - 0x4B400000; }
 
+// -(float)n for 0 <= n < 2^23, assembled from integer operations: for a
+// wave-uniform n the compiler keeps them, and the result, in scalar
+// registers (a float conversion would put it in a vector register).
+__device__ __forceinline__ float neg_float_of(int n)
+{
+    const int e = 31 - __builtin_clz((unsigned)n | 1u);
+    const unsigned bits = 0x80000000u | ((unsigned)(127 + e) << 23) | (((unsigned)n << (23 - e)) & 0x7FFFFFu);
+    return n == 0 ? -0.0f : __uint_as_float(bits);
+}
+
 // Entry rne_index(x) of an LDS table through a base biased by the bits of
 // 1.5 2^23 (lds_rne_base, once per table): the sample's address is then one
 // shift-add of x + 1.5 2^23's bits.  Unsigned 32-bit arithmetic wraps, so
@@ -917,7 +927,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
     __shared__ int s_umin[NA];
     __shared__ float s_trig[2 * NA];
     __shared__ float4 s_line[LINE ? NA : 1];
-    __shared__ float s_base[LINE ? NA : 1]; // LINE: window coordinate at the tile corner
+    __shared__ float s_base[REF ? 1 : NA]; // fast path: window coordinate at the super-tile's corner
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const ItemRange range = item_range(part, n_items);
@@ -959,11 +969,15 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
         // operation per sample; uniform over the slice).
         const bool clamp_u =
             !REF && !LINE && ((edge0 - 0.5f) != floorf(edge0 - 0.5f) || (edge1 - 0.5f) != floorf(edge1 - 0.5f));
-        // tile corner coordinates (pixel centres)
+        // Window corners (pixel centres): those of the 64 x 64 super-tile that
+        // holds the tile.  A tile of 32 rows stages its super-tile's window and,
+        // on the fast path, computes its super-tile's sample coordinates, so a
+        // pixel's result does not depend on the tile's rows or staged angles.
+        const int y0s = item.y0 & ~(kBpT - 1);
         const float cx0 = (float)item.x0 + 0.5f - origin_shift;
         const float cx1 = cx0 + (float)(kBpT - 1);
-        const float cy0 = (float)item.y0 + 0.5f - origin_shift;
-        const float cy1 = cy0 + (float)(4 * ROWS - 1);
+        const float cy0 = (float)y0s + 0.5f - origin_shift;
+        const float cy1 = cy0 + (float)(kBpT - 1);
         const float fY0 = (float)(item.y0 + wave) + 0.5f - origin_shift;
 
         float acc[ROWS], val[ROWS], acc2[PAIR ? ROWS : 1];
@@ -973,6 +987,15 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
             val[k] = 0.0f;
             if constexpr (PAIR)
                 acc2[k] = 0.0f;
+        }
+        // fast path: minus the offset of each of this thread's rows within the
+        // super-tile, uniform over a wave (scalar registers)
+        float nys[REF ? 1 : ROWS];
+        if constexpr (!REF) {
+            const int ys0 = item.y0 - y0s + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q)
+                nys[q] = neg_float_of(ys0 + 4 * q);
         }
         if (REF && !S->pix_w) {
             const TI *xin = uniform((const TI *)S->x);
@@ -1010,10 +1033,14 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                         const float *lt = uniform(S->line);
                         s_line[threadIdx.x] = make_float4(gload(lt, 8 * ang + 4), gload(lt, 8 * ang + 5),
                                                           gload(lt, 8 * ang + 6), gload(lt, 8 * ang + 7));
-                        // u - 1/2 - umin at the tile corner, rounded once: the
-                        // weights multiply coordinate errors by 1 / (M m)
+                    }
+                    if constexpr (!REF) {
+                        // u - 1/2 - umin at the super-tile's corner (joseph: less
+                        // another 1/2, as its entries are found by rounding to
+                        // nearest), rounded once; the line weights multiply
+                        // coordinate errors by 1 / (M m)
                         s_base[threadIdx.x] = (float)((double)cx0 * (double)ct - (double)cy0 * (double)st +
-                                                      (double)shift - 0.5 - (double)umin);
+                                                      (double)shift - (LINE ? 0.5 : 1.0) - (double)umin);
                     }
                 }
                 __syncthreads();
@@ -1102,17 +1129,15 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                             }
                         }
                     } else {
-                        // coordinates relative to the staged window, less 1/2: a
-                        // sample's entry is rne_index of it, one float add (floor-converts
-                        // and shifts issue at half the rate of float adds), and its address
-                        // one shift-add
-                        const uint32_t P = lds_rne_base(ldp + k * kBpWb);
-                        float u = __builtin_fmaf(fX, ct, __builtin_fmaf(-fY0, st, shift - 1.0f)) - (float)s_umin[k];
-                        const float du = -4.0f * st;
+                        // Window coordinates: row q's is fma(-ys_q, st, A), one FMA from
+                        // the thread's column term A and the row's offset ys_q in the
+                        // super-tile, whatever the tile's rows.  Joseph: less 1/2, so a
+                        // sample's entry is rne_index of it, one float add
+                        // (floor-converts and shifts issue at half the rate of float
+                        // adds), and its address one shift-add.
+                        const float A = __builtin_fmaf((float)lane, ct, s_base[k]);
                         if constexpr (LINE) {
                             const float4 L = s_line[k]; // (bp_k, bp_b0, bp_b1, 1/M)
-                            // this thread's column and first row, relative to the tile corner
-                            u = __builtin_fmaf((float)lane, ct, __builtin_fmaf(-(float)wave, st, s_base[k]));
                             if constexpr (PAIR) {
                                 const float2 *B2 = ldl + k * kBpWb;
 #pragma unroll
@@ -1121,7 +1146,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                     float fh[8];
 #pragma unroll
                                     for (int h = 0; h < 8; ++h) {
-                                        const float w = __builtin_fmaf((float)h, du, u); // >= 1: inside the window
+                                        const float w = __builtin_fmaf(nys[g + h], st, A); // >= 1: inside the window
                                         const int i0 = cvt_flr(w);
                                         p[h] = B2[i0];
                                         q[h] = B2[i0 + 1];
@@ -1137,7 +1162,6 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                         acc2[g + h] =
                                             __builtin_fmaf(w1, q[h].y, __builtin_fmaf(w0, p[h].y, acc2[g + h]));
                                     }
-                                    u = __builtin_fmaf(8.0f, du, u);
                                 }
                             } else {
                                 const float *B = lds + k * kBpWb;
@@ -1147,7 +1171,7 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                     float fh[8];
 #pragma unroll
                                     for (int h = 0; h < 8; ++h) {
-                                        const float w = __builtin_fmaf((float)h, du, u); // >= 1: inside the window
+                                        const float w = __builtin_fmaf(nys[g + h], st, A); // >= 1: inside the window
                                         const int i0 = cvt_flr(w);
                                         p[h] = make_float2(B[i0], B[i0 + 1]);
                                         fh[h] = fract_pos(w) - 0.5f;
@@ -1158,17 +1182,21 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                         const float w1 = __builtin_amdgcn_fmed3f(__builtin_fmaf(fh[h], L.x, L.z), 0.0f, L.w);
                                         acc[g + h] = __builtin_fmaf(w1, p[h].y, __builtin_fmaf(w0, p[h].x, acc[g + h]));
                                     }
-                                    u = __builtin_fmaf(8.0f, du, u);
                                 }
                             }
                         } else if (!clamp_u) {
+                            const uint32_t P = lds_rne_base(ldp + k * kBpWb);
+                            // a copy of A per branch: the compiler would otherwise hoist
+                            // every row's coordinate above the branch, all live at once
+                            float Ab = A;
+                            asm volatile("" : "+v"(Ab));
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
                                 E p[8];
                                 float w[8];
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
-                                    w[h] = __builtin_fmaf((float)h, du, u);
+                                    w[h] = __builtin_fmaf(nys[g + h], st, Ab);
                                     p[h] = lds_rne_read<E>(P, w[h]);
                                 }
 #pragma unroll
@@ -1177,19 +1205,21 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                     if constexpr (PAIR)
                                         acc2[g + h] = __builtin_fmaf(w[h], p[h].w, acc2[g + h] + p[h].z);
                                 }
-                                u = __builtin_fmaf(8.0f, du, u);
                             }
                         } else {
                             // u clamped to [edge0, edge1] (:203-210), in window coordinates less 1/2
                             const float wlo = (edge0 - 1.0f) - (float)s_umin[k];
                             const float whi = (edge1 - 1.0f) - (float)s_umin[k];
+                            const uint32_t P = lds_rne_base(ldp + k * kBpWb);
+                            float Ab = A;
+                            asm volatile("" : "+v"(Ab));
 #pragma unroll
                             for (int g = 0; g < ROWS; g += 8) {
                                 E p[8];
                                 float w[8];
 #pragma unroll
                                 for (int h = 0; h < 8; ++h) {
-                                    w[h] = __builtin_amdgcn_fmed3f(__builtin_fmaf((float)h, du, u), wlo, whi);
+                                    w[h] = __builtin_amdgcn_fmed3f(__builtin_fmaf(nys[g + h], st, Ab), wlo, whi);
                                     p[h] = lds_rne_read<E>(P, w[h]);
                                 }
 #pragma unroll
@@ -1198,7 +1228,6 @@ __global__ __launch_bounds__(256, REF ? 1 : (LINE ? 4 : (PAIR ? (NA == 32 ? 3 : 
                                     if constexpr (PAIR)
                                         acc2[g + h] = __builtin_fmaf(w[h], p[h].w, acc2[g + h] + p[h].z);
                                 }
-                                u = __builtin_fmaf(8.0f, du, u);
                             }
                         }
                     }

@@ -29,6 +29,13 @@ static int fp_shape(const uirk_tile *t)
 static const int k_fp_shape_angles[3] = {UIRK_FP_ANGLES, 8, 32};
 static int bp_shape(const uirk_tile *t) { return (t->bp_angles == 16) * 2 + (t->bp_rows == 32); }
 
+/* BP tile rows of a geometry in this batch: its tuned rows (the line model's
+ * 64), or 32 when the batch's table would not fill the device with them */
+static int bp_rows_of(const geom_t *G, int rows32)
+{
+    return rows32 ? 32 : (G->model ? UIRK_BP_TILE : G->tile.bp_rows);
+}
+
 #define TRY(x) do { uir_status s_ = (x); if (s_ != UIR_OK) return s_; } while (0)
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -356,14 +363,23 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "calloc");
         goto fail;
     }
+    /* BP tiles of 32 rows when the table at the tuned rows would not fill the
+     * device; a pixel's BP result does not depend on its tile's rows */
+    size_t bp_items = 0;
+    for (uint32_t i = 0; i < n_slices; ++i) {
+        const geom_t *G = &b->geoms[b->geom_of[i]];
+        const int rows = bp_rows_of(G, 0);
+        bp_items += (size_t)((G->n + UIRK_BP_TILE - 1) / UIRK_BP_TILE) * ((G->n + rows - 1) / rows);
+    }
+    const int rows32 = uirk_plan_bp_rows32(bp_items);
     /* consecutive slices of one geometry pair up in the BP (modes 3 and 4),
      * per model if its paired table still fills the device */
     for (int32_t model = 0; model < 2; ++model) {
         size_t paired_items = 0;
         for (uint32_t i = 0; i < n_slices; ++i) {
             const geom_t *G = &b->geoms[b->geom_of[i]];
-            const size_t tiles = (size_t)((G->n + UIRK_BP_TILE - 1) / UIRK_BP_TILE) *
-                                 ((G->n + G->tile.bp_rows - 1) / G->tile.bp_rows);
+            const int rows = bp_rows_of(G, rows32);
+            const size_t tiles = (size_t)((G->n + UIRK_BP_TILE - 1) / UIRK_BP_TILE) * ((G->n + rows - 1) / rows);
             if (G->model != model)
                 continue;
             paired_items += tiles;
@@ -410,7 +426,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             const size_t b0 = bvl.n;
             if (uirk_plan_fp_line(&fvl, (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos, G->runs,
                                   G->n_runs, UIRK_FP_ANGLES, UIRK_FP_WMAX) ||
-                (!second && uirk_plan_bp(&bvl, (int)i, (int)G->n, UIRK_BP_TILE))) {
+                (!second && uirk_plan_bp(&bvl, (int)i, (int)G->n, bp_rows_of(G, rows32)))) {
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
                 goto fail;
             }
@@ -420,11 +436,13 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             /* BP slice pairs: slice i + 1 of the same geometry joins slice i's
              * BP items (pair set) */
             const int second = i > 0 && lead[i - 1];
-            uirk_bp_vec *bvg = &bv[bp_shape(&G->tile)];
+            uirk_tile bt = G->tile;
+            bt.bp_rows = bp_rows_of(G, rows32);
+            uirk_bp_vec *bvg = &bv[bp_shape(&bt)];
             const size_t b0 = bvg->n;
             if (uirk_plan_fp(&fv[fp_shape(&G->tile)], (int)i, (int)G->n, (int)G->D, G->axis, G->h_sin, G->h_cos,
                              G->runs, G->n_runs, G->tile.fp_angles, UIRK_TILE_FP_WMAX(G->tile)) ||
-                (!second && uirk_plan_bp(bvg, (int)i, (int)G->n, G->tile.bp_rows))) {
+                (!second && uirk_plan_bp(bvg, (int)i, (int)G->n, bt.bp_rows))) {
                 s = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
                 goto fail;
             }
@@ -495,6 +513,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
         if (s == UIR_OK) {
             kgroup *k = &b->bpl;
             k->tile = (uirk_tile)UIRK_TILE_DEFAULT;
+            k->tile.bp_rows = rows32 ? 32 : UIRK_BP_TILE;
             k->mode = 2;
             for (size_t i = 0; i < bvl.n; ++i) {
                 cost[i] = (double)specs[bvl.v[i].slice].n_angles * (bvl.v[i].pair ? UIRK_PAIR_COST : 1.0);

@@ -257,6 +257,11 @@ int uirk_plan_pair_min_items(void)
     return n;
 }
 
+int uirk_plan_bp_rows32(size_t items)
+{
+    return items <= (size_t)uirk_plan_pair_min_items();
+}
+
 double uirk_fp_item_cost(const uirk_fp_item *it)
 {
     return (double)it->na * (double)(it->l_hi - it->l_lo) + 16.0; /* + epilogue */

@@ -67,6 +67,9 @@ typedef struct sgeom {
     size_t n_fp;
     uirk_bp_item *bp;
     size_t n_bp;
+    uirk_bp_item *bp32; /* tiles of 32 rows for steps whose BP table would not
+                         * fill the device (NULL: the tile has 32 already) */
+    size_t n_bp32;
     uint64_t bytes;
     int refs;
     double last_used;
@@ -269,6 +272,7 @@ static void geom_free(sgeom *g)
     free(g->fp);
     free(g->fp_cost);
     free(g->bp);
+    free(g->bp32);
     free(g);
 }
 
@@ -341,9 +345,12 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
                                                          UIRK_TILE_FP_WMAX(g->tile))
                                      : uirk_plan_fp(&fv, 0, (int)g->n, (int)g->D, g->axis, g->h_sin, g->h_cos, g->runs,
                                                     g->n_runs, g->tile.fp_angles, UIRK_TILE_FP_WMAX(g->tile));
-        if (fail_fp || uirk_plan_bp(&bv, 0, (int)g->n, g->tile.bp_rows)) {
+        uirk_bp_vec bv32 = {0};
+        if (fail_fp || uirk_plan_bp(&bv, 0, (int)g->n, g->tile.bp_rows) ||
+            (g->tile.bp_rows != 32 && uirk_plan_bp(&bv32, 0, (int)g->n, 32))) {
             free(fv.v);
             free(bv.v);
+            free(bv32.v);
             st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "work table");
             goto fail;
         }
@@ -351,6 +358,8 @@ static uir_status geom_create(uir_server *s, const uir_slice_spec *sp, sgeom **o
         g->n_fp = fv.n;
         g->bp = bv.v;
         g->n_bp = bv.n;
+        g->bp32 = bv32.v;
+        g->n_bp32 = bv32.n;
         g->fp_cost = (double *)malloc(sizeof(double) * (g->n_fp ? g->n_fp : 1));
         if (!g->fp_cost) {
             st = uir_set_error(UIR_ERR_OUT_OF_MEMORY, "malloc");
@@ -587,14 +596,33 @@ static uir_status dgrow(void **p, size_t *cap, size_t need, size_t es)
  * Descriptors are laid out geometry by geometry (slot j holds request
  * ord[j]), so that consecutive requests of one geometry share BP items as
  * slice pairs (modes 3 and 4). */
+/* A geometry's BP template items and group in a step: its tuned tile, or
+ * tiles of 32 rows when the step's table would not fill the device (rows32) */
+static const uirk_bp_item *geom_bp(const sgeom *g, int rows32, size_t *n)
+{
+    const int r32 = rows32 && g->bp32;
+    *n = r32 ? g->n_bp32 : g->n_bp;
+    return r32 ? g->bp32 : g->bp;
+}
+static int geom_bp_shape(const sgeom *g, int rows32)
+{
+    return g->bp_shape == SRV_BP_LINE ? SRV_BP_LINE : (rows32 ? g->bp_shape | 1 : g->bp_shape);
+}
+
 static uir_status tables_build(uir_server *s)
 {
     const uint32_t na = s->n_active;
-    size_t nf[SRV_FP_SHAPES] = {0}, nb[SRV_BP_SHAPES] = {0}, ncost = 1;
+    size_t nf[SRV_FP_SHAPES] = {0}, nb[SRV_BP_SHAPES] = {0}, ncost = 1, bp_items = 0;
+    for (uint32_t i = 0; i < na; ++i)
+        bp_items += s->active[i]->g->n_bp;
+    const int rows32 = uirk_plan_bp_rows32(bp_items);
+    s->bpg[SRV_BP_LINE].tile.bp_rows = rows32 ? 32 : UIRK_BP_TILE;
     for (uint32_t i = 0; i < na; ++i) {
         const sgeom *g = s->active[i]->g;
+        size_t n;
+        (void)geom_bp(g, rows32, &n);
         nf[g->fp_shape] += g->n_fp;
-        nb[g->bp_shape] += g->n_bp;
+        nb[geom_bp_shape(g, rows32)] += n;
     }
     for (int c = 0; c < SRV_FP_SHAPES; ++c)
         ncost = nf[c] > ncost ? nf[c] : ncost;
@@ -633,7 +661,9 @@ static uir_status tables_build(uir_server *s)
     size_t paired[2] = {0, 0};
     for (uint32_t i = 0; i < na; ++i) {
         const sgeom *g = s->active[s->ord[i]]->g;
-        paired[g->model != 0] += g->n_bp;
+        size_t n;
+        (void)geom_bp(g, rows32, &n);
+        paired[g->model != 0] += n;
         if (i + 1 < na && s->active[s->ord[i + 1]]->g == g)
             ++i;
     }
@@ -683,9 +713,11 @@ static uir_status tables_build(uir_server *s)
         } else {
             k->pix_scale = 1.0f / (float)g->A;
         }
-        sgroup *fg = &s->fpg[g->fp_shape], *bg = &s->bpg[g->bp_shape];
+        sgroup *fg = &s->fpg[g->fp_shape], *bg = &s->bpg[geom_bp_shape(g, rows32)];
         uirk_fp_item *fi = (uirk_fp_item *)fg->h_items;
         uirk_bp_item *bi = (uirk_bp_item *)bg->h_items;
+        size_t n_bp;
+        const uirk_bp_item *gbp = geom_bp(g, rows32, &n_bp);
         for (size_t j = 0; j < g->n_fp; ++j) {
             fi[fg->n] = g->fp[j];
             fi[fg->n].slice = (int32_t)i;
@@ -696,8 +728,8 @@ static uir_status tables_build(uir_server *s)
          * has no BP items of its own */
         const int second = lead;
         lead = pair_ok[g->model != 0] && !second && i + 1 < na && s->active[s->ord[i + 1]]->g == g;
-        for (size_t j = 0; j < g->n_bp && !second; ++j) {
-            bi[bg->n] = g->bp[j];
+        for (size_t j = 0; j < n_bp && !second; ++j) {
+            bi[bg->n] = gbp[j];
             bi[bg->n].slice = (int32_t)i;
             bi[bg->n].pair = lead;
             ++bg->n;
