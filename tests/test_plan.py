@@ -282,3 +282,82 @@ def test_pair_kernel_reads_stay_in_the_staged_window(n, d, a, axis_prop, max_ite
     assert lo >= 0, lo          # no entry left of the window
     assert hi <= -1, hi         # no entry at or right of position W
     assert widest <= 192
+
+
+def _fma32(a, b, c):
+    """float32 fused multiply-add: the float32 product is exact in float64."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
+
+
+def _bp_sample_entries(n, d, a, axis_prop, line):
+    """The fast BP's sample entries, recomputed in float32 the way the kernel
+    forms them (uir_fast.hip, k_bp_tab): per angle and 64 x 64 super-tile the
+    window start umin from the super-tile's corners, the window coordinate
+    base at its corner rounded once from double, and each pixel's coordinate
+    fma(-ys, sin, fma(lane, cos, base)), at the four corner pixels (the
+    coordinate is affine in the pixel, so they bound it up to a rounding).
+    Returns the smallest entry index, the largest index read (the line
+    model's second tap included) and the widest window the kernel's own
+    check sees."""
+    f32 = np.float32
+    L = uir.load()
+    s = np.zeros(a, np.float32)
+    c = np.zeros(a, np.float32)
+    fp = C.POINTER(C.c_float)
+    L.uir_geom_lut.argtypes = [C.c_uint32, C.c_float, fp, fp]
+    L.uir_geom_lut(a, float(np.float32(np.pi / a)), s.ctypes.data_as(fp), c.ctypes.data_as(fp))
+    L.uir_geom_axis.restype = C.c_float
+    L.uir_geom_axis.argtypes = [C.c_float, C.c_uint32]
+    axis = f32(L.uir_geom_axis(axis_prop, d))
+    D, N = f32(d), f32(n)
+    diff = D - axis * f32(2)
+    half = (D - axis) if diff < 0 else axis
+    edge0, edge1 = axis - half + f32(0.5), axis + half - f32(0.5)
+    shift = D / f32(2) - diff / f32(2)
+    clamp_u = not line and (np.floor(edge0 - f32(0.5)) != edge0 - f32(0.5) or
+                            np.floor(edge1 - f32(0.5)) != edge1 - f32(0.5))
+    t0 = np.arange(0, n, 64, dtype=np.float32) + f32(0.5) - N / f32(2)     # super-tile corners
+    cx0, cy0 = [m.ravel()[None, :] for m in np.meshgrid(t0, t0)]
+    cx1, cy1 = cx0 + f32(63), cy0 + f32(63)
+    st, ct = s[:, None], c[:, None]
+    us = [cx * ct - cy * st for cx in (cx0, cx1) for cy in (cy0, cy1)]       # float32, no contraction
+    umn = np.minimum(np.minimum(us[0], us[1]), np.minimum(us[2], us[3])) + shift
+    umx = np.maximum(np.maximum(us[0], us[1]), np.maximum(us[2], us[3])) + shift
+    if clamp_u:
+        umn, umx = np.clip(umn, edge0, edge1), np.clip(umx, edge0, edge1)
+    umin = np.floor(umn - f32(0.5)).astype(np.int64) - 1
+    widest = int((np.floor(umx - f32(0.5)).astype(np.int64) + 2 - umin).max())
+    base = ((cx0.astype(np.float64) * ct.astype(np.float64) - cy0.astype(np.float64) * st.astype(np.float64)) +
+            float(shift) - (0.5 if line else 1.0) - umin).astype(np.float32)
+    lo, hi = 1 << 30, -(1 << 30)
+    for lane in (0, 63):
+        for ys in (0, 63):
+            w = _fma32(-f32(ys), st, _fma32(f32(lane), ct, base))
+            if clamp_u:
+                w = np.clip(w, f32(edge0 - f32(1)) - umin.astype(np.float32),
+                            f32(edge1 - f32(1)) - umin.astype(np.float32))
+            for dw in (-2e-4, 2e-4):  # a rounding either way
+                x = w.astype(np.float64) + dw
+                i = np.floor(x) if line else np.rint(x)
+                lo, hi = min(lo, int(i.min())), max(hi, int(i.max()) + (1 if line else 0))
+    return lo, hi, widest
+
+
+@pytest.mark.parametrize("n,d,a,axis_prop,line", [
+    (1024, 1448, 720, -1.0, False),
+    (2048, 2897, 900, -1.0, False),
+    (97, 137, 60, 70.3, False),       # an edge inside a bin: the clamped coordinate
+    (256, 363, 180, 140.5, False),
+    (192, 272, 180, -1.0, True),
+    (1024, 1448, 720, -1.0, True),
+    (97, 137, 60, 40.0, True),
+])
+def test_bp_reads_stay_in_the_staged_window(n, d, a, axis_prop, line):
+    """Every sample of the fast BP reads an entry of its angle's staged
+    window (100 entries: the kernel stages bins umin .. umin + 99), for
+    every pixel of every 64 x 64 super-tile, the 32-row tiles included (they
+    take their super-tile's window and coordinates)."""
+    lo, hi, widest = _bp_sample_entries(n, d, a, axis_prop, line)
+    assert widest < 100, widest  # the kernel's own footprint check passes
+    assert lo >= 0, lo
+    assert hi <= 99, hi
