@@ -900,6 +900,11 @@ __global__ __launch_bounds__(256, APT == 8 ? 4 : 6) void k_fp_pairs(const uirk_s
 // ------------------------------------------------------------------------
 // Backprojection
 // ------------------------------------------------------------------------
+// A workgroup backprojects a tile of 64 x 4 ROWS pixels (one column per lane,
+// rows wave + 4 q), NA angles at a time staged in LDS.  The window and, on the
+// fast path, the sample coordinates belong to the 64 x 64 super-tile holding
+// the tile, and each pixel sums its angles in order, so the result of a pixel
+// does not depend on ROWS, NA or slice pairing.
 // LINE (fast path only): the line model.  Bins outside the detector read 0
 // (every ray is active, no clamp), bins are staged as single values (a
 // sample reads two adjacent words in one instruction), and a sample weighs
