@@ -1,5 +1,6 @@
 """The autotuner and the tile-shape variants on the GPU: every variant agrees
-with the FP64 oracle and with the default shape to FP32 rounding, batches
+with the FP64 oracle and with the default shape to FP32 rounding (the BP's
+shape changes no bit at all; the FP's moves its ray anchors), batches
 mixing geometries of different shapes launch one table per shape with each
 slice's result unchanged, and the server's autotune uses the tuned shape."""
 import itertools
