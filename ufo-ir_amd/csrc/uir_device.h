@@ -277,9 +277,6 @@ int uirk_plan_num_xcd(void);
  * (cost[i] >= 0; NULL = unit cost). */
 void uirk_plan_xcd(const double *cost, int n, int n_xcd, uirk_xcd_part *out);
 double uirk_fp_item_cost(const uirk_fp_item *it);
-/* Relative cost of a slice-pair BP item against a single-slice one (shared
- * coordinates and addresses). */
-#define UIRK_PAIR_COST 1.6
 /* BP slice pairs halve the workgroups of a launch and hold fewer waves per
  * SIMD, so they are used only while the paired table keeps at least this
  * many items: 4 workgroups per CU of the current device, or the value of the

@@ -489,8 +489,10 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             k->tile.bp_angles = g >= 2 ? 16 : 32;
             k->tile.bp_rows = (g & 1) ? 32 : UIRK_BP_TILE;
             k->mode = 1;
+            /* an item costs its angles: in a launch with pairs, an unpaired
+             * item runs the pair kernel and costs what a pair does */
             for (size_t i = 0; i < bv[g].n; ++i) {
-                cost[i] = (double)specs[bv[g].v[i].slice].n_angles * (bv[g].v[i].pair ? UIRK_PAIR_COST : 1.0);
+                cost[i] = (double)specs[bv[g].v[i].slice].n_angles;
                 k->mode = bv[g].v[i].pair ? 3 : k->mode;
             }
             uirk_plan_xcd(cost, (int)bv[g].n, uirk_plan_num_xcd(), &k->part);
@@ -516,7 +518,7 @@ uir_status uir_batch_new(uir_resources *res, uir_dtype storage, const uir_slice_
             k->tile.bp_rows = rows32 ? 32 : UIRK_BP_TILE;
             k->mode = 2;
             for (size_t i = 0; i < bvl.n; ++i) {
-                cost[i] = (double)specs[bvl.v[i].slice].n_angles * (bvl.v[i].pair ? UIRK_PAIR_COST : 1.0);
+                cost[i] = (double)specs[bvl.v[i].slice].n_angles;
                 k->mode = bvl.v[i].pair ? 4 : k->mode;
             }
             uirk_plan_xcd(cost, (int)bvl.n, uirk_plan_num_xcd(), &k->part);

@@ -748,8 +748,10 @@ static uir_status tables_build(uir_server *s)
         sgroup *grp = &s->bpg[c];
         const uirk_bp_item *it = (const uirk_bp_item *)grp->h_items;
         grp->mode = c == SRV_BP_LINE ? 2 : 1;
+        /* an item costs its angles, paired or not: in a launch with pairs,
+         * an unpaired item runs the pair kernel and costs what a pair does */
         for (int j = 0; j < grp->n; ++j) {
-            s->h_cost[j] = (double)s->h_bp_sl[it[j].slice].n_angles * (it[j].pair ? UIRK_PAIR_COST : 1.0);
+            s->h_cost[j] = (double)s->h_bp_sl[it[j].slice].n_angles;
             grp->mode = it[j].pair ? (c == SRV_BP_LINE ? 4 : 3) : grp->mode;
         }
         uirk_plan_xcd(s->h_cost, grp->n, uirk_plan_num_xcd(), &grp->part);
